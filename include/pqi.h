@@ -1,0 +1,172 @@
+/*
+ * pqi.h -- C ABI of libpqi.so, the MI355X (gfx950) cut-evaluation and
+ * bit-flip local-search library.
+ *
+ * The ABI replaces PyQrackIsing's OpenCL seam (pyqrackising/maxcut_tfim_util.py:9-301:
+ * OpenCLContext, setup_opencl, make_*_buf) and the host wrappers that drive the
+ * kernels in pyqrackising/kernels.cl.  Each entry point cites the reference
+ * function whose work it takes over.
+ *
+ * Conventions (all from the reference):
+ *   - bits are x in {0,1}^n, one uint8 per node for solver state; candidate
+ *     batches are packed B x ceil(n/32) uint32 words, LSB-first
+ *     (pyqrackising/maxcut_tfim_util.py:304-316);
+ *   - cut(x) = sum_{i<j} w_ij [x_i != x_j]; energy E(x) = 2 cut(x) - sum_{i<j} w_ij
+ *     (pyqrackising/maxcut_tfim_util.py:319-331);
+ *   - selection is the first maximum (np.argmax);
+ *   - dense W must be symmetric with a zero diagonal; CSR input must be strictly
+ *     upper triangular (pyqrackising/maxcut_tfim_util.py:533-542).
+ *
+ * Every function returns PQI_OK (0) or a negative code; pqi_last_error() returns
+ * a thread-local message.  No C++ exception crosses this boundary.  The caller
+ * owns every host array; nothing is retained after a call returns.  A context
+ * owns one HIP stream and is not re-entrant.
+ */
+#ifndef PQI_H
+#define PQI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define PQI_API __attribute__((visibility("default")))
+#else
+#define PQI_API
+#endif
+
+#define PQI_OK 0
+#define PQI_ERR_INVALID (-1)     /* bad argument (shape, dtype, range, symmetry) */
+#define PQI_ERR_NO_DEVICE (-2)   /* no HIP device, or the device index is out of range */
+#define PQI_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define PQI_ERR_OOM (-4)         /* device allocation failed */
+#define PQI_ERR_UNSUPPORTED (-5) /* valid request outside what this build implements */
+
+/* element types accepted for weights */
+#define PQI_DTYPE_I8 1
+#define PQI_DTYPE_I32 2
+#define PQI_DTYPE_F32 3
+#define PQI_DTYPE_F64 4
+
+/* how a graph's weights are held on the device */
+#define PQI_WEIGHTS_I8 1  /* integers in [-127, 127]: int8 storage, int32 fields, exact */
+#define PQI_WEIGHTS_F32 3 /* fp32 storage, fp64 fields and re-scores */
+#define PQI_WEIGHTS_F64 4 /* fp64 storage, fp64 fields and re-scores */
+
+/* double-flip semantics */
+#define PQI_DOUBLE_REFERENCE 0  /* pyqrackising/spin_glass_solver.py:55-92: thread t keeps its last pair */
+#define PQI_DOUBLE_EXHAUSTIVE 1 /* pyqrackising/kernels.cl:186-231: best over all n(n-1)/2 pairs */
+
+/* candidate-evaluation engine for Gray seeds */
+#define PQI_SEEDS_AUTO 0
+#define PQI_SEEDS_MFMA 1     /* E = rowsum((S W) .* S) on int8 MFMA (int8 weights only) */
+#define PQI_SEEDS_LOWRANK 2  /* cut difference from the maintained field, O(|F|^2) per seed */
+
+typedef struct pqi_ctx pqi_ctx;
+typedef struct pqi_graph pqi_graph;
+
+typedef struct {
+    int64_t n;
+    int32_t weights;  /* PQI_WEIGHTS_* */
+    int32_t has_dense;
+    int64_t nnz;      /* nonzeros of the full symmetric matrix */
+    double w_up;      /* sum_{i<j} w_ij */
+    int64_t device_bytes;
+} pqi_graph_info;
+
+/* Parameters of the spin_glass_solver local search
+ * (pyqrackising/spin_glass_solver.py:424-439, 556-560, 603). */
+typedef struct {
+    int32_t is_spin_glass;
+    int32_t double_mode;     /* PQI_DOUBLE_* */
+    int64_t gray_iterations; /* default n*n */
+    int64_t walkers;         /* thread_count = gray_seed_multiple in the reference */
+    int64_t seed_multiple;   /* gray_seed_multiple; block_size = walkers * seed_multiple */
+    int64_t max_rounds;      /* 0 = run until no phase improves */
+    int32_t seed_engine;     /* PQI_SEEDS_* */
+    int32_t flags;           /* PQI_FLAG_* */
+} pqi_search_params;
+
+/* pqi_search_params.flags */
+#define PQI_FLAG_TIME_KERNELS 1 /* time the walk and seed kernels with HIP events on the context stream */
+
+typedef struct {
+    double max_energy; /* reference bookkeeping: start value plus accepted gains */
+    double cut_value;  /* exact re-score of the final state */
+    double min_energy; /* -E (spin glass) or E (maxcut), exact re-score */
+    int64_t rounds;
+    int64_t single_accepts, double_accepts, seed_accepts, walk_accepts, reheat_accepts;
+    int64_t single_evals, double_evals, seed_evals, gray_evals;
+    double seconds_single, seconds_double, seconds_seeds, seconds_walk, seconds_reheat;
+    /* with PQI_FLAG_TIME_KERNELS: summed event-timed kernel durations and launch counts */
+    double walk_kernel_seconds, seed_kernel_seconds;
+    int64_t walk_launches, seed_launches;
+} pqi_search_stats;
+
+/* Thread-local message for the last failing call on this thread. */
+PQI_API const char* pqi_last_error(void);
+PQI_API int pqi_version(void);
+
+/* Number of visible HIP devices (0 with PQI_ERR_NO_DEVICE when none). */
+PQI_API int pqi_device_count(int* count);
+
+/* Replaces the import-time device pick, context and queue creation
+ * (pyqrackising/maxcut_tfim_util.py:96-137). */
+PQI_API int pqi_create(int device, pqi_ctx** ctx);
+PQI_API int pqi_destroy(pqi_ctx* ctx);
+
+/* Replaces make_G_m_buf (pyqrackising/maxcut_tfim_util.py:224-238).  W is n x n
+ * row-major of the given dtype, symmetric, zero diagonal.  Integer-valued
+ * weights in [-127,127] are held as int8 (exact); others as fp32/fp64. */
+PQI_API int pqi_graph_dense(pqi_ctx* ctx, const void* W, int dtype, int64_t n, pqi_graph** g);
+
+/* Replaces make_G_m_csr_buf (pyqrackising/maxcut_tfim_util.py:241-259).  The CSR
+ * is strictly upper triangular (the reference's format); it is symmetrised on
+ * the device side of the library. */
+PQI_API int pqi_graph_csr(pqi_ctx* ctx, const int64_t* indptr, const int64_t* indices, const void* data,
+                  int dtype, int64_t n, int64_t nnz, pqi_graph** g);
+PQI_API int pqi_graph_free(pqi_graph* g);
+PQI_API int pqi_graph_get_info(const pqi_graph* g, pqi_graph_info* info);
+
+/* Candidate scoring.  Replaces run_cut_opencl + the calculate_cut(_sparse)
+ * kernels (pyqrackising/maxcut_tfim.py:212-285, maxcut_tfim_sparse.py:227-306,
+ * kernels.cl:44-94, 233-285).  bits: B x ceil(n/32) uint32.  values (optional, B)
+ * receives cut(x_b) or E(x_b); *best_index the first maximum. */
+PQI_API int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass,
+                   double* values, int64_t* best_index, double* best_value);
+
+/* Single-flip phase.  Replaces run_single_bit_flips / run_bit_flips_opencl(False)
+ * (pyqrackising/spin_glass_solver.py:37-52, 156-241; kernels.cl:100-148).
+ * *gain is the best cut gain, x2 in spin-glass mode; *k its first argmax. */
+PQI_API int pqi_single_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, int64_t* k,
+                         double* gain);
+
+/* Double-flip phase.  Replaces run_double_bit_flips / run_bit_flips_opencl(True)
+ * (pyqrackising/spin_glass_solver.py:55-92, 156-241; kernels.cl:150-231). */
+PQI_API int pqi_double_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass,
+                         int64_t thread_count, int mode, int64_t* k, int64_t* l, double* gain);
+
+/* Gray-code phase.  Replaces pick_gray_seeds + run_gray_optimization
+ * (pyqrackising/spin_glass_solver.py:95-153) and run_gray_search_opencl
+ * (spin_glass_solver.py:244-307; kernels.cl:877-978).
+ * seed_gain: scaled cut difference of the best seed; if > 0 the best seed is
+ * written to state_out and the walk is not run (spin_glass_solver.py:655-661).
+ * Otherwise walk_gain is the best walker's scaled total and state_out its state. */
+PQI_API int pqi_gray_search(pqi_graph* g, const uint8_t* theta, const pqi_search_params* p,
+                    uint8_t* state_out, double* seed_gain, double* walk_gain);
+
+/* The whole spin_glass_solver loop (pyqrackising/spin_glass_solver.py:603-727)
+ * run on the device from theta (in: best_guess, out: final state). */
+PQI_API int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p,
+                     pqi_search_stats* stats);
+
+/* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
+PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PQI_H */

@@ -1,0 +1,760 @@
+/*
+ * pqi_oracle.c -- CPU restatement of PyQrackIsing's cut-evaluation and bit-flip
+ * local-search path (PyQrackIsing v9.16.6).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * pyqrackising_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links or calls it.
+ *
+ * Every function names the reference file:line it restates.  Paths are relative
+ * to the reference package: util = pyqrackising/maxcut_tfim_util.py,
+ * sgs = pyqrackising/spin_glass_solver.py,
+ * sgss = pyqrackising/spin_glass_solver_sparse.py,
+ * mt = pyqrackising/maxcut_tfim.py, kcl = pyqrackising/kernels.cl.
+ *
+ * Arithmetic is FP64 throughout (the FP64 oracle for real weights; exact for
+ * integer weights below 2^53).  The reference keeps its per-move values in
+ * float32 arrays; for integer weights of the sizes used here float32 is exact,
+ * so the two agree bit for bit.
+ *
+ * Deterministic restatement of the reference's implementation-defined points:
+ *   D2  sgs:55-92 double flips keep each thread's LAST pair (restated exactly,
+ *       via the closed form of that pair's index).
+ *   D3  sgs:111-112 top-k seed order on ties: energy descending, index ascending.
+ *   D4  sgs:131 the walkers' accumulators come from np.empty; here each walker
+ *       starts from its seed's own cut difference, as the sparse solver does
+ *       (sgss:176-186), so every accepted Gray result is a true improvement.
+ *   D5  util:734-735 a Gray flip index >= n is skipped.
+ *   D13 util:747 (gray >> i) for i >= 64 is 0 (Python integer semantics);
+ *       a negative index wraps once by +n (numba negative indexing).
+ *
+ * Parity pins for each function are in tests/test_oracle.py (brute force,
+ * networkx cut_size, closed forms, the reference scripts' known answers).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------- */
+/* Evaluation primitives                                                      */
+/* ------------------------------------------------------------------------- */
+
+/* util:327-331  compute_cut: sum over the full matrix of [x_i != x_j] W_ij, / 2 */
+ORC_API double orc_compute_cut(const double* W, int64_t n, const uint8_t* x) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* Wi = W + i * n;
+        for (int64_t j = 0; j < n; ++j)
+            if (x[i] != x[j]) acc += Wi[j];
+    }
+    return acc / 2.0;
+}
+
+/* util:319-324  compute_energy: sum of ((x_i ^ x_j)*2 - 1) W_ij over the full matrix, / 2 */
+ORC_API double orc_compute_energy(const double* W, int64_t n, const uint8_t* x) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* Wi = W + i * n;
+        for (int64_t j = 0; j < n; ++j)
+            acc += (x[i] != x[j]) ? Wi[j] : -Wi[j];
+    }
+    return acc / 2.0;
+}
+
+/* util:334-336  compute_cut_diff: x already has bit u flipped; returns the cut gain of that flip */
+ORC_API double orc_cut_diff(int64_t u, const uint8_t* x, const double* W, int64_t n) {
+    const double* Wu = W + u * n;
+    double acc = 0.0;
+    for (int64_t v = 0; v < n; ++v) acc += (x[u] != x[v]) ? Wu[v] : -Wu[v];
+    return acc;
+}
+
+/* util:339-346  compute_cut_diff_2: x already has bits k and l flipped; the k-l term is excluded */
+ORC_API double orc_cut_diff_2(int64_t k, int64_t l, const uint8_t* x, const double* W, int64_t n) {
+    const double* Wk = W + k * n;
+    const double* Wl = W + l * n;
+    double a = 0.0, b = 0.0;
+    for (int64_t v = 0; v < n; ++v) {
+        if (v != l) a += (x[k] != x[v]) ? Wk[v] : -Wk[v];
+        if (v != k) b += (x[l] != x[v]) ? Wl[v] : -Wl[v];
+    }
+    return a + b;
+}
+
+/* util:445-461  compute_cut_diff_between: cut(new) - cut(old) over i<j */
+ORC_API double orc_cut_diff_between(const uint8_t* o, const uint8_t* nw, const double* W, int64_t n) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* Wi = W + i * n;
+        for (int64_t j = i + 1; j < n; ++j) {
+            int od = o[i] != o[j], nd = nw[i] != nw[j];
+            if (od != nd) acc += nd ? Wi[j] : -Wi[j];
+        }
+    }
+    return acc;
+}
+
+/* util:349-358  compute_energy_sparse: every stored entry counts (+w if cut, -w otherwise) */
+ORC_API double orc_compute_energy_sparse(const uint8_t* x, const double* data, const int64_t* rows,
+                                         const int64_t* cols, int64_t n) {
+    double e = 0.0;
+    for (int64_t u = 0; u < n; ++u)
+        for (int64_t c = rows[u]; c < rows[u + 1]; ++c)
+            e += (x[u] == x[cols[c]]) ? -data[c] : data[c];
+    return e;
+}
+
+/* util:361-371  compute_cut_sparse: only entries with column > row count */
+ORC_API double orc_compute_cut_sparse(const uint8_t* x, const double* data, const int64_t* rows,
+                                      const int64_t* cols, int64_t n) {
+    double c = 0.0;
+    for (int64_t u = 0; u < n; ++u)
+        for (int64_t k = rows[u]; k < rows[u + 1]; ++k) {
+            int64_t j = cols[k];
+            if (j > u && x[u] != x[j]) c += data[k];
+        }
+    return c;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Candidate scoring: mt:212-285 with kcl:44-94 (value of every candidate,    */
+/* first maximum)                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* bits: B rows of ceil(n/32) uint32 words, LSB-first (util:304-316).
+ * values[b] = cut (maxcut) or energy (spin glass) of candidate b.
+ * Returns the first index of the maximum. */
+ORC_API int64_t orc_eval_batch(const double* W, int64_t n, const uint32_t* bits, int64_t B,
+                               int is_spin_glass, double* values) {
+    const int64_t nw = (n + 31) >> 5;
+#pragma omp parallel
+    {
+        uint8_t* x = (uint8_t*)malloc((size_t)n);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t b = 0; b < B; ++b) {
+            const uint32_t* row = bits + b * nw;
+            for (int64_t i = 0; i < n; ++i) x[i] = (row[i >> 5] >> (i & 31)) & 1u;
+            values[b] = is_spin_glass ? orc_compute_energy(W, n, x) : orc_compute_cut(W, n, x);
+        }
+        free(x);
+    }
+    int64_t best = 0;
+    for (int64_t b = 1; b < B; ++b)
+        if (values[b] > values[best]) best = b;
+    return best;
+}
+
+/* Sparse twin of orc_eval_batch: kcl:233-285 (absolute value per candidate). */
+ORC_API int64_t orc_eval_batch_sparse(const double* data, const int64_t* rows, const int64_t* cols,
+                                      int64_t n, const uint32_t* bits, int64_t B, int is_spin_glass,
+                                      double* values) {
+    const int64_t nw = (n + 31) >> 5;
+#pragma omp parallel
+    {
+        uint8_t* x = (uint8_t*)malloc((size_t)n);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t b = 0; b < B; ++b) {
+            const uint32_t* row = bits + b * nw;
+            for (int64_t i = 0; i < n; ++i) x[i] = (row[i >> 5] >> (i & 31)) & 1u;
+            values[b] = is_spin_glass ? orc_compute_energy_sparse(x, data, rows, cols, n)
+                                      : orc_compute_cut_sparse(x, data, rows, cols, n);
+        }
+        free(x);
+    }
+    int64_t best = 0;
+    for (int64_t b = 1; b < B; ++b)
+        if (values[b] > values[best]) best = b;
+    return best;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pair unranking and Gray helpers                                            */
+/* ------------------------------------------------------------------------- */
+
+/* sgs:66-76 (and sgs:224-234): pair index s -> (i, j), i < j, row-major over i */
+ORC_API void orc_unrank_pair(int64_t s, int64_t n, int64_t* i_out, int64_t* j_out) {
+    int64_t c = s, i = 0, lcv = n - 1;
+    while (c >= lcv) {
+        c -= lcv;
+        ++i;
+        --lcv;
+        if (!lcv) break;
+    }
+    *i_out = i;
+    *j_out = c + i + 1;
+}
+
+/* util:727-737 gray_code_next: the bit that changes between gray(i) and gray(i+1) */
+static inline int64_t gray_flip_pos(int64_t curr_idx) {
+    uint64_t prev = (uint64_t)curr_idx, curr = (uint64_t)curr_idx + 1u;
+    uint64_t diff = (prev ^ (prev >> 1)) ^ (curr ^ (curr >> 1));
+    int64_t p = 0;
+    while (!((diff >> p) & 1u)) ++p;
+    return p;
+}
+
+ORC_API int64_t orc_gray_flip_pos(int64_t curr_idx) { return gray_flip_pos(curr_idx); }
+
+/* util:740-749 gray_mutation (D13 for shifts >= 64 and negative indices). */
+ORC_API void orc_gray_mutation(int64_t index, const uint8_t* seed_bits, int64_t offset, int64_t n,
+                               uint8_t* out) {
+    memcpy(out, seed_bits, (size_t)n);
+    uint64_t gray = (uint64_t)index ^ ((uint64_t)index >> 1);
+    for (int64_t i = 0; i < n && i < 64; ++i) {
+        if (!((gray >> i) & 1u)) continue;
+        int64_t idx = n - offset - 1 - i;
+        while (idx < 0) idx += n;
+        out[idx] ^= 1u;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Local-search phases, dense (sgs:37-153)                                    */
+/* ------------------------------------------------------------------------- */
+
+/* sgs:37-52 run_single_bit_flips.  Returns the scaled best gain; *best_k the first argmax. */
+ORC_API double orc_single_bit_flips(const uint8_t* theta, int is_spin_glass, const double* W,
+                                    int64_t n, int64_t* best_k) {
+    double* e = (double*)malloc(sizeof(double) * (size_t)n);
+#pragma omp parallel
+    {
+        uint8_t* st = (uint8_t*)malloc((size_t)n);
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            memcpy(st, theta, (size_t)n);
+            st[i] ^= 1u;
+            e[i] = orc_cut_diff(i, st, W, n);
+        }
+        free(st);
+    }
+    int64_t b = 0;
+    for (int64_t i = 1; i < n; ++i)
+        if (e[i] > e[b]) b = i;
+    double best = e[b];
+    free(e);
+    *best_k = b;
+    return is_spin_glass ? best * 2.0 : best;
+}
+
+/* The pair index that thread t evaluates last in sgs:64-84 (D2). -1 if none. */
+ORC_API int64_t orc_double_last_index(int64_t t, int64_t combo_count, int64_t thread_count) {
+    int64_t tb = (combo_count + thread_count - 1) / thread_count;
+    if (t >= combo_count) return -1;
+    return t + tb * ((combo_count - 1 - t) / tb);
+}
+
+/* sgs:55-92 run_double_bit_flips.
+ * mode 0: reference CPU semantics (D2: thread t keeps its last pair; argmax over threads).
+ * mode 1: exhaustive (kcl:186-231: first maximum over all pairs).
+ * Returns the scaled gain; (*k,*l) the selected pair. */
+ORC_API double orc_double_bit_flips(const uint8_t* theta, int is_spin_glass, const double* W,
+                                    int64_t n, int64_t thread_count, int mode, int64_t* k_out,
+                                    int64_t* l_out) {
+    const int64_t C = n * (n - 1) / 2;
+    double best = -INFINITY;
+    int64_t bi = -1, bj = -1;
+    uint8_t* st = (uint8_t*)malloc((size_t)n);
+    if (mode == 0) {
+        for (int64_t t = 0; t < thread_count; ++t) {
+            int64_t s = orc_double_last_index(t, C, thread_count);
+            if (s < 0) continue; /* np.empty slot in the reference: never the max here */
+            int64_t i, j;
+            orc_unrank_pair(s, n, &i, &j);
+            memcpy(st, theta, (size_t)n);
+            st[i] ^= 1u;
+            st[j] ^= 1u;
+            double e = orc_cut_diff_2(i, j, st, W, n);
+            if (e > best) { best = e; bi = i; bj = j; }
+        }
+    } else {
+        /* O(n) per pair, first maximum in pair-index order */
+        double* rowbest = (double*)malloc(sizeof(double) * (size_t)n);
+        int64_t* rowj = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+#pragma omp parallel
+        {
+            uint8_t* s2 = (uint8_t*)malloc((size_t)n);
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t i = 0; i < n; ++i) {
+                rowbest[i] = -INFINITY;
+                rowj[i] = -1;
+                for (int64_t j = i + 1; j < n; ++j) {
+                    memcpy(s2, theta, (size_t)n);
+                    s2[i] ^= 1u;
+                    s2[j] ^= 1u;
+                    double e = orc_cut_diff_2(i, j, s2, W, n);
+                    if (e > rowbest[i]) { rowbest[i] = e; rowj[i] = j; }
+                }
+            }
+            free(s2);
+        }
+        for (int64_t i = 0; i < n; ++i)
+            if (rowj[i] >= 0 && rowbest[i] > best) { best = rowbest[i]; bi = i; bj = rowj[i]; }
+        free(rowbest);
+        free(rowj);
+    }
+    free(st);
+    *k_out = bi;
+    *l_out = bj;
+    return is_spin_glass ? best * 2.0 : best;
+}
+
+/* Stable top-k order: energy descending, index ascending (D3). */
+typedef struct { double e; int64_t i; } ekey;
+static int ekey_cmp(const void* a, const void* b) {
+    const ekey* x = (const ekey*)a;
+    const ekey* y = (const ekey*)b;
+    if (x->e > y->e) return -1;
+    if (x->e < y->e) return 1;
+    return (x->i < y->i) ? -1 : (x->i > y->i);
+}
+
+/* sgs:95-123 pick_gray_seeds with block_size = walkers * seed_multiple (the
+ * reference uses walkers == seed_multiple == gray_seed_multiple).
+ * seeds_out: walkers x n selected seeds (best first); energies_out: their cut
+ * differences against theta (unscaled).  all_energies (optional): every seed's
+ * cut difference, seed_count = block_size * ceil(n/64) entries. */
+ORC_API int64_t orc_pick_gray_seeds(const uint8_t* theta, int64_t walkers, int64_t seed_multiple,
+                                    const double* W, int64_t n, uint8_t* seeds_out,
+                                    double* energies_out, int64_t* index_out, double* all_energies) {
+    const int64_t blocks = (n + 63) >> 6;
+    const int64_t block_size = walkers * seed_multiple;
+    const int64_t seed_count = block_size * blocks;
+    ekey* keys = (ekey*)malloc(sizeof(ekey) * (size_t)seed_count);
+#pragma omp parallel
+    {
+        uint8_t* seed = (uint8_t*)malloc((size_t)n);
+#pragma omp for schedule(dynamic, 8)
+        for (int64_t s = 0; s < seed_count; ++s) {
+            int64_t i = s % block_size;
+            int64_t offset = (s / block_size) << 6;
+            orc_gray_mutation(i, theta, offset, n, seed);
+            keys[s].e = orc_cut_diff_between(theta, seed, W, n);
+            keys[s].i = s;
+        }
+        free(seed);
+    }
+    if (all_energies)
+        for (int64_t s = 0; s < seed_count; ++s) all_energies[s] = keys[s].e;
+    qsort(keys, (size_t)seed_count, sizeof(ekey), ekey_cmp);
+    for (int64_t w = 0; w < walkers; ++w) {
+        int64_t s = keys[w].i;
+        orc_gray_mutation(s % block_size, theta, (s / block_size) << 6, n, seeds_out + w * n);
+        energies_out[w] = keys[w].e;
+        if (index_out) index_out[w] = s;
+    }
+    free(keys);
+    return seed_count;
+}
+
+/* sgs:126-153 run_gray_optimization.  iterators (walkers x n) are walked in
+ * place; energies (walkers) enter holding each seed's cut difference (D4) and
+ * leave holding the walker's total.  Returns the first argmax walker; *gain_out
+ * its scaled energy.  *evals_out counts Gray steps evaluated (D5 skips excluded). */
+ORC_API int64_t orc_gray_optimization(uint8_t* iterators, double* energies, int64_t gray_iterations,
+                                      int64_t walkers, int is_spin_glass, const double* W, int64_t n,
+                                      double* gain_out, int64_t* evals_out) {
+    const int64_t thread_iterations = (gray_iterations + walkers - 1) / walkers;
+    const int64_t blocks = (n + 63) >> 6;
+    int64_t evals = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : evals)
+    for (int64_t w = 0; w < walkers; ++w) {
+        uint8_t* it = iterators + w * n;
+        for (int64_t ci = 0; ci < thread_iterations; ++ci) {
+            double best_energy = 0.0;
+            int64_t p = gray_flip_pos(ci);
+            for (int64_t b = 0; b < blocks; ++b) {
+                int64_t fb = p + (b << 6);
+                if (fb >= n) continue; /* D5 */
+                it[fb] ^= 1u;
+                double e = orc_cut_diff(fb, it, W, n);
+                ++evals;
+                if (e > 0.0) best_energy += e;
+                else it[fb] ^= 1u;
+            }
+            energies[w] += best_energy;
+        }
+    }
+    int64_t bw = 0;
+    for (int64_t w = 1; w < walkers; ++w)
+        if (energies[w] > energies[bw]) bw = w;
+    *gain_out = is_spin_glass ? energies[bw] * 2.0 : energies[bw];
+    if (evals_out) *evals_out = evals;
+    return bw;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Whole dense local search: sgs:603-717 (loop) + sgs:719-727 (finalise)      */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    int32_t is_spin_glass;
+    int32_t double_mode;      /* 0 reference CPU (D2), 1 exhaustive */
+    int64_t gray_iterations;  /* sgs:556-557 default n*n */
+    int64_t walkers;          /* thread_count (sgs:603) */
+    int64_t seed_multiple;    /* gray_seed_multiple */
+    int64_t max_rounds;       /* 0 = unbounded */
+} orc_params;
+
+typedef struct {
+    double max_energy;        /* reference bookkeeping (sgs:537-543 + accepted gains) */
+    double cut_value;         /* sgs:720-725 with an exact re-score */
+    double min_energy;
+    int64_t rounds;
+    int64_t single_accepts, double_accepts, seed_accepts, walk_accepts, reheat_accepts;
+    int64_t single_evals, double_evals, seed_evals, gray_evals;
+} orc_result;
+
+static void flip_copy(uint8_t* dst, const uint8_t* src, int64_t n, int64_t a, int64_t b) {
+    memcpy(dst, src, (size_t)n);
+    if (a >= 0) dst[a] ^= 1u;
+    if (b >= 0) dst[b] ^= 1u;
+}
+
+ORC_API int orc_dense_local_search(const double* W, int64_t n, uint8_t* best, const orc_params* p,
+                                   orc_result* r) {
+    const int sg = p->is_spin_glass;
+    const int64_t G = p->walkers;
+    const int64_t C = n * (n - 1) / 2;
+    memset(r, 0, sizeof(*r));
+    r->max_energy = sg ? orc_compute_energy(W, n, best) : orc_compute_cut(W, n, best);
+
+    uint8_t* state = (uint8_t*)malloc((size_t)n);
+    uint8_t* reheat = (uint8_t*)malloc((size_t)n);
+    uint8_t* seeds = (uint8_t*)malloc((size_t)(G * n));
+    double* seed_e = (double*)malloc(sizeof(double) * (size_t)G);
+    int64_t k, l;
+    int improved = 1;
+    while (improved) {
+        improved = 0;
+        if (p->max_rounds > 0 && r->rounds >= p->max_rounds) break;
+        r->rounds++;
+
+        double e = orc_single_bit_flips(best, sg, W, n, &k);
+        r->single_evals += n;
+        if (e > 0.0) {
+            r->max_energy += e;
+            best[k] ^= 1u;
+            r->single_accepts++;
+            improved = 1;
+            continue;
+        }
+
+        e = orc_double_bit_flips(best, sg, W, n, G, p->double_mode, &k, &l);
+        r->double_evals += p->double_mode ? C : (G < C ? G : C);
+        if (e > 0.0) {
+            r->max_energy += e;
+            best[k] ^= 1u;
+            best[l] ^= 1u;
+            r->double_accepts++;
+            improved = 1;
+            continue;
+        }
+
+        int64_t sc = orc_pick_gray_seeds(best, G, p->seed_multiple, W, n, seeds, seed_e, NULL, NULL);
+        r->seed_evals += sc;
+        e = sg ? seed_e[0] * 2.0 : seed_e[0];
+        if (e > 0.0) {
+            r->max_energy += e;
+            memcpy(best, seeds, (size_t)n);
+            r->seed_accepts++;
+            improved = 1;
+            continue;
+        }
+
+        int64_t ev = 0;
+        int64_t bw = orc_gray_optimization(seeds, seed_e, p->gray_iterations, G, sg, W, n, &e, &ev);
+        r->gray_evals += ev;
+        memcpy(state, seeds + bw * n, (size_t)n);
+        if (e > 0.0) {
+            r->max_energy += e;
+            memcpy(best, state, (size_t)n);
+            r->walk_accepts++;
+            improved = 1;
+            continue;
+        }
+
+        /* sgs:670-717 post-reheat phase */
+        memcpy(reheat, state, (size_t)n);
+        double reheat_energy = e;
+        e = orc_single_bit_flips(reheat, sg, W, n, &k);
+        r->single_evals += n;
+        if (e + reheat_energy > 0.0) {
+            r->max_energy += reheat_energy + e;
+            flip_copy(best, reheat, n, k, -1);
+            r->reheat_accepts++;
+            improved = 1;
+            continue;
+        }
+        if (e > 0.0) {
+            reheat[k] ^= 1u;
+            reheat_energy += e;
+        }
+        e = orc_double_bit_flips(reheat, sg, W, n, G, p->double_mode, &k, &l);
+        r->double_evals += p->double_mode ? C : (G < C ? G : C);
+        if (e + reheat_energy > 0.0) {
+            r->max_energy += reheat_energy + e;
+            flip_copy(best, reheat, n, k, l);
+            r->reheat_accepts++;
+            improved = 1;
+        }
+    }
+    free(state);
+    free(reheat);
+    free(seeds);
+    free(seed_e);
+
+    double cut = orc_compute_cut(W, n, best);
+    double en = orc_compute_energy(W, n, best);
+    r->cut_value = cut;
+    r->min_energy = sg ? -en : en;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Sparse local search: sgss:35-208 (phases) and sgss:699-864 (loop)          */
+/* The matrix is the caller's strictly upper-triangular CSR.                  */
+/* ------------------------------------------------------------------------- */
+
+static double sp_value(const uint8_t* x, const double* d, const int64_t* r, const int64_t* c,
+                       int64_t n, int sg) {
+    return sg ? orc_compute_energy_sparse(x, d, r, c, n) : orc_compute_cut_sparse(x, d, r, c, n);
+}
+
+/* sgss:35-57: absolute value of every single-flipped state; first argmax */
+ORC_API double orc_sparse_single(const uint8_t* theta, int sg, const double* d, const int64_t* rw,
+                                 const int64_t* cl, int64_t n, int64_t* best_k) {
+    double* e = (double*)malloc(sizeof(double) * (size_t)n);
+#pragma omp parallel
+    {
+        uint8_t* st = (uint8_t*)malloc((size_t)n);
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            memcpy(st, theta, (size_t)n);
+            st[i] ^= 1u;
+            e[i] = sp_value(st, d, rw, cl, n, sg);
+        }
+        free(st);
+    }
+    int64_t b = 0;
+    for (int64_t i = 1; i < n; ++i)
+        if (e[i] > e[b]) b = i;
+    double best = e[b];
+    free(e);
+    *best_k = b;
+    return best;
+}
+
+/* sgss:60-118 (D2 last pair per thread) or exhaustive (kcl:370-414) */
+ORC_API double orc_sparse_double(const uint8_t* theta, int sg, const double* d, const int64_t* rw,
+                                 const int64_t* cl, int64_t n, int64_t thread_count, int mode,
+                                 int64_t* k_out, int64_t* l_out) {
+    const int64_t C = n * (n - 1) / 2;
+    double best = -INFINITY;
+    int64_t bi = -1, bj = -1;
+    uint8_t* st = (uint8_t*)malloc((size_t)n);
+    if (mode == 0) {
+        for (int64_t t = 0; t < thread_count; ++t) {
+            int64_t s = orc_double_last_index(t, C, thread_count);
+            if (s < 0) continue;
+            int64_t i, j;
+            orc_unrank_pair(s, n, &i, &j);
+            flip_copy(st, theta, n, i, j);
+            double e = sp_value(st, d, rw, cl, n, sg);
+            if (e > best) { best = e; bi = i; bj = j; }
+        }
+    } else {
+        for (int64_t s = 0; s < C; ++s) {
+            int64_t i, j;
+            orc_unrank_pair(s, n, &i, &j);
+            flip_copy(st, theta, n, i, j);
+            double e = sp_value(st, d, rw, cl, n, sg);
+            if (e > best) { best = e; bi = i; bj = j; }
+        }
+    }
+    free(st);
+    *k_out = bi;
+    *l_out = bj;
+    return best;
+}
+
+/* sgss:121-154: absolute values of all seeds, stable top-`walkers` (D3) */
+ORC_API int64_t orc_sparse_pick_gray_seeds(const uint8_t* theta, int64_t walkers, int64_t seed_multiple,
+                                           const double* d, const int64_t* rw, const int64_t* cl,
+                                           int64_t n, int sg, uint8_t* seeds_out, double* energies_out) {
+    const int64_t blocks = (n + 63) >> 6;
+    const int64_t block_size = walkers * seed_multiple;
+    const int64_t seed_count = block_size * blocks;
+    ekey* keys = (ekey*)malloc(sizeof(ekey) * (size_t)seed_count);
+#pragma omp parallel
+    {
+        uint8_t* seed = (uint8_t*)malloc((size_t)n);
+#pragma omp for schedule(dynamic, 8)
+        for (int64_t s = 0; s < seed_count; ++s) {
+            orc_gray_mutation(s % block_size, theta, (s / block_size) << 6, n, seed);
+            keys[s].e = sp_value(seed, d, rw, cl, n, sg);
+            keys[s].i = s;
+        }
+        free(seed);
+    }
+    qsort(keys, (size_t)seed_count, sizeof(ekey), ekey_cmp);
+    for (int64_t w = 0; w < walkers; ++w) {
+        int64_t s = keys[w].i;
+        orc_gray_mutation(s % block_size, theta, (s / block_size) << 6, n, seeds_out + w * n);
+        energies_out[w] = keys[w].e;
+    }
+    free(keys);
+    return seed_count;
+}
+
+/* sgss:157-208: walkers accept a Gray flip iff the absolute value rises */
+ORC_API int64_t orc_sparse_gray_optimization(uint8_t* iterators, double* energies, int64_t gray_iterations,
+                                             int64_t walkers, int sg, const double* d, const int64_t* rw,
+                                             const int64_t* cl, int64_t n, double* best_out, int64_t* evals_out) {
+    const int64_t thread_iterations = (gray_iterations + walkers - 1) / walkers;
+    const int64_t blocks = (n + 63) >> 6;
+    int64_t evals = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : evals)
+    for (int64_t w = 0; w < walkers; ++w) {
+        uint8_t* it = iterators + w * n;
+        double be = energies[w];
+        for (int64_t ci = 0; ci < thread_iterations; ++ci) {
+            int64_t p = gray_flip_pos(ci);
+            for (int64_t b = 0; b < blocks; ++b) {
+                int64_t fb = p + (b << 6);
+                if (fb >= n) continue; /* D5 */
+                it[fb] ^= 1u;
+                double e = sp_value(it, d, rw, cl, n, sg);
+                ++evals;
+                if (e > be) be = e;
+                else it[fb] ^= 1u;
+            }
+            if (be > energies[w]) energies[w] = be;
+        }
+    }
+    int64_t bw = 0;
+    for (int64_t w = 1; w < walkers; ++w)
+        if (energies[w] > energies[bw]) bw = w;
+    *best_out = energies[bw];
+    if (evals_out) *evals_out = evals;
+    return bw;
+}
+
+ORC_API int orc_sparse_local_search(const double* d, const int64_t* rw, const int64_t* cl, int64_t n,
+                                    uint8_t* best, const orc_params* p, orc_result* r) {
+    const int sg = p->is_spin_glass;
+    const int64_t G = p->walkers;
+    const int64_t C = n * (n - 1) / 2;
+    memset(r, 0, sizeof(*r));
+    double max_energy = sp_value(best, d, rw, cl, n, sg);
+
+    uint8_t* state = (uint8_t*)malloc((size_t)n);
+    uint8_t* reheat = (uint8_t*)malloc((size_t)n);
+    uint8_t* seeds = (uint8_t*)malloc((size_t)(G * n));
+    double* seed_e = (double*)malloc(sizeof(double) * (size_t)G);
+    int64_t k, l;
+    int improved = 1;
+    while (improved) {
+        improved = 0;
+        if (p->max_rounds > 0 && r->rounds >= p->max_rounds) break;
+        r->rounds++;
+
+        double e = orc_sparse_single(best, sg, d, rw, cl, n, &k);
+        r->single_evals += n;
+        if (e > max_energy) {
+            max_energy = e;
+            best[k] ^= 1u;
+            r->single_accepts++;
+            improved = 1;
+            continue;
+        }
+        e = orc_sparse_double(best, sg, d, rw, cl, n, G, p->double_mode, &k, &l);
+        r->double_evals += p->double_mode ? C : (G < C ? G : C);
+        if (e > max_energy) {
+            max_energy = e;
+            best[k] ^= 1u;
+            best[l] ^= 1u;
+            r->double_accepts++;
+            improved = 1;
+            continue;
+        }
+        int64_t sc = orc_sparse_pick_gray_seeds(best, G, p->seed_multiple, d, rw, cl, n, sg, seeds, seed_e);
+        r->seed_evals += sc;
+        if (seed_e[0] > max_energy) {
+            max_energy = seed_e[0];
+            memcpy(best, seeds, (size_t)n);
+            r->seed_accepts++;
+            improved = 1;
+            continue;
+        }
+        int64_t ev = 0;
+        int64_t bw = orc_sparse_gray_optimization(seeds, seed_e, p->gray_iterations, G, sg, d, rw, cl, n, &e, &ev);
+        r->gray_evals += ev;
+        memcpy(state, seeds + bw * n, (size_t)n);
+        if (e > max_energy) {
+            max_energy = e;
+            memcpy(best, state, (size_t)n);
+            r->walk_accepts++;
+            improved = 1;
+            continue;
+        }
+        /* sgss:802-854 post-reheat phase */
+        memcpy(reheat, state, (size_t)n);
+        double reheat_energy = e;
+        e = orc_sparse_single(reheat, sg, d, rw, cl, n, &k);
+        r->single_evals += n;
+        if (e > max_energy) {
+            max_energy = e;
+            flip_copy(best, reheat, n, k, -1);
+            r->reheat_accepts++;
+            improved = 1;
+            continue;
+        }
+        if (e > reheat_energy) reheat[k] ^= 1u;
+        e = orc_sparse_double(reheat, sg, d, rw, cl, n, G, p->double_mode, &k, &l);
+        r->double_evals += p->double_mode ? C : (G < C ? G : C);
+        if (e > max_energy) {
+            max_energy = e;
+            flip_copy(best, reheat, n, k, l);
+            r->reheat_accepts++;
+            improved = 1;
+        }
+    }
+    free(state);
+    free(reheat);
+    free(seeds);
+    free(seed_e);
+    r->max_energy = max_energy;
+    r->cut_value = orc_compute_cut_sparse(best, d, rw, cl, n);
+    double en = orc_compute_energy_sparse(best, d, rw, cl, n);
+    r->min_energy = sg ? -en : en;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Exact solvers: mt:288-339 (strict '>' keeps the first maximum)             */
+/* ------------------------------------------------------------------------- */
+
+ORC_API double orc_exact(const double* W, int64_t n, int is_spin_glass, uint8_t* best_bits) {
+    double max_cut = -1.0;
+    uint64_t best_mask = 0;
+    for (uint64_t mask = 0; mask < (1ull << n); ++mask) {
+        double cut = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            uint64_t bi = (mask >> i) & 1u;
+            for (int64_t j = i + 1; j < n; ++j) {
+                uint64_t bj = (mask >> j) & 1u;
+                if (is_spin_glass) cut += (bi == bj) ? W[i * n + j] : -W[i * n + j];
+                else if (bi != bj) cut += W[i * n + j];
+            }
+        }
+        if (cut > max_cut) { max_cut = cut; best_mask = mask; }
+    }
+    for (int64_t i = 0; i < n; ++i) best_bits[i] = (best_mask >> i) & 1u;
+    return max_cut;
+}

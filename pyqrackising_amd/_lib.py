@@ -1,0 +1,346 @@
+"""ctypes binding of libpqi.so (the HIP C ABI declared in include/pqi.h).
+
+This module is the replacement for the reference's OpenCL seam
+(pyqrackising/maxcut_tfim_util.py:9-301).  There is no fallback: if the
+library is missing or no GPU is present, every entry point raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpqi.so")
+
+PQI_OK = 0
+PQI_ERR_INVALID = -1
+PQI_ERR_NO_DEVICE = -2
+PQI_ERR_HIP = -3
+PQI_ERR_OOM = -4
+PQI_ERR_UNSUPPORTED = -5
+
+DTYPE_I8, DTYPE_I32, DTYPE_F32, DTYPE_F64 = 1, 2, 3, 4
+WEIGHTS_I8, WEIGHTS_F32, WEIGHTS_F64 = 1, 3, 4
+DOUBLE_REFERENCE, DOUBLE_EXHAUSTIVE = 0, 1
+SEEDS_AUTO, SEEDS_MFMA, SEEDS_LOWRANK = 0, 1, 2
+FLAG_TIME_KERNELS = 1
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+
+
+class PqiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libpqi error {code}: {msg}")
+        self.code = code
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("weights", ctypes.c_int32),
+        ("has_dense", ctypes.c_int32),
+        ("nnz", ctypes.c_int64),
+        ("w_up", ctypes.c_double),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+class SearchParams(ctypes.Structure):
+    _fields_ = [
+        ("is_spin_glass", ctypes.c_int32),
+        ("double_mode", ctypes.c_int32),
+        ("gray_iterations", ctypes.c_int64),
+        ("walkers", ctypes.c_int64),
+        ("seed_multiple", ctypes.c_int64),
+        ("max_rounds", ctypes.c_int64),
+        ("seed_engine", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+    ]
+
+
+class SearchStats(ctypes.Structure):
+    _fields_ = [
+        ("max_energy", ctypes.c_double),
+        ("cut_value", ctypes.c_double),
+        ("min_energy", ctypes.c_double),
+        ("rounds", ctypes.c_int64),
+        ("single_accepts", ctypes.c_int64),
+        ("double_accepts", ctypes.c_int64),
+        ("seed_accepts", ctypes.c_int64),
+        ("walk_accepts", ctypes.c_int64),
+        ("reheat_accepts", ctypes.c_int64),
+        ("single_evals", ctypes.c_int64),
+        ("double_evals", ctypes.c_int64),
+        ("seed_evals", ctypes.c_int64),
+        ("gray_evals", ctypes.c_int64),
+        ("seconds_single", ctypes.c_double),
+        ("seconds_double", ctypes.c_double),
+        ("seconds_seeds", ctypes.c_double),
+        ("seconds_walk", ctypes.c_double),
+        ("seconds_reheat", ctypes.c_double),
+        ("walk_kernel_seconds", ctypes.c_double),
+        ("seed_kernel_seconds", ctypes.c_double),
+        ("walk_launches", ctypes.c_int64),
+        ("seed_launches", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# name -> (restype, argtypes); every symbol include/pqi.h declares
+SIGNATURES = {
+    "pqi_last_error": (ctypes.c_char_p, []),
+    "pqi_version": (_int, []),
+    "pqi_device_count": (_int, [_vp]),
+    "pqi_create": (_int, [_int, _vp]),
+    "pqi_destroy": (_int, [_vp]),
+    "pqi_graph_dense": (_int, [_vp, _vp, _int, _i64, _vp]),
+    "pqi_graph_csr": (_int, [_vp, _vp, _vp, _vp, _int, _i64, _i64, _vp]),
+    "pqi_graph_free": (_int, [_vp]),
+    "pqi_graph_get_info": (_int, [_vp, _vp]),
+    "pqi_eval_batch": (_int, [_vp, _vp, _i64, _int, _vp, _vp, _vp]),
+    "pqi_single_flip_best": (_int, [_vp, _vp, _int, _vp, _vp]),
+    "pqi_double_flip_best": (_int, [_vp, _vp, _int, _i64, _int, _vp, _vp, _vp]),
+    "pqi_gray_search": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "pqi_local_search": (_int, [_vp, _vp, _vp, _vp]),
+    "pqi_score": (_int, [_vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libpqi.so and bind every entry point (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PqiError(PQI_ERR_UNSUPPORTED,
+                           f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != PQI_OK:
+        msg = load().pqi_last_error()
+        raise PqiError(code, msg.decode() if msg else "")
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    r = load().pqi_device_count(ctypes.byref(c))
+    return c.value if r == PQI_OK else 0
+
+
+class Context:
+    """One device and one HIP stream (the reference's global OpenCL queue,
+    pyqrackising/maxcut_tfim_util.py:103-104).  Calls are serialised by a lock."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        check(load().pqi_create(int(device), ctypes.byref(self._h)))
+        self.lock = threading.RLock()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            load().pqi_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    with _lock:
+        if _default_ctx is None:
+            _default_ctx = Context(int(os.environ.get("PQI_DEVICE", "0")))
+        return _default_ctx
+
+
+def _dtype_code(a):
+    if a.dtype == np.int8:
+        return a, DTYPE_I8
+    if a.dtype == np.int32:
+        return a, DTYPE_I32
+    if a.dtype == np.float32:
+        return a, DTYPE_F32
+    if a.dtype == np.float64:
+        return a, DTYPE_F64
+    if a.dtype == np.bool_ or np.issubdtype(a.dtype, np.integer):
+        if a.size and (a.max() > np.iinfo(np.int32).max or a.min() < np.iinfo(np.int32).min):
+            return a.astype(np.float64), DTYPE_F64
+        return a.astype(np.int32), DTYPE_I32
+    return a.astype(np.float64), DTYPE_F64
+
+
+class DeviceGraph:
+    """A problem instance resident in HBM (replaces make_G_m_buf /
+    make_G_m_csr_buf, pyqrackising/maxcut_tfim_util.py:224-259)."""
+
+    def __init__(self, ctx, handle, n):
+        self.ctx = ctx
+        self._h = handle
+        self.n = n
+
+    @classmethod
+    def dense(cls, W, ctx=None):
+        ctx = ctx or default_context()
+        W = np.ascontiguousarray(W)
+        if W.ndim != 2 or W.shape[0] != W.shape[1]:
+            raise ValueError("W must be square")
+        W, code = _dtype_code(W)
+        W = np.ascontiguousarray(W)
+        h = ctypes.c_void_p()
+        with ctx.lock:
+            check(load().pqi_graph_dense(ctx.handle, _p(W), code, W.shape[0], ctypes.byref(h)))
+        return cls(ctx, h, W.shape[0])
+
+    @classmethod
+    def csr(cls, G, ctx=None):
+        """G: scipy CSR, strictly upper triangular (util:533-542)."""
+        ctx = ctx or default_context()
+        n = G.shape[0]
+        indptr = np.ascontiguousarray(G.indptr, dtype=np.int64)
+        indices = np.ascontiguousarray(G.indices, dtype=np.int64)
+        data, code = _dtype_code(np.ascontiguousarray(G.data))
+        data = np.ascontiguousarray(data)
+        h = ctypes.c_void_p()
+        with ctx.lock:
+            check(load().pqi_graph_csr(ctx.handle, _p(indptr), _p(indices), _p(data), code, n,
+                                       int(indices.size), ctypes.byref(h)))
+        return cls(ctx, h, n)
+
+    def info(self):
+        i = GraphInfo()
+        check(load().pqi_graph_get_info(self._h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def close(self):
+        if self._h:
+            load().pqi_graph_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- entry points -------------------------------------------------------
+
+    def _theta(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.uint8)
+        if t.shape != (self.n,):
+            raise ValueError(f"theta must have shape ({self.n},)")
+        return t
+
+    def eval_batch(self, bits, is_spin_glass, want_values=True):
+        bits = np.ascontiguousarray(bits, dtype=np.uint32)
+        B = bits.shape[0]
+        if bits.ndim != 2 or bits.shape[1] != (self.n + 31) >> 5:
+            raise ValueError("bits must be B x ceil(n/32) uint32")
+        vals = np.empty(B, dtype=np.float64) if want_values else None
+        bi = ctypes.c_int64()
+        bv = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_eval_batch(self._h, _p(bits), B, int(bool(is_spin_glass)),
+                                        _p(vals) if want_values else None, ctypes.byref(bi), ctypes.byref(bv)))
+        return bi.value, bv.value, vals
+
+    def single_flip_best(self, theta, is_spin_glass):
+        t = self._theta(theta)
+        k = ctypes.c_int64()
+        g = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_single_flip_best(self._h, _p(t), int(bool(is_spin_glass)), ctypes.byref(k),
+                                              ctypes.byref(g)))
+        return g.value, k.value
+
+    def double_flip_best(self, theta, is_spin_glass, thread_count, mode=DOUBLE_REFERENCE):
+        t = self._theta(theta)
+        k, l = ctypes.c_int64(), ctypes.c_int64()
+        g = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_double_flip_best(self._h, _p(t), int(bool(is_spin_glass)), int(thread_count), int(mode),
+                                              ctypes.byref(k), ctypes.byref(l), ctypes.byref(g)))
+        return g.value, k.value, l.value
+
+    def gray_search(self, theta, params):
+        t = self._theta(theta)
+        out = np.empty(self.n, dtype=np.uint8)
+        sg, wg = ctypes.c_double(), ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_gray_search(self._h, _p(t), ctypes.byref(params), _p(out), ctypes.byref(sg),
+                                         ctypes.byref(wg)))
+        return sg.value, wg.value, out
+
+    def local_search(self, theta, params):
+        t = self._theta(theta).copy()
+        st = SearchStats()
+        with self.ctx.lock:
+            check(load().pqi_local_search(self._h, _p(t), ctypes.byref(params), ctypes.byref(st)))
+        return t, st.as_dict()
+
+    def score(self, theta):
+        t = self._theta(theta)
+        c, e = ctypes.c_double(), ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_score(self._h, _p(t), ctypes.byref(c), ctypes.byref(e)))
+        return c.value, e.value
+
+
+def make_params(n, is_spin_glass=True, gray_iterations=None, walkers=None, seed_multiple=None,
+                double_mode=DOUBLE_REFERENCE, max_rounds=0, seed_engine=SEEDS_AUTO, flags=0):
+    """Defaults follow pyqrackising/spin_glass_solver.py:556-560, 603."""
+    if seed_multiple is None:
+        seed_multiple = max(2, os.cpu_count() or 2)
+    if walkers is None:
+        walkers = seed_multiple
+    p = SearchParams()
+    p.is_spin_glass = int(bool(is_spin_glass))
+    p.double_mode = int(double_mode)
+    p.gray_iterations = n * n if gray_iterations is None else int(gray_iterations)
+    p.walkers = int(walkers)
+    p.seed_multiple = int(seed_multiple)
+    p.max_rounds = int(max_rounds)
+    p.seed_engine = int(seed_engine)
+    p.flags = int(flags)
+    return p
+
+
+def pack_bits(X):
+    """B x n bool -> B x ceil(n/32) uint32, LSB-first
+    (pyqrackising/maxcut_tfim_util.py:304-316)."""
+    X = np.asarray(X, dtype=bool)
+    B, n = X.shape
+    nw = (n + 31) >> 5
+    pad = np.zeros((B, nw * 32), dtype=bool)
+    pad[:, :n] = X
+    by = np.packbits(pad.reshape(B, nw * 4, 8), axis=2, bitorder="little").reshape(B, nw * 4)
+    return np.ascontiguousarray(by).view(np.uint32).reshape(B, nw)

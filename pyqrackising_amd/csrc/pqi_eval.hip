@@ -1,0 +1,319 @@
+// Batched candidate energies q_b = s_b^T W s_b for gfx950.
+//
+// Candidates arrive packed (B rows of uint32 words, LSB-first, the layout of
+// pyqrackising/maxcut_tfim_util.py:304-316) and are unpacked to +-1 in LDS.
+// The contraction P = S W runs on MFMA and the epilogue forms
+// q_b = sum_j S_bj P_bj without writing P: this is the dense contraction
+// E = rowsum((S W) .* S) that replaces kernels.cl:44-94 (calculate_cut), which
+// walks the upper triangle of W once per candidate.
+//
+//   int8 weights: v_mfma_i32_32x32x32_i8, int32 accumulation, int64 row totals
+//   (exact); fp32/fp64 weights: FP64 tiles (exact to FP64 rounding).
+#include <hip/hip_runtime.h>
+
+#include "pqi_internal.h"
+
+namespace pqi {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int BM = 128;       // candidates per work group
+constexpr int BN = 128;       // output columns per work group
+constexpr int BK = 64;        // contraction depth per LDS stage
+constexpr int LDS_ROW = BK + 16;  // padded LDS row (bytes)
+
+// 4 bits -> 4 bytes holding 0/1.
+__device__ __forceinline__ uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
+
+// 0/1 bytes -> +-1 int8 bytes (x = 1 -> s = -1).
+__device__ __forceinline__ uint32_t to_pm1(uint32_t bytes01) { return (bytes01 * 0xFEu) | 0x01010101u; }
+
+}  // namespace
+
+// q[m] += sum over this block's 128 columns j of s_mj (S W)_mj.  q is zeroed by the caller.
+// Work-group tile 128 x 128, four waves in 2 x 2, each wave 64 x 64 as 2 x 2 MFMA 32x32x32.
+__global__ __launch_bounds__(256) void k_batch_i8(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
+                                                  const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
+                                                  unsigned long long* __restrict__ q) {
+    __shared__ __align__(16) int8_t As[BM * LDS_ROW];
+    __shared__ __align__(16) int8_t Bs[BN * LDS_ROW];
+    __shared__ uint32_t Sb[BM * (BN / 32)];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int64_t m0 = (int64_t)blockIdx.y * BM, j0 = (int64_t)blockIdx.x * BN;
+    const int lr = tid >> 1, lh = tid & 1;  // loader row / half
+
+    v16i acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+
+    const int64_t am = m0 + lr;
+    for (int64_t k0 = 0; k0 < n_pad; k0 += BK) {
+        // A: 32 candidate bits -> 32 bytes of +-1
+        {
+            const uint32_t word = (am < B) ? bits[am * wpr + (k0 >> 5) + lh] : 0u;
+            uint32_t o[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) o[t] = to_pm1(spread4((word >> (4 * t)) & 0xFu));
+            uint4* dst = (uint4*)&As[lr * LDS_ROW + lh * 32];
+            dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+            dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+        }
+        // B: rows j of W (W is symmetric, so row j holds column j)
+        {
+            const uint4* src = (const uint4*)(W + (j0 + lr) * ld + k0 + lh * 32);
+            uint4 a = src[0], b = src[1];
+            uint4* dst = (uint4*)&Bs[lr * LDS_ROW + lh * 32];
+            dst[0] = a;
+            dst[1] = b;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 32) {
+            v4i av[2], bv[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+                av[mi] = *(const v4i*)&As[(wm * 64 + mi * 32 + (lane & 31)) * LDS_ROW + kk + 16 * (lane >> 5)];
+#pragma unroll
+            for (int nj = 0; nj < 2; ++nj)
+                bv[nj] = *(const v4i*)&Bs[(wn * 64 + nj * 32 + (lane & 31)) * LDS_ROW + kk + 16 * (lane >> 5)];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int nj = 0; nj < 2; ++nj)
+                    acc[mi][nj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[mi], bv[nj], acc[mi][nj], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // candidate bits of this block's output columns
+    for (int t = tid; t < BM * (BN / 32); t += 256) {
+        const int r = t >> 2, c = t & 3;
+        const int64_t m = m0 + r;
+        Sb[t] = (m < B) ? bits[m * wpr + (j0 >> 5) + c] : 0u;
+    }
+    __syncthreads();
+
+    // 32x32 accumulator layout: lane l, register r -> column l%32,
+    // row 8*(r/4) + 4*(l/32) + r%4.
+    int vals[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) vals[i] = 0;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj) {
+            const int col = wn * 64 + nj * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const uint32_t bit = (Sb[row * 4 + (col >> 5)] >> (col & 31)) & 1u;
+                const int p = acc[mi][nj][r];
+                vals[mi * 16 + r] += bit ? -p : p;
+            }
+        }
+    // reduce-scatter over the 32 lanes that share rows
+#pragma unroll
+    for (int t = 0, half = 16; t < 5; ++t, half >>= 1) {
+        const int up = (lane >> t) & 1;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < half) {
+                const int send = up ? vals[i] : vals[i + half];
+                const int keep = up ? vals[i + half] : vals[i];
+                vals[i] = keep + __shfl_xor(send, 1 << t, 64);
+            }
+        }
+    }
+    const int qi = (lane & 1) * 16 + ((lane >> 1) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                   ((lane >> 4) & 1);
+    const int mi = qi >> 4, r = qi & 15;
+    const int row = wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int64_t m = m0 + row;
+    if (m < B) atomicAdd(&q[m], (unsigned long long)(long long)vals[0]);
+}
+
+// FP64 variant for real weights: tile 64 candidates x 64 columns, 4 x 4 per
+// thread; partial row sums per column block go to part[B][nblk] and are added
+// in fixed order by k_batch_sum (deterministic).
+template <typename WT>
+__global__ __launch_bounds__(256) void k_batch_real(const WT* __restrict__ W, int64_t ld, int64_t n,
+                                                    const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
+                                                    double* __restrict__ part, int64_t nblk) {
+    constexpr int TB = 64, TK = 16;
+    __shared__ double Ss[TK][TB + 1];
+    __shared__ double Ws[TK][TB + 1];
+    __shared__ double red[TB][17];
+    const int tid = threadIdx.x;
+    const int tr = tid >> 4, tc = tid & 15;  // 16 x 16 threads, 4 x 4 outputs each
+    const int64_t m0 = (int64_t)blockIdx.y * TB, j0 = (int64_t)blockIdx.x * TB;
+    double acc[4][4] = {};
+    for (int64_t i0 = 0; i0 < n; i0 += TK) {
+        for (int t = tid; t < TB * TK; t += 256) {
+            const int c = t / TK, i = t % TK;  // candidate c, contraction index i
+            const int64_t m = m0 + c, ii = i0 + i;
+            double sv = 0.0;
+            if (m < B && ii < n) sv = ((bits[m * wpr + (ii >> 5)] >> (ii & 31)) & 1u) ? -1.0 : 1.0;
+            Ss[i][c] = sv;
+            const int i2 = t / TB, jj = t % TB;
+            const int64_t wi = i0 + i2, wj = j0 + jj;
+            Ws[i2][jj] = (wi < n && wj < n) ? (double)W[wi * ld + wj] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TK; ++i) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = Ss[i][tr * 4 + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) b[u] = Ws[i][tc * 4 + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+        }
+        __syncthreads();
+    }
+    // epilogue: multiply by s_mj and sum this thread's 4 columns
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t m = m0 + tr * 4 + u;
+        double sacc = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t j = j0 + tc * 4 + v;
+            if (m < B && j < n) {
+                const bool bit = (bits[m * wpr + (j >> 5)] >> (j & 31)) & 1u;
+                sacc += bit ? -acc[u][v] : acc[u][v];
+            }
+        }
+        red[tr * 4 + u][tc] = sacc;
+    }
+    __syncthreads();
+    if (tid < TB) {
+        double s = 0.0;
+        for (int c = 0; c < 16; ++c) s += red[tid][c];
+        const int64_t m = m0 + tid;
+        if (m < B) part[m * nblk + blockIdx.x] = s;
+    }
+}
+
+__global__ void k_batch_sum(const double* part, int64_t nblk, int64_t B, double* q) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= B) return;
+    double s = 0.0;
+    for (int64_t b = 0; b < nblk; ++b) s += part[m * nblk + b];
+    q[m] = s;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0ull;
+}
+
+__global__ void k_i64_to_f64(const unsigned long long* qi, int64_t B, double* q) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < B) q[m] = (double)(long long)qi[m];
+}
+
+// CSR graphs: q_b = sum_u s_u sum_{e in adj(u)} w_e s_{v_e}.  One work group per
+// candidate; the candidate's bit row is staged in LDS, rows are strided over
+// threads, and the block total is formed in a fixed order (deterministic).
+template <typename WT, typename AT>
+__global__ __launch_bounds__(256) void k_batch_csr(const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
+                                                   const WT* __restrict__ aw, int64_t n,
+                                                   const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
+                                                   double* __restrict__ q, int stage) {
+    extern __shared__ uint32_t sbits[];
+    __shared__ AT part[4];
+    const int64_t b = blockIdx.x;
+    const uint32_t* row = bits + b * wpr;
+    const int64_t nw = (n + 31) >> 5;
+    const uint32_t* xb = row;
+    if (stage) {
+        for (int64_t t = threadIdx.x; t < nw; t += blockDim.x) sbits[t] = row[t];
+        __syncthreads();
+        xb = sbits;
+    }
+    AT acc = 0;
+    for (int64_t u = threadIdx.x; u < n; u += blockDim.x) {
+        AT r = 0;
+        for (int32_t e = ap[u]; e < ap[u + 1]; ++e) {
+            const int32_t v = ai[e];
+            const bool xv = (xb[v >> 5] >> (v & 31)) & 1u;
+            r += xv ? -(AT)aw[e] : (AT)aw[e];
+        }
+        const bool xu = (xb[u >> 5] >> (u & 31)) & 1u;
+        acc += xu ? -r : r;
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        if (sizeof(AT) == 8) {
+            long long lv;
+            __builtin_memcpy(&lv, &acc, 8);
+            int lo = __shfl_xor((int)(lv & 0xffffffff), m, 64), hi = __shfl_xor((int)(lv >> 32), m, 64);
+            long long ov = ((long long)hi << 32) | (unsigned)lo;
+            AT o;
+            __builtin_memcpy(&o, &ov, 8);
+            acc += o;
+        }
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) q[b] = (double)(part[0] + part[1] + part[2] + part[3]);
+}
+
+size_t batch_scratch_bytes(const Graph& g, int64_t B) {
+    if (g.W == nullptr) return 8;
+    if (g.kind == PQI_WEIGHTS_I8) return (size_t)B * 8;
+    int64_t nblk = (g.n + 63) / 64;
+    return (size_t)B * (size_t)nblk * 8;
+}
+
+void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, double* q, void* scratch,
+                            hipStream_t s) {
+    if (B <= 0) return;
+    if (g.W == nullptr) {
+        const size_t lds = (size_t)((g.n + 31) >> 5) * 4;
+        const int stage = lds <= 48 * 1024 ? 1 : 0;
+        const size_t dyn = stage ? lds : 0;
+        if (g.kind == PQI_WEIGHTS_I8)
+            hipLaunchKernelGGL((k_batch_csr<int8_t, int64_t>), dim3((unsigned)B), dim3(256), dyn, s, g.adj_ptr,
+                               g.adj_idx, (const int8_t*)g.adj_w, g.n, bits, wpr, B, q, stage);
+        else if (g.kind == PQI_WEIGHTS_F32)
+            hipLaunchKernelGGL((k_batch_csr<float, double>), dim3((unsigned)B), dim3(256), dyn, s, g.adj_ptr,
+                               g.adj_idx, (const float*)g.adj_w, g.n, bits, wpr, B, q, stage);
+        else
+            hipLaunchKernelGGL((k_batch_csr<double, double>), dim3((unsigned)B), dim3(256), dyn, s, g.adj_ptr,
+                               g.adj_idx, (const double*)g.adj_w, g.n, bits, wpr, B, q, stage);
+        return;
+    }
+    if (g.kind == PQI_WEIGHTS_I8) {
+        unsigned long long* qi = (unsigned long long*)scratch;
+        hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B);
+        dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
+        hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi);
+        hipLaunchKernelGGL(k_i64_to_f64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B, q);
+    } else {
+        const int64_t nblk = (g.n + 63) / 64;
+        double* part = (double*)scratch;
+        dim3 grid((unsigned)nblk, (unsigned)((B + 63) / 64));
+        if (g.kind == PQI_WEIGHTS_F32)
+            hipLaunchKernelGGL((k_batch_real<float>), grid, dim3(256), 0, s, (const float*)g.W, g.n_pad, g.n, bits,
+                               wpr, B, part, nblk);
+        else
+            hipLaunchKernelGGL((k_batch_real<double>), grid, dim3(256), 0, s, (const double*)g.W, g.n_pad, g.n, bits,
+                               wpr, B, part, nblk);
+        hipLaunchKernelGGL(k_batch_sum, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, part, nblk, B, q);
+    }
+}
+
+}  // namespace pqi

@@ -1,0 +1,122 @@
+// Internal declarations shared by the libpqi translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pqi.h"
+
+namespace pqi {
+
+// Device-side state of one problem instance.
+struct Graph {
+    pqi_ctx* ctx = nullptr;
+    int64_t n = 0;
+    int kind = PQI_WEIGHTS_I8;  // PQI_WEIGHTS_*
+    // Dense weights, n_pad x n_pad row-major, zero padded (nullptr for CSR input).
+    // int8_t for PQI_WEIGHTS_I8, float for F32, double for F64.
+    void* W = nullptr;
+    int64_t n_pad = 0;  // multiple of 128
+    // Full symmetric adjacency of the nonzeros, sorted columns per row.
+    int32_t* adj_ptr = nullptr;  // n+1
+    int32_t* adj_idx = nullptr;  // nnz
+    void* adj_w = nullptr;       // nnz, same element type as W
+    int64_t nnz = 0;
+    int64_t max_degree = 0;
+    double w_up = 0.0;           // sum_{i<j} w_ij
+    int64_t w_up_i = 0;          // same, exact, for int8 weights
+    bool use_adj = true;         // flips update fields through the adjacency lists
+    int64_t device_bytes = 0;
+};
+
+// Scratch buffer owned by a context, grown on demand.
+struct Buffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace pqi
+
+struct pqi_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    std::vector<pqi::Buffer> bufs;  // indexed by slot
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+struct pqi_graph : pqi::Graph {};
+
+namespace pqi {
+
+// error plumbing
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+// ---- kernels (launchers); all enqueue on ctx->stream -------------------------
+
+// h = W s for the state x (uint8 per node).  FT = int32 for int8 weights, double otherwise.
+void launch_field(const Graph& g, const uint8_t* x, void* h, hipStream_t s);
+
+// Exact score of the states x[b] (uint8, row stride n): out_e[b] = E(x_b) = -(1/2) s^T W s,
+// out_cut[b] = cut(x_b); FP64 (int64 for int8 weights).  hbuf holds one field.
+void launch_score_with(const Graph& g, const uint8_t* x, int64_t count, double* out_cut, double* out_e,
+                       void* hbuf, hipStream_t s);
+
+// Batch energies of packed candidates: q[b] = s_b^T W s_b (doubles; exact for int8).
+// bits: B rows of words_per_row uint32 (words_per_row >= n_pad/32 for dense graphs).
+// Dense graphs use MFMA (int8) or FP64 tiles; CSR graphs the LDS-staged CSR kernel.
+void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t words_per_row, int64_t B,
+                            double* q, void* scratch, hipStream_t s);
+size_t batch_scratch_bytes(const Graph& g, int64_t B);
+
+// One-work-group phase kernels over the state (x, h) of a graph.
+struct PhaseOut {
+    double gain;      // unscaled cut gain of the selected move
+    int64_t k, l;     // selected nodes (l = -1 for a single flip)
+    int64_t accepts;  // moves applied inside the kernel
+    int64_t evals;    // flip evaluations performed
+    double gain_sum;  // sum of applied gains
+};
+
+// Repeated single flips (first argmax, strictly positive gain) until none improves.
+void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s);
+// Best single flip only (no move applied).
+void launch_single_best(const Graph& g, const uint8_t* x, const void* h, PhaseOut* out, hipStream_t s);
+// Best double flip: reference semantics (last pair of each of thread_count threads) or exhaustive.
+void launch_double_best(const Graph& g, const uint8_t* x, const void* h, int64_t thread_count, int mode,
+                        PhaseOut* out, void* scratch, hipStream_t s);
+size_t double_scratch_bytes(const Graph& g);
+// Apply flips of nodes k (and l if >= 0) to (x, h).
+void launch_flip(const Graph& g, uint8_t* x, void* h, int64_t k, int64_t l, hipStream_t s);
+
+// Gray seeds: packed candidate rows for seeds [0, count) of theta.
+void launch_seed_bits(const uint8_t* x, int64_t n, int64_t words_per_row, int64_t block_size,
+                      int64_t count, uint32_t* bits, hipStream_t s);
+// Seed cut differences from the field (low-rank formula).
+void launch_seed_lowrank(const Graph& g, const uint8_t* x, const void* h, int64_t block_size, int64_t count,
+                         double* delta, hipStream_t s);
+// Convert quadratic forms to cut differences: delta[b] = (q_base - q[b]) / 4.
+void launch_q_to_delta(const double* q, const double* q_base, int64_t count, double* delta, hipStream_t s);
+// Stable top-k (value descending, index ascending): sel[r] = index of rank r, for r < k.
+void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, hipStream_t s);
+// out[i] = v[sel[i]] for i < k.
+void launch_gather(const double* v, const int64_t* sel, int64_t k, double* out, hipStream_t s);
+
+// Gray walk: walker w starts from seed sel[w] of (x, h) and walks; writes its
+// final state/field to wx/wh (row stride n) and its total cut difference to we[w].
+void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel,
+                      const double* sel_delta, int64_t block_size, int64_t walkers,
+                      int64_t thread_iterations, uint8_t* wx, void* wh, double* we, int64_t* wevals,
+                      hipStream_t s);
+
+// q = s^T h of the state (x, h), FP64 (exact int64 arithmetic for int8 weights).
+void launch_quad(const Graph& g, const uint8_t* x, const void* h, double* q, hipStream_t s);
+
+size_t field_bytes(const Graph& g);  // bytes per field entry (4 or 8)
+
+}  // namespace pqi

@@ -1,0 +1,791 @@
+// libpqi C ABI: contexts, graph upload, and the host side of the local search.
+//
+// The host code here takes the place of the reference's OpenCL plumbing
+// (pyqrackising/maxcut_tfim_util.py:9-301) and of the Python loops that drive
+// each phase (pyqrackising/spin_glass_solver.py:603-727).  Decisions between
+// phases are made on the host from a few bytes read back per phase; all
+// arrays stay resident in HBM for the whole search.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pqi_internal.h"
+
+namespace pqi {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return fail(_e == hipErrorOutOfMemory ? PQI_ERR_OOM : PQI_ERR_HIP,                   \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                      \
+    } while (0)
+
+#define CHK(expr)                       \
+    do {                                \
+        int _r = (expr);                \
+        if (_r != PQI_OK) return _r;    \
+    } while (0)
+
+// Scratch slots of a context.
+enum Slot {
+    S_X, S_H, S_RX, S_RH, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
+    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_COUNT
+};
+
+static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
+    if ((int)c->bufs.size() < S_COUNT) c->bufs.resize(S_COUNT);
+    Buffer& b = c->bufs[slot];
+    if (bytes == 0) bytes = 16;
+    if (b.bytes < bytes) {
+        if (b.p) HIPCHK(hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+        size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+        HIPCHK(hipMalloc(&b.p, want));
+        b.bytes = want;
+    }
+    *out = b.p;
+    return PQI_OK;
+}
+
+static int sync(pqi_ctx* c) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PQI_OK;
+}
+
+static int set_device(pqi_ctx* c) {
+    HIPCHK(hipSetDevice(c->device));
+    return PQI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Graph construction (host side)
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static double as_double(const void* p, int64_t i) {
+    return (double)((const T*)p)[i];
+}
+
+static double read_w(const void* p, int dtype, int64_t i) {
+    switch (dtype) {
+        case PQI_DTYPE_I8: return as_double<int8_t>(p, i);
+        case PQI_DTYPE_I32: return as_double<int32_t>(p, i);
+        case PQI_DTYPE_F32: return as_double<float>(p, i);
+        default: return as_double<double>(p, i);
+    }
+}
+
+static bool valid_dtype(int d) {
+    return d == PQI_DTYPE_I8 || d == PQI_DTYPE_I32 || d == PQI_DTYPE_F32 || d == PQI_DTYPE_F64;
+}
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// Upload the symmetric adjacency (rows sorted) in the graph's element type.
+static int upload_adjacency(pqi_graph* g, const std::vector<int32_t>& ptr, const std::vector<int32_t>& idx,
+                            const std::vector<double>& w) {
+    const int64_t nnz = (int64_t)idx.size();
+    g->nnz = nnz;
+    HIPCHK(hipMalloc(&g->adj_ptr, sizeof(int32_t) * ptr.size()));
+    HIPCHK(hipMalloc(&g->adj_idx, sizeof(int32_t) * std::max<int64_t>(1, nnz)));
+    size_t es = g->kind == PQI_WEIGHTS_I8 ? 1 : (g->kind == PQI_WEIGHTS_F32 ? 4 : 8);
+    HIPCHK(hipMalloc(&g->adj_w, es * std::max<int64_t>(1, nnz)));
+    HIPCHK(hipMemcpy(g->adj_ptr, ptr.data(), sizeof(int32_t) * ptr.size(), hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPCHK(hipMemcpy(g->adj_idx, idx.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+        if (g->kind == PQI_WEIGHTS_I8) {
+            std::vector<int8_t> t(nnz);
+            for (int64_t i = 0; i < nnz; ++i) t[i] = (int8_t)w[i];
+            HIPCHK(hipMemcpy(g->adj_w, t.data(), nnz, hipMemcpyHostToDevice));
+        } else if (g->kind == PQI_WEIGHTS_F32) {
+            std::vector<float> t(nnz);
+            for (int64_t i = 0; i < nnz; ++i) t[i] = (float)w[i];
+            HIPCHK(hipMemcpy(g->adj_w, t.data(), nnz * 4, hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(hipMemcpy(g->adj_w, w.data(), nnz * 8, hipMemcpyHostToDevice));
+        }
+    }
+    g->device_bytes += (int64_t)(sizeof(int32_t) * (ptr.size() + nnz) + es * nnz);
+    int64_t md = 0;
+    for (size_t i = 0; i + 1 < ptr.size(); ++i) md = std::max<int64_t>(md, ptr[i + 1] - ptr[i]);
+    g->max_degree = md;
+    return PQI_OK;
+}
+
+static int classify(double maxabs, bool all_int, int dtype) {
+    if (all_int && maxabs <= 127.0) return PQI_WEIGHTS_I8;
+    if (dtype == PQI_DTYPE_F32) return PQI_WEIGHTS_F32;
+    return PQI_WEIGHTS_F64;
+}
+
+static void free_graph(pqi_graph* g) {
+    if (!g) return;
+    if (g->W) (void)hipFree(g->W);
+    if (g->adj_ptr) (void)hipFree(g->adj_ptr);
+    if (g->adj_idx) (void)hipFree(g->adj_idx);
+    if (g->adj_w) (void)hipFree(g->adj_w);
+    delete g;
+}
+
+}  // namespace pqi
+
+using namespace pqi;
+
+extern "C" {
+
+const char* pqi_last_error(void) { return g_error.c_str(); }
+
+int pqi_version(void) { return 1; }
+
+int pqi_device_count(int* count) {
+    if (!count) return fail(PQI_ERR_INVALID, "count is null");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess || c <= 0) {
+        *count = 0;
+        return fail(PQI_ERR_NO_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    }
+    *count = c;
+    return PQI_OK;
+}
+
+int pqi_create(int device, pqi_ctx** out) {
+    if (!out) return fail(PQI_ERR_INVALID, "out is null");
+    *out = nullptr;
+    int count = 0;
+    CHK(pqi_device_count(&count));
+    if (device < 0 || device >= count)
+        return fail(PQI_ERR_NO_DEVICE, "device index " + std::to_string(device) + " out of range");
+    HIPCHK(hipSetDevice(device));
+    pqi_ctx* c = new pqi_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PQI_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    c->bufs.resize(S_COUNT);
+    *out = c;
+    return PQI_OK;
+}
+
+int pqi_destroy(pqi_ctx* c) {
+    if (!c) return PQI_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& b : c->bufs)
+        if (b.p) (void)hipFree(b.p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PQI_OK;
+}
+
+int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph** out) {
+    if (!c || !W || !out) return fail(PQI_ERR_INVALID, "null argument");
+    if (!valid_dtype(dtype)) return fail(PQI_ERR_INVALID, "unsupported dtype");
+    if (n < 1 || n > (1 << 20)) return fail(PQI_ERR_INVALID, "n out of range");
+    *out = nullptr;
+    CHK(set_device(c));
+    // validate and classify
+    double maxabs = 0.0;
+    bool all_int = true;
+    std::vector<int32_t> ptr(n + 1, 0);
+    std::vector<int32_t> idx;
+    std::vector<double> wv;
+    double w_up = 0.0;
+    long double w_up_l = 0.0L;
+    for (int64_t i = 0; i < n; ++i) {
+        if (read_w(W, dtype, i * n + i) != 0.0)
+            return fail(PQI_ERR_INVALID, "W has a nonzero diagonal entry at " + std::to_string(i));
+        for (int64_t j = 0; j < n; ++j) {
+            double a = read_w(W, dtype, i * n + j);
+            if (std::isnan(a) || std::isinf(a)) return fail(PQI_ERR_INVALID, "W has a non-finite entry");
+            if (j > i && a != read_w(W, dtype, j * n + i))
+                return fail(PQI_ERR_INVALID, "W is not symmetric");
+            if (a != 0.0) {
+                idx.push_back((int32_t)j);
+                wv.push_back(a);
+                maxabs = std::max(maxabs, std::fabs(a));
+                if (a != std::nearbyint(a)) all_int = false;
+                if (j > i) w_up_l += a;
+            }
+        }
+        ptr[i + 1] = (int32_t)idx.size();
+        if ((int64_t)idx.size() > INT32_MAX / 2) return fail(PQI_ERR_UNSUPPORTED, "too many nonzeros");
+    }
+    w_up = (double)w_up_l;
+    pqi_graph* g = new pqi_graph();
+    g->ctx = c;
+    g->n = n;
+    g->kind = classify(maxabs, all_int, dtype);
+    g->n_pad = round_up(n, 128);
+    g->w_up = w_up;
+    g->w_up_i = g->kind == PQI_WEIGHTS_I8 ? (int64_t)std::llround((double)w_up_l) : 0;
+    // dense copy, zero padded
+    size_t es = g->kind == PQI_WEIGHTS_I8 ? 1 : (g->kind == PQI_WEIGHTS_F32 ? 4 : 8);
+    size_t bytes = (size_t)g->n_pad * (size_t)g->n_pad * es;
+    hipError_t e = hipMalloc(&g->W, bytes);
+    if (e != hipSuccess) {
+        free_graph(g);
+        return fail(PQI_ERR_OOM, "device allocation of W failed");
+    }
+    g->device_bytes += (int64_t)bytes;
+    e = hipMemset(g->W, 0, bytes);
+    if (e == hipSuccess) {
+        // convert row by row into the device element type, then a pitched copy
+        std::vector<unsigned char> row((size_t)n * es);
+        for (int64_t i = 0; i < n && e == hipSuccess; ++i) {
+            for (int64_t j = 0; j < n; ++j) {
+                double a = read_w(W, dtype, i * n + j);
+                if (es == 1) ((int8_t*)row.data())[j] = (int8_t)a;
+                else if (es == 4) ((float*)row.data())[j] = (float)a;
+                else ((double*)row.data())[j] = a;
+            }
+            e = hipMemcpy((char*)g->W + (size_t)i * g->n_pad * es, row.data(), (size_t)n * es, hipMemcpyHostToDevice);
+        }
+    }
+    if (e != hipSuccess) {
+        free_graph(g);
+        return fail(PQI_ERR_HIP, std::string("uploading W: ") + hipGetErrorString(e));
+    }
+    int r = upload_adjacency(g, ptr, idx, wv);
+    if (r != PQI_OK) {
+        free_graph(g);
+        return r;
+    }
+    g->use_adj = g->nnz <= n * n / 8;
+    *out = g;
+    return PQI_OK;
+}
+
+int pqi_graph_csr(pqi_ctx* c, const int64_t* indptr, const int64_t* indices, const void* data, int dtype,
+                  int64_t n, int64_t nnz, pqi_graph** out) {
+    if (!c || !indptr || !out || (nnz > 0 && (!indices || !data))) return fail(PQI_ERR_INVALID, "null argument");
+    if (!valid_dtype(dtype)) return fail(PQI_ERR_INVALID, "unsupported dtype");
+    if (n < 1 || n >= INT32_MAX) return fail(PQI_ERR_INVALID, "n out of range");
+    if (indptr[0] != 0 || indptr[n] != nnz) return fail(PQI_ERR_INVALID, "indptr does not match nnz");
+    *out = nullptr;
+    CHK(set_device(c));
+    // symmetrise: count degrees
+    std::vector<int64_t> deg(n, 0);
+    double maxabs = 0.0;
+    bool all_int = true;
+    long double w_up_l = 0.0L;
+    for (int64_t u = 0; u < n; ++u) {
+        if (indptr[u + 1] < indptr[u]) return fail(PQI_ERR_INVALID, "indptr is not monotone");
+        for (int64_t e = indptr[u]; e < indptr[u + 1]; ++e) {
+            int64_t v = indices[e];
+            if (v <= u || v >= n) return fail(PQI_ERR_INVALID, "CSR must be strictly upper triangular");
+            double a = read_w(data, dtype, e);
+            if (std::isnan(a) || std::isinf(a)) return fail(PQI_ERR_INVALID, "non-finite weight");
+            if (a == 0.0) continue;
+            deg[u]++;
+            deg[v]++;
+            maxabs = std::max(maxabs, std::fabs(a));
+            if (a != std::nearbyint(a)) all_int = false;
+            w_up_l += a;
+        }
+    }
+    std::vector<int32_t> ptr(n + 1, 0);
+    for (int64_t u = 0; u < n; ++u) ptr[u + 1] = ptr[u] + (int32_t)deg[u];
+    const int64_t tot = ptr[n];
+    std::vector<int32_t> idx(tot);
+    std::vector<double> wv(tot);
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t u = 0; u < n; ++u)
+        for (int64_t e = indptr[u]; e < indptr[u + 1]; ++e) {
+            double a = read_w(data, dtype, e);
+            if (a == 0.0) continue;
+            int64_t v = indices[e];
+            idx[fill[u]] = (int32_t)v;
+            wv[fill[u]++] = a;
+            idx[fill[v]] = (int32_t)u;
+            wv[fill[v]++] = a;
+        }
+    // sort each row by column; reject duplicates
+    for (int64_t u = 0; u < n; ++u) {
+        std::vector<std::pair<int32_t, double>> rowv;
+        for (int32_t e = ptr[u]; e < ptr[u + 1]; ++e) rowv.emplace_back(idx[e], wv[e]);
+        std::sort(rowv.begin(), rowv.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (size_t t = 0; t < rowv.size(); ++t) {
+            if (t && rowv[t].first == rowv[t - 1].first) return fail(PQI_ERR_INVALID, "duplicate CSR entry");
+            idx[ptr[u] + t] = rowv[t].first;
+            wv[ptr[u] + t] = rowv[t].second;
+        }
+    }
+    pqi_graph* g = new pqi_graph();
+    g->ctx = c;
+    g->n = n;
+    g->kind = classify(maxabs, all_int, dtype);
+    g->n_pad = round_up(n, 128);
+    g->w_up = (double)w_up_l;
+    g->w_up_i = g->kind == PQI_WEIGHTS_I8 ? (int64_t)std::llround((double)w_up_l) : 0;
+    g->use_adj = true;
+    int r = upload_adjacency(g, ptr, idx, wv);
+    if (r != PQI_OK) {
+        free_graph(g);
+        return r;
+    }
+    *out = g;
+    return PQI_OK;
+}
+
+int pqi_graph_free(pqi_graph* g) {
+    if (!g) return PQI_OK;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    free_graph(g);
+    return PQI_OK;
+}
+
+int pqi_graph_get_info(const pqi_graph* g, pqi_graph_info* info) {
+    if (!g || !info) return fail(PQI_ERR_INVALID, "null argument");
+    info->n = g->n;
+    info->weights = g->kind;
+    info->has_dense = g->W != nullptr;
+    info->nnz = g->nnz;
+    info->w_up = g->w_up;
+    info->device_bytes = g->device_bytes;
+    return PQI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// State upload and scoring helpers
+// ---------------------------------------------------------------------------
+
+static int load_state(pqi_graph* g, const uint8_t* theta, uint8_t** x, void** h) {
+    pqi_ctx* c = g->ctx;
+    CHK(ensure(c, S_X, g->n, (void**)x));
+    CHK(ensure(c, S_H, g->n * field_bytes(*g), h));
+    for (int64_t i = 0; i < g->n; ++i)
+        if (theta[i] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
+    HIPCHK(hipMemcpyAsync(*x, theta, g->n, hipMemcpyHostToDevice, c->stream));
+    launch_field(*g, *x, *h, c->stream);
+    return PQI_OK;
+}
+
+static int score_device(pqi_graph* g, const uint8_t* x, double* cut, double* en) {
+    pqi_ctx* c = g->ctx;
+    double* out;
+    void* hb;
+    CHK(ensure(c, S_SCORE, 16, (void**)&out));
+    CHK(ensure(c, S_HB, g->n * field_bytes(*g), &hb));
+    launch_score_with(*g, x, 1, out, out + 1, hb, c->stream);
+    double hv[2];
+    HIPCHK(hipMemcpyAsync(hv, out, 16, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    *cut = hv[0];
+    *en = hv[1];
+    return PQI_OK;
+}
+
+static int read_out(pqi_ctx* c, const PhaseOut* d, PhaseOut* hst) {
+    HIPCHK(hipMemcpyAsync(hst, d, sizeof(PhaseOut), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy) {
+    if (!g || !theta || !cut || !energy) return fail(PQI_ERR_INVALID, "null argument");
+    CHK(set_device(g->ctx));
+    uint8_t* x;
+    void* h;
+    CHK(load_state(g, theta, &x, &h));
+    return score_device(g, x, cut, energy);
+}
+
+int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, double* values,
+                   int64_t* best_index, double* best_value) {
+    if (!g || (!bits && B > 0)) return fail(PQI_ERR_INVALID, "null argument");
+    if (B < 1) return fail(PQI_ERR_INVALID, "B must be >= 1");
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    const int64_t nw = (g->n + 31) >> 5;
+    const int64_t wpr = g->n_pad >> 5;
+    uint32_t* dbits;
+    double* q;
+    void* scratch;
+    CHK(ensure(c, S_BITS, (size_t)B * wpr * 4, (void**)&dbits));
+    CHK(ensure(c, S_Q, (size_t)B * 8, (void**)&q));
+    CHK(ensure(c, S_BATCH, batch_scratch_bytes(*g, B), &scratch));
+    // padding words stay zero: padded columns of W are zero, so they add nothing
+    HIPCHK(hipMemsetAsync(dbits, 0, (size_t)B * wpr * 4, c->stream));
+    HIPCHK(hipMemcpy2DAsync(dbits, wpr * 4, bits, nw * 4, nw * 4, B, hipMemcpyHostToDevice, c->stream));
+    launch_batch_quadratic(*g, dbits, wpr, B, q, scratch, c->stream);
+    std::vector<double> hq(B);
+    HIPCHK(hipMemcpyAsync(hq.data(), q, B * 8, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    int64_t best = 0;
+    double bv = -INFINITY;
+    for (int64_t b = 0; b < B; ++b) {
+        // cut = (2 W_up - q)/4, E = 2 cut - W_up = -q/2
+        double v = is_spin_glass ? -hq[b] / 2.0 : (2.0 * g->w_up - hq[b]) / 4.0;
+        if (g->kind == PQI_WEIGHTS_I8) {
+            long long qi = (long long)hq[b];
+            v = is_spin_glass ? (double)(-qi / 2) : (double)((2 * g->w_up_i - qi) / 4);
+        }
+        if (values) values[b] = v;
+        if (v > bv) { bv = v; best = b; }
+    }
+    if (best_index) *best_index = best;
+    if (best_value) *best_value = bv;
+    return PQI_OK;
+}
+
+int pqi_single_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, int64_t* k, double* gain) {
+    if (!g || !theta || !k || !gain) return fail(PQI_ERR_INVALID, "null argument");
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    uint8_t* x;
+    void* h;
+    CHK(load_state(g, theta, &x, &h));
+    PhaseOut* d;
+    CHK(ensure(c, S_OUT, sizeof(PhaseOut), (void**)&d));
+    launch_single_best(*g, x, h, d, c->stream);
+    PhaseOut o;
+    CHK(read_out(c, d, &o));
+    *k = o.k;
+    *gain = is_spin_glass ? 2.0 * o.gain : o.gain;
+    return PQI_OK;
+}
+
+int pqi_double_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, int64_t thread_count, int mode,
+                         int64_t* k, int64_t* l, double* gain) {
+    if (!g || !theta || !k || !l || !gain) return fail(PQI_ERR_INVALID, "null argument");
+    if (mode != PQI_DOUBLE_REFERENCE && mode != PQI_DOUBLE_EXHAUSTIVE) return fail(PQI_ERR_INVALID, "bad mode");
+    if (thread_count < 1) return fail(PQI_ERR_INVALID, "thread_count must be >= 1");
+    if (g->n < 2) return fail(PQI_ERR_INVALID, "n must be >= 2");
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    uint8_t* x;
+    void* h;
+    CHK(load_state(g, theta, &x, &h));
+    PhaseOut* d;
+    void* ds;
+    CHK(ensure(c, S_OUT, sizeof(PhaseOut), (void**)&d));
+    CHK(ensure(c, S_DBL, double_scratch_bytes(*g), &ds));
+    launch_double_best(*g, x, h, thread_count, mode, d, ds, c->stream);
+    PhaseOut o;
+    CHK(read_out(c, d, &o));
+    *k = o.k;
+    *l = o.l;
+    *gain = is_spin_glass ? 2.0 * o.gain : o.gain;
+    return PQI_OK;
+}
+
+}  // extern "C"
+
+namespace pqi {
+
+struct GrayBufs {
+    uint8_t* wx;
+    void* wh;
+    double* we;
+    int64_t* wev;
+    int64_t* sel;
+    double* seld;
+    double* delta;
+};
+
+static int check_params(const pqi_graph* g, const pqi_search_params* p) {
+    if (!p) return fail(PQI_ERR_INVALID, "params is null");
+    if (p->walkers < 1) return fail(PQI_ERR_INVALID, "walkers must be >= 1");
+    if (p->seed_multiple < 1) return fail(PQI_ERR_INVALID, "seed_multiple must be >= 1");
+    if (p->gray_iterations < 0) return fail(PQI_ERR_INVALID, "gray_iterations must be >= 0");
+    if (p->double_mode != PQI_DOUBLE_REFERENCE && p->double_mode != PQI_DOUBLE_EXHAUSTIVE)
+        return fail(PQI_ERR_INVALID, "bad double_mode");
+    int64_t nb = (g->n + 63) >> 6;
+    if (p->walkers > p->walkers * p->seed_multiple * nb) return fail(PQI_ERR_INVALID, "more walkers than seeds");
+    if (p->walkers > (1 << 20)) return fail(PQI_ERR_INVALID, "walkers too large");
+    return PQI_OK;
+}
+
+// Score every Gray seed of (x, h) and select the top `walkers` (stable order).
+static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_search_params* p, GrayBufs& b,
+                      int64_t* seed_count) {
+    pqi_ctx* c = g->ctx;
+    const int64_t nb = (g->n + 63) >> 6;
+    const int64_t bs = p->walkers * p->seed_multiple;
+    const int64_t count = bs * nb;
+    *seed_count = count;
+    CHK(ensure(c, S_DELTA, count * 8, (void**)&b.delta));
+    CHK(ensure(c, S_SEL, p->walkers * 8, (void**)&b.sel));
+    CHK(ensure(c, S_SELD, p->walkers * 8, (void**)&b.seld));
+    bool mfma = g->kind == PQI_WEIGHTS_I8 && g->W != nullptr &&
+                (p->seed_engine == PQI_SEEDS_MFMA || p->seed_engine == PQI_SEEDS_AUTO);
+    if (p->seed_engine == PQI_SEEDS_MFMA && !mfma)
+        return fail(PQI_ERR_UNSUPPORTED, "MFMA seed scoring needs dense int8 weights");
+    if (mfma) {
+        const int64_t wpr = g->n_pad >> 5;
+        uint32_t* bits;
+        double *q, *qb;
+        void* scratch;
+        CHK(ensure(c, S_BITS, (size_t)count * wpr * 4, (void**)&bits));
+        CHK(ensure(c, S_Q, count * 8, (void**)&q));
+        CHK(ensure(c, S_QB, 16, (void**)&qb));
+        CHK(ensure(c, S_BATCH, batch_scratch_bytes(*g, count + 1), &scratch));
+        launch_seed_bits(x, g->n, wpr, bs, count, bits, c->stream);
+        launch_batch_quadratic(*g, bits, wpr, count, q, scratch, c->stream);
+        // q of the base state from its maintained field: q_base = s^T h
+        launch_quad(*g, x, h, qb, c->stream);
+        launch_q_to_delta(q, qb, count, b.delta, c->stream);
+    } else {
+        launch_seed_lowrank(*g, x, h, bs, count, b.delta, c->stream);
+    }
+    launch_topk(b.delta, count, p->walkers, b.sel, c->stream);
+    launch_gather(b.delta, b.sel, p->walkers, b.seld, c->stream);
+    return PQI_OK;
+}
+
+static int gray_walk(pqi_graph* g, const uint8_t* x, const void* h, const pqi_search_params* p, GrayBufs& b,
+                     int64_t walkers, int64_t T) {
+    pqi_ctx* c = g->ctx;
+    const size_t fb = field_bytes(*g);
+    CHK(ensure(c, S_WX, (size_t)walkers * g->n, (void**)&b.wx));
+    CHK(ensure(c, S_WH, (size_t)walkers * g->n * fb, &b.wh));
+    CHK(ensure(c, S_WE, walkers * 8, (void**)&b.we));
+    CHK(ensure(c, S_WEV, walkers * 8, (void**)&b.wev));
+    launch_gray_walk(*g, x, h, b.sel, b.seld, p->walkers * p->seed_multiple, walkers, T, b.wx, b.wh, b.we, b.wev,
+                     c->stream);
+    return PQI_OK;
+}
+
+// Optional event timing of one kernel group on the context stream.
+struct EventTimer {
+    bool on = false;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit EventTimer(bool enable) : on(enable) {
+        if (on && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) on = false;
+    }
+    ~EventTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    void start(hipStream_t s) {
+        if (on) (void)hipEventRecord(a, s);
+    }
+    void stop(hipStream_t s) {
+        if (on) (void)hipEventRecord(b, s);
+    }
+    // call after the stream has been synchronised
+    double seconds() {
+        float ms = 0.f;
+        if (on && hipEventElapsedTime(&ms, a, b) == hipSuccess) return ms * 1e-3;
+        return 0.0;
+    }
+};
+
+static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace pqi
+
+extern "C" {
+
+int pqi_gray_search(pqi_graph* g, const uint8_t* theta, const pqi_search_params* p, uint8_t* state_out,
+                    double* seed_gain, double* walk_gain) {
+    if (!g || !theta || !state_out || !seed_gain || !walk_gain) return fail(PQI_ERR_INVALID, "null argument");
+    CHK(check_params(g, p));
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    uint8_t* x;
+    void* h;
+    CHK(load_state(g, theta, &x, &h));
+    GrayBufs b;
+    int64_t count;
+    CHK(gray_seeds(g, x, h, p, b, &count));
+    const double scale = p->is_spin_glass ? 2.0 : 1.0;
+    double top;
+    HIPCHK(hipMemcpyAsync(&top, b.seld, 8, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    *seed_gain = scale * top;
+    *walk_gain = 0.0;
+    if (top > 0.0) {
+        CHK(gray_walk(g, x, h, p, b, 1, 0));
+        HIPCHK(hipMemcpyAsync(state_out, b.wx, g->n, hipMemcpyDeviceToHost, c->stream));
+        return sync(c);
+    }
+    const int64_t T = (p->gray_iterations + p->walkers - 1) / p->walkers;
+    CHK(gray_walk(g, x, h, p, b, p->walkers, T));
+    std::vector<double> we(p->walkers);
+    HIPCHK(hipMemcpyAsync(we.data(), b.we, p->walkers * 8, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    int64_t bw = 0;
+    for (int64_t w = 1; w < p->walkers; ++w)
+        if (we[w] > we[bw]) bw = w;
+    *walk_gain = scale * we[bw];
+    HIPCHK(hipMemcpyAsync(state_out, b.wx + bw * g->n, g->n, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st) {
+    if (!g || !theta || !st) return fail(PQI_ERR_INVALID, "null argument");
+    CHK(check_params(g, p));
+    if (g->n < 2) return fail(PQI_ERR_INVALID, "n must be >= 2");
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    std::memset(st, 0, sizeof(*st));
+    const int64_t n = g->n;
+    const size_t fb = field_bytes(*g);
+    const double scale = p->is_spin_glass ? 2.0 : 1.0;
+    const int64_t C = n * (n - 1) / 2;
+    const int64_t G = p->walkers;
+    const int64_t T = (p->gray_iterations + G - 1) / G;
+
+    uint8_t* x;
+    void* h;
+    CHK(load_state(g, theta, &x, &h));
+    uint8_t* rx;
+    void* rh;
+    PhaseOut* dout;
+    void* dsc;
+    CHK(ensure(c, S_RX, n, (void**)&rx));
+    CHK(ensure(c, S_RH, n * fb, &rh));
+    CHK(ensure(c, S_OUT, sizeof(PhaseOut), (void**)&dout));
+    CHK(ensure(c, S_DBL, double_scratch_bytes(*g), &dsc));
+
+    double cut0, e0;
+    CHK(score_device(g, x, &cut0, &e0));
+    st->max_energy = p->is_spin_glass ? e0 : cut0;
+
+    PhaseOut o;
+    std::vector<double> we(G);
+    GrayBufs b;
+    const bool timing = (p->flags & PQI_FLAG_TIME_KERNELS) != 0;
+    EventTimer walk_timer(timing), seed_timer(timing);
+    for (;;) {
+        if (p->max_rounds > 0 && st->rounds >= p->max_rounds) break;
+        st->rounds++;
+
+        double t0 = now();
+        launch_descent(*g, x, h, INT64_MAX, dout, c->stream);
+        CHK(read_out(c, dout, &o));
+        st->single_evals += o.evals;
+        st->single_accepts += o.accepts;
+        st->max_energy += scale * o.gain_sum;
+        st->seconds_single += now() - t0;
+        // the descent ends on a failed single-flip check: go on to double flips
+
+        t0 = now();
+        launch_double_best(*g, x, h, G, p->double_mode, dout, dsc, c->stream);
+        CHK(read_out(c, dout, &o));
+        st->double_evals += p->double_mode == PQI_DOUBLE_EXHAUSTIVE ? C : std::min(G, C);
+        if (o.gain > 0.0) {
+            launch_flip(*g, x, h, o.k, o.l, c->stream);
+            st->max_energy += scale * o.gain;
+            st->double_accepts++;
+            st->seconds_double += now() - t0;
+            continue;
+        }
+        st->seconds_double += now() - t0;
+
+        t0 = now();
+        int64_t count;
+        seed_timer.start(c->stream);
+        CHK(gray_seeds(g, x, h, p, b, &count));
+        seed_timer.stop(c->stream);
+        st->seed_evals += count;
+        double top;
+        HIPCHK(hipMemcpyAsync(&top, b.seld, 8, hipMemcpyDeviceToHost, c->stream));
+        CHK(sync(c));
+        st->seed_kernel_seconds += seed_timer.seconds();
+        st->seed_launches += timing ? 1 : 0;
+        if (top > 0.0) {
+            CHK(gray_walk(g, x, h, p, b, 1, 0));
+            HIPCHK(hipMemcpyAsync(x, b.wx, n, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(h, b.wh, n * fb, hipMemcpyDeviceToDevice, c->stream));
+            st->max_energy += scale * top;
+            st->seed_accepts++;
+            st->seconds_seeds += now() - t0;
+            continue;
+        }
+        st->seconds_seeds += now() - t0;
+
+        t0 = now();
+        walk_timer.start(c->stream);
+        CHK(gray_walk(g, x, h, p, b, G, T));
+        walk_timer.stop(c->stream);
+        HIPCHK(hipMemcpyAsync(we.data(), b.we, G * 8, hipMemcpyDeviceToHost, c->stream));
+        std::vector<int64_t> wev(G);
+        HIPCHK(hipMemcpyAsync(wev.data(), b.wev, G * 8, hipMemcpyDeviceToHost, c->stream));
+        CHK(sync(c));
+        st->walk_kernel_seconds += walk_timer.seconds();
+        st->walk_launches += timing ? 1 : 0;
+        for (int64_t w = 0; w < G; ++w) st->gray_evals += wev[w];
+        int64_t bw = 0;
+        for (int64_t w = 1; w < G; ++w)
+            if (we[w] > we[bw]) bw = w;
+        const double walk = we[bw];
+        st->seconds_walk += now() - t0;
+        if (walk > 0.0) {
+            HIPCHK(hipMemcpyAsync(x, b.wx + bw * n, n, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(h, (char*)b.wh + bw * n * fb, n * fb, hipMemcpyDeviceToDevice, c->stream));
+            st->max_energy += scale * walk;
+            st->walk_accepts++;
+            continue;
+        }
+
+        // post-reheat phase (spin_glass_solver.py:670-717)
+        t0 = now();
+        HIPCHK(hipMemcpyAsync(rx, b.wx + bw * n, n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(rh, (char*)b.wh + bw * n * fb, n * fb, hipMemcpyDeviceToDevice, c->stream));
+        double reheat = walk;
+        launch_single_best(*g, rx, rh, dout, c->stream);
+        CHK(read_out(c, dout, &o));
+        st->single_evals += o.evals;
+        if (o.gain + reheat > 0.0) {
+            HIPCHK(hipMemcpyAsync(x, rx, n, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(h, rh, n * fb, hipMemcpyDeviceToDevice, c->stream));
+            launch_flip(*g, x, h, o.k, -1, c->stream);
+            st->max_energy += scale * (reheat + o.gain);
+            st->reheat_accepts++;
+            st->seconds_reheat += now() - t0;
+            continue;
+        }
+        if (o.gain > 0.0) {
+            launch_flip(*g, rx, rh, o.k, -1, c->stream);
+            reheat += o.gain;
+        }
+        launch_double_best(*g, rx, rh, G, p->double_mode, dout, dsc, c->stream);
+        CHK(read_out(c, dout, &o));
+        st->double_evals += p->double_mode == PQI_DOUBLE_EXHAUSTIVE ? C : std::min(G, C);
+        if (o.gain + reheat > 0.0) {
+            HIPCHK(hipMemcpyAsync(x, rx, n, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(h, rh, n * fb, hipMemcpyDeviceToDevice, c->stream));
+            launch_flip(*g, x, h, o.k, o.l, c->stream);
+            st->max_energy += scale * (reheat + o.gain);
+            st->reheat_accepts++;
+            st->seconds_reheat += now() - t0;
+            continue;
+        }
+        st->seconds_reheat += now() - t0;
+        break;
+    }
+    double cut, en;
+    CHK(score_device(g, x, &cut, &en));
+    st->cut_value = cut;
+    st->min_energy = p->is_spin_glass ? -en : en;
+    HIPCHK(hipMemcpyAsync(theta, x, n, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+}  // extern "C"

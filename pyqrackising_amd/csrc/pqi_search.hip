@@ -1,0 +1,712 @@
+// Local-search kernels for gfx950: local fields, single-flip descent with
+// work-group argmax, double flips, Gray seeds, top-k and Gray-code walkers.
+//
+// State of a search = (x, h): x[v] in {0,1} (uint8) and the local field
+// h[v] = sum_u W[v][u] s_u with s = 1 - 2x.  The cut gain of flipping k is
+// Delta_k = s_k h_k (pyqrackising/maxcut_tfim_util.py:334-336 computes the same
+// value in O(n)); a pair gains Delta_k + Delta_l - 2 w_kl s_k s_l
+// (maxcut_tfim_util.py:339-346).  Flipping k changes h_v by -2 w_vk s_k(old).
+// Fields are int32 for int8 weights (exact) and FP64 otherwise.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+
+#include "pqi_internal.h"
+
+namespace pqi {
+
+template <typename WT> struct FieldOf;
+template <> struct FieldOf<int8_t> { using T = int32_t; };
+template <> struct FieldOf<float> { using T = double; };
+template <> struct FieldOf<double> { using T = double; };
+
+template <typename T> __device__ __forceinline__ T lowest();
+template <> __device__ __forceinline__ int32_t lowest<int32_t>() { return INT_MIN; }
+template <> __device__ __forceinline__ int64_t lowest<int64_t>() { return LLONG_MIN; }
+template <> __device__ __forceinline__ double lowest<double>() { return -INFINITY; }
+
+// Plain-data view of a Graph for kernels.
+template <typename WT>
+struct GView {
+    const WT* W;
+    int64_t ld;
+    int64_t n;
+    const int32_t* ap;
+    const int32_t* ai;
+    const WT* aw;
+    int use_adj;
+};
+
+template <typename WT>
+static GView<WT> view(const Graph& g) {
+    GView<WT> v;
+    v.W = (const WT*)g.W;
+    v.ld = g.n_pad;
+    v.n = g.n;
+    v.ap = g.adj_ptr;
+    v.ai = g.adj_idx;
+    v.aw = (const WT*)g.adj_w;
+    v.use_adj = (g.use_adj || g.W == nullptr) ? 1 : 0;
+    return v;
+}
+
+// w_kl: dense lookup or binary search in the sorted adjacency row k.
+template <typename WT>
+__device__ __forceinline__ WT weight_of(const GView<WT>& g, int64_t k, int64_t l) {
+    if (g.W) return g.W[k * g.ld + l];
+    int32_t lo = g.ap[k], hi = g.ap[k + 1] - 1;
+    while (lo <= hi) {
+        int32_t mid = (lo + hi) >> 1;
+        int32_t c = g.ai[mid];
+        if (c == l) return g.aw[mid];
+        if (c < l) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return WT(0);
+}
+
+template <typename FT>
+__device__ __forceinline__ FT gain_of(const uint8_t* x, const FT* h, int64_t k) {
+    return x[k] ? -h[k] : h[k];
+}
+
+// (value, index) order: larger value first, then smaller index.
+template <typename FT>
+__device__ __forceinline__ bool better(FT av, int64_t ak, FT bv, int64_t bk) {
+    return av > bv || (av == bv && ak < bk);
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor_t(T v, int m) {
+    return __shfl_xor(v, m, 64);
+}
+template <>
+__device__ __forceinline__ int64_t shfl_xor_t<int64_t>(int64_t v, int m) {
+    int lo = __shfl_xor((int)(v & 0xffffffff), m, 64);
+    int hi = __shfl_xor((int)(v >> 32), m, 64);
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
+template <typename FT>
+__device__ __forceinline__ void wave_argmax(FT& v, int64_t& k) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        FT ov = shfl_xor_t(v, m);
+        int64_t ok = shfl_xor_t(k, m);
+        if (better(ov, ok, v, k)) { v = ov; k = ok; }
+    }
+}
+
+// Work-group argmax; every thread returns the winner.  Uses 2*nwaves LDS slots.
+template <typename FT>
+__device__ void block_argmax(FT& v, int64_t& k, FT* sv, int64_t* sk) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    wave_argmax(v, k);
+    __syncthreads();
+    if (lane == 0) { sv[wid] = v; sk[wid] = k; }
+    __syncthreads();
+    v = sv[0];
+    k = sk[0];
+    for (int i = 1; i < nw; ++i)
+        if (better(sv[i], sk[i], v, k)) { v = sv[i]; k = sk[i]; }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += shfl_xor_t(v, m);
+    return v;
+}
+
+// Apply the flip of node k: every thread of the calling group takes part
+// (stride = group size).  Caller synchronises before and after.
+template <typename WT, typename FT>
+__device__ __forceinline__ void flip_update(const GView<WT>& g, uint8_t* x, FT* h, int64_t k, int s_old,
+                                            int tid, int nthreads) {
+    const FT two_s = (FT)(2 * s_old);
+    if (g.use_adj) {
+        for (int32_t e = g.ap[k] + tid; e < g.ap[k + 1]; e += nthreads) h[g.ai[e]] -= two_s * (FT)g.aw[e];
+    } else {
+        const WT* Wk = g.W + k * g.ld;
+        for (int64_t v = tid; v < g.n; v += nthreads) h[v] -= two_s * (FT)Wk[v];
+    }
+    if (tid == 0) x[k] ^= 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Local field h = W s
+// ---------------------------------------------------------------------------
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void k_field_dense(GView<WT> g, const uint8_t* x, FT* h) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= g.n) return;
+    const WT* Wr = g.W + row * g.ld;
+    FT acc = 0;
+    for (int64_t u = lane; u < g.n; u += 64) acc += x[u] ? -(FT)Wr[u] : (FT)Wr[u];
+    acc = wave_sum(acc);
+    if (lane == 0) h[row] = acc;
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void k_field_adj(GView<WT> g, const uint8_t* x, FT* h) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g.n) return;
+    FT acc = 0;
+    for (int32_t e = g.ap[row]; e < g.ap[row + 1]; ++e) acc += x[g.ai[e]] ? -(FT)g.aw[e] : (FT)g.aw[e];
+    h[row] = acc;
+}
+
+template <typename WT>
+static void field_impl(const Graph& g, const uint8_t* x, void* h, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    GView<WT> v = view<WT>(g);
+    if (g.W && !g.use_adj) {
+        hipLaunchKernelGGL((k_field_dense<WT, FT>), dim3((unsigned)((g.n + 3) / 4)), dim3(256), 0, s, v, x, (FT*)h);
+    } else {
+        hipLaunchKernelGGL((k_field_adj<WT, FT>), dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, v, x, (FT*)h);
+    }
+}
+
+#define PQI_DISPATCH(g, FN, ...)                                   \
+    do {                                                           \
+        switch ((g).kind) {                                        \
+            case PQI_WEIGHTS_I8: FN<int8_t>(__VA_ARGS__); break;   \
+            case PQI_WEIGHTS_F32: FN<float>(__VA_ARGS__); break;   \
+            default: FN<double>(__VA_ARGS__); break;               \
+        }                                                          \
+    } while (0)
+
+void launch_field(const Graph& g, const uint8_t* x, void* h, hipStream_t s) {
+    PQI_DISPATCH(g, field_impl, g, x, h, s);
+}
+
+size_t field_bytes(const Graph& g) { return g.kind == PQI_WEIGHTS_I8 ? 4 : 8; }
+
+// ---------------------------------------------------------------------------
+// Exact score: q = s^T h  ->  cut = (2 W_up - q)/4, E = -q/2.
+// One work group, fixed reduction order (deterministic).
+// ---------------------------------------------------------------------------
+
+template <typename FT> struct AccOf { using T = double; };
+template <> struct AccOf<int32_t> { using T = int64_t; };
+
+template <typename FT>
+__global__ __launch_bounds__(1024) void k_score(const uint8_t* x, const FT* h, int64_t n, double w_up,
+                                               int64_t w_up_i, double* out_cut, double* out_e) {
+    using AT = typename AccOf<FT>::T;
+    __shared__ AT part[16];
+    AT acc = 0;
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) acc += x[v] ? -(AT)h[v] : (AT)h[v];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        AT q = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) q += part[i];
+        if (sizeof(AT) == 8 && sizeof(FT) == 4) {
+            // exact integer arithmetic: 2*W_up - q is a multiple of 4
+            int64_t qi = (int64_t)q;
+            *out_cut = (double)((2 * w_up_i - qi) / 4);
+            *out_e = (double)(-qi / 2);
+        } else {
+            double qd = (double)q;
+            *out_cut = (2.0 * w_up - qd) / 4.0;
+            *out_e = -qd / 2.0;
+        }
+    }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(1024) void k_quad(const uint8_t* x, const FT* h, int64_t n, double* q) {
+    using AT = typename AccOf<FT>::T;
+    __shared__ AT part[16];
+    AT acc = 0;
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) acc += x[v] ? -(AT)h[v] : (AT)h[v];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        AT t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[i];
+        q[0] = (double)t;
+    }
+}
+
+void launch_quad(const Graph& g, const uint8_t* x, const void* h, double* q, hipStream_t s) {
+    if (g.kind == PQI_WEIGHTS_I8)
+        hipLaunchKernelGGL((k_quad<int32_t>), dim3(1), dim3(1024), 0, s, x, (const int32_t*)h, g.n, q);
+    else
+        hipLaunchKernelGGL((k_quad<double>), dim3(1), dim3(1024), 0, s, x, (const double*)h, g.n, q);
+}
+
+template <typename WT>
+static void score_impl(const Graph& g, const uint8_t* x, int64_t count, double* out_cut, double* out_e,
+                       void* hbuf, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    for (int64_t b = 0; b < count; ++b) {
+        field_impl<WT>(g, x + b * g.n, hbuf, s);
+        hipLaunchKernelGGL((k_score<FT>), dim3(1), dim3(1024), 0, s, x + b * g.n, (const FT*)hbuf, g.n, g.w_up,
+                           g.w_up_i, out_cut + b, out_e + b);
+    }
+}
+
+void launch_score_with(const Graph& g, const uint8_t* x, int64_t count, double* out_cut, double* out_e,
+                       void* hbuf, hipStream_t s) {
+    PQI_DISPATCH(g, score_impl, g, x, count, out_cut, out_e, hbuf, s);
+}
+
+// ---------------------------------------------------------------------------
+// Single-flip descent: argmax of Delta over all k (first maximum), apply while
+// positive.  One work group of 1024 threads.  max_steps = 0 only reports the
+// best move (pyqrackising/spin_glass_solver.py:37-52).
+// ---------------------------------------------------------------------------
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h, int64_t max_steps,
+                                                 PhaseOut* out) {
+    __shared__ FT sv[16];
+    __shared__ int64_t sk[16];
+    const int tid = threadIdx.x;
+    int64_t steps = 0, evals = 0;
+    double gsum = 0.0;
+    FT bv;
+    int64_t bk;
+    for (;;) {
+        bv = lowest<FT>();
+        bk = LLONG_MAX;
+        for (int64_t k = tid; k < g.n; k += 1024) {
+            FT d = gain_of(x, h, k);
+            if (better(d, k, bv, bk)) { bv = d; bk = k; }
+        }
+        block_argmax(bv, bk, sv, sk);
+        evals += g.n;
+        if (!(bv > 0) || steps >= max_steps) break;
+        const int s_old = x[bk] ? -1 : 1;
+        __syncthreads();
+        flip_update(g, x, h, bk, s_old, tid, 1024);
+        __syncthreads();
+        ++steps;
+        gsum += (double)bv;
+    }
+    if (tid == 0) {
+        out->gain = (double)bv;
+        out->k = bk;
+        out->l = -1;
+        out->accepts = steps;
+        out->evals = evals;
+        out->gain_sum = gsum;
+    }
+}
+
+template <typename WT>
+static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_descent<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), x, (FT*)h, max_steps, out);
+}
+
+void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
+    PQI_DISPATCH(g, descent_impl, g, x, h, max_steps, out, s);
+}
+
+void launch_single_best(const Graph& g, const uint8_t* x, const void* h, PhaseOut* out, hipStream_t s) {
+    PQI_DISPATCH(g, descent_impl, g, (uint8_t*)x, (void*)h, 0, out, s);
+}
+
+// ---------------------------------------------------------------------------
+// Flip application (host-selected moves)
+// ---------------------------------------------------------------------------
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_flip(GView<WT> g, uint8_t* x, FT* h, int64_t k, int64_t l) {
+    int s_old = x[k] ? -1 : 1;
+    __syncthreads();
+    flip_update(g, x, h, k, s_old, threadIdx.x, blockDim.x);
+    __syncthreads();
+    if (l >= 0) {
+        s_old = x[l] ? -1 : 1;
+        __syncthreads();
+        flip_update(g, x, h, l, s_old, threadIdx.x, blockDim.x);
+    }
+}
+
+template <typename WT>
+static void flip_impl(const Graph& g, uint8_t* x, void* h, int64_t k, int64_t l, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_flip<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), x, (FT*)h, k, l);
+}
+
+void launch_flip(const Graph& g, uint8_t* x, void* h, int64_t k, int64_t l, hipStream_t s) {
+    PQI_DISPATCH(g, flip_impl, g, x, h, k, l, s);
+}
+
+// ---------------------------------------------------------------------------
+// Double flips
+// ---------------------------------------------------------------------------
+
+// Row-major pair index s -> (i, j), i < j (pyqrackising/spin_glass_solver.py:66-76).
+__device__ __forceinline__ void unrank_pair(int64_t s, int64_t n, int64_t& i, int64_t& j) {
+    int64_t c = s, r = 0, lcv = n - 1;
+    while (c >= lcv) {
+        c -= lcv;
+        ++r;
+        --lcv;
+        if (!lcv) break;
+    }
+    i = r;
+    j = c + r + 1;
+}
+
+template <typename WT, typename FT>
+__device__ __forceinline__ FT pair_gain(const GView<WT>& g, const uint8_t* x, const FT* h, int64_t i, int64_t j) {
+    const int si = x[i] ? -1 : 1, sj = x[j] ? -1 : 1;
+    return (FT)si * h[i] + (FT)sj * h[j] - (FT)(2 * si * sj) * (FT)weight_of(g, i, j);
+}
+
+// Reference CPU semantics (D2): thread t of thread_count evaluates the last
+// pair index t + tb*floor((C-1-t)/tb) of its strided loop; argmax over t.
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_double_ref(GView<WT> g, const uint8_t* x, const FT* h,
+                                                    int64_t thread_count, PhaseOut* out) {
+    __shared__ FT sv[16];
+    __shared__ int64_t sk[16];
+    const int64_t n = g.n, C = n * (n - 1) / 2;
+    const int64_t tb = (C + thread_count - 1) / thread_count;
+    FT bv = lowest<FT>();
+    int64_t bt = LLONG_MAX;
+    for (int64_t t = threadIdx.x; t < thread_count; t += blockDim.x) {
+        if (t >= C) continue;
+        int64_t s = t + tb * ((C - 1 - t) / tb), i, j;
+        unrank_pair(s, n, i, j);
+        FT d = pair_gain(g, x, h, i, j);
+        if (better(d, t, bv, bt)) { bv = d; bt = t; }
+    }
+    block_argmax(bv, bt, sv, sk);
+    if (threadIdx.x == 0) {
+        int64_t s = bt + tb * ((C - 1 - bt) / tb), i, j;
+        unrank_pair(s, n, i, j);
+        out->gain = (double)bv;
+        out->k = i;
+        out->l = j;
+        out->accepts = 0;
+        out->evals = thread_count < C ? thread_count : C;
+        out->gain_sum = 0.0;
+    }
+}
+
+// Exhaustive: first maximum over all pairs in pair-index order (kernels.cl:186-231).
+// Stage 1: each work group scans rows i = blockIdx.x, +gridDim.x, ...
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void k_double_exh(GView<WT> g, const uint8_t* x, const FT* h, FT* pv,
+                                                   int64_t* ps) {
+    __shared__ FT sv[4];
+    __shared__ int64_t sk[4];
+    const int64_t n = g.n;
+    FT bv = lowest<FT>();
+    int64_t bs = LLONG_MAX;
+    for (int64_t i = blockIdx.x; i < n - 1; i += gridDim.x) {
+        const int64_t base = i * (2 * n - i - 1) / 2 - i - 1;  // pair index of (i, j) = base + j
+        const int si = x[i] ? -1 : 1;
+        const FT di = (FT)si * h[i];
+        for (int64_t j = i + 1 + threadIdx.x; j < n; j += blockDim.x) {
+            const int sj = x[j] ? -1 : 1;
+            FT d = di + (FT)sj * h[j];
+            WT w = g.W ? g.W[i * g.ld + j] : weight_of(g, i, j);
+            d -= (FT)(2 * si * sj) * (FT)w;
+            int64_t s = base + j;
+            if (better(d, s, bv, bs)) { bv = d; bs = s; }
+        }
+    }
+    block_argmax(bv, bs, sv, sk);
+    if (threadIdx.x == 0) {
+        pv[blockIdx.x] = bv;
+        ps[blockIdx.x] = bs;
+    }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(1024) void k_double_exh_final(const FT* pv, const int64_t* ps, int nparts, int64_t n,
+                                                          PhaseOut* out) {
+    __shared__ FT sv[16];
+    __shared__ int64_t sk[16];
+    FT bv = lowest<FT>();
+    int64_t bs = LLONG_MAX;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+        if (better(pv[i], ps[i], bv, bs)) { bv = pv[i]; bs = ps[i]; }
+    block_argmax(bv, bs, sv, sk);
+    if (threadIdx.x == 0) {
+        int64_t i = -1, j = -1;
+        if (bs != LLONG_MAX) unrank_pair(bs, n, i, j);
+        out->gain = (double)bv;
+        out->k = i;
+        out->l = j;
+        out->accepts = 0;
+        out->evals = n * (n - 1) / 2;
+        out->gain_sum = 0.0;
+    }
+}
+
+static const int kExhParts = 1024;
+
+size_t double_scratch_bytes(const Graph&) { return (size_t)kExhParts * 16; }
+
+template <typename WT>
+static void double_impl(const Graph& g, const uint8_t* x, const void* h, int64_t thread_count, int mode,
+                        PhaseOut* out, void* scratch, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    GView<WT> v = view<WT>(g);
+    if (mode == PQI_DOUBLE_REFERENCE) {
+        hipLaunchKernelGGL((k_double_ref<WT, FT>), dim3(1), dim3(1024), 0, s, v, x, (const FT*)h, thread_count, out);
+    } else {
+        int parts = (int)std::min<int64_t>(kExhParts, std::max<int64_t>(1, g.n - 1));
+        FT* pv = (FT*)scratch;
+        int64_t* ps = (int64_t*)((char*)scratch + (size_t)kExhParts * 8);
+        hipLaunchKernelGGL((k_double_exh<WT, FT>), dim3(parts), dim3(256), 0, s, v, x, (const FT*)h, pv, ps);
+        hipLaunchKernelGGL((k_double_exh_final<FT>), dim3(1), dim3(1024), 0, s, pv, ps, parts, g.n, out);
+    }
+}
+
+void launch_double_best(const Graph& g, const uint8_t* x, const void* h, int64_t thread_count, int mode,
+                        PhaseOut* out, void* scratch, hipStream_t s) {
+    PQI_DISPATCH(g, double_impl, g, x, h, thread_count, mode, out, scratch, s);
+}
+
+// ---------------------------------------------------------------------------
+// Gray seeds (pyqrackising/spin_glass_solver.py:95-123 with the mutation of
+// maxcut_tfim_util.py:740-749): seed b mutates theta at positions
+// n - offset - 1 - j for the set bits j of gray(b mod block_size), offset =
+// 64*floor(b / block_size); a negative position wraps once by +n.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int64_t seed_pos(int64_t n, int64_t offset, int j) {
+    int64_t idx = n - offset - 1 - j;
+    while (idx < 0) idx += n;
+    return idx;
+}
+
+__global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs,
+                                                   int64_t count, uint32_t* bits) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * wpr) return;
+    const int64_t b = t / wpr, w = t % wpr;
+    uint32_t word = 0;
+    for (int q = 0; q < 32; ++q) {
+        int64_t v = w * 32 + q;
+        if (v < n && x[v]) word |= 1u << q;
+    }
+    const uint64_t i = (uint64_t)(b % bs);
+    const uint64_t gray = i ^ (i >> 1);
+    const int64_t offset = (b / bs) << 6;
+    for (int j = 0; j < 64 && j < n; ++j) {
+        if (!((gray >> j) & 1u)) continue;
+        int64_t idx = seed_pos(n, offset, j);
+        if ((idx >> 5) == w) word ^= 1u << (idx & 31);
+    }
+    bits[b * wpr + w] = word;
+}
+
+void launch_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs, int64_t count, uint32_t* bits,
+                      hipStream_t s) {
+    int64_t threads = count * wpr;
+    hipLaunchKernelGGL(k_seed_bits, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, x, n, wpr, bs, count,
+                       bits);
+}
+
+// Cut difference of flipping the set F from (x, h):
+//   sum_{k in F} s_k h_k - 2 sum_{k<l in F} w_kl s_k s_l.
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void k_seed_lowrank(GView<WT> g, const uint8_t* x, const FT* h, int64_t bs,
+                                                      int64_t count, double* delta) {
+    using AT = typename AccOf<FT>::T;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= count) return;
+    const int64_t n = g.n;
+    const uint64_t i = (uint64_t)(b % bs);
+    const uint64_t gray = i ^ (i >> 1);
+    const int64_t offset = (b / bs) << 6;
+    int64_t F[64];
+    int m = 0;
+    for (int j = 0; j < 64 && j < n; ++j) {
+        if (!((gray >> j) & 1u)) continue;
+        int64_t idx = seed_pos(n, offset, j);
+        int dup = -1;
+        for (int q = 0; q < m; ++q)
+            if (F[q] == idx) dup = q;
+        if (dup >= 0) F[dup] = F[--m];  // XOR twice cancels
+        else F[m++] = idx;
+    }
+    AT acc = 0;
+    for (int q = 0; q < m; ++q) {
+        const int64_t k = F[q];
+        const int sk = x[k] ? -1 : 1;
+        acc += (AT)sk * (AT)h[k];
+        for (int r = q + 1; r < m; ++r) {
+            const int64_t l = F[r];
+            const int sl = x[l] ? -1 : 1;
+            acc -= (AT)(2 * sk * sl) * (AT)weight_of(g, k, l);
+        }
+    }
+    delta[b] = (double)acc;
+}
+
+template <typename WT>
+static void lowrank_impl(const Graph& g, const uint8_t* x, const void* h, int64_t bs, int64_t count, double* delta,
+                         hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_seed_lowrank<WT, FT>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                       view<WT>(g), x, (const FT*)h, bs, count, delta);
+}
+
+void launch_seed_lowrank(const Graph& g, const uint8_t* x, const void* h, int64_t bs, int64_t count, double* delta,
+                         hipStream_t s) {
+    PQI_DISPATCH(g, lowrank_impl, g, x, h, bs, count, delta, s);
+}
+
+__global__ void k_q_to_delta(const double* q, const double* q_base, int64_t count, double* delta) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < count) delta[b] = (q_base[0] - q[b]) * 0.25;
+}
+
+void launch_q_to_delta(const double* q, const double* q_base, int64_t count, double* delta, hipStream_t s) {
+    hipLaunchKernelGGL(k_q_to_delta, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, q, q_base, count, delta);
+}
+
+__global__ void k_gather(const double* v, const int64_t* sel, int64_t k, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) out[i] = v[sel[i]];
+}
+
+void launch_gather(const double* v, const int64_t* sel, int64_t k, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, v, sel, k, out);
+}
+
+// Stable top-k by rank counting: rank(a) = #{b : v_b > v_a or (v_b == v_a and b < a)}.
+__global__ __launch_bounds__(256) void k_topk(const double* v, int64_t count, int64_t k, int64_t* sel) {
+    __shared__ double tile[256];
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double va = a < count ? v[a] : 0.0;
+    int64_t rank = 0;
+    for (int64_t t0 = 0; t0 < count; t0 += 256) {
+        __syncthreads();
+        if (t0 + threadIdx.x < count) tile[threadIdx.x] = v[t0 + threadIdx.x];
+        __syncthreads();
+        const int lim = (int)std::min<int64_t>(256, count - t0);
+        for (int q = 0; q < lim; ++q) {
+            const double vb = tile[q];
+            const int64_t b = t0 + q;
+            rank += (vb > va || (vb == va && b < a)) ? 1 : 0;
+        }
+    }
+    if (a < count && rank < k) sel[rank] = a;
+}
+
+void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, hipStream_t s) {
+    hipLaunchKernelGGL(k_topk, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, v, count, k, sel);
+}
+
+// ---------------------------------------------------------------------------
+// Gray-code walkers (pyqrackising/spin_glass_solver.py:126-153).
+//
+// One wavefront per walker.  Iteration i flips, for block b = 0..nb-1 in order,
+// the node k = ctz(i+1) + 64 b and keeps it iff its gain is positive.  All nb
+// candidate gains of an iteration are evaluated at once, one block per lane;
+// a ballot finds the first positive one, which is applied, and the lanes after
+// it re-evaluate.  This is the reference's sequential order exactly.
+// The walker's state lives in LDS when it fits, else in its global slot.
+// ---------------------------------------------------------------------------
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0, const FT* h0, const int64_t* sel,
+                                                 const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx,
+                                                 FT* wh, double* we, int64_t* wevals, int use_lds) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x;
+    const int64_t w = blockIdx.x;
+    const int64_t n = g.n;
+    const int64_t nb = (n + 63) >> 6;
+    uint8_t* gx = wx + w * n;
+    FT* gh = wh + w * n;
+    FT* h = use_lds ? (FT*)lds : gh;
+    uint8_t* x = use_lds ? (uint8_t*)(lds + ((n * sizeof(FT) + 15) & ~(size_t)15)) : gx;
+
+    for (int64_t v = lane; v < n; v += 64) {
+        h[v] = h0[v];
+        x[v] = x0[v];
+    }
+    __syncthreads();
+
+    // seed mutation
+    const int64_t seed = sel[w];
+    const uint64_t si = (uint64_t)(seed % bs);
+    const uint64_t gray = si ^ (si >> 1);
+    const int64_t offset = (seed / bs) << 6;
+    for (int j = 0; j < 64 && j < n; ++j) {
+        if (!((gray >> j) & 1u)) continue;
+        const int64_t k = seed_pos(n, offset, j);
+        const int s_old = x[k] ? -1 : 1;
+        __syncthreads();
+        flip_update(g, x, h, k, s_old, lane, 64);
+        __syncthreads();
+    }
+
+    double energy = sel_delta[w];
+    int64_t evals = 0;
+    for (int64_t ci = 0; ci < T; ++ci) {
+        const int p = __builtin_ctzll((unsigned long long)(ci + 1));
+        double gain_i = 0.0;
+        for (int64_t b0 = 0; b0 < nb; b0 += 64) {
+            const int64_t b = b0 + lane;
+            const int64_t k = (int64_t)p + 64 * b;
+            const bool valid = (b < nb) && (k < n);
+            FT d = valid ? gain_of(x, h, k) : (FT)0;
+            evals += __popcll(__ballot(valid));
+            uint64_t mask = __ballot(valid && d > 0);
+            while (mask) {
+                const int bl = __builtin_ctzll(mask);
+                const int64_t kk = (int64_t)p + 64 * (b0 + bl);
+                const FT dd = __shfl(d, bl, 64);
+                gain_i += (double)dd;
+                const int s_old = x[kk] ? -1 : 1;
+                __syncthreads();
+                flip_update(g, x, h, kk, s_old, lane, 64);
+                __syncthreads();
+                d = valid ? gain_of(x, h, k) : (FT)0;
+                mask = __ballot(valid && d > 0 && lane > bl);
+            }
+        }
+        energy += gain_i;
+    }
+    __syncthreads();
+    if (use_lds) {
+        for (int64_t v = lane; v < n; v += 64) {
+            gh[v] = h[v];
+            gx[v] = x[v];
+        }
+    }
+    if (lane == 0) {
+        we[w] = energy;
+        wevals[w] = evals;
+    }
+}
+
+template <typename WT>
+static void walk_impl(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel, const double* sel_delta,
+                      int64_t bs, int64_t walkers, int64_t T, uint8_t* wx, void* wh, double* we, int64_t* wevals,
+                      hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    size_t lds = ((g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
+    int use_lds = lds <= 64 * 1024 ? 1 : 0;
+    hipLaunchKernelGGL((k_gray_walk<WT, FT>), dim3((unsigned)walkers), dim3(64), use_lds ? lds : 0, s, view<WT>(g),
+                       x, (const FT*)h, sel, sel_delta, bs, T, wx, (FT*)wh, we, wevals, use_lds);
+}
+
+void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel, const double* sel_delta,
+                      int64_t block_size, int64_t walkers, int64_t thread_iterations, uint8_t* wx, void* wh,
+                      double* we, int64_t* wevals, hipStream_t s) {
+    PQI_DISPATCH(g, walk_impl, g, x, h, sel, sel_delta, block_size, walkers, thread_iterations, wx, wh, we, wevals, s);
+}
+
+}  // namespace pqi

@@ -1,0 +1,207 @@
+"""Host-side candidate sampler: the TFIM Hamming-weight prior and the
+neighbour-repulsion bit picker that produce the candidate bitstrings the GPU
+scores (pyqrackising/maxcut_tfim_util.py:545-724, maxcut_tfim.py:36-75,
+139-145, 189-209; maxcut_tfim_sparse.py:37-88, 204-224).
+
+The arithmetic is restated in float64 numpy.  Two deliberate differences from
+the reference: the random stream is a seeded numpy PCG64 generator (the
+reference draws from numba's unseedable per-thread generator, so its candidate
+streams cannot be reproduced anyway), and shots are drawn vectorised over a
+chunk of shots instead of one numba thread per shot.
+"""
+import math
+
+import numpy as np
+
+EPSILON = 2.0 ** -23  # pyqrackising/maxcut_tfim_util.py:87-88 (float32 build)
+
+
+# ---- prior ------------------------------------------------------------------
+
+def init_J_and_z_dense(G_m):
+    """pyqrackising/maxcut_tfim.py:189-209."""
+    G = np.asarray(G_m, dtype=np.float64)
+    G_min = G.min()
+    val = G - G_min
+    mask = val > EPSILON
+    degrees = mask.sum(axis=1).astype(np.uint32)
+    J = np.where(mask, val, 0.0).sum(axis=1)
+    J_eff = np.where(degrees > 0, -J / np.maximum(degrees, 1), 0.0)
+    return J_eff, degrees
+
+
+def init_J_and_z_sparse(G_m):
+    """pyqrackising/maxcut_tfim_sparse.py:204-224 on the upper-triangular CSR:
+    every one of the n entries of a row counts, implicit zeros included."""
+    n = G_m.shape[0]
+    data = np.asarray(G_m.data, dtype=np.float64)
+    indptr = np.asarray(G_m.indptr)
+    nnz_row = np.diff(indptr)
+    G_min = min(data.min() if data.size else 0.0, 0.0 if (nnz_row < n).any() else np.inf)
+    rows = np.repeat(np.arange(n), nnz_row)
+    val = data - G_min
+    m = val > EPSILON
+    degrees = np.bincount(rows[m], minlength=n).astype(np.int64)
+    J = np.bincount(rows[m], weights=val[m], minlength=n)
+    zval = 0.0 - G_min
+    if zval > EPSILON:
+        implicit = n - nnz_row
+        degrees += implicit
+        J += implicit * zval
+    J_eff = np.where(degrees > 0, -J / np.maximum(degrees, 1), 0.0)
+    return J_eff, degrees.astype(np.uint32)
+
+
+def init_thresholds(n_qubits):
+    """pyqrackising/maxcut_tfim_util.py:558-575."""
+    n_bias = n_qubits + 1
+    thresholds = np.empty(n_bias, dtype=np.float64)
+    normalizer = 0
+    for q in range(n_qubits >> 1):
+        normalizer += math.comb(n_qubits, q) << 1
+    if n_qubits & 1:
+        normalizer += math.comb(n_qubits, n_qubits >> 1)
+    p = 1
+    for q in range(n_qubits >> 1):
+        val = p / normalizer
+        thresholds[q] = val
+        thresholds[n_bias - (q + 1)] = val
+        p = math.comb(n_qubits, q + 1)
+    if n_qubits & 1:
+        thresholds[n_qubits >> 1] = p / normalizer
+    return thresholds
+
+
+def _theta_c(J, h, z):
+    zJ = z * J
+    if abs(zJ) < EPSILON:
+        return (np.pi if J > 0 else -np.pi) / 2
+    return np.arcsin(max(-1.0, min(1.0, h / zJ)))
+
+
+def _time_envelope(bias, t, n_qubits):
+    """pyqrackising/maxcut_tfim_util.py:578-623 (alpha = 1)."""
+    half = n_qubits / 2.0
+    t_eff = t if t > 1e-9 else 1e-9
+    k = np.arange(n_qubits + 1, dtype=np.float64)
+    out = bias * np.exp(-np.abs(k - half) / t_eff)
+    tot = out.sum()
+    if tot <= 0.0:
+        out = np.zeros(n_qubits + 1)
+        out[n_qubits // 2] = 1.0
+        return out
+    return out / tot
+
+
+def probability_by_hamming_weight(J, h, z, theta, t, n_bias, normalized=True, omega=1.5 * np.pi):
+    """pyqrackising/maxcut_tfim_util.py:626-651."""
+    theta_c = _theta_c(J, h, z)
+    p = (2.0 ** (abs(J / h) - 1.0) * (1.0 + np.sin(theta - theta_c) * np.cos(omega * J * t + theta)
+                                       / (1.0 + np.sqrt(t))) - 0.5)
+    factor = 2.0 ** -(p / n_bias)
+    bias = factor ** np.arange(1, n_bias + 1, dtype=np.float64)
+    if J > 0.0:
+        return bias[::-1]
+    if normalized:
+        bias = bias / bias.sum()
+    return _time_envelope(bias / bias.sum(), t, n_bias - 1)
+
+
+def maxcut_hamming_cdf(n_qubits, J_func, degrees, quality, tot_t, h_mult, omega=1.5 * np.pi):
+    """pyqrackising/maxcut_tfim_util.py:654-687 (with init_theta, :545-555).
+    Nodes sharing (J, degree) share every bias vector, so each distinct pair
+    is evaluated once per step and weighted by its multiplicity."""
+    hamming_prob = init_thresholds(n_qubits)
+    n_steps = 1 << quality
+    delta_t = tot_t / n_steps
+    n_bias = n_qubits + 1
+    h_abs = abs(h_mult)
+    pairs = {}
+    for q in range(n_qubits):
+        J = float(J_func[q])
+        if abs(J) < EPSILON:
+            continue
+        key = (J, int(degrees[q]))
+        pairs[key] = pairs.get(key, 0) + 1
+    for (J, z), mult in pairs.items():
+        theta_eff = _theta_c(J, h_abs, z)
+        for step in range(1, n_steps):
+            t = step * delta_t
+            tm1 = (step - 1) * delta_t
+            h_t = h_mult * (tot_t - t)
+            bias = probability_by_hamming_weight(J, h_t, z, theta_eff, t, n_bias, False, omega)
+            last = probability_by_hamming_weight(J, h_t, z, theta_eff, tm1, n_bias, False, omega)
+            hamming_prob += mult * (bias - last)
+    hamming_prob /= hamming_prob.sum() - (hamming_prob[0] + hamming_prob[-1])
+    cum_prob = np.cumsum(hamming_prob[1:-1])
+    cum_prob[-1] = 1.0
+    return cum_prob
+
+
+# ---- sampler ----------------------------------------------------------------
+
+class RepulsionSampler:
+    """pyqrackising/maxcut_tfim.py:36-75 (local_repulsion_choice) with
+    sample_mag / bit_pick (maxcut_tfim_util.py:690-724), vectorised over shots.
+
+    Semantics kept: the first node of shot s is s mod n; a sampled m <= 1 picks
+    as the reference does (m == 1 gives one node, m == 0 gives two); a node is
+    "selected" when its weight is exactly 0 at the end (weights == 0.0)."""
+
+    def __init__(self, W, cum_prob, repulsion_base, rng, chunk=None):
+        self.W = np.asarray(W, dtype=np.float64)  # symmetric dense
+        self.n = self.W.shape[0]
+        self.cum_prob = np.asarray(cum_prob, dtype=np.float64)
+        self.base = float(repulsion_base)
+        self.rng = rng
+        self.no_repulsion = abs(1.0 - self.base) <= EPSILON
+        if not self.no_repulsion:
+            self.factor = self.base ** (-self.W)  # row k: multiplier after picking k
+        self.chunk = chunk or max(1, min(4096, (1 << 24) // max(1, self.n)))
+
+    def sample(self, shots, start=0):
+        n = self.n
+        out = np.empty((shots, n), dtype=bool)
+        for c0 in range(0, shots, self.chunk):
+            c1 = min(shots, c0 + self.chunk)
+            out[c0:c1] = self._chunk(np.arange(start + c0, start + c1))
+        return out
+
+    def _pick(self, weights):
+        p = weights.sum(axis=1) * self.rng.random(weights.shape[0])
+        cum = np.cumsum(weights, axis=1)
+        hit = p[:, None] < cum
+        node = np.argmax(hit, axis=1)
+        node[~hit.any(axis=1)] = 0
+        return node
+
+    def _update(self, weights, rows, node):
+        weights[rows, node] = 0.0
+        if not self.no_repulsion:
+            weights[rows] *= self.factor[node]
+
+    def _chunk(self, s_idx):
+        n = self.n
+        S = s_idx.size
+        m = np.searchsorted(self.cum_prob, self.rng.random(S), side="left")
+        first = s_idx % n
+        weights = np.ones((S, n), dtype=np.float64)
+        rows = np.arange(S)
+        one = m == 1
+        self._update(weights, rows, first)
+        # picks 1 .. m-2 update, pick m-1 (or 1 when m == 0) only zeroes
+        last_pick = np.maximum(m - 1, 1)
+        tmax = int(last_pick.max()) if S else 0
+        for t in range(1, tmax + 1):
+            act = (t <= last_pick) & ~one
+            if not act.any():
+                break
+            r = rows[act]
+            node = self._pick(weights[r])
+            upd = t < last_pick[r]
+            self._update(weights, r[upd], node[upd])
+            weights[r[~upd], node[~upd]] = 0.0
+        sel = weights == 0.0
+        sel[one] = False
+        sel[rows[one], first[one]] = True
+        return sel

@@ -1,0 +1,325 @@
+"""GPU parity: every libpqi entry point against the CPU oracle on the same
+seeded inputs.  Integer weights must match bit for bit (values, selected
+indices, accepted moves, final states); real weights within 1e-9 relative of
+the FP64 oracle."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from pyqrackising_amd import _lib, graphs
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-9  # FP64 fields and re-scores: summation-order differences only
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def regular_pm1(n, d, seed, dtype=np.int8):
+    rng = np.random.default_rng(seed)
+    lo, hi = graphs.random_regular_edges(n, d, rng)
+    return graphs.dense_from_edges(n, lo, hi, graphs.pm1(lo.size, rng), dtype=dtype)
+
+
+def er_weights(n, p, seed, kind):
+    rng = np.random.default_rng(seed)
+    lo, hi = graphs.erdos_renyi_edges(n, p, rng)
+    if kind == "int":
+        w = rng.integers(-5, 6, lo.size).astype(np.float64)
+    elif kind == "unit":
+        w = np.ones(lo.size)
+    else:
+        w = rng.normal(size=lo.size)
+    return graphs.dense_from_edges(n, lo, hi, w, dtype=np.float64)
+
+
+def upper(W):
+    n = W.shape[0]
+    iu, ju = np.nonzero(np.triu(W, 1))
+    return graphs.upper_csr(n, iu, ju, W[iu, ju], dtype=np.float64)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert _lib.device_count() > 0, "no GPU visible"
+    return _lib.default_context()
+
+
+# ---------------------------------------------------------------------------
+# candidate scoring
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n,B", [(24, 7), (100, 129), (130, 300), (257, 1000), (2048, 256)])
+@pytest.mark.parametrize("sg", [0, 1])
+def test_eval_batch_int8_mfma(ctx, orc, n, B, sg):
+    log(f"eval_batch int8 n={n} B={B} sg={sg}")
+    W = regular_pm1(n, 3, seed=n) if n % 2 == 0 else er_weights(n, 0.1, n, "int")
+    rng = np.random.default_rng(n + B)
+    X = rng.integers(0, 2, (B, n)).astype(bool)
+    X[B // 2] = X[0]  # a tie: first index wins
+    dg = _lib.DeviceGraph.dense(W.astype(np.float32))
+    assert dg.info()["weights"] == _lib.WEIGHTS_I8
+    bi, bv, vals = dg.eval_batch(_lib.pack_bits(X), sg)
+    obi, ovals = orc.eval_batch(W, X, sg)
+    assert np.array_equal(vals, ovals)
+    assert bi == obi and bv == ovals[obi]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("sg", [0, 1])
+def test_eval_batch_real(ctx, orc, dtype, sg):
+    n, B = 150, 200
+    log(f"eval_batch real n={n} dtype={dtype.__name__} sg={sg}")
+    W = er_weights(n, 0.3, 5, "real").astype(dtype).astype(np.float64)
+    X = np.random.default_rng(6).integers(0, 2, (B, n)).astype(bool)
+    dg = _lib.DeviceGraph.dense(W.astype(dtype))
+    bi, bv, vals = dg.eval_batch(_lib.pack_bits(X), sg)
+    obi, ovals = orc.eval_batch(W, X, sg)
+    np.testing.assert_allclose(vals, ovals, rtol=REL, atol=REL * np.abs(ovals).max())
+    assert bi == obi
+
+
+@pytest.mark.parametrize("kind", ["int", "real"])
+def test_eval_batch_csr(ctx, orc, kind):
+    n, B = 300, 64
+    log(f"eval_batch csr kind={kind}")
+    W = er_weights(n, 0.05, 7, kind)
+    G = upper(W)
+    X = np.random.default_rng(8).integers(0, 2, (B, n)).astype(bool)
+    dg = _lib.DeviceGraph.csr(G)
+    for sg in (0, 1):
+        bi, bv, vals = dg.eval_batch(_lib.pack_bits(X), sg)
+        obi, ovals = orc.eval_batch_sparse(G, X, sg)
+        if kind == "int":
+            assert np.array_equal(vals, ovals) and bi == obi
+        else:
+            np.testing.assert_allclose(vals, ovals, rtol=REL, atol=1e-9)
+            assert bi == obi
+
+
+# ---------------------------------------------------------------------------
+# single / double flips
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind", ["int", "real"])
+@pytest.mark.parametrize("sg", [0, 1])
+def test_single_flip(ctx, orc, kind, sg):
+    n = 90
+    log(f"single flip kind={kind} sg={sg}")
+    W = er_weights(n, 0.2, 9, kind)
+    dg = _lib.DeviceGraph.dense(W)
+    rng = np.random.default_rng(10)
+    for _ in range(5):
+        x = rng.integers(0, 2, n).astype(np.uint8)
+        g, k = dg.single_flip_best(x, sg)
+        og, ok = orc.single_bit_flips(x, sg, W)
+        assert k == ok
+        assert g == pytest.approx(og, rel=REL, abs=1e-9) if kind == "real" else g == og
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("kind", ["int", "real"])
+def test_double_flip(ctx, orc, mode, kind):
+    n = 70
+    log(f"double flip mode={mode} kind={kind}")
+    W = er_weights(n, 0.2, 11, kind)
+    dg = _lib.DeviceGraph.dense(W)
+    dcsr = _lib.DeviceGraph.csr(upper(W))
+    rng = np.random.default_rng(12)
+    for T in (3, 8, 64):
+        x = rng.integers(0, 2, n).astype(np.uint8)
+        og, ok, ol = orc.double_bit_flips(x, 1, W, T, mode)
+        for d in (dg, dcsr):
+            g, k, l = d.double_flip_best(x, 1, T, mode)
+            assert (k, l) == (ok, ol)
+            assert g == pytest.approx(og, rel=REL, abs=1e-9) if kind == "real" else g == og
+
+
+# ---------------------------------------------------------------------------
+# Gray phase
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("engine", [_lib.SEEDS_MFMA, _lib.SEEDS_LOWRANK])
+@pytest.mark.parametrize("n", [64, 70, 200])
+def test_gray_search(ctx, orc, engine, n):
+    log(f"gray search engine={engine} n={n}")
+    W = er_weights(n, 0.08, 13 + n, "int")
+    dg = _lib.DeviceGraph.dense(W)
+    rng = np.random.default_rng(14)
+    walkers, mult, iters = 6, 3, 600
+    # start from a single-flip local optimum so the walk phase is reached
+    x = rng.integers(0, 2, n).astype(np.uint8)
+    x, _ = orc.dense_local_search(W, x, walkers=walkers, seed_multiple=mult, gray_iterations=0, max_rounds=200)
+    p = _lib.make_params(n, is_spin_glass=1, gray_iterations=iters, walkers=walkers, seed_multiple=mult,
+                         seed_engine=engine)
+    sg_gain, wg_gain, state = dg.gray_search(x, p)
+    seeds, en, idx = orc.pick_gray_seeds(x, walkers, mult, W)
+    assert sg_gain == 2 * en[0]
+    if en[0] > 0:
+        assert np.array_equal(state, seeds[0])
+        return
+    bw, gain, it, en2, _ = orc.gray_optimization(seeds, en, iters, walkers, 1, W)
+    assert wg_gain == gain
+    assert np.array_equal(state, it[bw])
+
+
+# ---------------------------------------------------------------------------
+# the whole local search
+# ---------------------------------------------------------------------------
+
+def _cmp_search(dg, orc_fn, W_or_G, x0, sg, real, **kw):
+    p = _lib.make_params(len(x0), is_spin_glass=sg, **kw)
+    xg, st = dg.local_search(x0, p)
+    okw = {k: v for k, v in kw.items() if k != "seed_engine"}
+    xc, r = orc_fn(W_or_G, x0, is_spin_glass=sg, **okw)
+    assert np.array_equal(xg, xc), "final states differ"
+    for key in ("single_accepts", "double_accepts", "seed_accepts", "walk_accepts", "reheat_accepts", "gray_evals"):
+        assert st[key] == r[key], key
+    if real:
+        for key in ("cut_value", "min_energy", "max_energy"):
+            assert st[key] == pytest.approx(r[key], rel=REL, abs=1e-9), key
+    else:
+        for key in ("cut_value", "min_energy", "max_energy"):
+            assert st[key] == r[key], key
+    return st
+
+
+@pytest.mark.parametrize("sg", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_local_search_int_dense(ctx, orc, sg, mode):
+    n = 128
+    log(f"local search int dense sg={sg} mode={mode}")
+    W = regular_pm1(n, 3, seed=15)
+    dg = _lib.DeviceGraph.dense(W)
+    x0 = np.random.default_rng(16).integers(0, 2, n).astype(np.uint8)
+    _cmp_search(dg, orc.dense_local_search, W.astype(np.float64), x0, sg, False, walkers=8, seed_multiple=4,
+                gray_iterations=2000, double_mode=mode)
+
+
+@pytest.mark.parametrize("engine", [_lib.SEEDS_MFMA, _lib.SEEDS_LOWRANK])
+def test_local_search_seed_engines_agree(ctx, orc, engine):
+    n = 200
+    log(f"local search engine={engine}")
+    W = er_weights(n, 0.05, 17, "int")
+    dg = _lib.DeviceGraph.dense(W)
+    x0 = np.random.default_rng(18).integers(0, 2, n).astype(np.uint8)
+    _cmp_search(dg, orc.dense_local_search, W, x0, 1, False, walkers=5, seed_multiple=2, gray_iterations=1500,
+                seed_engine=engine)
+
+
+@pytest.mark.parametrize("sg", [0, 1])
+def test_local_search_real_dense(ctx, orc, sg):
+    n = 100
+    log(f"local search real dense sg={sg}")
+    W = er_weights(n, 0.3, 19, "real")
+    dg = _lib.DeviceGraph.dense(W)
+    x0 = np.random.default_rng(20).integers(0, 2, n).astype(np.uint8)
+    _cmp_search(dg, orc.dense_local_search, W, x0, sg, True, walkers=4, seed_multiple=4, gray_iterations=1000)
+
+
+@pytest.mark.parametrize("kind", ["int", "real"])
+def test_local_search_sparse(ctx, orc, kind):
+    n = 300
+    log(f"local search sparse kind={kind}")
+    W = er_weights(n, 8.0 / n, 21, kind)
+    G = upper(W)
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = np.random.default_rng(22).integers(0, 2, n).astype(np.uint8)
+    p = _lib.make_params(n, is_spin_glass=1, walkers=8, seed_multiple=2, gray_iterations=3000)
+    xg, st = dg.local_search(x0, p)
+    xc, r = orc.sparse_local_search(G, x0, is_spin_glass=1, walkers=8, seed_multiple=2, gray_iterations=3000)
+    assert np.array_equal(xg, xc)
+    if kind == "int":
+        assert st["cut_value"] == r["cut_value"] and st["min_energy"] == r["min_energy"]
+    else:
+        assert st["cut_value"] == pytest.approx(r["cut_value"], rel=REL)
+        assert st["min_energy"] == pytest.approx(r["min_energy"], rel=REL)
+
+
+def test_local_search_c2_shape(ctx, orc):
+    """The benchmark's instance shape (3-regular, n=2048, +-1) with the Gray
+    phase capped so the oracle finishes in seconds."""
+    n = 2048
+    log("local search C2 shape")
+    W = regular_pm1(n, 3, seed=23)
+    dg = _lib.DeviceGraph.dense(W)
+    x0 = np.random.default_rng(24).integers(0, 2, n).astype(np.uint8)
+    st = _cmp_search(dg, orc.dense_local_search, W.astype(np.float64), x0, 1, False, walkers=16, seed_multiple=8,
+                     gray_iterations=4096, seed_engine=_lib.SEEDS_MFMA)
+    assert st["seed_evals"] > 0
+
+
+def test_determinism(ctx):
+    n = 256
+    W = regular_pm1(n, 5 if False else 4, seed=25)
+    dg = _lib.DeviceGraph.dense(W)
+    x0 = np.random.default_rng(26).integers(0, 2, n).astype(np.uint8)
+    p = _lib.make_params(n, walkers=8, seed_multiple=2, gray_iterations=2048)
+    a, sa = dg.local_search(x0, p)
+    b, sb = dg.local_search(x0, p)
+    assert np.array_equal(a, b) and sa["cut_value"] == sb["cut_value"]
+
+
+# ---------------------------------------------------------------------------
+# Python entry points
+# ---------------------------------------------------------------------------
+
+def test_exact_small_graphs_known_answers(ctx):
+    """maxcut_tfim below the heuristic threshold is the 2^n brute force
+    (pyqrackising/maxcut_tfim.py:364-372); known answers from the reference's
+    scripts (Petersen 12, icosahedral 20, K_{m,n} m*n)."""
+    import json
+    import os
+
+    import networkx as nx
+
+    from pyqrackising_amd import maxcut_tfim, maxcut_tfim_sparse
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        G = nx.Graph()
+        G.add_nodes_from(range(c["n"]))
+        for (u, v), w in zip(c["edges"], c["weights"]):
+            G.add_edge(u, v, weight=w)
+        for fn in (maxcut_tfim, maxcut_tfim_sparse):
+            bits, value, (l, r) = fn(G)
+            assert value == c["maxcut"], (c["name"], fn.__name__)
+            x = [b == "1" for b in bits]
+            assert nx.cut_size(G, [i for i in range(c["n"]) if x[i]], weight="weight") == value
+
+
+def test_spin_glass_solver_matches_oracle(ctx, orc):
+    from pyqrackising_amd import spin_glass_solver, spin_glass_solver_sparse
+
+    n = 96
+    W = regular_pm1(n, 3, seed=27).astype(np.float32)
+    guess = np.random.default_rng(28).integers(0, 2, n).astype(bool).tolist()
+    bits, cut, (l, r), e = spin_glass_solver(W, best_guess=guess, gray_iterations=900, gray_seed_multiple=4)
+    xc, res = orc.dense_local_search(W, np.array(guess, np.uint8), is_spin_glass=1, walkers=4, seed_multiple=4,
+                                     gray_iterations=900)
+    assert bits == "".join("1" if b else "0" for b in xc)
+    assert cut == res["cut_value"] and e == res["min_energy"]
+    assert sorted(l + r) == list(range(n))
+    G = upper(W.astype(np.float64))
+    bits2, cut2, _, e2 = spin_glass_solver_sparse(G, best_guess=guess, gray_iterations=900, gray_seed_multiple=4)
+    assert (bits2, cut2, e2) == (bits, cut, e)
+
+
+def test_maxcut_tfim_heuristic_path(ctx, orc):
+    """Above the threshold: sampled candidates scored on the GPU; the reported
+    cut equals the cut of the returned bitstring (the reference's own check,
+    scripts/maxcut_benchmarks.py:127-129)."""
+    from pyqrackising_amd import maxcut_tfim
+
+    n = 40
+    W = er_weights(n, 0.5, 29, "unit").astype(np.float32)
+    bits, value, (l, r) = maxcut_tfim(W, quality=3, seed=1)
+    x = np.array([b == "1" for b in bits], np.uint8)
+    assert value == orc.compute_cut(W, x)
+    assert value > 0
