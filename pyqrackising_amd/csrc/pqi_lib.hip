@@ -677,11 +677,15 @@ int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, p
         st->rounds++;
 
         double t0 = now();
-        launch_descent(*g, x, h, INT64_MAX, dout, c->stream);
-        CHK(read_out(c, dout, &o));
-        st->single_evals += o.evals;
-        st->single_accepts += o.accepts;
-        st->max_energy += scale * o.gain_sum;
+        // bounded launches keep every kernel short; repeat while the bound was hit
+        const int64_t cap = 16 * n + 4096;
+        do {
+            launch_descent(*g, x, h, cap, dout, c->stream);
+            CHK(read_out(c, dout, &o));
+            st->single_evals += o.evals;
+            st->single_accepts += o.accepts;
+            st->max_energy += scale * o.gain_sum;
+        } while (o.accepts == cap);
         st->seconds_single += now() - t0;
         // the descent ends on a failed single-flip check: go on to double flips
 
