@@ -98,7 +98,9 @@ typedef struct {
     double min_energy; /* -E (spin glass) or E (maxcut), exact re-score */
     int64_t rounds;
     int64_t single_accepts, double_accepts, seed_accepts, walk_accepts, reheat_accepts;
-    int64_t single_evals, double_evals, seed_evals, gray_evals;
+    int64_t single_evals, double_evals, seed_evals;
+    int64_t gray_evals;          /* Gray steps of the reference walk, each resolved exactly */
+    int64_t gray_evals_computed; /* of those, gains computed (the rest repeat a rejected step) */
     double seconds_single, seconds_double, seconds_seeds, seconds_walk, seconds_reheat;
     /* with PQI_FLAG_TIME_KERNELS: summed event-timed kernel durations and launch counts */
     double walk_kernel_seconds, seed_kernel_seconds;

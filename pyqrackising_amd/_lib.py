@@ -77,6 +77,7 @@ class SearchStats(ctypes.Structure):
         ("double_evals", ctypes.c_int64),
         ("seed_evals", ctypes.c_int64),
         ("gray_evals", ctypes.c_int64),
+        ("gray_evals_computed", ctypes.c_int64),
         ("seconds_single", ctypes.c_double),
         ("seconds_double", ctypes.c_double),
         ("seconds_seeds", ctypes.c_double),

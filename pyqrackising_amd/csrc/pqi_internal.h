@@ -103,12 +103,14 @@ void launch_seed_lowrank(const Graph& g, const uint8_t* x, const void* h, int64_
 // Convert quadratic forms to cut differences: delta[b] = (q_base - q[b]) / 4.
 void launch_q_to_delta(const double* q, const double* q_base, int64_t count, double* delta, hipStream_t s);
 // Stable top-k (value descending, index ascending): sel[r] = index of rank r, for r < k.
-void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, hipStream_t s);
+void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, void* scratch, hipStream_t s);
+size_t topk_scratch_bytes(int64_t count);
 // out[i] = v[sel[i]] for i < k.
 void launch_gather(const double* v, const int64_t* sel, int64_t k, double* out, hipStream_t s);
 
 // Gray walk: walker w starts from seed sel[w] of (x, h) and walks; writes its
-// final state/field to wx/wh (row stride n) and its total cut difference to we[w].
+// final state/field to wx/wh (row stride n), its total cut difference to we[w]
+// and the number of gain evaluations it computed to wevals[w].
 void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel,
                       const double* sel_delta, int64_t block_size, int64_t walkers,
                       int64_t thread_iterations, uint8_t* wx, void* wh, double* we, int64_t* wevals,

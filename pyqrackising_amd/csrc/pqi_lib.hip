@@ -44,7 +44,7 @@ int fail(int code, const std::string& msg) {
 // Scratch slots of a context.
 enum Slot {
     S_X, S_H, S_RX, S_RH, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
-    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_COUNT
+    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_COUNT
 };
 
 static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
@@ -550,7 +550,9 @@ static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_s
     } else {
         launch_seed_lowrank(*g, x, h, bs, count, b.delta, c->stream);
     }
-    launch_topk(b.delta, count, p->walkers, b.sel, c->stream);
+    void* rank;
+    CHK(ensure(c, S_RANK, topk_scratch_bytes(count), &rank));
+    launch_topk(b.delta, count, p->walkers, b.sel, rank, c->stream);
     launch_gather(b.delta, b.sel, p->walkers, b.seld, c->stream);
     return PQI_OK;
 }
@@ -592,6 +594,21 @@ struct EventTimer {
         return 0.0;
     }
 };
+
+// Gray steps one walker of the reference performs in T iterations: iteration
+// i flips ctz(i+1) + 64 b for every block b, skipping positions >= n.
+static int64_t reference_gray_steps(int64_t T, int64_t n) {
+    const int64_t nb = (n + 63) >> 6;
+    int64_t total = 0;
+    for (int p = 0; p < 63; ++p) {
+        const int64_t cnt = (T >> p) - (T >> (p + 1));
+        if (!cnt) continue;
+        int64_t valid = 0;
+        for (int64_t b = 0; b < nb; ++b) valid += (p + 64 * b < n) ? 1 : 0;
+        total += cnt * valid;
+    }
+    return total;
+}
 
 static double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -734,7 +751,8 @@ int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, p
         CHK(sync(c));
         st->walk_kernel_seconds += walk_timer.seconds();
         st->walk_launches += timing ? 1 : 0;
-        for (int64_t w = 0; w < G; ++w) st->gray_evals += wev[w];
+        for (int64_t w = 0; w < G; ++w) st->gray_evals_computed += wev[w];
+        st->gray_evals += G * reference_gray_steps(T, n);
         int64_t bw = 0;
         for (int64_t w = 1; w < G; ++w)
             if (we[w] > we[bw]) bw = w;

@@ -302,10 +302,103 @@ __global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h
     }
 }
 
+// Single-wavefront variant.  LDS holds the gains D_v = s_v h_v themselves, so
+// the argmax scan is one independent LDS load per element (unrolled), and a
+// flip of k updates D_k -> -D_k and D_v -= 2 w_vk s_k s_v for its neighbours.
+// No work-group barriers, 32-bit indices; same selection rule and result.
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
+                                                    PhaseOut* out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x;
+    const int n = (int)g.n;
+    FT* D = (FT*)lds;
+    uint8_t* x = lds + (((size_t)n * sizeof(FT) + 15) & ~(size_t)15);
+    for (int v = lane; v < n; v += 64) {
+        const uint8_t xv = xg[v];
+        x[v] = xv;
+        D[v] = xv ? -hg[v] : hg[v];
+    }
+    __syncthreads();
+    int64_t steps = 0, evals = 0;
+    double gsum = 0.0;
+    FT bv;
+    int bk;
+    const int full = n & ~511;  // unrolled part: 8 elements per lane per pass
+    for (;;) {
+        bv = lowest<FT>();
+        bk = INT_MAX;
+        for (int v0 = lane; v0 < full; v0 += 512) {
+            FT d[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) d[u] = D[v0 + 64 * u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (d[u] > bv) { bv = d[u]; bk = v0 + 64 * u; }
+        }
+        for (int v = full + lane; v < n; v += 64) {
+            const FT d = D[v];
+            if (d > bv) { bv = d; bk = v; }
+        }
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+            const FT ov = shfl_xor_t(bv, m);
+            const int ok = __shfl_xor(bk, m, 64);
+            if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
+        }
+        evals += n;
+        if (!(bv > 0) || steps >= max_steps) break;
+        const int sk = x[bk] ? -1 : 1;  // s_k before the flip
+        __syncthreads();
+        if (g.use_adj) {
+            for (int32_t e = g.ap[bk] + lane; e < g.ap[bk + 1]; e += 64) {
+                const int v = g.ai[e];
+                const int sv = x[v] ? -1 : 1;
+                D[v] -= (FT)(2 * sk * sv) * (FT)g.aw[e];
+            }
+        } else {
+            const WT* Wk = g.W + (int64_t)bk * g.ld;
+            for (int v = lane; v < n; v += 64) {
+                if (v == bk) continue;
+                const int sv = x[v] ? -1 : 1;
+                D[v] -= (FT)(2 * sk * sv) * (FT)Wk[v];
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            D[bk] = -D[bk];
+            x[bk] ^= 1u;
+        }
+        __syncthreads();
+        ++steps;
+        gsum += (double)bv;
+    }
+    if (steps) {
+        for (int v = lane; v < n; v += 64) {
+            const uint8_t xv = x[v];
+            hg[v] = xv ? -D[v] : D[v];
+            xg[v] = xv;
+        }
+    }
+    if (lane == 0) {
+        out->gain = (double)bv;
+        out->k = bk;
+        out->l = -1;
+        out->accepts = steps;
+        out->evals = evals;
+        out->gain_sum = gsum;
+    }
+}
+
 template <typename WT>
 static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
-    hipLaunchKernelGGL((k_descent<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), x, (FT*)h, max_steps, out);
+    const size_t lds = (((size_t)g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
+    if (lds <= 64 * 1024)
+        hipLaunchKernelGGL((k_descent_wave<WT, FT>), dim3(1), dim3(64), lds, s, view<WT>(g), x, (FT*)h, max_steps,
+                           out);
+    else
+        hipLaunchKernelGGL((k_descent<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), x, (FT*)h, max_steps, out);
 }
 
 void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
@@ -347,17 +440,20 @@ void launch_flip(const Graph& g, uint8_t* x, void* h, int64_t k, int64_t l, hipS
 // Double flips
 // ---------------------------------------------------------------------------
 
-// Row-major pair index s -> (i, j), i < j (pyqrackising/spin_glass_solver.py:66-76).
+// Row-major pair index s -> (i, j), i < j (pyqrackising/spin_glass_solver.py:66-76
+// walks the rows one by one).  Row i starts at base(i) = i(2n-i-1)/2; i is the
+// root of base(i) <= s, taken in FP64 and corrected by exact integer checks.
+__device__ __forceinline__ int64_t pair_base(int64_t i, int64_t n) { return i * (2 * n - i - 1) / 2; }
+
 __device__ __forceinline__ void unrank_pair(int64_t s, int64_t n, int64_t& i, int64_t& j) {
-    int64_t c = s, r = 0, lcv = n - 1;
-    while (c >= lcv) {
-        c -= lcv;
-        ++r;
-        --lcv;
-        if (!lcv) break;
-    }
+    const double b = 2.0 * (double)n - 1.0;
+    int64_t r = (int64_t)((b - sqrt(b * b - 8.0 * (double)s)) * 0.5);
+    if (r < 0) r = 0;
+    if (r > n - 2) r = n - 2;
+    while (r > 0 && pair_base(r, n) > s) --r;
+    while (r < n - 2 && pair_base(r + 1, n) <= s) ++r;
     i = r;
-    j = c + r + 1;
+    j = s - pair_base(r, n) + r + 1;
 }
 
 template <typename WT, typename FT>
@@ -584,27 +680,50 @@ void launch_gather(const double* v, const int64_t* sel, int64_t k, double* out, 
 }
 
 // Stable top-k by rank counting: rank(a) = #{b : v_b > v_a or (v_b == v_a and b < a)}.
-__global__ __launch_bounds__(256) void k_topk(const double* v, int64_t count, int64_t k, int64_t* sel) {
+// Stage 1 splits the comparisons of each candidate over gridDim.y work groups
+// (integer atomics, so the count is exact); stage 2 scatters the first k.
+__global__ __launch_bounds__(256) void k_rank(const double* v, int64_t count, int64_t span, int32_t* rank) {
     __shared__ double tile[256];
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const double va = a < count ? v[a] : 0.0;
-    int64_t rank = 0;
-    for (int64_t t0 = 0; t0 < count; t0 += 256) {
+    const int64_t lo = (int64_t)blockIdx.y * span;
+    const int64_t hi = lo + span < count ? lo + span : count;
+    int32_t r = 0;
+    for (int64_t t0 = lo; t0 < hi; t0 += 256) {
         __syncthreads();
-        if (t0 + threadIdx.x < count) tile[threadIdx.x] = v[t0 + threadIdx.x];
+        if (t0 + threadIdx.x < hi) tile[threadIdx.x] = v[t0 + threadIdx.x];
         __syncthreads();
-        const int lim = (int)std::min<int64_t>(256, count - t0);
+        const int lim = (int)(hi - t0 < 256 ? hi - t0 : 256);
         for (int q = 0; q < lim; ++q) {
             const double vb = tile[q];
             const int64_t b = t0 + q;
-            rank += (vb > va || (vb == va && b < a)) ? 1 : 0;
+            r += (vb > va || (vb == va && b < a)) ? 1 : 0;
         }
     }
-    if (a < count && rank < k) sel[rank] = a;
+    if (a < count && r) atomicAdd(&rank[a], r);
 }
 
-void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, hipStream_t s) {
-    hipLaunchKernelGGL(k_topk, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, v, count, k, sel);
+__global__ void k_rank_scatter(const int32_t* rank, int64_t count, int64_t k, int64_t* sel) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < count && rank[a] < k) sel[rank[a]] = a;
+}
+
+__global__ void k_zero_i32(int32_t* p, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
+size_t topk_scratch_bytes(int64_t count) { return (size_t)count * 4; }
+
+void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, void* scratch, hipStream_t s) {
+    int32_t* rank = (int32_t*)scratch;
+    const unsigned gx = (unsigned)((count + 255) / 256);
+    int64_t splits = (count + 511) / 512;  // about 512 comparisons per thread
+    if (splits > 64) splits = 64;
+    const int64_t span = (count + splits - 1) / splits;
+    hipLaunchKernelGGL(k_zero_i32, dim3(gx), dim3(256), 0, s, rank, count);
+    hipLaunchKernelGGL(k_rank, dim3(gx, (unsigned)splits), dim3(256), 0, s, v, count, span, rank);
+    hipLaunchKernelGGL(k_rank_scatter, dim3(gx), dim3(256), 0, s, rank, count, k, sel);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,8 +773,30 @@ __global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0
 
     double energy = sel_delta[w];
     int64_t evals = 0;
-    for (int64_t ci = 0; ci < T; ++ci) {
+    // clean bit p: every flip at positions p + 64 b has been evaluated and
+    // rejected since the walker last changed.  A rejected flip is reverted, so
+    // until the next accepted flip those iterations repeat exactly and are
+    // skipped (their gain is 0).  pmax: largest p with an iteration below T.
+    uint64_t clean = 0;
+    const int pmax = 63 - __builtin_clzll((unsigned long long)(T > 0 ? T : 1));
+    for (int64_t ci = 0; ci < T;) {
         const int p = __builtin_ctzll((unsigned long long)(ci + 1));
+        if ((clean >> p) & 1u) {
+            // next iteration index whose p is not clean: i+1 = q 2^p with q odd
+            const uint64_t v = (uint64_t)ci + 1;
+            uint64_t best = ~0ull;
+            for (int pp = 0; pp <= pmax; ++pp) {
+                if ((clean >> pp) & 1u) continue;
+                uint64_t q = (v + ((1ull << pp) - 1)) >> pp;
+                q |= 1ull;
+                const uint64_t u = q << pp;
+                if (u < best) best = u;
+            }
+            if (best == ~0ull || best > (uint64_t)T) break;
+            ci = (int64_t)best - 1;
+            continue;
+        }
+        bool changed = false;
         double gain_i = 0.0;
         for (int64_t b0 = 0; b0 < nb; b0 += 64) {
             const int64_t b = b0 + lane;
@@ -669,6 +810,7 @@ __global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0
                 const int64_t kk = (int64_t)p + 64 * (b0 + bl);
                 const FT dd = __shfl(d, bl, 64);
                 gain_i += (double)dd;
+                changed = true;
                 const int s_old = x[kk] ? -1 : 1;
                 __syncthreads();
                 flip_update(g, x, h, kk, s_old, lane, 64);
@@ -678,6 +820,8 @@ __global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0
             }
         }
         energy += gain_i;
+        clean = changed ? 0ull : (clean | (1ull << p));
+        ++ci;
     }
     __syncthreads();
     if (use_lds) {

@@ -53,9 +53,11 @@ def init_J_and_z_sparse(G_m):
 
 
 def init_thresholds(n_qubits):
-    """pyqrackising/maxcut_tfim_util.py:558-575."""
+    """pyqrackising/maxcut_tfim_util.py:558-575.  The reference allocates with
+    np.empty and never writes one entry (index n/2 for even n, (n+1)/2 for odd
+    n), so that entry holds whatever the allocator returns; it is 0 here."""
     n_bias = n_qubits + 1
-    thresholds = np.empty(n_bias, dtype=np.float64)
+    thresholds = np.zeros(n_bias, dtype=np.float64)
     normalizer = 0
     for q in range(n_qubits >> 1):
         normalizer += math.comb(n_qubits, q) << 1

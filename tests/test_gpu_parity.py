@@ -323,3 +323,37 @@ def test_maxcut_tfim_heuristic_path(ctx, orc):
     x = np.array([b == "1" for b in bits], np.uint8)
     assert value == orc.compute_cut(W, x)
     assert value > 0
+
+
+def test_c1_pipeline_matches_oracle(ctx, orc):
+    """The CPU-runnable configuration: ER(64, 0.5) unit weights, int32 dense,
+    quality 6 -> maxcut_tfim then spin_glass_solver.  The candidate rounds are
+    replayed with the same seeded sampler and scored by the oracle; the local
+    search is compared on the resulting guess."""
+    from pyqrackising_amd import maxcut_tfim, spin_glass_solver
+    from pyqrackising_amd import sampler as smp
+
+    n, seed = 64, 31
+    W = er_weights(n, 0.5, 1001, "unit").astype(np.int32)
+    bits, value, _ = maxcut_tfim(W, quality=6, seed=seed)
+    # replay (pyqrackising/maxcut_tfim.py:148-186 semantics)
+    Wd = W.astype(np.float64)
+    J, z = smp.init_J_and_z_dense(Wd)
+    cum = smp.maxcut_hamming_cdf(n, J, z, 6, 8.0, 8.0)
+    shots = ((max(1, (n << 6) >> 1) + 31) >> 5) << 5
+    sampler = smp.RepulsionSampler(Wd, cum, 5.0, np.random.default_rng(seed))
+    best, best_e = np.zeros(n, bool), 0.0
+    while True:
+        X = sampler.sample(shots, 0)
+        bi, vals = orc.eval_batch(Wd, X, 0)
+        if vals[bi] > best_e:
+            best_e, best = vals[bi], X[bi].copy()
+        else:
+            break
+    assert bits == "".join("1" if b else "0" for b in best)
+    assert value == best_e
+    out = spin_glass_solver(W, best_guess=bits, is_spin_glass=False, gray_seed_multiple=8, gray_iterations=4096)
+    xc, r = orc.dense_local_search(Wd, best.astype(np.uint8), is_spin_glass=0, walkers=8, seed_multiple=8,
+                                   gray_iterations=4096)
+    assert out[0] == "".join("1" if b else "0" for b in xc)
+    assert out[1] == r["cut_value"] and out[3] == r["min_energy"]
