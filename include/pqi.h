@@ -59,6 +59,11 @@ extern "C" {
 #define PQI_DOUBLE_REFERENCE 0  /* pyqrackising/spin_glass_solver.py:55-92: thread t keeps its last pair */
 #define PQI_DOUBLE_EXHAUSTIVE 1 /* pyqrackising/kernels.cl:186-231: best over all n(n-1)/2 pairs */
 
+/* candidate-evaluation engine for pqi_eval_batch_ex */
+#define PQI_ENGINE_AUTO 0  /* int8: MFMA (exact); fp32: split BF16 + FP64 re-score of the top 16; fp64: FP64 */
+#define PQI_ENGINE_EXACT 1 /* int8: MFMA (exact); real weights: FP64 tiles */
+#define PQI_ENGINE_BF16 2  /* fp32 weights only: split hi/lo BF16 MFMA, FP32 accumulation */
+
 /* candidate-evaluation engine for Gray seeds */
 #define PQI_SEEDS_AUTO 0
 #define PQI_SEEDS_MFMA 1     /* E = rowsum((S W) .* S) on int8 MFMA (int8 weights only) */
@@ -138,6 +143,14 @@ PQI_API int pqi_graph_get_info(const pqi_graph* g, pqi_graph_info* info);
  * receives cut(x_b) or E(x_b); *best_index the first maximum. */
 PQI_API int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass,
                    double* values, int64_t* best_index, double* best_value);
+
+/* pqi_eval_batch with an explicit engine (PQI_ENGINE_*).  With the split-BF16
+ * engine, values are approximate; the 16 best by approximate value are
+ * re-scored in FP64 on the device, best_index/best_value come from those FP64
+ * values, and *max_rel_err (optional) receives the largest relative difference
+ * between the approximate and the FP64 value among them. */
+PQI_API int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, int engine,
+                              double* values, int64_t* best_index, double* best_value, double* max_rel_err);
 
 /* Single-flip phase.  Replaces run_single_bit_flips / run_bit_flips_opencl(False)
  * (pyqrackising/spin_glass_solver.py:37-52, 156-241; kernels.cl:100-148).

@@ -25,6 +25,7 @@ WEIGHTS_I8, WEIGHTS_F32, WEIGHTS_F64 = 1, 3, 4
 DOUBLE_REFERENCE, DOUBLE_EXHAUSTIVE = 0, 1
 SEEDS_AUTO, SEEDS_MFMA, SEEDS_LOWRANK = 0, 1, 2
 FLAG_TIME_KERNELS = 1
+ENGINE_AUTO, ENGINE_EXACT, ENGINE_BF16 = 0, 1, 2
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -105,6 +106,7 @@ SIGNATURES = {
     "pqi_graph_free": (_int, [_vp]),
     "pqi_graph_get_info": (_int, [_vp, _vp]),
     "pqi_eval_batch": (_int, [_vp, _vp, _i64, _int, _vp, _vp, _vp]),
+    "pqi_eval_batch_ex": (_int, [_vp, _vp, _i64, _int, _int, _vp, _vp, _vp, _vp]),
     "pqi_single_flip_best": (_int, [_vp, _vp, _int, _vp, _vp]),
     "pqi_double_flip_best": (_int, [_vp, _vp, _int, _i64, _int, _vp, _vp, _vp]),
     "pqi_gray_search": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -261,7 +263,7 @@ class DeviceGraph:
             raise ValueError(f"theta must have shape ({self.n},)")
         return t
 
-    def eval_batch(self, bits, is_spin_glass, want_values=True):
+    def eval_batch(self, bits, is_spin_glass, want_values=True, engine=ENGINE_AUTO, return_error=False):
         bits = np.ascontiguousarray(bits, dtype=np.uint32)
         B = bits.shape[0]
         if bits.ndim != 2 or bits.shape[1] != (self.n + 31) >> 5:
@@ -269,9 +271,13 @@ class DeviceGraph:
         vals = np.empty(B, dtype=np.float64) if want_values else None
         bi = ctypes.c_int64()
         bv = ctypes.c_double()
+        err = ctypes.c_double()
         with self.ctx.lock:
-            check(load().pqi_eval_batch(self._h, _p(bits), B, int(bool(is_spin_glass)),
-                                        _p(vals) if want_values else None, ctypes.byref(bi), ctypes.byref(bv)))
+            check(load().pqi_eval_batch_ex(self._h, _p(bits), B, int(bool(is_spin_glass)), int(engine),
+                                           _p(vals) if want_values else None, ctypes.byref(bi), ctypes.byref(bv),
+                                           ctypes.byref(err)))
+        if return_error:
+            return bi.value, bv.value, vals, err.value
         return bi.value, bv.value, vals
 
     def single_flip_best(self, theta, is_spin_glass):

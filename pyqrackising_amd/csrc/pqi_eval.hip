@@ -15,6 +15,8 @@
 
 namespace pqi {
 
+__global__ void k_batch_sum(const double* part, int64_t nblk, int64_t B, double* q);
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -139,6 +141,173 @@ __global__ __launch_bounds__(256) void k_batch_i8(const int8_t* __restrict__ W, 
     const int row = wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     const int64_t m = m0 + row;
     if (m < B) atomicAdd(&q[m], (unsigned long long)(long long)vals[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 variant for fp32 weights: W = W_hi + W_lo + r with
+// W_hi = bf16(W), W_lo = bf16(W - W_hi), |r| <= 2^-18 |W|.  The weights are
+// stored as one n_pad x 2 n_pad bf16 matrix [W_hi | W_lo]; the contraction runs
+// over K = 2 n_pad with the candidate repeated, on v_mfma_f32_32x32x16_bf16
+// with FP32 accumulation.  +-1 is exact in bf16.  Row partial sums per wave
+// column block go to part[] and are added in FP64 in fixed order.
+// ---------------------------------------------------------------------------
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// round-to-nearest-even fp32 -> bf16 bits (finite inputs)
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__global__ void k_split_bf16(const float* __restrict__ W, int64_t n_pad, uint16_t* __restrict__ Wcat) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_pad * n_pad) return;
+    const int64_t j = t / n_pad, k = t % n_pad;
+    const float w = W[t];
+    const uint16_t hi = bf16_rne(w);
+    const uint16_t lo = bf16_rne(w - bf16_to_f(hi));
+    Wcat[j * 2 * n_pad + k] = hi;
+    Wcat[j * 2 * n_pad + n_pad + k] = lo;
+}
+
+void launch_split_bf16(const float* W, int64_t n_pad, uint16_t* Wcat, hipStream_t s) {
+    const int64_t tot = n_pad * n_pad;
+    hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, W, n_pad, Wcat);
+}
+
+__global__ __launch_bounds__(256) void k_batch_bf16(const uint16_t* __restrict__ Wcat, int64_t n_pad,
+                                                    const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
+                                                    double* __restrict__ part, int64_t nparts) {
+    constexpr int BKB = 32;              // bf16 elements per stage
+    constexpr int ROWB = BKB * 2 + 16;   // padded LDS row in bytes
+    __shared__ __align__(16) unsigned char As[BM * ROWB];
+    __shared__ __align__(16) unsigned char Bs[BN * ROWB];
+    __shared__ uint32_t Sb[BM * (BN / 32)];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int64_t m0 = (int64_t)blockIdx.y * BM, j0 = (int64_t)blockIdx.x * BN;
+    const int lr = tid >> 1, lh = tid & 1;
+    const int64_t K = 2 * n_pad;
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+
+    const int64_t am = m0 + lr;
+    for (int64_t k0 = 0; k0 < K; k0 += BKB) {
+        const int64_t kk = k0 < n_pad ? k0 : k0 - n_pad;
+        {
+            // each of the two loader threads of a row unpacks 16 of the 32 bits
+            const uint32_t word = (am < B) ? bits[am * wpr + (kk >> 5)] : 0u;
+            const uint32_t half = (word >> (16 * lh)) & 0xFFFFu;
+            uint32_t o[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const uint32_t b0 = (half >> (2 * t)) & 1u, b1 = (half >> (2 * t + 1)) & 1u;
+                o[t] = (0x3F80u | (b0 << 15)) | ((0x3F80u | (b1 << 15)) << 16);
+            }
+            uint4* dst = (uint4*)&As[lr * ROWB + lh * 32];
+            dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+            dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+        }
+        {
+            const uint4* src = (const uint4*)(Wcat + (j0 + lr) * K + k0 + lh * 16);
+            uint4 a = src[0], b = src[1];
+            uint4* dst = (uint4*)&Bs[lr * ROWB + lh * 32];
+            dst[0] = a;
+            dst[1] = b;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BKB; ks += 16) {
+            v8bf av[2], bv[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+                av[mi] = *(const v8bf*)&As[(wm * 64 + mi * 32 + (lane & 31)) * ROWB + (ks + 8 * (lane >> 5)) * 2];
+#pragma unroll
+            for (int nj = 0; nj < 2; ++nj)
+                bv[nj] = *(const v8bf*)&Bs[(wn * 64 + nj * 32 + (lane & 31)) * ROWB + (ks + 8 * (lane >> 5)) * 2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int nj = 0; nj < 2; ++nj)
+                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[mi], bv[nj], acc[mi][nj], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < BM * (BN / 32); t += 256) {
+        const int r = t >> 2, c = t & 3;
+        const int64_t m = m0 + r;
+        Sb[t] = (m < B) ? bits[m * wpr + (j0 >> 5) + c] : 0u;
+    }
+    __syncthreads();
+    float vals[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) vals[i] = 0.f;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj) {
+            const int col = wn * 64 + nj * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const uint32_t bit = (Sb[row * 4 + (col >> 5)] >> (col & 31)) & 1u;
+                const float p = acc[mi][nj][r];
+                vals[mi * 16 + r] += bit ? -p : p;
+            }
+        }
+#pragma unroll
+    for (int t = 0, half = 16; t < 5; ++t, half >>= 1) {
+        const int up = (lane >> t) & 1;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < half) {
+                const float send = up ? vals[i] : vals[i + half];
+                const float keep = up ? vals[i + half] : vals[i];
+                vals[i] = keep + __shfl_xor(send, 1 << t, 64);
+            }
+        }
+    }
+    const int qi = (lane & 1) * 16 + ((lane >> 1) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                   ((lane >> 4) & 1);
+    const int mi = qi >> 4, r = qi & 15;
+    const int row = wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int64_t m = m0 + row;
+    if (m < B) part[m * nparts + blockIdx.x * 2 + wn] = (double)vals[0];
+}
+
+void launch_batch_bf16(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, double* q, void* scratch,
+                       hipStream_t s) {
+    const int64_t nparts = 2 * (g.n_pad / BN);
+    double* part = (double*)scratch;
+    dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
+    hipLaunchKernelGGL(k_batch_bf16, grid, dim3(256), 0, s, (const uint16_t*)g.Wbf, g.n_pad, bits, wpr, B, part,
+                       nparts);
+    hipLaunchKernelGGL(k_batch_sum, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, part, nparts, B, q);
+}
+
+size_t batch_bf16_scratch_bytes(const Graph& g, int64_t B) { return (size_t)B * 2 * (g.n_pad / BN) * 8; }
+
+__global__ void k_gather_rows(const uint32_t* bits, int64_t wpr, const int64_t* sel, int64_t k, uint32_t* out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k * wpr) return;
+    const int64_t r = t / wpr, w = t % wpr;
+    out[t] = bits[sel[r] * wpr + w];
+}
+
+void launch_gather_rows(const uint32_t* bits, int64_t wpr, const int64_t* sel, int64_t k, uint32_t* out,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((k * wpr + 255) / 256)), dim3(256), 0, s, bits, wpr, sel, k,
+                       out);
 }
 
 // FP64 variant for real weights: tile 64 candidates x 64 columns, 4 x 4 per

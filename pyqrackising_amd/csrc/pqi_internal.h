@@ -29,6 +29,7 @@ struct Graph {
     double w_up = 0.0;           // sum_{i<j} w_ij
     int64_t w_up_i = 0;          // same, exact, for int8 weights
     bool use_adj = true;         // flips update fields through the adjacency lists
+    void* Wbf = nullptr;         // fp32 weights only: [W_hi | W_lo] bf16, n_pad x 2 n_pad
     int64_t device_bytes = 0;
 };
 
@@ -73,6 +74,14 @@ void launch_score_with(const Graph& g, const uint8_t* x, int64_t count, double* 
 void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t words_per_row, int64_t B,
                             double* q, void* scratch, hipStream_t s);
 size_t batch_scratch_bytes(const Graph& g, int64_t B);
+// Split-BF16 MFMA energies for fp32 weights (needs g.Wbf); q approximate (see pqi_eval.hip).
+void launch_batch_bf16(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, double* q, void* scratch,
+                       hipStream_t s);
+size_t batch_bf16_scratch_bytes(const Graph& g, int64_t B);
+void launch_split_bf16(const float* W, int64_t n_pad, uint16_t* Wcat, hipStream_t s);
+// out[r] = bits row sel[r], r < k.
+void launch_gather_rows(const uint32_t* bits, int64_t wpr, const int64_t* sel, int64_t k, uint32_t* out,
+                        hipStream_t s);
 
 // One-work-group phase kernels over the state (x, h) of a graph.
 struct PhaseOut {

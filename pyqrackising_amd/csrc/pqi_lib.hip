@@ -103,6 +103,7 @@ static int upload_adjacency(pqi_graph* g, const std::vector<int32_t>& ptr, const
                             const std::vector<double>& w) {
     const int64_t nnz = (int64_t)idx.size();
     g->nnz = nnz;
+    if (nnz > INT32_MAX / 2) return fail(PQI_ERR_UNSUPPORTED, "too many nonzeros for the adjacency lists");
     HIPCHK(hipMalloc(&g->adj_ptr, sizeof(int32_t) * ptr.size()));
     HIPCHK(hipMalloc(&g->adj_idx, sizeof(int32_t) * std::max<int64_t>(1, nnz)));
     size_t es = g->kind == PQI_WEIGHTS_I8 ? 1 : (g->kind == PQI_WEIGHTS_F32 ? 4 : 8);
@@ -138,6 +139,7 @@ static int classify(double maxabs, bool all_int, int dtype) {
 static void free_graph(pqi_graph* g) {
     if (!g) return;
     if (g->W) (void)hipFree(g->W);
+    if (g->Wbf) (void)hipFree(g->Wbf);
     if (g->adj_ptr) (void)hipFree(g->adj_ptr);
     if (g->adj_idx) (void)hipFree(g->adj_idx);
     if (g->adj_w) (void)hipFree(g->adj_w);
@@ -205,74 +207,97 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
     if (n < 1 || n > (1 << 20)) return fail(PQI_ERR_INVALID, "n out of range");
     *out = nullptr;
     CHK(set_device(c));
-    // validate and classify
+    // pass 1 (64 x 64 tiles for cache locality): diagonal, finiteness, symmetry,
+    // value class, nonzero count and W_up = sum_{i<j} w_ij (long double)
     double maxabs = 0.0;
     bool all_int = true;
-    std::vector<int32_t> ptr(n + 1, 0);
-    std::vector<int32_t> idx;
-    std::vector<double> wv;
-    double w_up = 0.0;
+    int64_t nnz = 0;
     long double w_up_l = 0.0L;
-    for (int64_t i = 0; i < n; ++i) {
-        if (read_w(W, dtype, i * n + i) != 0.0)
-            return fail(PQI_ERR_INVALID, "W has a nonzero diagonal entry at " + std::to_string(i));
-        for (int64_t j = 0; j < n; ++j) {
-            double a = read_w(W, dtype, i * n + j);
-            if (std::isnan(a) || std::isinf(a)) return fail(PQI_ERR_INVALID, "W has a non-finite entry");
-            if (j > i && a != read_w(W, dtype, j * n + i))
-                return fail(PQI_ERR_INVALID, "W is not symmetric");
-            if (a != 0.0) {
-                idx.push_back((int32_t)j);
-                wv.push_back(a);
-                maxabs = std::max(maxabs, std::fabs(a));
-                if (a != std::nearbyint(a)) all_int = false;
-                if (j > i) w_up_l += a;
-            }
-        }
-        ptr[i + 1] = (int32_t)idx.size();
-        if ((int64_t)idx.size() > INT32_MAX / 2) return fail(PQI_ERR_UNSUPPORTED, "too many nonzeros");
-    }
-    w_up = (double)w_up_l;
+    const int64_t TB = 64;
+    for (int64_t i0 = 0; i0 < n; i0 += TB)
+        for (int64_t j0 = i0; j0 < n; j0 += TB)
+            for (int64_t i = i0; i < std::min(n, i0 + TB); ++i)
+                for (int64_t j = std::max(j0, i); j < std::min(n, j0 + TB); ++j) {
+                    const double a = read_w(W, dtype, i * n + j);
+                    if (j == i) {
+                        if (a != 0.0)
+                            return fail(PQI_ERR_INVALID, "W has a nonzero diagonal entry at " + std::to_string(i));
+                        continue;
+                    }
+                    if (std::isnan(a) || std::isinf(a)) return fail(PQI_ERR_INVALID, "W has a non-finite entry");
+                    if (a != read_w(W, dtype, j * n + i)) return fail(PQI_ERR_INVALID, "W is not symmetric");
+                    if (a != 0.0) {
+                        nnz += 2;
+                        maxabs = std::max(maxabs, std::fabs(a));
+                        if (all_int && a != std::nearbyint(a)) all_int = false;
+                        w_up_l += a;
+                    }
+                }
     pqi_graph* g = new pqi_graph();
     g->ctx = c;
     g->n = n;
     g->kind = classify(maxabs, all_int, dtype);
     g->n_pad = round_up(n, 128);
-    g->w_up = w_up;
+    g->w_up = (double)w_up_l;
     g->w_up_i = g->kind == PQI_WEIGHTS_I8 ? (int64_t)std::llround((double)w_up_l) : 0;
-    // dense copy, zero padded
-    size_t es = g->kind == PQI_WEIGHTS_I8 ? 1 : (g->kind == PQI_WEIGHTS_F32 ? 4 : 8);
-    size_t bytes = (size_t)g->n_pad * (size_t)g->n_pad * es;
-    hipError_t e = hipMalloc(&g->W, bytes);
-    if (e != hipSuccess) {
+    g->nnz = nnz;
+    g->use_adj = nnz <= n * n / 8;
+    // dense copy, zero padded, in the storage type (one pitched copy)
+    const size_t es = g->kind == PQI_WEIGHTS_I8 ? 1 : (g->kind == PQI_WEIGHTS_F32 ? 4 : 8);
+    const size_t bytes = (size_t)g->n_pad * (size_t)g->n_pad * es;
+    auto bail = [&](int code, const std::string& m) {
         free_graph(g);
-        return fail(PQI_ERR_OOM, "device allocation of W failed");
-    }
+        return fail(code, m);
+    };
+    if (hipMalloc(&g->W, bytes) != hipSuccess) return bail(PQI_ERR_OOM, "device allocation of W failed");
     g->device_bytes += (int64_t)bytes;
-    e = hipMemset(g->W, 0, bytes);
-    if (e == hipSuccess) {
-        // convert row by row into the device element type, then a pitched copy
-        std::vector<unsigned char> row((size_t)n * es);
-        for (int64_t i = 0; i < n && e == hipSuccess; ++i) {
+    if (hipMemset(g->W, 0, bytes) != hipSuccess) return bail(PQI_ERR_HIP, "clearing W failed");
+    const bool same = (es == 1 && dtype == PQI_DTYPE_I8) || (es == 4 && dtype == PQI_DTYPE_F32) ||
+                      (es == 8 && dtype == PQI_DTYPE_F64);
+    std::vector<unsigned char> conv;
+    const void* src = W;
+    if (!same) {
+        conv.resize((size_t)n * n * es);
+        for (int64_t t = 0; t < n * n; ++t) {
+            const double a = read_w(W, dtype, t);
+            if (es == 1) ((int8_t*)conv.data())[t] = (int8_t)a;
+            else if (es == 4) ((float*)conv.data())[t] = (float)a;
+            else ((double*)conv.data())[t] = a;
+        }
+        src = conv.data();
+    }
+    if (hipMemcpy2D(g->W, (size_t)g->n_pad * es, src, (size_t)n * es, (size_t)n * es, (size_t)n,
+                    hipMemcpyHostToDevice) != hipSuccess)
+        return bail(PQI_ERR_HIP, "uploading W failed");
+    std::vector<unsigned char>().swap(conv);
+    if (g->kind == PQI_WEIGHTS_F32) {
+        const size_t bb = (size_t)g->n_pad * (size_t)g->n_pad * 4;  // two bf16 copies
+        if (hipMalloc(&g->Wbf, bb) != hipSuccess) return bail(PQI_ERR_OOM, "device allocation of bf16 W failed");
+        g->device_bytes += (int64_t)bb;
+        launch_split_bf16((const float*)g->W, g->n_pad, (uint16_t*)g->Wbf, c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(PQI_ERR_HIP, "bf16 split failed");
+    }
+    if (g->use_adj) {
+        std::vector<int32_t> ptr(n + 1, 0), idx;
+        std::vector<double> wv;
+        idx.reserve(nnz);
+        wv.reserve(nnz);
+        for (int64_t i = 0; i < n; ++i) {
             for (int64_t j = 0; j < n; ++j) {
-                double a = read_w(W, dtype, i * n + j);
-                if (es == 1) ((int8_t*)row.data())[j] = (int8_t)a;
-                else if (es == 4) ((float*)row.data())[j] = (float)a;
-                else ((double*)row.data())[j] = a;
+                const double a = read_w(W, dtype, i * n + j);
+                if (a != 0.0) {
+                    idx.push_back((int32_t)j);
+                    wv.push_back(a);
+                }
             }
-            e = hipMemcpy((char*)g->W + (size_t)i * g->n_pad * es, row.data(), (size_t)n * es, hipMemcpyHostToDevice);
+            ptr[i + 1] = (int32_t)idx.size();
+        }
+        int r = upload_adjacency(g, ptr, idx, wv);
+        if (r != PQI_OK) {
+            free_graph(g);
+            return r;
         }
     }
-    if (e != hipSuccess) {
-        free_graph(g);
-        return fail(PQI_ERR_HIP, std::string("uploading W: ") + hipGetErrorString(e));
-    }
-    int r = upload_adjacency(g, ptr, idx, wv);
-    if (r != PQI_OK) {
-        free_graph(g);
-        return r;
-    }
-    g->use_adj = g->nnz <= n * n / 8;
     *out = g;
     return PQI_OK;
 }
@@ -412,10 +437,15 @@ int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy) {
     return score_device(g, x, cut, energy);
 }
 
-int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, double* values,
-                   int64_t* best_index, double* best_value) {
+int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, int engine,
+                      double* values, int64_t* best_index, double* best_value, double* max_rel_err) {
     if (!g || (!bits && B > 0)) return fail(PQI_ERR_INVALID, "null argument");
     if (B < 1) return fail(PQI_ERR_INVALID, "B must be >= 1");
+    if (engine != PQI_ENGINE_AUTO && engine != PQI_ENGINE_EXACT && engine != PQI_ENGINE_BF16)
+        return fail(PQI_ERR_INVALID, "bad engine");
+    const bool bf16 = g->Wbf != nullptr && engine != PQI_ENGINE_EXACT;
+    if (engine == PQI_ENGINE_BF16 && !bf16)
+        return fail(PQI_ERR_UNSUPPORTED, "the split-BF16 engine needs dense fp32 weights");
     pqi_ctx* c = g->ctx;
     CHK(set_device(c));
     const int64_t nw = (g->n + 31) >> 5;
@@ -425,29 +455,73 @@ int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_gl
     void* scratch;
     CHK(ensure(c, S_BITS, (size_t)B * wpr * 4, (void**)&dbits));
     CHK(ensure(c, S_Q, (size_t)B * 8, (void**)&q));
-    CHK(ensure(c, S_BATCH, batch_scratch_bytes(*g, B), &scratch));
+    CHK(ensure(c, S_BATCH, bf16 ? batch_bf16_scratch_bytes(*g, B) : batch_scratch_bytes(*g, B), &scratch));
     // padding words stay zero: padded columns of W are zero, so they add nothing
     HIPCHK(hipMemsetAsync(dbits, 0, (size_t)B * wpr * 4, c->stream));
     HIPCHK(hipMemcpy2DAsync(dbits, wpr * 4, bits, nw * 4, nw * 4, B, hipMemcpyHostToDevice, c->stream));
-    launch_batch_quadratic(*g, dbits, wpr, B, q, scratch, c->stream);
+    if (bf16) launch_batch_bf16(*g, dbits, wpr, B, q, scratch, c->stream);
+    else launch_batch_quadratic(*g, dbits, wpr, B, q, scratch, c->stream);
     std::vector<double> hq(B);
     HIPCHK(hipMemcpyAsync(hq.data(), q, B * 8, hipMemcpyDeviceToHost, c->stream));
     CHK(sync(c));
-    int64_t best = 0;
-    double bv = -INFINITY;
-    for (int64_t b = 0; b < B; ++b) {
-        // cut = (2 W_up - q)/4, E = 2 cut - W_up = -q/2
-        double v = is_spin_glass ? -hq[b] / 2.0 : (2.0 * g->w_up - hq[b]) / 4.0;
+    // cut = (2 W_up - q)/4, E = 2 cut - W_up = -q/2
+    auto value_of = [&](double qv) {
         if (g->kind == PQI_WEIGHTS_I8) {
-            long long qi = (long long)hq[b];
-            v = is_spin_glass ? (double)(-qi / 2) : (double)((2 * g->w_up_i - qi) / 4);
+            const long long qi = (long long)qv;
+            return is_spin_glass ? (double)(-qi / 2) : (double)((2 * g->w_up_i - qi) / 4);
         }
-        if (values) values[b] = v;
-        if (v > bv) { bv = v; best = b; }
+        return is_spin_glass ? -qv / 2.0 : (2.0 * g->w_up - qv) / 4.0;
+    };
+    std::vector<double> v(B);
+    for (int64_t b = 0; b < B; ++b) v[b] = value_of(hq[b]);
+    int64_t best = 0;
+    if (bf16) {
+        // FP64 re-score of the K best approximate values (one pass over W)
+        const int64_t K = std::min<int64_t>(16, B);
+        std::vector<int64_t> order(B);
+        for (int64_t b = 0; b < B; ++b) order[b] = b;
+        std::partial_sort(order.begin(), order.begin() + K, order.end(), [&](int64_t a, int64_t b2) {
+            return v[a] > v[b2] || (v[a] == v[b2] && a < b2);
+        });
+        order.resize(K);
+        std::sort(order.begin(), order.end());
+        int64_t* dsel;
+        uint32_t* rows;
+        double* q64;
+        void* sc2;
+        CHK(ensure(c, S_SEL, K * 8, (void**)&dsel));
+        CHK(ensure(c, S_RANK, (size_t)K * wpr * 4, (void**)&rows));
+        CHK(ensure(c, S_QB, K * 8, (void**)&q64));
+        CHK(ensure(c, S_DBL, std::max(batch_scratch_bytes(*g, K), double_scratch_bytes(*g)), &sc2));
+        HIPCHK(hipMemcpyAsync(dsel, order.data(), K * 8, hipMemcpyHostToDevice, c->stream));
+        launch_gather_rows(dbits, wpr, dsel, K, rows, c->stream);
+        launch_batch_quadratic(*g, rows, wpr, K, q64, sc2, c->stream);
+        std::vector<double> hq64(K);
+        HIPCHK(hipMemcpyAsync(hq64.data(), q64, K * 8, hipMemcpyDeviceToHost, c->stream));
+        CHK(sync(c));
+        double err = 0.0, bv = -INFINITY;
+        for (int64_t t = 0; t < K; ++t) {
+            const double exact = value_of(hq64[t]);
+            const double den = std::max(std::fabs(exact), 1e-300);
+            err = std::max(err, std::fabs(v[order[t]] - exact) / den);
+            v[order[t]] = exact;
+            if (exact > bv) { bv = exact; best = order[t]; }
+        }
+        if (max_rel_err) *max_rel_err = err;
+    } else {
+        for (int64_t b = 1; b < B; ++b)
+            if (v[b] > v[best]) best = b;
+        if (max_rel_err) *max_rel_err = 0.0;
     }
+    if (values) std::memcpy(values, v.data(), B * 8);
     if (best_index) *best_index = best;
-    if (best_value) *best_value = bv;
+    if (best_value) *best_value = v[best];
     return PQI_OK;
+}
+
+int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, double* values,
+                   int64_t* best_index, double* best_value) {
+    return pqi_eval_batch_ex(g, bits, B, is_spin_glass, PQI_ENGINE_AUTO, values, best_index, best_value, nullptr);
 }
 
 int pqi_single_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, int64_t* k, double* gain) {

@@ -78,10 +78,68 @@ def test_eval_batch_real(ctx, orc, dtype, sg):
     W = er_weights(n, 0.3, 5, "real").astype(dtype).astype(np.float64)
     X = np.random.default_rng(6).integers(0, 2, (B, n)).astype(bool)
     dg = _lib.DeviceGraph.dense(W.astype(dtype))
-    bi, bv, vals = dg.eval_batch(_lib.pack_bits(X), sg)
+    bi, bv, vals = dg.eval_batch(_lib.pack_bits(X), sg, engine=_lib.ENGINE_EXACT)
     obi, ovals = orc.eval_batch(W, X, sg)
     np.testing.assert_allclose(vals, ovals, rtol=REL, atol=REL * np.abs(ovals).max())
     assert bi == obi
+    # default engine: fp32 weights use split BF16 + FP64 re-score of the best 16
+    bi2, bv2, _ = dg.eval_batch(_lib.pack_bits(X), sg)
+    assert bi2 == obi and bv2 == pytest.approx(ovals[obi], rel=REL)
+
+
+def sk_matrix(n, seed):
+    """Sherrington-Kirkpatrick couplings: w_ij ~ N(0, 1/n) i.i.d. for i < j, fp32."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n), dtype=np.float32) / np.float32(np.sqrt(n))
+    W = np.triu(A, 1)
+    del A
+    W += W.T
+    return W
+
+
+# split-BF16 tolerance: |approx - FP64| <= BF16_TOL * sum_{i<j} |w_ij|.  The
+# split leaves |r| <= 2^-18 |w| per weight; the rest is FP32 accumulation.
+BF16_TOL = 2e-5
+
+
+@pytest.mark.parametrize("sg", [0, 1])
+def test_eval_batch_bf16_split(ctx, orc, sg):
+    n, B = 1000, 300
+    log(f"eval_batch bf16 split n={n} sg={sg}")
+    W = sk_matrix(n, 41)
+    X = np.random.default_rng(42).integers(0, 2, (B, n)).astype(bool)
+    dg = _lib.DeviceGraph.dense(W)
+    assert dg.info()["weights"] == _lib.WEIGHTS_F32
+    bi, bv, vals, err = dg.eval_batch(_lib.pack_bits(X), sg, engine=_lib.ENGINE_BF16, return_error=True)
+    obi, ovals = orc.eval_batch(W.astype(np.float64), X, sg)
+    scale = np.abs(np.triu(W.astype(np.float64), 1)).sum()
+    approx = np.ones(B, bool)
+    top = np.argsort(-vals, kind="stable")[:16]
+    approx[top] = False
+    assert np.all(np.abs(vals[approx] - ovals[approx]) <= BF16_TOL * scale)
+    np.testing.assert_allclose(vals[top], ovals[top], rtol=1e-9)  # FP64 re-scored
+    assert bi == obi and bv == pytest.approx(ovals[obi], rel=1e-9)
+    log(f"  max rel err among re-scored: {err:.3e}")
+
+
+def test_eval_batch_sk_full_size(ctx, orc):
+    """C4 shape: SK n=16384, fp32, 1024 random candidates.  The split-BF16
+    energies against the FP64 engine on the device (size-independent
+    agreement), the FP64 re-score of the selected best against the oracle."""
+    n, B = 16384, 1024
+    log("eval_batch SK n=16384 B=1024")
+    W = sk_matrix(n, 43)
+    X = np.random.default_rng(44).integers(0, 2, (B, n)).astype(bool)
+    bits = _lib.pack_bits(X)
+    dg = _lib.DeviceGraph.dense(W)
+    bi, bv, vals, err = dg.eval_batch(bits, 1, engine=_lib.ENGINE_BF16, return_error=True)
+    ei, ev, evals = dg.eval_batch(bits, 1, engine=_lib.ENGINE_EXACT)
+    scale = float(np.abs(np.triu(W, 1)).sum(dtype=np.float64))
+    assert np.all(np.abs(vals - evals) <= BF16_TOL * scale)
+    assert bi == ei and bv == pytest.approx(ev, rel=1e-9)
+    e64 = orc.compute_energy(W, X[bi].astype(np.uint8))
+    assert bv == pytest.approx(e64, rel=1e-9)
+    log(f"  best={bi} E={bv:.6f} oracle={e64:.6f} bf16 max rel err (top16)={err:.3e}")
 
 
 @pytest.mark.parametrize("kind", ["int", "real"])
