@@ -71,6 +71,7 @@ extern "C" {
 
 typedef struct pqi_ctx pqi_ctx;
 typedef struct pqi_graph pqi_graph;
+typedef struct pqi_batch pqi_batch;
 
 typedef struct {
     int64_t n;
@@ -176,6 +177,26 @@ PQI_API int pqi_gray_search(pqi_graph* g, const uint8_t* theta, const pqi_search
  * run on the device from theta (in: best_guess, out: final state). */
 PQI_API int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p,
                      pqi_search_stats* stats);
+
+/* ---- many instances at once ------------------------------------------------
+ * Candidate scoring for `count` dense int8 instances of uneven sizes in one
+ * persistent launch captured in a hipGraph (zero, grouped MFMA tiles, per-
+ * instance first-best).  Replaces one run_cut_opencl call per instance
+ * (pyqrackising/maxcut_tfim.py:212-285). */
+PQI_API int pqi_batch_create(pqi_ctx* ctx, pqi_graph* const* graphs, int64_t count, int64_t cands,
+                             pqi_batch** out);
+PQI_API int pqi_batch_free(pqi_batch* b);
+/* bits: for each instance in order, cands x ceil(n_i/32) uint32 (LSB-first). */
+PQI_API int pqi_batch_set_candidates(pqi_batch* b, const uint32_t* bits);
+/* best_index/best_value: one per instance; values (optional): cands per instance, in order. */
+PQI_API int pqi_batch_eval(pqi_batch* b, int is_spin_glass, int64_t* best_index, double* best_value,
+                           double* values);
+/* Times every tile variant (contraction depth x persistent grid) `reps` times,
+ * keeps the fastest and re-captures the graph.  ms_per_variant (optional). */
+PQI_API int pqi_batch_autotune(pqi_batch* b, int reps, int* chosen, double* ms_per_variant, int max_variants);
+PQI_API int pqi_batch_set_variant(pqi_batch* b, int variant);
+PQI_API int pqi_batch_info(const pqi_batch* b, int64_t* ntiles, int64_t* algorithmic_ops, int64_t* executed_ops,
+                           int64_t* weight_bytes, int* variant);
 
 /* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
 PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);

@@ -112,6 +112,13 @@ SIGNATURES = {
     "pqi_gray_search": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pqi_local_search": (_int, [_vp, _vp, _vp, _vp]),
     "pqi_score": (_int, [_vp, _vp, _vp, _vp]),
+    "pqi_batch_create": (_int, [_vp, _vp, _i64, _i64, _vp]),
+    "pqi_batch_free": (_int, [_vp]),
+    "pqi_batch_set_candidates": (_int, [_vp, _vp]),
+    "pqi_batch_eval": (_int, [_vp, _int, _vp, _vp, _vp]),
+    "pqi_batch_autotune": (_int, [_vp, _int, _vp, _vp, _int]),
+    "pqi_batch_set_variant": (_int, [_vp, _int]),
+    "pqi_batch_info": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -320,6 +327,69 @@ class DeviceGraph:
         with self.ctx.lock:
             check(load().pqi_score(self._h, _p(t), ctypes.byref(c), ctypes.byref(e)))
         return c.value, e.value
+
+
+class Batch:
+    """Candidate scoring for many dense int8 instances in one persistent,
+    graph-captured launch (pqi_batch_*)."""
+
+    N_VARIANTS = 6
+
+    def __init__(self, graphs, cands):
+        self.graphs = list(graphs)
+        self.ctx = self.graphs[0].ctx
+        self.cands = int(cands)
+        arr = (ctypes.c_void_p * len(self.graphs))(*[g._h.value for g in self.graphs])
+        self._h = ctypes.c_void_p()
+        with self.ctx.lock:
+            check(load().pqi_batch_create(self.ctx.handle, arr, len(self.graphs), self.cands, ctypes.byref(self._h)))
+
+    def set_candidates(self, bits_list):
+        """bits_list[i]: cands x ceil(n_i/32) uint32 for instance i."""
+        flat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(b, dtype=np.uint32).ravel()
+                                                    for b in bits_list]))
+        with self.ctx.lock:
+            check(load().pqi_batch_set_candidates(self._h, _p(flat)))
+
+    def eval(self, is_spin_glass, want_values=False):
+        k = len(self.graphs)
+        bi = np.empty(k, dtype=np.int64)
+        bv = np.empty(k, dtype=np.float64)
+        vals = np.empty(k * self.cands, dtype=np.float64) if want_values else None
+        with self.ctx.lock:
+            check(load().pqi_batch_eval(self._h, int(bool(is_spin_glass)), _p(bi), _p(bv),
+                                        _p(vals) if want_values else None))
+        return bi, bv, (vals.reshape(k, self.cands) if want_values else None)
+
+    def autotune(self, reps=3):
+        ms = np.zeros(self.N_VARIANTS, dtype=np.float64)
+        ch = ctypes.c_int()
+        with self.ctx.lock:
+            check(load().pqi_batch_autotune(self._h, int(reps), ctypes.byref(ch), _p(ms), self.N_VARIANTS))
+        return ch.value, ms
+
+    def set_variant(self, v):
+        with self.ctx.lock:
+            check(load().pqi_batch_set_variant(self._h, int(v)))
+
+    def info(self):
+        nt, ops, exe, wb = (ctypes.c_int64() for _ in range(4))
+        v = ctypes.c_int()
+        check(load().pqi_batch_info(self._h, ctypes.byref(nt), ctypes.byref(ops), ctypes.byref(exe),
+                                    ctypes.byref(wb), ctypes.byref(v)))
+        return {"tiles": nt.value, "algorithmic_ops": ops.value, "executed_ops": exe.value,
+                "weight_bytes": wb.value, "variant": v.value}
+
+    def close(self):
+        if self._h:
+            load().pqi_batch_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def make_params(n, is_spin_glass=True, gray_iterations=None, walkers=None, seed_multiple=None,

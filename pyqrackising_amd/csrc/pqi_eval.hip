@@ -12,135 +12,27 @@
 #include <hip/hip_runtime.h>
 
 #include "pqi_internal.h"
+#include "pqi_tile_i8.h"
 
 namespace pqi {
 
 __global__ void k_batch_sum(const double* part, int64_t nblk, int64_t B, double* q);
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
 
 namespace {
-
-constexpr int BM = 128;       // candidates per work group
-constexpr int BN = 128;       // output columns per work group
-constexpr int BK = 64;        // contraction depth per LDS stage
-constexpr int LDS_ROW = BK + 16;  // padded LDS row (bytes)
-
-// 4 bits -> 4 bytes holding 0/1.
-__device__ __forceinline__ uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
-
-// 0/1 bytes -> +-1 int8 bytes (x = 1 -> s = -1).
-__device__ __forceinline__ uint32_t to_pm1(uint32_t bytes01) { return (bytes01 * 0xFEu) | 0x01010101u; }
-
+constexpr int BM = TILE_BM;  // candidates per work group
+constexpr int BN = TILE_BN;  // output columns per work group
 }  // namespace
 
 // q[m] += sum over this block's 128 columns j of s_mj (S W)_mj.  q is zeroed by the caller.
-// Work-group tile 128 x 128, four waves in 2 x 2, each wave 64 x 64 as 2 x 2 MFMA 32x32x32.
 __global__ __launch_bounds__(256) void k_batch_i8(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
                                                   const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
                                                   unsigned long long* __restrict__ q) {
-    __shared__ __align__(16) int8_t As[BM * LDS_ROW];
-    __shared__ __align__(16) int8_t Bs[BN * LDS_ROW];
-    __shared__ uint32_t Sb[BM * (BN / 32)];
-
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = wid >> 1, wn = wid & 1;
-    const int64_t m0 = (int64_t)blockIdx.y * BM, j0 = (int64_t)blockIdx.x * BN;
-    const int lr = tid >> 1, lh = tid & 1;  // loader row / half
-
-    v16i acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
-
-    const int64_t am = m0 + lr;
-    for (int64_t k0 = 0; k0 < n_pad; k0 += BK) {
-        // A: 32 candidate bits -> 32 bytes of +-1
-        {
-            const uint32_t word = (am < B) ? bits[am * wpr + (k0 >> 5) + lh] : 0u;
-            uint32_t o[8];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) o[t] = to_pm1(spread4((word >> (4 * t)) & 0xFu));
-            uint4* dst = (uint4*)&As[lr * LDS_ROW + lh * 32];
-            dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-            dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
-        }
-        // B: rows j of W (W is symmetric, so row j holds column j)
-        {
-            const uint4* src = (const uint4*)(W + (j0 + lr) * ld + k0 + lh * 32);
-            uint4 a = src[0], b = src[1];
-            uint4* dst = (uint4*)&Bs[lr * LDS_ROW + lh * 32];
-            dst[0] = a;
-            dst[1] = b;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 32) {
-            v4i av[2], bv[2];
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
-                av[mi] = *(const v4i*)&As[(wm * 64 + mi * 32 + (lane & 31)) * LDS_ROW + kk + 16 * (lane >> 5)];
-#pragma unroll
-            for (int nj = 0; nj < 2; ++nj)
-                bv[nj] = *(const v4i*)&Bs[(wn * 64 + nj * 32 + (lane & 31)) * LDS_ROW + kk + 16 * (lane >> 5)];
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                for (int nj = 0; nj < 2; ++nj)
-                    acc[mi][nj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[mi], bv[nj], acc[mi][nj], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-
-    // candidate bits of this block's output columns
-    for (int t = tid; t < BM * (BN / 32); t += 256) {
-        const int r = t >> 2, c = t & 3;
-        const int64_t m = m0 + r;
-        Sb[t] = (m < B) ? bits[m * wpr + (j0 >> 5) + c] : 0u;
-    }
-    __syncthreads();
-
-    // 32x32 accumulator layout: lane l, register r -> column l%32,
-    // row 8*(r/4) + 4*(l/32) + r%4.
-    int vals[32];
-#pragma unroll
-    for (int i = 0; i < 32; ++i) vals[i] = 0;
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int nj = 0; nj < 2; ++nj) {
-            const int col = wn * 64 + nj * 32 + (lane & 31);
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const uint32_t bit = (Sb[row * 4 + (col >> 5)] >> (col & 31)) & 1u;
-                const int p = acc[mi][nj][r];
-                vals[mi * 16 + r] += bit ? -p : p;
-            }
-        }
-    // reduce-scatter over the 32 lanes that share rows
-#pragma unroll
-    for (int t = 0, half = 16; t < 5; ++t, half >>= 1) {
-        const int up = (lane >> t) & 1;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            if (i < half) {
-                const int send = up ? vals[i] : vals[i + half];
-                const int keep = up ? vals[i + half] : vals[i];
-                vals[i] = keep + __shfl_xor(send, 1 << t, 64);
-            }
-        }
-    }
-    const int qi = (lane & 1) * 16 + ((lane >> 1) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 3) & 1) * 2 +
-                   ((lane >> 4) & 1);
-    const int mi = qi >> 4, r = qi & 15;
-    const int row = wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    const int64_t m = m0 + row;
-    if (m < B) atomicAdd(&q[m], (unsigned long long)(long long)vals[0]);
+    __shared__ __align__(16) int8_t As[TILE_BM * (64 + 16)];
+    __shared__ __align__(16) int8_t Bs[TILE_BN * (64 + 16)];
+    __shared__ uint32_t Sb[TILE_BM * (TILE_BN / 32)];
+    tile_i8<64>(W, ld, n_pad, bits, wpr, B, (int64_t)blockIdx.y * TILE_BM, (int64_t)blockIdx.x * TILE_BN, q, As, Bs,
+                Sb);
 }
 
 // ---------------------------------------------------------------------------

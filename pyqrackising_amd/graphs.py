@@ -150,3 +150,27 @@ def erdos_renyi_edges_sparse(n, p, rng):
 def pm1(size, rng):
     """Uniform +-1 couplings (int8)."""
     return (rng.integers(0, 2, size=size, dtype=np.int8) * 2 - 1).astype(np.int8)
+
+
+def dense_int8_from_edges(n, lo, hi, w):
+    """Symmetric int8 weights from an edge list (zero diagonal)."""
+    W = np.zeros((n, n), dtype=np.int8)
+    W[lo, hi] = w
+    W[hi, lo] = w
+    return W
+
+
+def batched_regular_instances(count, seed, n_lo=256, n_hi=4096, d_lo=3, d_hi=10):
+    """The batched workload: `count` random d-regular graphs with n uniform in
+    [n_lo, n_hi] and d uniform in {d_lo..d_hi} (n += 1 when n*d is odd), +-1
+    couplings.  Returns a list of int8 dense matrices."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(n_lo, n_hi + 1))
+        d = int(rng.integers(d_lo, d_hi + 1))
+        if (n * d) % 2:
+            n += 1
+        lo, hi = random_regular_edges(n, d, rng)
+        out.append(dense_int8_from_edges(n, lo, hi, pm1(lo.size, rng)))
+    return out

@@ -415,3 +415,69 @@ def test_c1_pipeline_matches_oracle(ctx, orc):
                                    gray_iterations=4096)
     assert out[0] == "".join("1" if b else "0" for b in xc)
     assert out[1] == r["cut_value"] and out[3] == r["min_energy"]
+
+
+# ---------------------------------------------------------------------------
+# many instances in one persistent, graph-captured launch
+# ---------------------------------------------------------------------------
+
+def test_batch_eval_uneven_instances(ctx, orc):
+    log("batch eval uneven")
+    Ws = graphs.batched_regular_instances(12, seed=51, n_lo=24, n_hi=600)
+    cands = 300
+    rng = np.random.default_rng(52)
+    dgs = [_lib.DeviceGraph.dense(W) for W in Ws]
+    batch = _lib.Batch(dgs, cands)
+    Xs = [rng.integers(0, 2, (cands, W.shape[0])).astype(bool) for W in Ws]
+    batch.set_candidates([_lib.pack_bits(X) for X in Xs])
+    expect = [orc.eval_batch(W.astype(np.float64), X, sg) for sg in (0, 1) for W, X in zip(Ws, Xs)]
+    for v in range(_lib.Batch.N_VARIANTS):
+        batch.set_variant(v)
+        for sg in (0, 1):
+            bi, bv, vals = batch.eval(sg, want_values=True)
+            for i in range(len(Ws)):
+                obi, ovals = expect[sg * len(Ws) + i]
+                assert np.array_equal(vals[i], ovals), (v, sg, i)
+                assert bi[i] == obi and bv[i] == ovals[obi]
+    chosen, ms = batch.autotune(reps=2)
+    assert 0 <= chosen < _lib.Batch.N_VARIANTS and ms[chosen] == ms.min()
+    # graph replays with new candidates, eager single-instance work in between
+    for rep in range(3):
+        Xs = [rng.integers(0, 2, (cands, W.shape[0])).astype(bool) for W in Ws]
+        batch.set_candidates([_lib.pack_bits(X) for X in Xs])
+        bi, bv, vals = batch.eval(0, want_values=True)
+        for i, (dg, X) in enumerate(zip(dgs, Xs)):
+            ebi, ebv, evals = dg.eval_batch(_lib.pack_bits(X), 0)
+            assert np.array_equal(vals[i], evals) and bi[i] == ebi, (rep, i)
+
+
+def test_batch_eval_c3_shape(ctx, orc):
+    """The batched configuration at full size: 256 instances, n in [256, 4096],
+    d in {3..10}, 1024 candidates each.  Every instance against the
+    single-instance path (itself pinned to the oracle); a few candidates of the
+    largest instances against the oracle directly."""
+    log("batch eval C3 shape")
+    Ws = graphs.batched_regular_instances(256, seed=1003)
+    cands = 1024
+    rng = np.random.default_rng(53)
+    dgs = [_lib.DeviceGraph.dense(W) for W in Ws]
+    bits = [rng.integers(0, 1 << 32, (cands, (W.shape[0] + 31) // 32), dtype=np.uint64).astype(np.uint32)
+            for W in Ws]
+    for b, W in zip(bits, Ws):  # clear bits beyond n
+        n = W.shape[0]
+        if n % 32:
+            b[:, -1] &= np.uint32((1 << (n % 32)) - 1)
+    batch = _lib.Batch(dgs, cands)
+    batch.set_candidates(bits)
+    chosen, ms = batch.autotune(reps=2)
+    log(f"  autotune chose {chosen}: {np.round(ms, 3).tolist()} ms")
+    bi, bv, vals = batch.eval(1, want_values=True)
+    for i in range(0, 256, 17):
+        ebi, ebv, evals = dgs[i].eval_batch(bits[i], 1)
+        assert np.array_equal(vals[i], evals) and bi[i] == ebi and bv[i] == ebv, i
+    big = np.argsort([W.shape[0] for W in Ws])[-2:]
+    for i in big:
+        n = Ws[i].shape[0]
+        X = ((bits[i][:4, :, None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(4, -1)[:, :n].astype(bool)
+        _, ov = orc.eval_batch(Ws[i].astype(np.float64), X, 1)
+        assert np.array_equal(vals[i][:4], ov), i
