@@ -198,6 +198,15 @@ PQI_API int pqi_batch_set_variant(pqi_batch* b, int variant);
 PQI_API int pqi_batch_info(const pqi_batch* b, int64_t* ntiles, int64_t* algorithmic_ops, int64_t* executed_ops,
                            int64_t* weight_bytes, int* variant);
 
+/* The spin_glass_solver loop on `count` instances at once (all of one weight
+ * kind, any sizes): every phase is one launch over the instances that take
+ * that branch of the reference loop.  Each instance makes exactly the moves
+ * pqi_local_search makes on it alone.  thetas[i] (n_i bytes) in: start, out:
+ * final state; stats: count records.  gray_iterations < 0 means n_i^2 per
+ * instance.  Only the reference double-flip mode. */
+PQI_API int pqi_batch_local_search(pqi_ctx* ctx, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
+                                   const pqi_search_params* p, pqi_search_stats* stats);
+
 /* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
 PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);
 

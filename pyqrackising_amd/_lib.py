@@ -119,6 +119,7 @@ SIGNATURES = {
     "pqi_batch_autotune": (_int, [_vp, _int, _vp, _vp, _int]),
     "pqi_batch_set_variant": (_int, [_vp, _int]),
     "pqi_batch_info": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "pqi_batch_local_search": (_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -390,6 +391,23 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_local_search(graphs, thetas, params):
+    """pqi_batch_local_search: returns (states, list of stats dicts)."""
+    graphs = list(graphs)
+    ctx = graphs[0].ctx
+    k = len(graphs)
+    states = [np.ascontiguousarray(t, dtype=np.uint8).copy() for t in thetas]
+    for g, t in zip(graphs, states):
+        if t.shape != (g.n,):
+            raise ValueError("theta shape does not match its graph")
+    garr = (ctypes.c_void_p * k)(*[g._h.value for g in graphs])
+    tarr = (ctypes.c_void_p * k)(*[t.ctypes.data for t in states])
+    st = (SearchStats * k)()
+    with ctx.lock:
+        check(load().pqi_batch_local_search(ctx.handle, garr, k, tarr, ctypes.byref(params), st))
+    return states, [s.as_dict() for s in st]
 
 
 def make_params(n, is_spin_glass=True, gray_iterations=None, walkers=None, seed_multiple=None,

@@ -1,0 +1,437 @@
+// Batched local search: the spin_glass_solver loop
+// (pyqrackising/spin_glass_solver.py:603-727) run on many instances at once.
+//
+// Every phase is one launch over the list of instances that take that branch
+// of the reference loop in the current round; the host reads back one small
+// record per instance per phase and builds the next list.  Each instance's
+// moves are exactly those pqi_local_search makes on it alone: the kernels are
+// the same bodies, only indexed by instance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pqi_internal.h"
+
+namespace pqi {
+
+namespace {
+
+struct Arena {
+    void* base = nullptr;
+    size_t used = 0, cap = 0;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+int64_t reference_steps(int64_t T, int64_t n) {
+    const int64_t nb = (n + 63) >> 6;
+    int64_t total = 0;
+    for (int p = 0; p < 63; ++p) {
+        const int64_t cnt = (T >> p) - (T >> (p + 1));
+        if (!cnt) continue;
+        int64_t valid = 0;
+        for (int64_t b = 0; b < nb; ++b) valid += (p + 64 * b < n) ? 1 : 0;
+        total += cnt * valid;
+    }
+    return total;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+}  // namespace pqi
+
+using namespace pqi;
+
+#define BCHK(expr)                                                                  \
+    do {                                                                            \
+        hipError_t _e = (expr);                                                     \
+        if (_e != hipSuccess) {                                                     \
+            std::string _m = std::string(#expr) + ": " + hipGetErrorString(_e);     \
+            if (arena) (void)hipFree(arena);                                        \
+            return fail(_e == hipErrorOutOfMemory ? PQI_ERR_OOM : PQI_ERR_HIP, _m); \
+        }                                                                           \
+    } while (0)
+
+extern "C" {
+
+int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
+                           const pqi_search_params* p, pqi_search_stats* stats) {
+    if (!c || !graphs || !thetas || !p || !stats) return fail(PQI_ERR_INVALID, "null argument");
+    if (count < 1) return fail(PQI_ERR_INVALID, "count must be >= 1");
+    if (p->walkers < 1 || p->seed_multiple < 1) return fail(PQI_ERR_INVALID, "walkers and seed_multiple must be >= 1");
+    if (p->double_mode != PQI_DOUBLE_REFERENCE)
+        return fail(PQI_ERR_UNSUPPORTED, "the batched search implements the reference double-flip mode");
+    const int kind = graphs[0]->kind;
+    for (int64_t i = 0; i < count; ++i) {
+        if (!graphs[i] || graphs[i]->ctx != c) return fail(PQI_ERR_INVALID, "graph of another context");
+        if (graphs[i]->kind != kind) return fail(PQI_ERR_INVALID, "all graphs must share one weight kind");
+        if (graphs[i]->n < 2) return fail(PQI_ERR_INVALID, "n must be >= 2");
+        if (!thetas[i]) return fail(PQI_ERR_INVALID, "null theta");
+        for (int64_t v = 0; v < graphs[i]->n; ++v)
+            if (thetas[i][v] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return fail(PQI_ERR_HIP, "hipSetDevice failed");
+    hipStream_t s = c->stream;
+    void* arena = nullptr;
+    const size_t fb = kind == PQI_WEIGHTS_I8 ? 4 : 8;
+    const int64_t G = p->walkers;
+    const double scale = p->is_spin_glass ? 2.0 : 1.0;
+
+    // per-instance sizes and arena layout
+    std::vector<BInstRaw> h_inst(count);
+    std::vector<size_t> off_x(count);
+    size_t total = 0;
+    int64_t max_n = 0, max_seeds = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        const pqi_graph* g = graphs[i];
+        const int64_t n = g->n, nb = (n + 63) >> 6;
+        const int64_t bs = G * p->seed_multiple, seeds = bs * nb;
+        if (G > seeds) return fail(PQI_ERR_INVALID, "more walkers than seeds");
+        const int64_t gi = p->gray_iterations < 0 ? n * n : p->gray_iterations;
+        BInstRaw& r = h_inst[i];
+        std::memset(&r, 0, sizeof(r));
+        r.W = g->W;
+        r.ld = g->n_pad;
+        r.n = n;
+        r.ap = g->adj_ptr;
+        r.ai = g->adj_idx;
+        r.aw = g->adj_w;
+        r.use_adj = (g->use_adj || g->W == nullptr) ? 1 : 0;
+        r.seed_count = seeds;
+        r.bs = bs;
+        r.G = G;
+        r.T = (gi + G - 1) / G;
+        max_n = std::max(max_n, n);
+        max_seeds = std::max(max_seeds, seeds);
+        off_x[i] = total;
+        total += align256(n) * 2 + align256(n * fb) * 2 + align256(G * n) + align256(G * n * fb) + align256(G * 8) * 4 +
+                 align256(seeds * 8) + align256(seeds * 4) + align256(sizeof(PhaseOut));
+    }
+    const int64_t L0 = count;
+    const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
+                        align256(L0 * sizeof(BInstRaw));
+    BCHK(hipMalloc(&arena, total + misc));
+    char* a = (char*)arena;
+    for (int64_t i = 0; i < count; ++i) {
+        BInstRaw& r = h_inst[i];
+        const int64_t n = r.n;
+        char* q = a + off_x[i];
+        auto take = [&](size_t bytes) {
+            char* t = q;
+            q += align256(bytes);
+            return t;
+        };
+        r.x = (uint8_t*)take(n);
+        r.h = take(n * fb);
+        r.rx = (uint8_t*)take(n);
+        r.rh = take(n * fb);
+        r.wx = (uint8_t*)take(G * n);
+        r.wh = take(G * n * fb);
+        r.we = (double*)take(G * 8);
+        r.wev = (int64_t*)take(G * 8);
+        r.sel = (int64_t*)take(G * 8);
+        r.seld = (double*)take(G * 8);
+        r.delta = (double*)take(r.seed_count * 8);
+        r.rank = (int32_t*)take(r.seed_count * 4);
+        r.out = (PhaseOut*)take(sizeof(PhaseOut));
+    }
+    char* m = a + total;
+    int32_t* d_list = (int32_t*)m;
+    m += align256(L0 * 4);
+    int64_t* d_moves = (int64_t*)m;
+    m += align256(L0 * 16);
+    int32_t* d_src = (int32_t*)m;
+    m += align256(L0 * 4);
+    double* d_q = (double*)m;
+    m += align256(L0 * 8);
+    BInstRaw* d_inst = (BInstRaw*)m;
+    BCHK(hipMemcpyAsync(d_inst, h_inst.data(), count * sizeof(BInstRaw), hipMemcpyHostToDevice, s));
+    for (int64_t i = 0; i < count; ++i)
+        BCHK(hipMemcpyAsync(h_inst[i].x, thetas[i], h_inst[i].n, hipMemcpyHostToDevice, s));
+
+    std::vector<int32_t> list(count);
+    for (int64_t i = 0; i < count; ++i) list[i] = (int32_t)i;
+    std::vector<PhaseOut> outs(count);
+    std::vector<int64_t> moves(2 * count);
+    std::vector<int32_t> src(count);
+    std::vector<double> qv(count);
+    std::memset(stats, 0, sizeof(pqi_search_stats) * count);
+
+    auto upload_list = [&](const std::vector<int32_t>& l) -> hipError_t {
+        return hipMemcpyAsync(d_list, l.data(), l.size() * 4, hipMemcpyHostToDevice, s);
+    };
+    auto read_outs = [&](const std::vector<int32_t>& l) -> hipError_t {
+        for (size_t t = 0; t < l.size(); ++t) {
+            hipError_t e = hipMemcpyAsync(&outs[t], h_inst[l[t]].out, sizeof(PhaseOut), hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipStreamSynchronize(s);
+    };
+    auto flips = [&](const std::vector<int32_t>& l, const std::vector<int64_t>& mv, int which) -> hipError_t {
+        hipError_t e = upload_list(l);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_moves, mv.data(), l.size() * 16, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) launch_b_flip(kind, d_inst, d_list, (int64_t)l.size(), d_moves, which, s);
+        return e;
+    };
+    auto copies = [&](const std::vector<int32_t>& l, const std::vector<int32_t>& sr, int dst) -> hipError_t {
+        hipError_t e = upload_list(l);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_src, sr.data(), l.size() * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) launch_b_copy(kind, d_inst, d_list, (int64_t)l.size(), max_n, d_src, dst, s);
+        return e;
+    };
+    auto score = [&](std::vector<int32_t>& l) -> hipError_t {
+        hipError_t e = upload_list(l);
+        if (e != hipSuccess) return e;
+        launch_b_quad(kind, d_inst, d_list, (int64_t)l.size(), d_q, s);
+        e = hipMemcpyAsync(qv.data(), d_q, l.size() * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e;
+    };
+
+    // fields and starting values
+    BCHK(upload_list(list));
+    launch_b_field(kind, d_inst, d_list, count, max_n, s);
+    BCHK(score(list));
+    for (int64_t i = 0; i < count; ++i) {
+        const pqi_graph* g = graphs[i];
+        const double cut = kind == PQI_WEIGHTS_I8 ? (double)((2 * g->w_up_i - (long long)qv[i]) / 4)
+                                                  : (2.0 * g->w_up - qv[i]) / 4.0;
+        const double en = kind == PQI_WEIGHTS_I8 ? (double)(-(long long)qv[i] / 2) : -qv[i] / 2.0;
+        stats[i].max_energy = p->is_spin_glass ? en : cut;
+    }
+
+    std::vector<int32_t> active = list;
+    std::vector<double> walk_gain(count), reheat(count);
+    std::vector<double> we(G);
+    std::vector<int64_t> wev(G);
+    while (!active.empty()) {
+        std::vector<int32_t> next;
+        if (p->max_rounds > 0) {
+            std::vector<int32_t> keep;
+            for (int32_t i : active)
+                if (stats[i].rounds < p->max_rounds) keep.push_back(i);
+            active.swap(keep);
+            if (active.empty()) break;
+        }
+        for (int32_t i : active) stats[i].rounds++;
+
+        // single-flip descent, bounded launches repeated while the bound is hit
+        double t0 = now_s();
+        std::vector<int32_t> run = active;
+        while (!run.empty()) {
+            const int64_t cap = 16 * max_n + 4096;
+            BCHK(upload_list(run));
+            launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, s);
+            BCHK(read_outs(run));
+            std::vector<int32_t> again;
+            for (size_t t = 0; t < run.size(); ++t) {
+                pqi_search_stats& st = stats[run[t]];
+                st.single_evals += outs[t].evals;
+                st.single_accepts += outs[t].accepts;
+                st.max_energy += scale * outs[t].gain_sum;
+                if (outs[t].accepts == cap) again.push_back(run[t]);
+            }
+            run.swap(again);
+        }
+        const double t_single = now_s() - t0;
+
+        // double flips
+        t0 = now_s();
+        BCHK(upload_list(active));
+        launch_b_double(kind, d_inst, d_list, (int64_t)active.size(), 0, s);
+        BCHK(read_outs(active));
+        std::vector<int32_t> acc, rest;
+        std::vector<int64_t> mv;
+        for (size_t t = 0; t < active.size(); ++t) {
+            const int32_t i = active[t];
+            const int64_t n = h_inst[i].n, C = n * (n - 1) / 2;
+            stats[i].double_evals += std::min(G, C);
+            if (outs[t].gain > 0.0) {
+                acc.push_back(i);
+                mv.push_back(outs[t].k);
+                mv.push_back(outs[t].l);
+                stats[i].max_energy += scale * outs[t].gain;
+                stats[i].double_accepts++;
+            } else {
+                rest.push_back(i);
+            }
+        }
+        if (!acc.empty()) BCHK(flips(acc, mv, 0));
+        next.insert(next.end(), acc.begin(), acc.end());
+        const double t_double = now_s() - t0;
+
+        // Gray seeds
+        t0 = now_s();
+        std::vector<int32_t> rest2, seeded;
+        if (!rest.empty()) {
+            BCHK(upload_list(rest));
+            launch_b_seeds(kind, d_inst, d_list, (int64_t)rest.size(), max_seeds, G, s);
+            std::vector<double> top(rest.size());
+            for (size_t t = 0; t < rest.size(); ++t)
+                BCHK(hipMemcpyAsync(&top[t], h_inst[rest[t]].seld, 8, hipMemcpyDeviceToHost, s));
+            BCHK(hipStreamSynchronize(s));
+            for (size_t t = 0; t < rest.size(); ++t) {
+                const int32_t i = rest[t];
+                stats[i].seed_evals += h_inst[i].seed_count;
+                if (top[t] > 0.0) {
+                    seeded.push_back(i);
+                    stats[i].max_energy += scale * top[t];
+                    stats[i].seed_accepts++;
+                } else {
+                    rest2.push_back(i);
+                }
+            }
+            if (!seeded.empty()) {
+                BCHK(upload_list(seeded));
+                launch_b_walk(kind, d_inst, d_list, (int64_t)seeded.size(), max_n, G, 0, 1, s);
+                std::vector<int32_t> zero(seeded.size(), 0);
+                BCHK(copies(seeded, zero, 0));
+                next.insert(next.end(), seeded.begin(), seeded.end());
+            }
+        }
+        const double t_seeds = now_s() - t0;
+
+        // Gray walkers
+        t0 = now_s();
+        std::vector<int32_t> rest3, walked;
+        std::vector<int32_t> best_walker(count, 0);
+        if (!rest2.empty()) {
+            BCHK(upload_list(rest2));
+            launch_b_walk(kind, d_inst, d_list, (int64_t)rest2.size(), max_n, G, -1, -1, s);
+            std::vector<double> all_we(rest2.size() * G);
+            std::vector<int64_t> all_wev(rest2.size() * G);
+            for (size_t t = 0; t < rest2.size(); ++t) {
+                BCHK(hipMemcpyAsync(&all_we[t * G], h_inst[rest2[t]].we, G * 8, hipMemcpyDeviceToHost, s));
+                BCHK(hipMemcpyAsync(&all_wev[t * G], h_inst[rest2[t]].wev, G * 8, hipMemcpyDeviceToHost, s));
+            }
+            BCHK(hipStreamSynchronize(s));
+            std::vector<int32_t> wsrc;
+            for (size_t t = 0; t < rest2.size(); ++t) {
+                const int32_t i = rest2[t];
+                int64_t bw = 0;
+                for (int64_t w = 0; w < G; ++w) {
+                    stats[i].gray_evals_computed += all_wev[t * G + w];
+                    if (all_we[t * G + w] > all_we[t * G + bw]) bw = w;
+                }
+                stats[i].gray_evals += G * reference_steps(h_inst[i].T, h_inst[i].n);
+                best_walker[i] = (int32_t)bw;
+                walk_gain[i] = all_we[t * G + bw];
+                if (walk_gain[i] > 0.0) {
+                    walked.push_back(i);
+                    wsrc.push_back((int32_t)bw);
+                    stats[i].max_energy += scale * walk_gain[i];
+                    stats[i].walk_accepts++;
+                } else {
+                    rest3.push_back(i);
+                }
+            }
+            if (!walked.empty()) BCHK(copies(walked, wsrc, 0));
+            next.insert(next.end(), walked.begin(), walked.end());
+        }
+        const double t_walk = now_s() - t0;
+
+        // post-reheat phase
+        t0 = now_s();
+        if (!rest3.empty()) {
+            std::vector<int32_t> wsrc;
+            for (int32_t i : rest3) {
+                wsrc.push_back(best_walker[i]);
+                reheat[i] = walk_gain[i];
+            }
+            BCHK(copies(rest3, wsrc, 1));
+            BCHK(upload_list(rest3));
+            launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, s);
+            BCHK(read_outs(rest3));
+            std::vector<int32_t> take1, improve1, rest4;
+            std::vector<int64_t> mv1, mvi;
+            for (size_t t = 0; t < rest3.size(); ++t) {
+                const int32_t i = rest3[t];
+                stats[i].single_evals += outs[t].evals;
+                if (outs[t].gain + reheat[i] > 0.0) {
+                    take1.push_back(i);
+                    mv1.push_back(outs[t].k);
+                    mv1.push_back(-1);
+                    stats[i].max_energy += scale * (reheat[i] + outs[t].gain);
+                    stats[i].reheat_accepts++;
+                } else {
+                    if (outs[t].gain > 0.0) {
+                        improve1.push_back(i);
+                        mvi.push_back(outs[t].k);
+                        mvi.push_back(-1);
+                        reheat[i] += outs[t].gain;
+                    }
+                    rest4.push_back(i);
+                }
+            }
+            if (!take1.empty()) {
+                std::vector<int32_t> rsrc(take1.size(), -1);
+                BCHK(copies(take1, rsrc, 0));
+                BCHK(flips(take1, mv1, 0));
+                next.insert(next.end(), take1.begin(), take1.end());
+            }
+            if (!improve1.empty()) BCHK(flips(improve1, mvi, 1));
+            if (!rest4.empty()) {
+                BCHK(upload_list(rest4));
+                launch_b_double(kind, d_inst, d_list, (int64_t)rest4.size(), 1, s);
+                BCHK(read_outs(rest4));
+                std::vector<int32_t> take2;
+                std::vector<int64_t> mv2;
+                for (size_t t = 0; t < rest4.size(); ++t) {
+                    const int32_t i = rest4[t];
+                    const int64_t n = h_inst[i].n, C = n * (n - 1) / 2;
+                    stats[i].double_evals += std::min(G, C);
+                    if (outs[t].gain + reheat[i] > 0.0) {
+                        take2.push_back(i);
+                        mv2.push_back(outs[t].k);
+                        mv2.push_back(outs[t].l);
+                        stats[i].max_energy += scale * (reheat[i] + outs[t].gain);
+                        stats[i].reheat_accepts++;
+                    }
+                }
+                if (!take2.empty()) {
+                    std::vector<int32_t> rsrc(take2.size(), -1);
+                    BCHK(copies(take2, rsrc, 0));
+                    BCHK(flips(take2, mv2, 0));
+                    next.insert(next.end(), take2.begin(), take2.end());
+                }
+            }
+        }
+        const double t_reheat = now_s() - t0;
+        const double share = 1.0 / (double)active.size();
+        for (int32_t i : active) {
+            stats[i].seconds_single += t_single * share;
+            stats[i].seconds_double += t_double * share;
+            stats[i].seconds_seeds += t_seeds * share;
+            stats[i].seconds_walk += t_walk * share;
+            stats[i].seconds_reheat += t_reheat * share;
+        }
+        std::sort(next.begin(), next.end());
+        active.swap(next);
+    }
+
+    // exact final scores and states
+    BCHK(score(list));
+    for (int64_t i = 0; i < count; ++i) {
+        const pqi_graph* g = graphs[i];
+        const double cut = kind == PQI_WEIGHTS_I8 ? (double)((2 * g->w_up_i - (long long)qv[i]) / 4)
+                                                  : (2.0 * g->w_up - qv[i]) / 4.0;
+        const double en = kind == PQI_WEIGHTS_I8 ? (double)(-(long long)qv[i] / 2) : -qv[i] / 2.0;
+        stats[i].cut_value = cut;
+        stats[i].min_energy = p->is_spin_glass ? -en : en;
+        BCHK(hipMemcpyAsync(thetas[i], h_inst[i].x, h_inst[i].n, hipMemcpyDeviceToHost, s));
+    }
+    BCHK(hipStreamSynchronize(s));
+    (void)hipFree(arena);
+    return PQI_OK;
+}
+
+}  // extern "C"

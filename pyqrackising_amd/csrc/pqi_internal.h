@@ -130,4 +130,44 @@ void launch_quad(const Graph& g, const uint8_t* x, const void* h, double* q, hip
 
 size_t field_bytes(const Graph& g);  // bytes per field entry (4 or 8)
 
+// ---- batched search: per-instance descriptors in device memory ----------------
+struct BInstRaw {
+    const void* W;  // dense weights (or nullptr)
+    int64_t ld, n;
+    const int32_t* ap;
+    const int32_t* ai;
+    const void* aw;
+    int32_t use_adj, pad;
+    uint8_t* x;  // best state and field
+    void* h;
+    uint8_t* rx;  // reheat state and field
+    void* rh;
+    uint8_t* wx;  // walkers: G x n states and fields
+    void* wh;
+    double* we;
+    int64_t* wev;
+    double* delta;  // seed_count seed gains
+    int32_t* rank;
+    int64_t* sel;   // G selected seeds
+    double* seld;
+    PhaseOut* out;
+    int64_t seed_count, bs, G, T;
+};
+
+// which = 0: the best state (x, h); 1: the reheat state (rx, rh)
+void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
+void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
+                      int which, hipStream_t s);
+void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s);
+void launch_b_flip(int kind, const BInstRaw* d, const int32_t* list, int64_t L, const int64_t* moves, int which,
+                   hipStream_t s);
+void launch_b_seeds(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_seeds, int64_t max_G,
+                    hipStream_t s);
+void launch_b_walk(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_G,
+                   int64_t T_override, int64_t G_override, hipStream_t s);
+// src[list position] = walker index, or -1 for the reheat state; dst 0 = best, 1 = reheat
+void launch_b_copy(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, const int32_t* src,
+                   int dst, hipStream_t s);
+void launch_b_quad(int kind, const BInstRaw* d, const int32_t* list, int64_t L, double* q, hipStream_t s);
+
 }  // namespace pqi

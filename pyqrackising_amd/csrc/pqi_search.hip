@@ -180,6 +180,15 @@ static void field_impl(const Graph& g, const uint8_t* x, void* h, hipStream_t s)
         }                                                          \
     } while (0)
 
+// Dynamic LDS above 64 KiB must be requested per kernel; gfx950 has 160 KiB
+// per work group.  Returns the limit that applies to `fn`.
+static size_t lds_limit(const void* fn) {
+    const int want = 160 * 1024;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess) return want;
+    (void)hipGetLastError();
+    return 64 * 1024;
+}
+
 void launch_field(const Graph& g, const uint8_t* x, void* h, hipStream_t s) {
     PQI_DISPATCH(g, field_impl, g, x, h, s);
 }
@@ -221,7 +230,7 @@ __global__ __launch_bounds__(1024) void k_score(const uint8_t* x, const FT* h, i
 }
 
 template <typename FT>
-__global__ __launch_bounds__(1024) void k_quad(const uint8_t* x, const FT* h, int64_t n, double* q) {
+__device__ void quad_body(const uint8_t* x, const FT* h, int64_t n, double* q) {
     using AT = typename AccOf<FT>::T;
     __shared__ AT part[16];
     AT acc = 0;
@@ -234,6 +243,11 @@ __global__ __launch_bounds__(1024) void k_quad(const uint8_t* x, const FT* h, in
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[i];
         q[0] = (double)t;
     }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(1024) void k_quad(const uint8_t* x, const FT* h, int64_t n, double* q) {
+    quad_body<FT>(x, h, n, q);
 }
 
 void launch_quad(const Graph& g, const uint8_t* x, const void* h, double* q, hipStream_t s) {
@@ -266,8 +280,7 @@ void launch_score_with(const Graph& g, const uint8_t* x, int64_t count, double* 
 // ---------------------------------------------------------------------------
 
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h, int64_t max_steps,
-                                                 PhaseOut* out) {
+__device__ void descent_body(const GView<WT>& g, uint8_t* x, FT* h, int64_t max_steps, PhaseOut* out) {
     __shared__ FT sv[16];
     __shared__ int64_t sk[16];
     const int tid = threadIdx.x;
@@ -302,14 +315,19 @@ __global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h
     }
 }
 
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h, int64_t max_steps,
+                                                 PhaseOut* out) {
+    descent_body<WT, FT>(g, x, h, max_steps, out);
+}
+
 // Single-wavefront variant.  LDS holds the gains D_v = s_v h_v themselves, so
 // the argmax scan is one independent LDS load per element (unrolled), and a
 // flip of k updates D_k -> -D_k and D_v -= 2 w_vk s_k s_v for its neighbours.
 // No work-group barriers, 32-bit indices; same selection rule and result.
 template <typename WT, typename FT>
-__global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
-                                                    PhaseOut* out) {
-    extern __shared__ __align__(16) unsigned char lds[];
+__device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64_t max_steps, PhaseOut* out,
+                                  unsigned char* lds) {
     const int lane = threadIdx.x;
     const int n = (int)g.n;
     FT* D = (FT*)lds;
@@ -390,11 +408,19 @@ __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, F
     }
 }
 
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
+                                                    PhaseOut* out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds);
+}
+
 template <typename WT>
 static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
     const size_t lds = (((size_t)g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
-    if (lds <= 64 * 1024)
+    static const size_t limit = lds_limit((const void*)k_descent_wave<WT, FT>);
+    if (lds <= limit)
         hipLaunchKernelGGL((k_descent_wave<WT, FT>), dim3(1), dim3(64), lds, s, view<WT>(g), x, (FT*)h, max_steps,
                            out);
     else
@@ -414,7 +440,8 @@ void launch_single_best(const Graph& g, const uint8_t* x, const void* h, PhaseOu
 // ---------------------------------------------------------------------------
 
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_flip(GView<WT> g, uint8_t* x, FT* h, int64_t k, int64_t l) {
+__device__ void flip_body(const GView<WT>& g, uint8_t* x, FT* h, int64_t k, int64_t l) {
+    if (k < 0) return;
     int s_old = x[k] ? -1 : 1;
     __syncthreads();
     flip_update(g, x, h, k, s_old, threadIdx.x, blockDim.x);
@@ -424,6 +451,11 @@ __global__ __launch_bounds__(1024) void k_flip(GView<WT> g, uint8_t* x, FT* h, i
         __syncthreads();
         flip_update(g, x, h, l, s_old, threadIdx.x, blockDim.x);
     }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_flip(GView<WT> g, uint8_t* x, FT* h, int64_t k, int64_t l) {
+    flip_body<WT, FT>(g, x, h, k, l);
 }
 
 template <typename WT>
@@ -465,8 +497,8 @@ __device__ __forceinline__ FT pair_gain(const GView<WT>& g, const uint8_t* x, co
 // Reference CPU semantics (D2): thread t of thread_count evaluates the last
 // pair index t + tb*floor((C-1-t)/tb) of its strided loop; argmax over t.
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_double_ref(GView<WT> g, const uint8_t* x, const FT* h,
-                                                    int64_t thread_count, PhaseOut* out) {
+__device__ void double_ref_body(const GView<WT>& g, const uint8_t* x, const FT* h, int64_t thread_count,
+                                PhaseOut* out) {
     __shared__ FT sv[16];
     __shared__ int64_t sk[16];
     const int64_t n = g.n, C = n * (n - 1) / 2;
@@ -491,6 +523,12 @@ __global__ __launch_bounds__(1024) void k_double_ref(GView<WT> g, const uint8_t*
         out->evals = thread_count < C ? thread_count : C;
         out->gain_sum = 0.0;
     }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_double_ref(GView<WT> g, const uint8_t* x, const FT* h,
+                                                    int64_t thread_count, PhaseOut* out) {
+    double_ref_body<WT, FT>(g, x, h, thread_count, out);
 }
 
 // Exhaustive: first maximum over all pairs in pair-index order (kernels.cl:186-231).
@@ -614,11 +652,8 @@ void launch_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs, int6
 // Cut difference of flipping the set F from (x, h):
 //   sum_{k in F} s_k h_k - 2 sum_{k<l in F} w_kl s_k s_l.
 template <typename WT, typename FT>
-__global__ __launch_bounds__(256) void k_seed_lowrank(GView<WT> g, const uint8_t* x, const FT* h, int64_t bs,
-                                                      int64_t count, double* delta) {
+__device__ double seed_lowrank_one(const GView<WT>& g, const uint8_t* x, const FT* h, int64_t bs, int64_t b) {
     using AT = typename AccOf<FT>::T;
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= count) return;
     const int64_t n = g.n;
     const uint64_t i = (uint64_t)(b % bs);
     const uint64_t gray = i ^ (i >> 1);
@@ -645,7 +680,14 @@ __global__ __launch_bounds__(256) void k_seed_lowrank(GView<WT> g, const uint8_t
             acc -= (AT)(2 * sk * sl) * (AT)weight_of(g, k, l);
         }
     }
-    delta[b] = (double)acc;
+    return (double)acc;
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void k_seed_lowrank(GView<WT> g, const uint8_t* x, const FT* h, int64_t bs,
+                                                      int64_t count, double* delta) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < count) delta[b] = seed_lowrank_one<WT, FT>(g, x, h, bs, b);
 }
 
 template <typename WT>
@@ -682,11 +724,11 @@ void launch_gather(const double* v, const int64_t* sel, int64_t k, double* out, 
 // Stable top-k by rank counting: rank(a) = #{b : v_b > v_a or (v_b == v_a and b < a)}.
 // Stage 1 splits the comparisons of each candidate over gridDim.y work groups
 // (integer atomics, so the count is exact); stage 2 scatters the first k.
-__global__ __launch_bounds__(256) void k_rank(const double* v, int64_t count, int64_t span, int32_t* rank) {
+__device__ void rank_body(const double* v, int64_t count, int64_t span, int32_t* rank, int64_t bx, int64_t by) {
     __shared__ double tile[256];
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t a = bx * blockDim.x + threadIdx.x;
     const double va = a < count ? v[a] : 0.0;
-    const int64_t lo = (int64_t)blockIdx.y * span;
+    const int64_t lo = by * span;
     const int64_t hi = lo + span < count ? lo + span : count;
     int32_t r = 0;
     for (int64_t t0 = lo; t0 < hi; t0 += 256) {
@@ -701,6 +743,10 @@ __global__ __launch_bounds__(256) void k_rank(const double* v, int64_t count, in
         }
     }
     if (a < count && r) atomicAdd(&rank[a], r);
+}
+
+__global__ __launch_bounds__(256) void k_rank(const double* v, int64_t count, int64_t span, int32_t* rank) {
+    rank_body(v, count, span, rank, blockIdx.x, blockIdx.y);
 }
 
 __global__ void k_rank_scatter(const int32_t* rank, int64_t count, int64_t k, int64_t* sel) {
@@ -738,12 +784,10 @@ void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, void* 
 // ---------------------------------------------------------------------------
 
 template <typename WT, typename FT>
-__global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0, const FT* h0, const int64_t* sel,
-                                                 const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx,
-                                                 FT* wh, double* we, int64_t* wevals, int use_lds) {
-    extern __shared__ __align__(16) unsigned char lds[];
+__device__ void gray_walk_body(const GView<WT>& g, const uint8_t* x0, const FT* h0, const int64_t* sel,
+                               const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx, FT* wh, double* we,
+                               int64_t* wevals, int use_lds, unsigned char* lds, int64_t w) {
     const int lane = threadIdx.x;
-    const int64_t w = blockIdx.x;
     const int64_t n = g.n;
     const int64_t nb = (n + 63) >> 6;
     uint8_t* gx = wx + w * n;
@@ -836,13 +880,22 @@ __global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0
     }
 }
 
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0, const FT* h0, const int64_t* sel,
+                                                 const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx,
+                                                 FT* wh, double* we, int64_t* wevals, int use_lds) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    gray_walk_body<WT, FT>(g, x0, h0, sel, sel_delta, bs, T, wx, wh, we, wevals, use_lds, lds, blockIdx.x);
+}
+
 template <typename WT>
 static void walk_impl(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel, const double* sel_delta,
                       int64_t bs, int64_t walkers, int64_t T, uint8_t* wx, void* wh, double* we, int64_t* wevals,
                       hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
     size_t lds = ((g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
-    int use_lds = lds <= 64 * 1024 ? 1 : 0;
+    static const size_t limit = lds_limit((const void*)k_gray_walk<WT, FT>);
+    int use_lds = lds <= limit ? 1 : 0;
     hipLaunchKernelGGL((k_gray_walk<WT, FT>), dim3((unsigned)walkers), dim3(64), use_lds ? lds : 0, s, view<WT>(g),
                        x, (const FT*)h, sel, sel_delta, bs, T, wx, (FT*)wh, we, wevals, use_lds);
 }
@@ -851,6 +904,243 @@ void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int
                       int64_t block_size, int64_t walkers, int64_t thread_iterations, uint8_t* wx, void* wh,
                       double* we, int64_t* wevals, hipStream_t s) {
     PQI_DISPATCH(g, walk_impl, g, x, h, sel, sel_delta, block_size, walkers, thread_iterations, wx, wh, we, wevals, s);
+}
+
+// ===========================================================================
+// Batched search: the same phase bodies over a list of instances, one work
+// group (or a row of work groups) per listed instance.  Per-instance pointers
+// come from BInstRaw descriptors in device memory.
+// ===========================================================================
+
+template <typename WT>
+__device__ __forceinline__ GView<WT> gview_of(const BInstRaw& r) {
+    GView<WT> v;
+    v.W = (const WT*)r.W;
+    v.ld = r.ld;
+    v.n = r.n;
+    v.ap = r.ap;
+    v.ai = r.ai;
+    v.aw = (const WT*)r.aw;
+    v.use_adj = r.use_adj;
+    return v;
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void kb_descent_lds(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
+                                                    int which) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    descent_wave_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out, lds);
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void kb_descent_glb(const BInstRaw* insts, const int32_t* list,
+                                                      int64_t max_steps, int which) {
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    descent_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out);
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void kb_double(const BInstRaw* insts, const int32_t* list, int which) {
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    double_ref_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (const FT*)(which ? r.rh : r.h), r.G, r.out);
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void kb_flip(const BInstRaw* insts, const int32_t* list, const int64_t* moves,
+                                               int which) {
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    flip_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), moves[2 * blockIdx.x],
+                      moves[2 * blockIdx.x + 1]);
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void kb_seed_lowrank(const BInstRaw* insts, const int32_t* list) {
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < r.seed_count) r.delta[b] = seed_lowrank_one<WT, FT>(gview_of<WT>(r), r.x, (const FT*)r.h, r.bs, b);
+    if (b < r.seed_count) r.rank[b] = 0;
+}
+
+__global__ __launch_bounds__(256) void kb_rank(const BInstRaw* insts, const int32_t* list, int64_t splits) {
+    const BInstRaw& r = insts[list[blockIdx.z]];
+    if ((int64_t)blockIdx.x * blockDim.x >= r.seed_count) return;  // whole work group idle
+    const int64_t span = (r.seed_count + splits - 1) / splits;
+    if ((int64_t)blockIdx.y * span >= r.seed_count) return;
+    rank_body(r.delta, r.seed_count, span, r.rank, blockIdx.x, blockIdx.y);
+}
+
+__global__ void kb_rank_scatter(const BInstRaw* insts, const int32_t* list) {
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < r.seed_count && r.rank[a] < r.G) r.sel[r.rank[a]] = a;
+}
+
+__global__ void kb_gather(const BInstRaw* insts, const int32_t* list) {
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < r.G) r.seld[i] = r.delta[r.sel[i]];
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void kb_walk(const BInstRaw* insts, const int32_t* list, int64_t T_override,
+                                             int64_t G_override, int use_lds) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const int64_t G = G_override > 0 ? G_override : r.G;
+    if ((int64_t)blockIdx.x >= G) return;
+    const int64_t T = T_override >= 0 ? T_override : r.T;
+    gray_walk_body<WT, FT>(gview_of<WT>(r), r.x, (const FT*)r.h, r.sel, r.seld, r.bs, T, r.wx, (FT*)r.wh, r.we,
+                           r.wev, use_lds, lds, blockIdx.x);
+}
+
+// copy a state: src >= 0 -> walker src, src == -1 -> reheat state; dst 0 -> best, 1 -> reheat
+template <typename FT>
+__global__ __launch_bounds__(256) void kb_copy(const BInstRaw* insts, const int32_t* list, const int32_t* src,
+                                              int dst) {
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const int sidx = src[blockIdx.y];
+    const uint8_t* sx = sidx >= 0 ? r.wx + (int64_t)sidx * r.n : r.rx;
+    const FT* sh = sidx >= 0 ? (const FT*)r.wh + (int64_t)sidx * r.n : (const FT*)r.rh;
+    uint8_t* dx = dst ? r.rx : r.x;
+    FT* dh = (FT*)(dst ? r.rh : r.h);
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < r.n; v += (int64_t)gridDim.x * blockDim.x) {
+        dx[v] = sx[v];
+        dh[v] = sh[v];
+    }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(1024) void kb_quad(const BInstRaw* insts, const int32_t* list, double* q) {
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    quad_body<FT>(r.x, (const FT*)r.h, r.n, q + blockIdx.x);
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void kb_field(const BInstRaw* insts, const int32_t* list) {
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const GView<WT> g = gview_of<WT>(r);
+    FT* h = (FT*)r.h;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < r.n;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        FT acc = 0;
+        if (g.use_adj) {
+            for (int32_t e = g.ap[row]; e < g.ap[row + 1]; ++e) acc += r.x[g.ai[e]] ? -(FT)g.aw[e] : (FT)g.aw[e];
+        } else {
+            const WT* Wr = g.W + row * g.ld;
+            for (int64_t u = 0; u < r.n; ++u) acc += r.x[u] ? -(FT)Wr[u] : (FT)Wr[u];
+        }
+        h[row] = acc;
+    }
+}
+
+template <typename WT>
+static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((kb_field<WT, FT>), dim3((unsigned)std::min<int64_t>(64, (max_n + 255) / 256), (unsigned)L),
+                       dim3(256), 0, s, d, list);
+}
+
+template <typename WT>
+static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
+                           int which, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
+    static const size_t limit = lds_limit((const void*)kb_descent_lds<WT, FT>);
+    if (lds <= limit)
+        hipLaunchKernelGGL((kb_descent_lds<WT, FT>), dim3((unsigned)L), dim3(64), lds, s, d, list, max_steps, which);
+    else
+        hipLaunchKernelGGL((kb_descent_glb<WT, FT>), dim3((unsigned)L), dim3(1024), 0, s, d, list, max_steps, which);
+}
+
+template <typename WT>
+static void b_double_impl(const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((kb_double<WT, FT>), dim3((unsigned)L), dim3(1024), 0, s, d, list, which);
+}
+
+template <typename WT>
+static void b_flip_impl(const BInstRaw* d, const int32_t* list, int64_t L, const int64_t* moves, int which,
+                        hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((kb_flip<WT, FT>), dim3((unsigned)L), dim3(1024), 0, s, d, list, moves, which);
+}
+
+template <typename WT>
+static void b_seeds_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_seeds, int64_t max_G,
+                         hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    const unsigned gx = (unsigned)((max_seeds + 255) / 256);
+    hipLaunchKernelGGL((kb_seed_lowrank<WT, FT>), dim3(gx, (unsigned)L), dim3(256), 0, s, d, list);
+    int64_t splits = std::min<int64_t>(64, (max_seeds + 511) / 512);
+    hipLaunchKernelGGL(kb_rank, dim3(gx, (unsigned)splits, (unsigned)L), dim3(256), 0, s, d, list, splits);
+    hipLaunchKernelGGL(kb_rank_scatter, dim3(gx, (unsigned)L), dim3(256), 0, s, d, list);
+    hipLaunchKernelGGL(kb_gather, dim3((unsigned)((max_G + 255) / 256), (unsigned)L), dim3(256), 0, s, d, list);
+}
+
+template <typename WT>
+static void b_walk_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_G,
+                        int64_t T_override, int64_t G_override, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
+    static const size_t limit = lds_limit((const void*)kb_walk<WT, FT>);
+    const int use_lds = lds <= limit ? 1 : 0;
+    const int64_t gx = G_override > 0 ? G_override : max_G;
+    hipLaunchKernelGGL((kb_walk<WT, FT>), dim3((unsigned)gx, (unsigned)L), dim3(64), use_lds ? lds : 0, s, d, list,
+                       T_override, G_override, use_lds);
+}
+
+template <typename WT>
+static void b_copy_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, const int32_t* src, int dst,
+                        hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((kb_copy<FT>), dim3((unsigned)std::min<int64_t>(64, (max_n + 255) / 256), (unsigned)L),
+                       dim3(256), 0, s, d, list, src, dst);
+}
+
+template <typename WT>
+static void b_quad_impl(const BInstRaw* d, const int32_t* list, int64_t L, double* q, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((kb_quad<FT>), dim3((unsigned)L), dim3(1024), 0, s, d, list, q);
+}
+
+#define PQI_KIND_DISPATCH(kind, FN, ...)                            \
+    do {                                                            \
+        switch (kind) {                                             \
+            case PQI_WEIGHTS_I8: FN<int8_t>(__VA_ARGS__); break;    \
+            case PQI_WEIGHTS_F32: FN<float>(__VA_ARGS__); break;    \
+            default: FN<double>(__VA_ARGS__); break;                \
+        }                                                           \
+    } while (0)
+
+void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_field_impl, d, list, L, max_n, s);
+}
+void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
+                      int which, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, s);
+}
+void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_double_impl, d, list, L, which, s);
+}
+void launch_b_flip(int kind, const BInstRaw* d, const int32_t* list, int64_t L, const int64_t* moves, int which,
+                   hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_flip_impl, d, list, L, moves, which, s);
+}
+void launch_b_seeds(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_seeds, int64_t max_G,
+                    hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_seeds_impl, d, list, L, max_seeds, max_G, s);
+}
+void launch_b_walk(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_G,
+                   int64_t T_override, int64_t G_override, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_walk_impl, d, list, L, max_n, max_G, T_override, G_override, s);
+}
+void launch_b_copy(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, const int32_t* src,
+                   int dst, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_copy_impl, d, list, L, max_n, src, dst, s);
+}
+void launch_b_quad(int kind, const BInstRaw* d, const int32_t* list, int64_t L, double* q, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_quad_impl, d, list, L, q, s);
 }
 
 }  // namespace pqi

@@ -481,3 +481,58 @@ def test_batch_eval_c3_shape(ctx, orc):
         X = ((bits[i][:4, :, None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(4, -1)[:, :n].astype(bool)
         _, ov = orc.eval_batch(Ws[i].astype(np.float64), X, 1)
         assert np.array_equal(vals[i][:4], ov), i
+
+
+# ---------------------------------------------------------------------------
+# batched local search
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind", ["int", "real"])
+def test_batch_local_search_matches_single(ctx, orc, kind):
+    """Each instance of the batched search makes exactly the moves of the
+    single-instance search (and of the oracle)."""
+    log(f"batch local search kind={kind}")
+    rng = np.random.default_rng(61)
+    Ws, x0s = [], []
+    for t in range(7):
+        n = int(rng.integers(40, 260))
+        if kind == "int":
+            W = er_weights(n, 4.0 / n, 100 + t, "int")
+        else:
+            W = er_weights(n, 0.1, 100 + t, "real")
+        Ws.append(W)
+        x0s.append(rng.integers(0, 2, n).astype(np.uint8))
+    dgs = [_lib.DeviceGraph.dense(W) for W in Ws]
+    p = _lib.make_params(0, is_spin_glass=1, gray_iterations=1500, walkers=6, seed_multiple=3)
+    states, stats = _lib.batch_local_search(dgs, x0s, p)
+    for i, (dg, W, x0) in enumerate(zip(dgs, Ws, x0s)):
+        xs, st = dg.local_search(x0, _lib.make_params(len(x0), is_spin_glass=1, gray_iterations=1500, walkers=6,
+                                                      seed_multiple=3))
+        assert np.array_equal(states[i], xs), i
+        for key in ("single_accepts", "double_accepts", "seed_accepts", "walk_accepts", "reheat_accepts",
+                    "gray_evals", "rounds"):
+            assert stats[i][key] == st[key], (i, key)
+        assert stats[i]["cut_value"] == st["cut_value"] and stats[i]["min_energy"] == st["min_energy"]
+        if i < 2:
+            xc, r = orc.dense_local_search(W, x0, is_spin_glass=1, walkers=6, seed_multiple=3, gray_iterations=1500)
+            assert np.array_equal(states[i], xc)
+
+
+def test_batch_local_search_csr_default_iterations(ctx):
+    """CSR instances, gray_iterations = n^2 per instance (negative value)."""
+    log("batch local search csr")
+    rng = np.random.default_rng(62)
+    Gs, x0s = [], []
+    for t in range(5):
+        n = int(rng.integers(64, 400))
+        W = er_weights(n, 6.0 / n, 200 + t, "real")
+        Gs.append(upper(W))
+        x0s.append(rng.integers(0, 2, n).astype(np.uint8))
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    p = _lib.make_params(0, is_spin_glass=1, walkers=8, seed_multiple=2)
+    p.gray_iterations = -1
+    states, stats = _lib.batch_local_search(dgs, x0s, p)
+    for i, (dg, x0) in enumerate(zip(dgs, x0s)):
+        xs, st = dg.local_search(x0, _lib.make_params(len(x0), is_spin_glass=1, walkers=8, seed_multiple=2))
+        assert np.array_equal(states[i], xs), i
+        assert stats[i]["cut_value"] == st["cut_value"]
