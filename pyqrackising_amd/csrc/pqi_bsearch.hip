@@ -418,7 +418,9 @@ int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, 
         active.swap(next);
     }
 
-    // exact final scores and states
+    // exact final scores (field recomputed from the state, as pqi_score does) and states
+    BCHK(upload_list(list));
+    launch_b_field(kind, d_inst, d_list, count, max_n, s);
     BCHK(score(list));
     for (int64_t i = 0; i < count; ++i) {
         const pqi_graph* g = graphs[i];

@@ -1016,29 +1016,36 @@ __global__ __launch_bounds__(1024) void kb_quad(const BInstRaw* insts, const int
     quad_body<FT>(r.x, (const FT*)r.h, r.n, q + blockIdx.x);
 }
 
+// Same summation order as k_field_dense / k_field_adj, so every instance's
+// field is bit-identical to the single-instance path.
 template <typename WT, typename FT>
 __global__ __launch_bounds__(256) void kb_field(const BInstRaw* insts, const int32_t* list) {
     const BInstRaw& r = insts[list[blockIdx.y]];
     const GView<WT> g = gview_of<WT>(r);
     FT* h = (FT*)r.h;
-    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < r.n;
-         row += (int64_t)gridDim.x * blockDim.x) {
+    if (g.use_adj) {
+        const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (row >= r.n) return;
         FT acc = 0;
-        if (g.use_adj) {
-            for (int32_t e = g.ap[row]; e < g.ap[row + 1]; ++e) acc += r.x[g.ai[e]] ? -(FT)g.aw[e] : (FT)g.aw[e];
-        } else {
-            const WT* Wr = g.W + row * g.ld;
-            for (int64_t u = 0; u < r.n; ++u) acc += r.x[u] ? -(FT)Wr[u] : (FT)Wr[u];
-        }
+        for (int32_t e = g.ap[row]; e < g.ap[row + 1]; ++e) acc += r.x[g.ai[e]] ? -(FT)g.aw[e] : (FT)g.aw[e];
         h[row] = acc;
+    } else {
+        const int lane = threadIdx.x & 63;
+        for (int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < r.n; row += (int64_t)gridDim.x * 4) {
+            const WT* Wr = g.W + row * g.ld;
+            FT acc = 0;
+            for (int64_t u = lane; u < r.n; u += 64) acc += r.x[u] ? -(FT)Wr[u] : (FT)Wr[u];
+            acc = wave_sum(acc);
+            if (lane == 0) h[row] = acc;
+        }
     }
 }
 
 template <typename WT>
 static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
-    hipLaunchKernelGGL((kb_field<WT, FT>), dim3((unsigned)std::min<int64_t>(64, (max_n + 255) / 256), (unsigned)L),
-                       dim3(256), 0, s, d, list);
+    hipLaunchKernelGGL((kb_field<WT, FT>), dim3((unsigned)((max_n + 3) / 4), (unsigned)L), dim3(256), 0, s, d,
+                       list);
 }
 
 template <typename WT>
