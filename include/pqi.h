@@ -72,6 +72,7 @@ extern "C" {
 typedef struct pqi_ctx pqi_ctx;
 typedef struct pqi_graph pqi_graph;
 typedef struct pqi_batch pqi_batch;
+typedef struct pqi_mpnn pqi_mpnn;
 
 typedef struct {
     int64_t n;
@@ -206,6 +207,24 @@ PQI_API int pqi_batch_info(const pqi_batch* b, int64_t* ntiles, int64_t* algorit
  * instance.  Only the reference double-flip mode. */
 PQI_API int pqi_batch_local_search(pqi_ctx* ctx, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
                                    const pqi_search_params* p, pqi_search_stats* stats);
+
+/* ---- flip-proposal network --------------------------------------------------
+ * No counterpart in the reference: a 3-layer message-passing net (hidden 64)
+ * that scores which spins to flip; every proposal it makes is checked by the
+ * exact gain kernels.  Parameters are FP32 masters; layer transforms run on
+ * BF16 MFMA (precision 1, default) or FP32 (precision 0).  Features per node:
+ * (gain / strength, degree / 16, spin, min(age, 64) / 64). */
+PQI_API int pqi_mpnn_create(pqi_ctx* ctx, uint64_t seed, pqi_mpnn** out);
+PQI_API int pqi_mpnn_free(pqi_mpnn* m);
+PQI_API int pqi_mpnn_num_params(void);
+PQI_API int pqi_mpnn_set_precision(pqi_mpnn* m, int precision);
+PQI_API int pqi_mpnn_get_params(pqi_mpnn* m, float* out);
+PQI_API int pqi_mpnn_set_params(pqi_mpnn* m, const float* in);
+/* Concatenate the graphs' adjacencies into one block-diagonal batch. */
+PQI_API int pqi_mpnn_bind(pqi_mpnn* m, pqi_graph* const* graphs, int64_t count, int64_t* total_nodes);
+PQI_API int pqi_mpnn_forward(pqi_mpnn* m, const float* feats, float* scores);
+PQI_API int pqi_mpnn_loss_grad(pqi_mpnn* m, const float* feats, const float* labels, double* loss, float* grad);
+PQI_API int pqi_mpnn_train_step(pqi_mpnn* m, const float* feats, const float* labels, float lr, double* loss);
 
 /* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
 PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);

@@ -120,6 +120,16 @@ SIGNATURES = {
     "pqi_batch_set_variant": (_int, [_vp, _int]),
     "pqi_batch_info": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pqi_batch_local_search": (_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "pqi_mpnn_create": (_int, [_vp, ctypes.c_uint64, _vp]),
+    "pqi_mpnn_free": (_int, [_vp]),
+    "pqi_mpnn_num_params": (_int, []),
+    "pqi_mpnn_set_precision": (_int, [_vp, _int]),
+    "pqi_mpnn_get_params": (_int, [_vp, _vp]),
+    "pqi_mpnn_set_params": (_int, [_vp, _vp]),
+    "pqi_mpnn_bind": (_int, [_vp, _vp, _i64, _vp]),
+    "pqi_mpnn_forward": (_int, [_vp, _vp, _vp]),
+    "pqi_mpnn_loss_grad": (_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pqi_mpnn_train_step": (_int, [_vp, _vp, _vp, ctypes.c_float, _vp]),
 }
 
 _lib = None
@@ -408,6 +418,76 @@ def batch_local_search(graphs, thetas, params):
     with ctx.lock:
         check(load().pqi_batch_local_search(ctx.handle, garr, k, tarr, ctypes.byref(params), st))
     return states, [s.as_dict() for s in st]
+
+
+class Mpnn:
+    """The flip-proposal network (pqi_mpnn_*)."""
+
+    def __init__(self, seed=0, ctx=None, precision=1):
+        self.ctx = ctx or default_context()
+        self._h = ctypes.c_void_p()
+        check(load().pqi_mpnn_create(self.ctx.handle, ctypes.c_uint64(seed), ctypes.byref(self._h)))
+        self.set_precision(precision)
+        self.n_params = load().pqi_mpnn_num_params()
+        self.N = 0
+
+    def set_precision(self, precision):
+        check(load().pqi_mpnn_set_precision(self._h, int(precision)))
+
+    def params(self):
+        out = np.empty(self.n_params, dtype=np.float32)
+        check(load().pqi_mpnn_get_params(self._h, _p(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        check(load().pqi_mpnn_set_params(self._h, _p(p)))
+
+    def bind(self, graphs):
+        graphs = list(graphs)
+        arr = (ctypes.c_void_p * len(graphs))(*[g._h.value for g in graphs])
+        N = ctypes.c_int64()
+        with self.ctx.lock:
+            check(load().pqi_mpnn_bind(self._h, arr, len(graphs), ctypes.byref(N)))
+        self.N = N.value
+        return self.N
+
+    def forward(self, feats):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        assert f.shape == (self.N, 4)
+        out = np.empty(self.N, dtype=np.float32)
+        with self.ctx.lock:
+            check(load().pqi_mpnn_forward(self._h, _p(f), _p(out)))
+        return out
+
+    def loss_grad(self, feats, labels):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_mpnn_loss_grad(self._h, _p(f), _p(y), ctypes.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train_step(self, feats, labels, lr=1e-3):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        loss = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_mpnn_train_step(self._h, _p(f), _p(y), ctypes.c_float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            load().pqi_mpnn_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def make_params(n, is_spin_glass=True, gray_iterations=None, walkers=None, seed_multiple=None,
