@@ -112,6 +112,9 @@ typedef struct {
     /* with PQI_FLAG_TIME_KERNELS: summed event-timed kernel durations and launch counts */
     double walk_kernel_seconds, seed_kernel_seconds;
     int64_t walk_launches, seed_launches;
+    /* learned phase (pqi_learned_search) */
+    int64_t learned_accepts, learned_passes;
+    double seconds_learned;
 } pqi_search_stats;
 
 /* Thread-local message for the last failing call on this thread. */
@@ -225,6 +228,15 @@ PQI_API int pqi_mpnn_bind(pqi_mpnn* m, pqi_graph* const* graphs, int64_t count, 
 PQI_API int pqi_mpnn_forward(pqi_mpnn* m, const float* feats, float* scores);
 PQI_API int pqi_mpnn_loss_grad(pqi_mpnn* m, const float* feats, const float* labels, double* loss, float* grad);
 PQI_API int pqi_mpnn_train_step(pqi_mpnn* m, const float* feats, const float* labels, float lr, double* loss);
+
+/* Batched search with learned proposals: up to `passes` rounds of network
+ * scores -> score order -> one first-improvement pass with exact gains, then
+ * the loop of pqi_batch_local_search.  The network must be bound to exactly
+ * these graphs, in this order.  targets (optional, count values: energy in
+ * spin-glass mode, cut otherwise): an instance stops once it reaches its own. */
+PQI_API int pqi_learned_search(pqi_ctx* ctx, pqi_mpnn* net, pqi_graph* const* graphs, int64_t count,
+                               uint8_t* const* thetas, const pqi_search_params* p, const double* targets,
+                               int64_t passes, pqi_search_stats* stats);
 
 /* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
 PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);

@@ -88,6 +88,9 @@ class SearchStats(ctypes.Structure):
         ("seed_kernel_seconds", ctypes.c_double),
         ("walk_launches", ctypes.c_int64),
         ("seed_launches", ctypes.c_int64),
+        ("learned_accepts", ctypes.c_int64),
+        ("learned_passes", ctypes.c_int64),
+        ("seconds_learned", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -130,6 +133,7 @@ SIGNATURES = {
     "pqi_mpnn_forward": (_int, [_vp, _vp, _vp]),
     "pqi_mpnn_loss_grad": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "pqi_mpnn_train_step": (_int, [_vp, _vp, _vp, ctypes.c_float, _vp]),
+    "pqi_learned_search": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
 }
 
 _lib = None
@@ -488,6 +492,22 @@ class Mpnn:
             self.close()
         except Exception:
             pass
+
+
+def learned_search(net, graphs, thetas, params, targets=None, passes=8):
+    """pqi_learned_search: returns (states, list of stats dicts)."""
+    graphs = list(graphs)
+    ctx = graphs[0].ctx
+    k = len(graphs)
+    states = [np.ascontiguousarray(t, dtype=np.uint8).copy() for t in thetas]
+    garr = (ctypes.c_void_p * k)(*[g._h.value for g in graphs])
+    tarr = (ctypes.c_void_p * k)(*[t.ctypes.data for t in states])
+    tg = None if targets is None else np.ascontiguousarray(targets, dtype=np.float64)
+    st = (SearchStats * k)()
+    with ctx.lock:
+        check(load().pqi_learned_search(ctx.handle, net._h, garr, k, tarr, ctypes.byref(params),
+                                        None if tg is None else _p(tg), int(passes), st))
+    return states, [s.as_dict() for s in st]
 
 
 def make_params(n, is_spin_glass=True, gray_iterations=None, walkers=None, seed_multiple=None,

@@ -61,10 +61,16 @@ using namespace pqi;
         }                                                                           \
     } while (0)
 
-extern "C" {
+namespace pqi {
 
-int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
-                           const pqi_search_params* p, pqi_search_stats* stats) {
+// The batched loop.  With `net`, a learned phase runs first: up to `passes`
+// rounds of (features, network scores, score order, one first-improvement pass
+// with exact gains), stopping early when no instance flips.  With `targets`,
+// an instance stops as soon as its value (energy in spin-glass mode, cut
+// otherwise) reaches its target.
+static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
+                       const pqi_search_params* p, pqi_search_stats* stats, pqi_mpnn* net, const double* targets,
+                       int64_t passes) {
     if (!c || !graphs || !thetas || !p || !stats) return fail(PQI_ERR_INVALID, "null argument");
     if (count < 1) return fail(PQI_ERR_INVALID, "count must be >= 1");
     if (p->walkers < 1 || p->seed_multiple < 1) return fail(PQI_ERR_INVALID, "walkers and seed_multiple must be >= 1");
@@ -115,6 +121,16 @@ int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, 
         off_x[i] = total;
         total += align256(n) * 2 + align256(n * fb) * 2 + align256(G * n) + align256(G * n * fb) + align256(G * 8) * 4 +
                  align256(seeds * 8) + align256(seeds * 4) + align256(sizeof(PhaseOut));
+        if (net) total += align256(n * 4) + align256(n);
+    }
+    if (net) {
+        int64_t off = 0;
+        for (int64_t i = 0; i < count; ++i) {
+            if (!graphs[i]->adj_ptr) return fail(PQI_ERR_UNSUPPORTED, "learned proposals need adjacency lists");
+            if (mpnn_node_offset(net, i) != off) return fail(PQI_ERR_INVALID, "the network is bound to other graphs");
+            off += graphs[i]->n;
+        }
+        if (mpnn_total_nodes(net) != off) return fail(PQI_ERR_INVALID, "the network is bound to other graphs");
     }
     const int64_t L0 = count;
     const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
@@ -143,6 +159,12 @@ int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, 
         r.delta = (double*)take(r.seed_count * 8);
         r.rank = (int32_t*)take(r.seed_count * 4);
         r.out = (PhaseOut*)take(sizeof(PhaseOut));
+        if (net) {
+            r.perm = (int32_t*)take(n * 4);
+            r.age = (uint8_t*)take(n);
+            r.feat = mpnn_features(net) + mpnn_node_offset(net, i) * 4;
+            r.score = mpnn_scores(net) + mpnn_node_offset(net, i);
+        }
     }
     char* m = a + total;
     int32_t* d_list = (int32_t*)m;
@@ -209,7 +231,43 @@ int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, 
         stats[i].max_energy = p->is_spin_glass ? en : cut;
     }
 
-    std::vector<int32_t> active = list;
+    auto reached = [&](int32_t i) { return targets && stats[i].max_energy >= targets[i]; };
+
+    // learned phase
+    if (net && passes > 0) {
+        const double t0 = now_s();
+        std::vector<uint8_t> zeros(max_n, 64);  // every node starts "long ago flipped"
+        for (int64_t i = 0; i < count; ++i)
+            BCHK(hipMemcpyAsync(h_inst[i].age, zeros.data(), h_inst[i].n, hipMemcpyHostToDevice, s));
+        BCHK(hipStreamSynchronize(s));
+        std::vector<int32_t> run;
+        for (int32_t i : list)
+            if (!reached(i)) run.push_back(i);
+        for (int64_t pass = 0; pass < passes && !run.empty(); ++pass) {
+            BCHK(upload_list(list));
+            launch_b_features(kind, d_inst, d_list, count, max_n, s);
+            mpnn_forward_device(net, s);
+            BCHK(upload_list(run));
+            launch_b_order(d_inst, d_list, (int64_t)run.size(), s);
+            launch_b_learned_pass(kind, d_inst, d_list, (int64_t)run.size(), max_n, s);
+            BCHK(read_outs(run));
+            std::vector<int32_t> again;
+            for (size_t t = 0; t < run.size(); ++t) {
+                pqi_search_stats& st = stats[run[t]];
+                st.learned_accepts += outs[t].accepts;
+                st.learned_passes++;
+                st.max_energy += scale * outs[t].gain_sum;
+                if (outs[t].accepts > 0 && !reached(run[t])) again.push_back(run[t]);
+            }
+            run.swap(again);
+        }
+        const double dt = now_s() - t0;
+        for (int64_t i = 0; i < count; ++i) stats[i].seconds_learned += dt / (double)count;
+    }
+
+    std::vector<int32_t> active;
+    for (int32_t i : list)
+        if (!reached(i)) active.push_back(i);
     std::vector<double> walk_gain(count), reheat(count);
     std::vector<double> we(G);
     std::vector<int64_t> wev(G);
@@ -415,7 +473,10 @@ int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, 
             stats[i].seconds_reheat += t_reheat * share;
         }
         std::sort(next.begin(), next.end());
-        active.swap(next);
+        std::vector<int32_t> still;
+        for (int32_t i : next)
+            if (!reached(i)) still.push_back(i);
+        active.swap(still);
     }
 
     // exact final scores (field recomputed from the state, as pqi_score does) and states
@@ -434,6 +495,22 @@ int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, 
     BCHK(hipStreamSynchronize(s));
     (void)hipFree(arena);
     return PQI_OK;
+}
+
+}  // namespace pqi
+
+extern "C" {
+
+int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
+                           const pqi_search_params* p, pqi_search_stats* stats) {
+    return run_batched(c, graphs, count, thetas, p, stats, nullptr, nullptr, 0);
+}
+
+int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
+                       const pqi_search_params* p, const double* targets, int64_t passes, pqi_search_stats* stats) {
+    if (!net) return fail(PQI_ERR_INVALID, "null network");
+    if (passes < 0) return fail(PQI_ERR_INVALID, "passes must be >= 0");
+    return run_batched(c, graphs, count, thetas, p, stats, net, targets, passes);
 }
 
 }  // extern "C"

@@ -51,6 +51,7 @@ struct pqi_ctx {
 };
 
 struct pqi_graph : pqi::Graph {};
+struct pqi_mpnn;
 
 namespace pqi {
 
@@ -152,6 +153,11 @@ struct BInstRaw {
     double* seld;
     PhaseOut* out;
     int64_t seed_count, bs, G, T;
+    // learned proposals (nullptr when unused)
+    float* feat;         // n x 4 features, a slice of the network's batch input
+    const float* score;  // n scores, a slice of the network's batch output
+    int32_t* perm;       // n: nodes in descending score order
+    uint8_t* age;        // n: passes since each node last flipped (saturating)
 };
 
 // which = 0: the best state (x, h); 1: the reheat state (rx, rh)
@@ -169,5 +175,17 @@ void launch_b_walk(int kind, const BInstRaw* d, const int32_t* list, int64_t L, 
 void launch_b_copy(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, const int32_t* src,
                    int dst, hipStream_t s);
 void launch_b_quad(int kind, const BInstRaw* d, const int32_t* list, int64_t L, double* q, hipStream_t s);
+// learned proposals: node features from (x, h, age); score order; one first-improvement
+// pass over the nodes in that order with exact gains (out->accepts, out->gain_sum)
+void launch_b_features(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
+void launch_b_order(const BInstRaw* d, const int32_t* list, int64_t L, hipStream_t s);
+void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
+
+// network internals used by the learned search (pqi_mpnn.hip)
+float* mpnn_features(pqi_mpnn* m);
+const float* mpnn_scores(pqi_mpnn* m);
+int64_t mpnn_node_offset(pqi_mpnn* m, int64_t i);
+int64_t mpnn_total_nodes(pqi_mpnn* m);
+void mpnn_forward_device(pqi_mpnn* m, hipStream_t s);
 
 }  // namespace pqi

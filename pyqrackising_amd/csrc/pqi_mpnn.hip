@@ -383,6 +383,16 @@ static int upload_inputs(pqi_mpnn* m, const float* feats, const float* labels) {
 
 }  // namespace pqi
 
+namespace pqi {
+float* mpnn_features(pqi_mpnn* m) { return m->X0; }
+const float* mpnn_scores(pqi_mpnn* m) { return m->score; }
+int64_t mpnn_node_offset(pqi_mpnn* m, int64_t i) {
+    return i < (int64_t)m->node_off.size() ? m->node_off[i] : -1;
+}
+int64_t mpnn_total_nodes(pqi_mpnn* m) { return m->N; }
+void mpnn_forward_device(pqi_mpnn* m, hipStream_t s) { forward(m, s); }
+}  // namespace pqi
+
 using namespace pqi;
 
 extern "C" {

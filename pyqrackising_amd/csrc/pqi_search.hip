@@ -1150,4 +1150,173 @@ void launch_b_quad(int kind, const BInstRaw* d, const int32_t* list, int64_t L, 
     PQI_KIND_DISPATCH(kind, b_quad_impl, d, list, L, q, s);
 }
 
+// ---------------------------------------------------------------------------
+// Learned proposals
+// ---------------------------------------------------------------------------
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void kb_features(const BInstRaw* insts, const int32_t* list) {
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= r.n) return;
+    const FT* h = (const FT*)r.h;
+    const int sv = r.x[v] ? -1 : 1;
+    const double d = (double)sv * (double)h[v];
+    double str = 0.0;
+    const WT* aw = (const WT*)r.aw;
+    for (int32_t e = r.ap[v]; e < r.ap[v + 1]; ++e) str += fabs((double)aw[e]);
+    float* f = r.feat + v * 4;
+    f[0] = str > 0.0 ? (float)(d / str) : 0.f;
+    f[1] = (float)(r.ap[v + 1] - r.ap[v]) * (1.f / 16.f);
+    f[2] = (float)sv;
+    f[3] = (float)(r.age[v] < 64 ? r.age[v] : 64) * (1.f / 64.f);
+}
+
+// Stable counting sort into 256 score buckets, highest scores first; within a
+// bucket nodes keep index order.  One work group per instance.
+__global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int32_t* list) {
+    __shared__ float lo_s[256], hi_s[256];
+    __shared__ int32_t cnt[256];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    const int t = threadIdx.x;
+    float lo = INFINITY, hi = -INFINITY;
+    for (int64_t v = t; v < r.n; v += 256) {
+        const float sc = r.score[v];
+        lo = fminf(lo, sc);
+        hi = fmaxf(hi, sc);
+    }
+    lo_s[t] = lo;
+    hi_s[t] = hi;
+    cnt[t] = 0;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if (t < m) {
+            lo_s[t] = fminf(lo_s[t], lo_s[t + m]);
+            hi_s[t] = fmaxf(hi_s[t], hi_s[t + m]);
+        }
+        __syncthreads();
+    }
+    lo = lo_s[0];
+    hi = hi_s[0];
+    const float span = hi > lo ? hi - lo : 1.f;
+    auto bucket = [&](float sc) {
+        int b = (int)((hi - sc) / span * 255.f);
+        return b < 0 ? 0 : (b > 255 ? 255 : b);
+    };
+    for (int64_t v = t; v < r.n; v += 256) atomicAdd(&cnt[bucket(r.score[v])], 1);
+    __syncthreads();
+    if (t == 0) {
+        int32_t run = 0;
+        for (int b = 0; b < 256; ++b) {
+            const int32_t c = cnt[b];
+            cnt[b] = run;
+            run += c;
+        }
+        for (int64_t v = 0; v < r.n; ++v) r.perm[cnt[bucket(r.score[v])]++] = (int32_t)v;
+    }
+}
+
+// One first-improvement pass in score order: nodes are visited in perm order
+// and flipped iff their exact gain is positive at that moment.  64 consecutive
+// candidates are evaluated at once (one per lane); a ballot finds the first
+// positive one, it is applied, and the later lanes re-evaluate.
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, const int32_t* list, int use_lds) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    const GView<WT> g = gview_of<WT>(r);
+    const int lane = threadIdx.x;
+    const int64_t n = r.n;
+    FT* hg = (FT*)r.h;
+    FT* D = use_lds ? (FT*)lds : hg;  // gains (LDS) or the field itself (global)
+    uint8_t* x = use_lds ? lds + (((size_t)n * sizeof(FT) + 15) & ~(size_t)15) : r.x;
+    if (use_lds) {
+        for (int64_t v = lane; v < n; v += 64) {
+            const uint8_t xv = r.x[v];
+            x[v] = xv;
+            D[v] = xv ? -hg[v] : hg[v];
+        }
+    }
+    __syncthreads();
+    auto gain = [&](int64_t v) -> FT { return use_lds ? D[v] : (x[v] ? -hg[v] : hg[v]); };
+    int64_t flips = 0;
+    double gsum = 0.0;
+    for (int64_t c0 = 0; c0 < n; c0 += 64) {
+        const bool valid = c0 + lane < n;
+        const int64_t v = valid ? r.perm[c0 + lane] : 0;
+        FT d = valid ? gain(v) : (FT)0;
+        uint64_t mask = __ballot(valid && d > 0);
+        while (mask) {
+            const int bl = __builtin_ctzll(mask);
+            const int64_t k = __shfl(v, bl, 64);
+            const FT dd = __shfl(d, bl, 64);
+            gsum += (double)dd;
+            ++flips;
+            const int sk = x[k] ? -1 : 1;
+            __syncthreads();
+            if (use_lds) {
+                for (int32_t e = g.ap[k] + lane; e < g.ap[k + 1]; e += 64) {
+                    const int32_t u = g.ai[e];
+                    const int su = x[u] ? -1 : 1;
+                    D[u] -= (FT)(2 * sk * su) * (FT)g.aw[e];
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    D[k] = -D[k];
+                    x[k] ^= 1u;
+                }
+            } else {
+                flip_update(g, x, hg, k, sk, lane, 64);
+            }
+            if (lane == 0) r.age[k] = 0;
+            __syncthreads();
+            d = valid ? gain(v) : (FT)0;
+            mask = __ballot(valid && d > 0 && lane > bl);
+        }
+    }
+    __syncthreads();
+    for (int64_t u = lane; u < n; u += 64) {
+        if (use_lds) {
+            const uint8_t xv = x[u];
+            hg[u] = xv ? -D[u] : D[u];
+            r.x[u] = xv;
+        }
+        if (r.age[u] < 255) r.age[u]++;
+    }
+    if (lane == 0) {
+        r.out->accepts = flips;
+        r.out->gain_sum = gsum;
+        r.out->evals = n;
+        r.out->gain = 0.0;
+        r.out->k = -1;
+        r.out->l = -1;
+    }
+}
+
+template <typename WT>
+static void b_features_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((kb_features<WT, FT>), dim3((unsigned)((max_n + 255) / 256), (unsigned)L), dim3(256), 0, s, d,
+                       list);
+}
+
+template <typename WT>
+static void b_learned_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
+    static const size_t limit = lds_limit((const void*)kb_learned_pass<WT, FT>);
+    const int use_lds = lds <= limit ? 1 : 0;
+    hipLaunchKernelGGL((kb_learned_pass<WT, FT>), dim3((unsigned)L), dim3(64), use_lds ? lds : 0, s, d, list, use_lds);
+}
+
+void launch_b_features(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_features_impl, d, list, L, max_n, s);
+}
+void launch_b_order(const BInstRaw* d, const int32_t* list, int64_t L, hipStream_t s) {
+    hipLaunchKernelGGL(kb_order, dim3((unsigned)L), dim3(256), 0, s, d, list);
+}
+void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_learned_impl, d, list, L, max_n, s);
+}
+
 }  // namespace pqi
