@@ -230,13 +230,14 @@ PQI_API int pqi_mpnn_loss_grad(pqi_mpnn* m, const float* feats, const float* lab
 PQI_API int pqi_mpnn_train_step(pqi_mpnn* m, const float* feats, const float* labels, float lr, double* loss);
 
 /* Batched search with learned proposals: up to `passes` rounds of network
- * scores -> score order -> one first-improvement pass with exact gains, then
+ * scores -> score order -> one first-improvement pass with exact gains over the
+ * spins whose score (a logit) exceeds min_score, then
  * the loop of pqi_batch_local_search.  The network must be bound to exactly
  * these graphs, in this order.  targets (optional, count values: energy in
  * spin-glass mode, cut otherwise): an instance stops once it reaches its own. */
 PQI_API int pqi_learned_search(pqi_ctx* ctx, pqi_mpnn* net, pqi_graph* const* graphs, int64_t count,
                                uint8_t* const* thetas, const pqi_search_params* p, const double* targets,
-                               int64_t passes, pqi_search_stats* stats);
+                               int64_t passes, float min_score, pqi_search_stats* stats);
 
 /* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
 PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);

@@ -4,9 +4,11 @@ This module is the replacement for the reference's OpenCL seam
 (pyqrackising/maxcut_tfim_util.py:9-301).  There is no fallback: if the
 library is missing or no GPU is present, every entry point raises.
 """
+import atexit
 import ctypes
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -133,11 +135,35 @@ SIGNATURES = {
     "pqi_mpnn_forward": (_int, [_vp, _vp, _vp]),
     "pqi_mpnn_loss_grad": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "pqi_mpnn_train_step": (_int, [_vp, _vp, _vp, ctypes.c_float, _vp]),
-    "pqi_learned_search": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
+    "pqi_learned_search": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, ctypes.c_float, _vp]),
 }
 
 _lib = None
 _lock = threading.Lock()
+
+# Every live device object, so that they can be released in order (objects
+# before their context) while the HIP runtime is still up.  After release at
+# interpreter exit, close() calls are no-ops.
+_live = weakref.WeakSet()
+_shutdown = False
+
+
+def _release_all():
+    global _shutdown, _default_ctx
+    if _shutdown:
+        return
+    for rank in (0, 1, 2):
+        for obj in list(_live):
+            if getattr(obj, "_release_rank", 1) == rank:
+                try:
+                    obj.close()
+                except Exception:
+                    pass
+    _default_ctx = None
+    _shutdown = True
+
+
+atexit.register(_release_all)
 
 
 def load():
@@ -176,19 +202,22 @@ class Context:
     """One device and one HIP stream (the reference's global OpenCL queue,
     pyqrackising/maxcut_tfim_util.py:103-104).  Calls are serialised by a lock."""
 
+    _release_rank = 2
+
     def __init__(self, device=0):
         self._h = ctypes.c_void_p()
         check(load().pqi_create(int(device), ctypes.byref(self._h)))
         self.lock = threading.RLock()
+        _live.add(self)
 
     @property
     def handle(self):
         return self._h
 
     def close(self):
-        if self._h:
+        if self._h and not _shutdown:
             load().pqi_destroy(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -228,10 +257,13 @@ class DeviceGraph:
     """A problem instance resident in HBM (replaces make_G_m_buf /
     make_G_m_csr_buf, pyqrackising/maxcut_tfim_util.py:224-259)."""
 
+    _release_rank = 1
+
     def __init__(self, ctx, handle, n):
         self.ctx = ctx
         self._h = handle
         self.n = n
+        _live.add(self)
 
     @classmethod
     def dense(cls, W, ctx=None):
@@ -267,9 +299,9 @@ class DeviceGraph:
         return {f: getattr(i, f) for f, _ in i._fields_}
 
     def close(self):
-        if self._h:
+        if self._h and not _shutdown:
             load().pqi_graph_free(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -358,6 +390,9 @@ class Batch:
         self._h = ctypes.c_void_p()
         with self.ctx.lock:
             check(load().pqi_batch_create(self.ctx.handle, arr, len(self.graphs), self.cands, ctypes.byref(self._h)))
+        _live.add(self)
+
+    _release_rank = 0
 
     def set_candidates(self, bits_list):
         """bits_list[i]: cands x ceil(n_i/32) uint32 for instance i."""
@@ -396,9 +431,9 @@ class Batch:
                 "weight_bytes": wb.value, "variant": v.value}
 
     def close(self):
-        if self._h:
+        if self._h and not _shutdown:
             load().pqi_batch_free(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -434,6 +469,9 @@ class Mpnn:
         self.set_precision(precision)
         self.n_params = load().pqi_mpnn_num_params()
         self.N = 0
+        _live.add(self)
+
+    _release_rank = 0
 
     def set_precision(self, precision):
         check(load().pqi_mpnn_set_precision(self._h, int(precision)))
@@ -483,9 +521,9 @@ class Mpnn:
         return loss.value
 
     def close(self):
-        if self._h:
+        if self._h and not _shutdown:
             load().pqi_mpnn_free(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -494,7 +532,7 @@ class Mpnn:
             pass
 
 
-def learned_search(net, graphs, thetas, params, targets=None, passes=8):
+def learned_search(net, graphs, thetas, params, targets=None, passes=8, min_score=-np.inf):
     """pqi_learned_search: returns (states, list of stats dicts)."""
     graphs = list(graphs)
     ctx = graphs[0].ctx
@@ -506,7 +544,8 @@ def learned_search(net, graphs, thetas, params, targets=None, passes=8):
     st = (SearchStats * k)()
     with ctx.lock:
         check(load().pqi_learned_search(ctx.handle, net._h, garr, k, tarr, ctypes.byref(params),
-                                        None if tg is None else _p(tg), int(passes), st))
+                                        None if tg is None else _p(tg), int(passes), ctypes.c_float(min_score),
+                                        st))
     return states, [s.as_dict() for s in st]
 
 

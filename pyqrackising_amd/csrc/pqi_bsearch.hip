@@ -65,12 +65,13 @@ namespace pqi {
 
 // The batched loop.  With `net`, a learned phase runs first: up to `passes`
 // rounds of (features, network scores, score order, one first-improvement pass
-// with exact gains), stopping early when no instance flips.  With `targets`,
+// with exact gains over the spins scored above min_score), stopping early when
+// no instance flips.  With `targets`,
 // an instance stops as soon as its value (energy in spin-glass mode, cut
 // otherwise) reaches its target.
 static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
                        const pqi_search_params* p, pqi_search_stats* stats, pqi_mpnn* net, const double* targets,
-                       int64_t passes) {
+                       int64_t passes, float min_score) {
     if (!c || !graphs || !thetas || !p || !stats) return fail(PQI_ERR_INVALID, "null argument");
     if (count < 1) return fail(PQI_ERR_INVALID, "count must be >= 1");
     if (p->walkers < 1 || p->seed_multiple < 1) return fail(PQI_ERR_INVALID, "walkers and seed_multiple must be >= 1");
@@ -249,7 +250,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             mpnn_forward_device(net, s);
             BCHK(upload_list(run));
             launch_b_order(d_inst, d_list, (int64_t)run.size(), s);
-            launch_b_learned_pass(kind, d_inst, d_list, (int64_t)run.size(), max_n, s);
+            launch_b_learned_pass(kind, d_inst, d_list, (int64_t)run.size(), max_n, min_score, s);
             BCHK(read_outs(run));
             std::vector<int32_t> again;
             for (size_t t = 0; t < run.size(); ++t) {
@@ -503,14 +504,15 @@ extern "C" {
 
 int pqi_batch_local_search(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
                            const pqi_search_params* p, pqi_search_stats* stats) {
-    return run_batched(c, graphs, count, thetas, p, stats, nullptr, nullptr, 0);
+    return run_batched(c, graphs, count, thetas, p, stats, nullptr, nullptr, 0, 0.f);
 }
 
 int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
-                       const pqi_search_params* p, const double* targets, int64_t passes, pqi_search_stats* stats) {
+                       const pqi_search_params* p, const double* targets, int64_t passes, float min_score,
+                       pqi_search_stats* stats) {
     if (!net) return fail(PQI_ERR_INVALID, "null network");
     if (passes < 0) return fail(PQI_ERR_INVALID, "passes must be >= 0");
-    return run_batched(c, graphs, count, thetas, p, stats, net, targets, passes);
+    return run_batched(c, graphs, count, thetas, p, stats, net, targets, passes, min_score);
 }
 
 }  // extern "C"

@@ -179,7 +179,8 @@ void launch_b_quad(int kind, const BInstRaw* d, const int32_t* list, int64_t L, 
 // pass over the nodes in that order with exact gains (out->accepts, out->gain_sum)
 void launch_b_features(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
 void launch_b_order(const BInstRaw* d, const int32_t* list, int64_t L, hipStream_t s);
-void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
+void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n,
+                           float min_score, hipStream_t s);
 
 // network internals used by the learned search (pqi_mpnn.hip)
 float* mpnn_features(pqi_mpnn* m);

@@ -1221,7 +1221,8 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
 // candidates are evaluated at once (one per lane); a ballot finds the first
 // positive one, it is applied, and the later lanes re-evaluate.
 template <typename WT, typename FT>
-__global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, const int32_t* list, int use_lds) {
+__global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, const int32_t* list, int use_lds,
+                                                     float min_score) {
     extern __shared__ __align__(16) unsigned char lds[];
     const BInstRaw& r = insts[list[blockIdx.x]];
     const GView<WT> g = gview_of<WT>(r);
@@ -1242,8 +1243,8 @@ __global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, con
     int64_t flips = 0;
     double gsum = 0.0;
     for (int64_t c0 = 0; c0 < n; c0 += 64) {
-        const bool valid = c0 + lane < n;
-        const int64_t v = valid ? r.perm[c0 + lane] : 0;
+        const int64_t v = c0 + lane < n ? r.perm[c0 + lane] : 0;
+        const bool valid = c0 + lane < n && r.score[v] > min_score;
         FT d = valid ? gain(v) : (FT)0;
         uint64_t mask = __ballot(valid && d > 0);
         while (mask) {
@@ -1301,12 +1302,14 @@ static void b_features_impl(const BInstRaw* d, const int32_t* list, int64_t L, i
 }
 
 template <typename WT>
-static void b_learned_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+static void b_learned_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, float min_score,
+                           hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
     const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
     static const size_t limit = lds_limit((const void*)kb_learned_pass<WT, FT>);
     const int use_lds = lds <= limit ? 1 : 0;
-    hipLaunchKernelGGL((kb_learned_pass<WT, FT>), dim3((unsigned)L), dim3(64), use_lds ? lds : 0, s, d, list, use_lds);
+    hipLaunchKernelGGL((kb_learned_pass<WT, FT>), dim3((unsigned)L), dim3(64), use_lds ? lds : 0, s, d, list, use_lds,
+                       min_score);
 }
 
 void launch_b_features(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
@@ -1315,8 +1318,9 @@ void launch_b_features(int kind, const BInstRaw* d, const int32_t* list, int64_t
 void launch_b_order(const BInstRaw* d, const int32_t* list, int64_t L, hipStream_t s) {
     hipLaunchKernelGGL(kb_order, dim3((unsigned)L), dim3(256), 0, s, d, list);
 }
-void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
-    PQI_KIND_DISPATCH(kind, b_learned_impl, d, list, L, max_n, s);
+void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n,
+                           float min_score, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_learned_impl, d, list, L, max_n, min_score, s);
 }
 
 }  // namespace pqi

@@ -90,6 +90,11 @@ class LearnedSolver:
         dgs, feats, labels = trajectory_dataset(train_graphs, starts, params)
         return train(self.net, dgs, feats, labels, steps=steps, lr=lr, log=log)
 
-    def solve(self, dgs, starts, params, targets=None, passes=8):
-        self.net.bind(dgs)
-        return _lib.learned_search(self.net, dgs, starts, params, targets=targets, passes=passes)
+    def bind(self, dgs):
+        return self.net.bind(dgs)
+
+    def solve(self, dgs, starts, params, targets=None, passes=8, min_score=-np.inf, bind=True):
+        if bind:
+            self.net.bind(dgs)
+        return _lib.learned_search(self.net, dgs, starts, params, targets=targets, passes=passes,
+                                   min_score=min_score)
