@@ -109,3 +109,95 @@ def test_mpnn_training_reduces_loss():
     assert last < 0.5 * first
     pred = net.forward(feats) > 0
     assert (pred == (labels > 0)).mean() > 0.9
+
+
+# ---------------------------------------------------------------------------
+# learned-proposal search
+# ---------------------------------------------------------------------------
+
+def _er_int(n, deg, seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = graphs.erdos_renyi_edges(n, deg / (n - 1), rng)
+    w = rng.integers(-3, 4, lo.size).astype(np.float64)
+    keep = w != 0
+    G = graphs.upper_csr(n, lo[keep], hi[keep], w[keep], dtype=np.float32)
+    W = (G + G.T).toarray().astype(np.float64)
+    return G, W
+
+
+def test_learned_passes_in_index_order_match_oracle(orc):
+    """With all scores equal the score order is the index order: the learned
+    phase is index-order first-improvement passes, then the reference loop."""
+    log("learned search, index order vs oracle")
+    rng = np.random.default_rng(81)
+    Gs, Ws, x0s = [], [], []
+    for t in range(5):
+        G, W = _er_int(int(rng.integers(60, 200)), 5.0, 300 + t)
+        Gs.append(G)
+        Ws.append(W)
+        x0s.append(rng.integers(0, 2, W.shape[0]).astype(np.uint8))
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    net = _lib.Mpnn(seed=1)
+    net.set_params(np.zeros(net.n_params, np.float32))
+    net.bind(dgs)
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=800)
+    states, stats = _lib.learned_search(net, dgs, x0s, p, passes=50)
+    for i, (W, x0) in enumerate(zip(Ws, x0s)):
+        x1, flips, passes, gain = orc.first_improvement(W, x0, np.arange(W.shape[0]), 50)
+        assert stats[i]["learned_accepts"] == flips and stats[i]["learned_passes"] == passes, i
+        xc, r = orc.dense_local_search(W, x1, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=800)
+        assert np.array_equal(states[i], xc), i
+        assert stats[i]["min_energy"] == r["min_energy"]
+        assert stats[i]["max_energy"] == -r["min_energy"]  # every accepted move was a true gain
+
+
+def test_learned_search_moves_are_exact(orc):
+    """Arbitrary network scores: every accepted learned move is a true
+    improvement (exact bookkeeping) and the search ends single-flip optimal."""
+    log("learned search exactness")
+    rng = np.random.default_rng(82)
+    Gs, Ws, x0s = [], [], []
+    for t in range(4):
+        G, W = _er_int(int(rng.integers(80, 300)), 6.0, 400 + t)
+        Gs.append(G)
+        Ws.append(W)
+        x0s.append(rng.integers(0, 2, W.shape[0]).astype(np.uint8))
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    net = _lib.Mpnn(seed=9)
+    net.bind(dgs)
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=500)
+    states, stats = _lib.learned_search(net, dgs, x0s, p, passes=6, min_score=0.0)
+    for i, W in enumerate(Ws):
+        e = orc.compute_energy(W, states[i])
+        assert stats[i]["max_energy"] == e and stats[i]["min_energy"] == -e
+        g, _ = orc.single_bit_flips(states[i], 1, W)
+        assert g <= 0
+
+
+def test_trained_network_gain():
+    """What training buys: the same learned search (confident proposals only,
+    four passes, exact gains) with the trained network and with the untrained
+    one, on held-out Erdos-Renyi graphs with U[-1, 1] couplings; the measure is
+    how many instances reach the plain batched search's final energy."""
+    from pyqrackising_amd import learned
+
+    log("trained vs untrained network")
+    P = _lib.make_params(0, is_spin_glass=1, walkers=16, seed_multiple=8)
+    P.gray_iterations = -1
+    solver = learned.LearnedSolver(seed=11)
+    losses = solver.fit(learned.regular_training_graphs(32, seed=3001, n_hi=1536), P, seed=5, steps=250)
+    assert losses[-1] < 0.6 * losses[0]
+    G5 = learned.er_graphs(32, 2048, 8.0, seed=5006)
+    dgs = [_lib.DeviceGraph.csr(G) for G in G5]
+    rng = np.random.default_rng(78)
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in G5]
+    _, plain = _lib.batch_local_search(dgs, starts, P)
+    targets = np.array([s["max_energy"] for s in plain])
+    solver.bind(dgs)
+    _, tr = solver.solve(dgs, starts, P, targets=targets, passes=4, min_score=1.0, bind=False)
+    trained = sum(s["max_energy"] >= t for s, t in zip(tr, targets))
+    solver.net.set_params(_lib.Mpnn(seed=11).params())
+    _, un = solver.solve(dgs, starts, P, targets=targets, passes=4, min_score=1.0, bind=False)
+    untrained = sum(s["max_energy"] >= t for s, t in zip(un, targets))
+    log(f"  reached the plain energy: trained {trained}/32, untrained {untrained}/32")
+    assert trained >= untrained

@@ -347,3 +347,29 @@ def test_known_answers():
         bits, v = orc.exact(W, 0)
         assert v == c["maxcut"], c["name"]
         assert nx_cut(W, bits) == c["maxcut"]
+
+
+def test_first_improvement_passes(orc):
+    """Each pass visits the order once and flips only improving nodes; the
+    result is single-flip optimal when the last pass flips nothing."""
+    rng = np.random.default_rng(13)
+    n = 45
+    W = rand_graph(n, rng, p=0.2)
+    x = rng.integers(0, 2, n).astype(np.uint8)
+    order = rng.permutation(n)
+    xs, flips, passes, gain = orc.first_improvement(W, x, order, 100)
+    assert gain == nx_cut(W, xs) - nx_cut(W, x)
+    assert flips > 0 and passes >= 2
+    for k in range(n):
+        y = xs.copy()
+        y[k] ^= 1
+        assert nx_cut(W, y) <= nx_cut(W, xs)
+    # one pass, literally: visit in order, keep a flip iff the cut rises
+    y = x.copy()
+    for v in order:
+        z = y.copy()
+        z[v] ^= 1
+        if nx_cut(W, z) > nx_cut(W, y):
+            y = z
+    x1, _, p1, _ = orc.first_improvement(W, x, order, 1)
+    assert p1 == 1 and np.array_equal(x1, y)
