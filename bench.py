@@ -61,6 +61,126 @@ def cpu_baseline(W, x0, walkers, seed_multiple, gray_iterations):
     }
 
 
+def time_to_equal_cut(log):
+    """64 Erdos-Renyi graphs, n = 8192, mean degree 8, couplings U[-1, 1] (fp32),
+    spin-glass mode, from fixed random starts: the plain HIP local search one
+    instance at a time (the reference loop per instance) against the batched
+    search (all instances per launch, identical moves) and the learned+batched
+    search (network-ordered proposals with exact gains, stopping at the plain
+    search's final energy; instances that end short are re-solved by the
+    batched plain search from their start, so every instance reaches it)."""
+    from pyqrackising_amd import _lib, learned
+
+    P = _lib.make_params(0, is_spin_glass=1, walkers=16, seed_multiple=8)
+    P.gray_iterations = -1
+    solver = learned.LearnedSolver(seed=11)
+    t = time.perf_counter()
+    solver.fit(learned.regular_training_graphs(48, seed=3001), P, seed=5, steps=300, lr=2e-3)
+    train_s = time.perf_counter() - t
+    G5 = learned.er_graphs(64, 8192, 8.0, seed=5005)
+    dgs = [_lib.DeviceGraph.csr(G) for G in G5]
+    rng = np.random.default_rng(77)
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in G5]
+    solver.bind(dgs)
+    # warm up every path once
+    dgs[0].local_search(starts[0], _lib.make_params(8192, is_spin_glass=1, walkers=16, seed_multiple=8))
+    _lib.batch_local_search(dgs[:2], starts[:2], P)
+    t = time.perf_counter()
+    plain = [dg.local_search(x, _lib.make_params(dg.n, is_spin_glass=1, walkers=16, seed_multiple=8))
+             for dg, x in zip(dgs, starts)]
+    t_plain = time.perf_counter() - t
+    targets = np.array([st["max_energy"] for _, st in plain])
+    log("# time-to-equal-cut: plain sequential done")
+    t = time.perf_counter()
+    bs, _ = _lib.batch_local_search(dgs, starts, P)
+    t_batch = time.perf_counter() - t
+    identical = all(np.array_equal(a, b[0]) for a, b in zip(bs, plain))
+    t = time.perf_counter()
+    _, lst = solver.solve(dgs, starts, P, targets=targets, passes=4, min_score=1.0, bind=False)
+    short = [i for i, s in enumerate(lst) if s["max_energy"] < targets[i]]
+    if short:
+        _lib.batch_local_search([dgs[i] for i in short], [starts[i] for i in short], P)
+    t_learned = time.perf_counter() - t
+    return {
+        "workload": "64 x ER(n=8192, mean degree 8), U[-1,1] fp32 couplings, spin glass, gray_iterations=n^2, "
+                    "walkers=16, seed_multiple=8",
+        "plain_sequential_s": t_plain,
+        "batched_s": t_batch,
+        "batched_identical_to_plain": identical,
+        "learned_batched_s": t_learned,
+        "learned_reached_without_fallback": 64 - len(short),
+        "speedup_batched": t_plain / t_batch,
+        "speedup_learned_batched": t_plain / t_learned,
+        "network_training_s": train_s,
+    }
+
+
+def batched_eval(log):
+    """256 instances, n ~ U[256, 4096], d ~ U{3..10}, +-1 int8, 1024 candidates
+    each, one graph-captured persistent launch with the autotuned variant."""
+    from pyqrackising_amd import _lib, graphs
+
+    Ws = graphs.batched_regular_instances(256, seed=1003)
+    dgs = [_lib.DeviceGraph.dense(W) for W in Ws]
+    rng = np.random.default_rng(53)
+    bits = [rng.integers(0, 1 << 32, (1024, (W.shape[0] + 31) // 32), dtype=np.uint64).astype(np.uint32)
+            for W in Ws]
+    batch = _lib.Batch(dgs, 1024)
+    batch.set_candidates(bits)
+    chosen, ms = batch.autotune(reps=3)
+    info = batch.info()
+    reps = 10
+    batch.eval(1)
+    t = time.perf_counter()
+    for _ in range(reps):
+        batch.eval(1)
+    dt = (time.perf_counter() - t) / reps
+    best_ms = float(ms[chosen])
+    log("# batched evaluation done")
+    return {
+        "workload": "256 instances, n~U[256,4096], d~U{3..10}, +-1 int8, 1024 candidates each",
+        "autotune_ms_per_variant": [round(float(v), 4) for v in ms],
+        "variant": chosen,
+        "kernel_ms": best_ms,
+        "call_ms": dt * 1e3,
+        "algorithmic_ops": info["algorithmic_ops"],
+        "executed_ops": info["executed_ops"],
+        "weight_bytes": info["weight_bytes"],
+        "mfma_int8_tops": info["algorithmic_ops"] / (best_ms * 1e-3) / 1e12,
+        "mfma_frac_of_5000_tops": info["algorithmic_ops"] / (best_ms * 1e-3) / 5.0e15,
+    }
+
+
+def sk_bf16(log):
+    """SK couplings w_ij ~ N(0, 1/n), n = 16384, fp32; 1024 random candidates
+    scored on split-BF16 MFMA with an FP64 re-score of the best 16."""
+    from pyqrackising_amd import _lib
+
+    n, B = 16384, 1024
+    rng = np.random.default_rng(43)
+    A = rng.standard_normal((n, n), dtype=np.float32) / np.float32(np.sqrt(n))
+    W = np.triu(A, 1)
+    del A
+    W += W.T
+    dg = _lib.DeviceGraph.dense(W)
+    del W
+    bits = _lib.pack_bits(rng.integers(0, 2, (B, n)).astype(bool))
+    dg.eval_batch(bits, 1, engine=_lib.ENGINE_BF16)
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        bi, bv, _, err = dg.eval_batch(bits, 1, engine=_lib.ENGINE_BF16, return_error=True)
+    dt = (time.perf_counter() - t) / reps
+    log("# SK split-BF16 done")
+    return {
+        "workload": "SK n=16384 fp32, 1024 candidates, split hi/lo BF16 MFMA + FP64 re-score of the best 16",
+        "call_ms": dt * 1e3,
+        "algorithmic_flop": 2.0 * B * n * n,
+        "best_energy_fp64": bv,
+        "bf16_max_rel_err_top16": err,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +192,7 @@ def main():
     ap.add_argument("--gray-iterations", type=int, default=None)
     ap.add_argument("--cpu-gray-iterations", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched, learned and SK legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -161,6 +282,14 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_extra:
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        for key, fn in (("time_to_equal_cut", time_to_equal_cut), ("batched_eval", batched_eval),
+                        ("sk_bf16", sk_bf16)):
+            try:
+                out[key] = fn(log)
+            except Exception as e:  # a failing leg is reported, not fatal to the headline
+                out[key] = {"error": repr(e)}
     if not args.no_cpu:
         cgi = n * n // 4 if args.cpu_gray_iterations is None else args.cpu_gray_iterations
         cb = cpu_baseline(W, x0, 16, 8, cgi)

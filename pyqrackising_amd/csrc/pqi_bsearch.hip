@@ -133,6 +133,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         }
         if (mpnn_total_nodes(net) != off) return fail(PQI_ERR_INVALID, "the network is bound to other graphs");
     }
+    int all_sparse = 1;
+    for (int64_t i = 0; i < count; ++i) all_sparse &= h_inst[i].use_adj;
     const int64_t L0 = count;
     const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
                         align256(L0 * sizeof(BInstRaw));
@@ -289,7 +291,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         while (!run.empty()) {
             const int64_t cap = 16 * max_n + 4096;
             BCHK(upload_list(run));
-            launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, s);
+            launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, all_sparse, s);
             BCHK(read_outs(run));
             std::vector<int32_t> again;
             for (size_t t = 0; t < run.size(); ++t) {
@@ -408,7 +410,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             }
             BCHK(copies(rest3, wsrc, 1));
             BCHK(upload_list(rest3));
-            launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, s);
+            launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, all_sparse, s);
             BCHK(read_outs(rest3));
             std::vector<int32_t> take1, improve1, rest4;
             std::vector<int64_t> mv1, mvi;

@@ -162,8 +162,10 @@ struct BInstRaw {
 
 // which = 0: the best state (x, h); 1: the reheat state (rx, rh)
 void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
+// all_sparse: every listed instance updates through adjacency lists (enables the
+// two-level argmax variant)
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                      int which, hipStream_t s);
+                      int which, int all_sparse, hipStream_t s);
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s);
 void launch_b_flip(int kind, const BInstRaw* d, const int32_t* list, int64_t L, const int64_t* moves, int which,
                    hipStream_t s);

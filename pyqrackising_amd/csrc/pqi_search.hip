@@ -408,6 +408,125 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
     }
 }
 
+// Sparse-graph variant with a two-level argmax: LDS keeps, for every group of
+// 64 consecutive nodes, the group's first maximum gain and its node.  A step
+// takes the first maximum over the group maxima (ties -> lowest node, since
+// groups are in node order), and a flip of k only recomputes the groups of k
+// and of its neighbours.  Same selection rule and result as the full scan.
+template <typename FT>
+__device__ __forceinline__ void group_max(const FT* D, int64_t n, int64_t grp, FT* gmax, int32_t* gidx) {
+    const int lane = threadIdx.x;
+    const int64_t v = grp * 64 + lane;
+    FT bv = v < n ? D[v] : lowest<FT>();
+    int bk = v < n ? (int)v : INT_MAX;
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        const FT ov = shfl_xor_t(bv, m);
+        const int ok = __shfl_xor(bk, m, 64);
+        if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
+    }
+    if (lane == 0) {
+        gmax[grp] = bv;
+        gidx[grp] = bk;
+    }
+}
+
+template <typename WT, typename FT>
+__device__ void descent_wave2_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64_t max_steps, PhaseOut* out,
+                                   unsigned char* lds) {
+    const int lane = threadIdx.x;
+    const int n = (int)g.n;
+    const int ng = (n + 63) >> 6;
+    size_t off = 0;
+    FT* D = (FT*)(lds + off);
+    off += ((size_t)n * sizeof(FT) + 15) & ~(size_t)15;
+    FT* gmax = (FT*)(lds + off);
+    off += ((size_t)ng * sizeof(FT) + 15) & ~(size_t)15;
+    int32_t* gidx = (int32_t*)(lds + off);
+    off += ((size_t)ng * 4 + 15) & ~(size_t)15;
+    uint8_t* x = lds + off;
+    for (int v = lane; v < n; v += 64) {
+        const uint8_t xv = xg[v];
+        x[v] = xv;
+        D[v] = xv ? -hg[v] : hg[v];
+    }
+    __syncthreads();
+    for (int grp = 0; grp < ng; ++grp) group_max(D, n, grp, gmax, gidx);
+    __syncthreads();
+    int64_t steps = 0, evals = 0;
+    double gsum = 0.0;
+    FT bv;
+    int bk;
+    for (;;) {
+        bv = lowest<FT>();
+        bk = INT_MAX;
+        for (int t = lane; t < ng; t += 64) {
+            const FT d = gmax[t];
+            const int k = gidx[t];
+            if (d > bv || (d == bv && k < bk)) { bv = d; bk = k; }
+        }
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+            const FT ov = shfl_xor_t(bv, m);
+            const int ok = __shfl_xor(bk, m, 64);
+            if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
+        }
+        evals += n;
+        if (!(bv > 0) || steps >= max_steps) break;
+        const int sk = x[bk] ? -1 : 1;
+        __syncthreads();
+        const int32_t e0 = g.ap[bk], e1 = g.ap[bk + 1];
+        for (int32_t e = e0 + lane; e < e1; e += 64) {
+            const int v = g.ai[e];
+            const int sv = x[v] ? -1 : 1;
+            D[v] -= (FT)(2 * sk * sv) * (FT)g.aw[e];
+        }
+        __syncthreads();
+        if (lane == 0) {
+            D[bk] = -D[bk];
+            x[bk] ^= 1u;
+        }
+        __syncthreads();
+        group_max(D, n, bk >> 6, gmax, gidx);
+        for (int32_t e = e0; e < e1; ++e) {
+            const int grp = g.ai[e] >> 6;
+            if (grp != (bk >> 6)) group_max(D, n, grp, gmax, gidx);
+        }
+        __syncthreads();
+        ++steps;
+        gsum += (double)bv;
+    }
+    if (steps) {
+        for (int v = lane; v < n; v += 64) {
+            const uint8_t xv = x[v];
+            hg[v] = xv ? -D[v] : D[v];
+            xg[v] = xv;
+        }
+    }
+    if (lane == 0) {
+        out->gain = (double)bv;
+        out->k = bk;
+        out->l = -1;
+        out->accepts = steps;
+        out->evals = evals;
+        out->gain_sum = gsum;
+    }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void k_descent_wave2(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
+                                                     PhaseOut* out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    descent_wave2_body<WT, FT>(g, xg, hg, max_steps, out, lds);
+}
+
+template <typename FT>
+static size_t descent2_lds(int64_t n) {
+    const int64_t ng = (n + 63) >> 6;
+    return (((size_t)n * sizeof(FT) + 15) & ~(size_t)15) + (((size_t)ng * sizeof(FT) + 15) & ~(size_t)15) +
+           (((size_t)ng * 4 + 15) & ~(size_t)15) + (size_t)n;
+}
+
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
                                                     PhaseOut* out) {
@@ -420,7 +539,12 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
     using FT = typename FieldOf<WT>::T;
     const size_t lds = (((size_t)g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
     static const size_t limit = lds_limit((const void*)k_descent_wave<WT, FT>);
-    if (lds <= limit)
+    static const size_t limit2 = lds_limit((const void*)k_descent_wave2<WT, FT>);
+    const bool sparse = g.use_adj || g.W == nullptr;
+    if (sparse && descent2_lds<FT>(g.n) <= limit2)
+        hipLaunchKernelGGL((k_descent_wave2<WT, FT>), dim3(1), dim3(64), descent2_lds<FT>(g.n), s, view<WT>(g), x,
+                           (FT*)h, max_steps, out);
+    else if (lds <= limit)
         hipLaunchKernelGGL((k_descent_wave<WT, FT>), dim3(1), dim3(64), lds, s, view<WT>(g), x, (FT*)h, max_steps,
                            out);
     else
@@ -934,6 +1058,14 @@ __global__ __launch_bounds__(64) void kb_descent_lds(const BInstRaw* insts, cons
 }
 
 template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void kb_descent_lds2(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
+                                                     int which) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    descent_wave2_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out, lds);
+}
+
+template <typename WT, typename FT>
 __global__ __launch_bounds__(1024) void kb_descent_glb(const BInstRaw* insts, const int32_t* list,
                                                       int64_t max_steps, int which) {
     const BInstRaw& r = insts[list[blockIdx.x]];
@@ -1050,11 +1182,15 @@ static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int6
 
 template <typename WT>
 static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                           int which, hipStream_t s) {
+                           int which, int all_sparse, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
     const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
     static const size_t limit = lds_limit((const void*)kb_descent_lds<WT, FT>);
-    if (lds <= limit)
+    static const size_t limit2 = lds_limit((const void*)kb_descent_lds2<WT, FT>);
+    if (all_sparse && descent2_lds<FT>(max_n) <= limit2)
+        hipLaunchKernelGGL((kb_descent_lds2<WT, FT>), dim3((unsigned)L), dim3(64), descent2_lds<FT>(max_n), s, d, list,
+                           max_steps, which);
+    else if (lds <= limit)
         hipLaunchKernelGGL((kb_descent_lds<WT, FT>), dim3((unsigned)L), dim3(64), lds, s, d, list, max_steps, which);
     else
         hipLaunchKernelGGL((kb_descent_glb<WT, FT>), dim3((unsigned)L), dim3(1024), 0, s, d, list, max_steps, which);
@@ -1124,8 +1260,8 @@ void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
     PQI_KIND_DISPATCH(kind, b_field_impl, d, list, L, max_n, s);
 }
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                      int which, hipStream_t s) {
-    PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, s);
+                      int which, int all_sparse, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, all_sparse, s);
 }
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s) {
     PQI_KIND_DISPATCH(kind, b_double_impl, d, list, L, which, s);
