@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 
 #include "pqi_internal.h"
@@ -120,6 +121,48 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Wavefront reductions on DPP row operations (a few cycles per step instead of
+// an LDS-path shuffle): quad swaps, row shifts, then row broadcasts; the
+// result is complete in lane 63 and read back to every lane.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false);
+}
+
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const int lo = dpp_mov<CTRL, ROWS>(__double2loint(v));
+    const int hi = dpp_mov<CTRL, ROWS>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce_dpp(T v, Op op) {
+    v = op(v, dpp_mov<0xB1>(v));        // quad_perm [1,0,3,2]
+    v = op(v, dpp_mov<0x4E>(v));        // quad_perm [2,3,0,1]
+    v = op(v, dpp_mov<0x114>(v));       // row_shr:4
+    v = op(v, dpp_mov<0x118>(v));       // row_shr:8
+    v = op(v, dpp_mov<0x142, 0xA>(v));  // row_bcast:15
+    v = op(v, dpp_mov<0x143, 0xC>(v));  // row_bcast:31
+    if constexpr (sizeof(T) == 8) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+        return __hiloint2double(hi, lo);
+    } else {
+        return __builtin_amdgcn_readlane(v, 63);
+    }
+}
+
+// First maximum over the wave: the largest value, and among the lanes holding
+// it the smallest index.  Every lane returns the winner.
+template <typename FT>
+__device__ __forceinline__ void wave_argmax_dpp(FT& v, int& k) {
+    const FT m = wave_reduce_dpp(v, [](FT a, FT b) { return a > b ? a : b; });
+    const int c = v == m ? k : INT_MAX;
+    k = wave_reduce_dpp(c, [](int a, int b) { return a < b ? a : b; });
+    v = m;
+}
+
 // Apply the flip of node k: every thread of the calling group takes part
 // (stride = group size).  Caller synchronises before and after.
 template <typename WT, typename FT>
@@ -179,6 +222,16 @@ static void field_impl(const Graph& g, const uint8_t* x, void* h, hipStream_t s)
             default: FN<double>(__VA_ARGS__); break;               \
         }                                                          \
     } while (0)
+
+// Descent variant: 0 or 1 the full scan (measured faster on MI355X), 2 the
+// two-level argmax for sparse graphs.  Read once from PQI_DESCENT_VARIANT.
+static int descent_variant() {
+    static const int v = [] {
+        const char* e = getenv("PQI_DESCENT_VARIANT");
+        return e ? atoi(e) : 0;
+    }();
+    return v;
+}
 
 // Dynamic LDS above 64 KiB must be requested per kernel; gfx950 has 160 KiB
 // per work group.  Returns the limit that applies to `fn`.
@@ -358,12 +411,7 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
             const FT d = D[v];
             if (d > bv) { bv = d; bk = v; }
         }
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1) {
-            const FT ov = shfl_xor_t(bv, m);
-            const int ok = __shfl_xor(bk, m, 64);
-            if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
-        }
+        wave_argmax_dpp(bv, bk);
         evals += n;
         if (!(bv > 0) || steps >= max_steps) break;
         const int sk = x[bk] ? -1 : 1;  // s_k before the flip
@@ -419,12 +467,7 @@ __device__ __forceinline__ void group_max(const FT* D, int64_t n, int64_t grp, F
     const int64_t v = grp * 64 + lane;
     FT bv = v < n ? D[v] : lowest<FT>();
     int bk = v < n ? (int)v : INT_MAX;
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) {
-        const FT ov = shfl_xor_t(bv, m);
-        const int ok = __shfl_xor(bk, m, 64);
-        if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
-    }
+    wave_argmax_dpp(bv, bk);
     if (lane == 0) {
         gmax[grp] = bv;
         gidx[grp] = bk;
@@ -465,21 +508,22 @@ __device__ void descent_wave2_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             const int k = gidx[t];
             if (d > bv || (d == bv && k < bk)) { bv = d; bk = k; }
         }
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1) {
-            const FT ov = shfl_xor_t(bv, m);
-            const int ok = __shfl_xor(bk, m, 64);
-            if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
-        }
+        wave_argmax_dpp(bv, bk);
         evals += n;
         if (!(bv > 0) || steps >= max_steps) break;
         const int sk = x[bk] ? -1 : 1;
         __syncthreads();
         const int32_t e0 = g.ap[bk], e1 = g.ap[bk + 1];
-        for (int32_t e = e0 + lane; e < e1; e += 64) {
-            const int v = g.ai[e];
-            const int sv = x[v] ? -1 : 1;
-            D[v] -= (FT)(2 * sk * sv) * (FT)g.aw[e];
+        // first 64 neighbours stay in registers for the group refresh below
+        int nb0 = -1;
+        for (int32_t c = e0; c < e1; c += 64) {
+            const int32_t e = c + lane;
+            if (e < e1) {
+                const int v = g.ai[e];
+                if (c == e0) nb0 = v;
+                const int sv = x[v] ? -1 : 1;
+                D[v] -= (FT)(2 * sk * sv) * (FT)g.aw[e];
+            }
         }
         __syncthreads();
         if (lane == 0) {
@@ -487,10 +531,17 @@ __device__ void descent_wave2_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             x[bk] ^= 1u;
         }
         __syncthreads();
-        group_max(D, n, bk >> 6, gmax, gidx);
-        for (int32_t e = e0; e < e1; ++e) {
-            const int grp = g.ai[e] >> 6;
-            if (grp != (bk >> 6)) group_max(D, n, grp, gmax, gidx);
+        const int gk = bk >> 6;
+        group_max(D, n, gk, gmax, gidx);
+        for (int32_t c = e0; c < e1; c += 64) {
+            const int nbv = c == e0 ? nb0 : (c + lane < e1 ? g.ai[c + lane] : -1);
+            const int cnt = e1 - c < 64 ? e1 - c : 64;
+            int prev = gk;
+            for (int t = 0; t < cnt; ++t) {
+                const int grp = __shfl(nbv, t, 64) >> 6;
+                if (grp != prev && grp != gk) group_max(D, n, grp, gmax, gidx);
+                prev = grp;
+            }
         }
         __syncthreads();
         ++steps;
@@ -540,7 +591,7 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
     const size_t lds = (((size_t)g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
     static const size_t limit = lds_limit((const void*)k_descent_wave<WT, FT>);
     static const size_t limit2 = lds_limit((const void*)k_descent_wave2<WT, FT>);
-    const bool sparse = g.use_adj || g.W == nullptr;
+    const bool sparse = (g.use_adj || g.W == nullptr) && descent_variant() == 2;
     if (sparse && descent2_lds<FT>(g.n) <= limit2)
         hipLaunchKernelGGL((k_descent_wave2<WT, FT>), dim3(1), dim3(64), descent2_lds<FT>(g.n), s, view<WT>(g), x,
                            (FT*)h, max_steps, out);
@@ -1187,7 +1238,7 @@ static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, in
     const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
     static const size_t limit = lds_limit((const void*)kb_descent_lds<WT, FT>);
     static const size_t limit2 = lds_limit((const void*)kb_descent_lds2<WT, FT>);
-    if (all_sparse && descent2_lds<FT>(max_n) <= limit2)
+    if (all_sparse && descent_variant() == 2 && descent2_lds<FT>(max_n) <= limit2)
         hipLaunchKernelGGL((kb_descent_lds2<WT, FT>), dim3((unsigned)L), dim3(64), descent2_lds<FT>(max_n), s, d, list,
                            max_steps, which);
     else if (lds <= limit)
