@@ -163,6 +163,15 @@ PQI_API int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int
 PQI_API int pqi_single_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, int64_t* k,
                          double* gain);
 
+/* Single-flip phase loop, resident on the device: the `while improved` loop of
+ * pyqrackising/spin_glass_solver.py:604-624 restricted to its single-flip
+ * branch -- take the first-argmax single flip while its gain is positive, at
+ * most max_steps times.  theta (0/1 bytes) is updated in place; *accepts is
+ * the number of flips applied, *gain their summed cut gain (x2 in spin-glass
+ * mode, as the reference's max_energy increments). */
+PQI_API int pqi_single_flip_descent(pqi_graph* g, uint8_t* theta, int is_spin_glass, int64_t max_steps,
+                                    int64_t* accepts, double* gain);
+
 /* Double-flip phase.  Replaces run_double_bit_flips / run_bit_flips_opencl(True)
  * (pyqrackising/spin_glass_solver.py:55-92, 156-241; kernels.cl:150-231). */
 PQI_API int pqi_double_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass,

@@ -113,6 +113,7 @@ SIGNATURES = {
     "pqi_eval_batch": (_int, [_vp, _vp, _i64, _int, _vp, _vp, _vp]),
     "pqi_eval_batch_ex": (_int, [_vp, _vp, _i64, _int, _int, _vp, _vp, _vp, _vp]),
     "pqi_single_flip_best": (_int, [_vp, _vp, _int, _vp, _vp]),
+    "pqi_single_flip_descent": (_int, [_vp, _vp, _int, _i64, _vp, _vp]),
     "pqi_double_flip_best": (_int, [_vp, _vp, _int, _i64, _int, _vp, _vp, _vp]),
     "pqi_gray_search": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pqi_local_search": (_int, [_vp, _vp, _vp, _vp]),
@@ -342,6 +343,17 @@ class DeviceGraph:
             check(load().pqi_single_flip_best(self._h, _p(t), int(bool(is_spin_glass)), ctypes.byref(k),
                                               ctypes.byref(g)))
         return g.value, k.value
+
+    def single_flip_descent(self, theta, is_spin_glass, max_steps=2 ** 62):
+        """Repeated first-argmax single flips while the gain is positive.
+        Returns (new theta, flips applied, summed gain)."""
+        t = self._theta(theta).copy()
+        a = ctypes.c_int64()
+        g = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_single_flip_descent(self._h, _p(t), int(bool(is_spin_glass)), int(max_steps),
+                                                 ctypes.byref(a), ctypes.byref(g)))
+        return t, a.value, g.value
 
     def double_flip_best(self, theta, is_spin_glass, thread_count, mode=DOUBLE_REFERENCE):
         t = self._theta(theta)

@@ -113,6 +113,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         r.ai = g->adj_idx;
         r.aw = g->adj_w;
         r.use_adj = (g->use_adj || g->W == nullptr) ? 1 : 0;
+        r.kadj = g->key_adj;
+        r.kstride = g->key_adj ? g->key_adj_stride : 0;
         r.seed_count = seeds;
         r.bs = bs;
         r.G = G;
@@ -135,6 +137,22 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     }
     int all_sparse = 1;
     for (int64_t i = 0; i < count; ++i) all_sparse &= h_inst[i].use_adj;
+    // packed-key descent (int8 weights): one LDS size for the launch, the
+    // largest staged layout when that fits, else the largest unstaged one;
+    // each work group stages its adjacency when its own layout fits
+    size_t key_lds = 0;
+    {
+        const size_t limit = b_key_lds_limit();
+        size_t staged = 0, base = 0;
+        bool keyed = kind == PQI_WEIGHTS_I8;
+        for (int64_t i = 0; i < count && keyed; ++i) {
+            h_inst[i].key_shift = key_shift(*graphs[i]);
+            keyed = h_inst[i].key_shift > 0;
+            staged = std::max(staged, key_lds_bytes(*graphs[i], SIZE_MAX));
+            base = std::max(base, key_lds_bytes(*graphs[i], 0));
+        }
+        if (keyed) key_lds = staged <= limit ? staged : (base <= limit ? base : 0);
+    }
     const int64_t L0 = count;
     const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
                         align256(L0 * sizeof(BInstRaw));
@@ -291,7 +309,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         while (!run.empty()) {
             const int64_t cap = 16 * max_n + 4096;
             BCHK(upload_list(run));
-            launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, all_sparse, s);
+            launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, all_sparse, key_lds, s);
             BCHK(read_outs(run));
             std::vector<int32_t> again;
             for (size_t t = 0; t < run.size(); ++t) {
@@ -410,7 +428,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             }
             BCHK(copies(rest3, wsrc, 1));
             BCHK(upload_list(rest3));
-            launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, all_sparse, s);
+            launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, all_sparse, key_lds, s);
             BCHK(read_outs(rest3));
             std::vector<int32_t> take1, improve1, rest4;
             std::vector<int64_t> mv1, mvi;

@@ -26,6 +26,12 @@ struct Graph {
     void* adj_w = nullptr;       // nnz, same element type as W
     int64_t nnz = 0;
     int64_t max_degree = 0;
+    int64_t max_strength = -1;   // int8 weights: max_v sum_u |w_vu| (exact); -1 otherwise
+    // int8 weights with adjacency lists: fixed-stride table, row v at v * stride,
+    // entries (w << 24) | u, zero padded; stride = max degree rounded up so that
+    // n * stride is a multiple of 4 (nullptr when not built)
+    int32_t* key_adj = nullptr;
+    int64_t key_adj_stride = 0;
     double w_up = 0.0;           // sum_{i<j} w_ij
     int64_t w_up_i = 0;          // same, exact, for int8 weights
     bool use_adj = true;         // flips update fields through the adjacency lists
@@ -131,6 +137,11 @@ void launch_quad(const Graph& g, const uint8_t* x, const void* h, double* q, hip
 
 size_t field_bytes(const Graph& g);  // bytes per field entry (4 or 8)
 
+// Packed-key descent for int8 weights: key width (0 when it does not apply)
+// and the dynamic LDS it uses (adjacency staged when it fits in `limit`).
+int key_shift(const Graph& g);
+size_t key_lds_bytes(const Graph& g, size_t limit);
+
 // ---- batched search: per-instance descriptors in device memory ----------------
 struct BInstRaw {
     const void* W;  // dense weights (or nullptr)
@@ -138,7 +149,10 @@ struct BInstRaw {
     const int32_t* ap;
     const int32_t* ai;
     const void* aw;
-    int32_t use_adj, pad;
+    int32_t use_adj;
+    int32_t key_shift;     // packed-key descent width for int8 weights (0: not used)
+    const int32_t* kadj;   // Graph::key_adj and its stride
+    int64_t kstride;
     uint8_t* x;  // best state and field
     void* h;
     uint8_t* rx;  // reheat state and field
@@ -163,9 +177,12 @@ struct BInstRaw {
 // which = 0: the best state (x, h); 1: the reheat state (rx, rh)
 void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
 // all_sparse: every listed instance updates through adjacency lists (enables the
-// two-level argmax variant)
+// two-level argmax variant); key_lds > 0: every instance has key_shift > 0 and
+// the packed-key descent runs with that much dynamic LDS
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                      int which, int all_sparse, hipStream_t s);
+                      int which, int all_sparse, size_t key_lds, hipStream_t s);
+// dynamic LDS limit of the batched packed-key descent
+size_t b_key_lds_limit();
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s);
 void launch_b_flip(int kind, const BInstRaw* d, const int32_t* list, int64_t L, const int64_t* moves, int which,
                    hipStream_t s);

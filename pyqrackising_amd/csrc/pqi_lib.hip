@@ -125,8 +125,30 @@ static int upload_adjacency(pqi_graph* g, const std::vector<int32_t>& ptr, const
     }
     g->device_bytes += (int64_t)(sizeof(int32_t) * (ptr.size() + nnz) + es * nnz);
     int64_t md = 0;
-    for (size_t i = 0; i + 1 < ptr.size(); ++i) md = std::max<int64_t>(md, ptr[i + 1] - ptr[i]);
+    double ms = 0.0;
+    for (size_t i = 0; i + 1 < ptr.size(); ++i) {
+        md = std::max<int64_t>(md, ptr[i + 1] - ptr[i]);
+        double r = 0.0;
+        for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) r += std::fabs(w[e]);
+        ms = std::max(ms, r);
+    }
     g->max_degree = md;
+    if (g->kind == PQI_WEIGHTS_I8) {
+        g->max_strength = (int64_t)ms;
+        // fixed-stride table for the packed-key descent when padding stays small
+        const int64_t n = (int64_t)ptr.size() - 1;
+        if (md > 0 && md <= 64 && n < (1 << 24) && n * md <= 4 * nnz) {
+            const int64_t words = (n * md + 3) & ~int64_t(3);
+            std::vector<int32_t> t(words, 0);
+            for (int64_t v = 0; v < n; ++v)
+                for (int32_t e = ptr[v]; e < ptr[v + 1]; ++e)
+                    t[v * md + (e - ptr[v])] = (int32_t)((uint32_t)(uint8_t)(int8_t)w[e] << 24) | idx[e];
+            HIPCHK(hipMalloc(&g->key_adj, words * 4));
+            HIPCHK(hipMemcpy(g->key_adj, t.data(), words * 4, hipMemcpyHostToDevice));
+            g->key_adj_stride = md;
+            g->device_bytes += words * 4;
+        }
+    }
     return PQI_OK;
 }
 
@@ -143,6 +165,7 @@ static void free_graph(pqi_graph* g) {
     if (g->adj_ptr) (void)hipFree(g->adj_ptr);
     if (g->adj_idx) (void)hipFree(g->adj_idx);
     if (g->adj_w) (void)hipFree(g->adj_w);
+    if (g->key_adj) (void)hipFree(g->key_adj);
     delete g;
 }
 
@@ -213,6 +236,7 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
     bool all_int = true;
     int64_t nnz = 0;
     long double w_up_l = 0.0L;
+    std::vector<double> strength(n, 0.0);  // row sums of |w|
     const int64_t TB = 64;
     for (int64_t i0 = 0; i0 < n; i0 += TB)
         for (int64_t j0 = i0; j0 < n; j0 += TB)
@@ -231,6 +255,8 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
                         maxabs = std::max(maxabs, std::fabs(a));
                         if (all_int && a != std::nearbyint(a)) all_int = false;
                         w_up_l += a;
+                        strength[i] += std::fabs(a);
+                        strength[j] += std::fabs(a);
                     }
                 }
     pqi_graph* g = new pqi_graph();
@@ -242,6 +268,8 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
     g->w_up_i = g->kind == PQI_WEIGHTS_I8 ? (int64_t)std::llround((double)w_up_l) : 0;
     g->nnz = nnz;
     g->use_adj = nnz <= n * n / 8;
+    if (g->kind == PQI_WEIGHTS_I8) g->max_strength = (int64_t)*std::max_element(strength.begin(), strength.end());
+    std::vector<double>().swap(strength);
     // dense copy, zero padded, in the storage type (one pitched copy)
     const size_t es = g->kind == PQI_WEIGHTS_I8 ? 1 : (g->kind == PQI_WEIGHTS_F32 ? 4 : 8);
     const size_t bytes = (size_t)g->n_pad * (size_t)g->n_pad * es;
@@ -538,6 +566,37 @@ int pqi_single_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, 
     CHK(read_out(c, d, &o));
     *k = o.k;
     *gain = is_spin_glass ? 2.0 * o.gain : o.gain;
+    return PQI_OK;
+}
+
+int pqi_single_flip_descent(pqi_graph* g, uint8_t* theta, int is_spin_glass, int64_t max_steps, int64_t* accepts,
+                            double* gain) {
+    if (!g || !theta || !accepts || !gain) return fail(PQI_ERR_INVALID, "null argument");
+    if (max_steps < 0) return fail(PQI_ERR_INVALID, "max_steps must be >= 0");
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    uint8_t* x;
+    void* h;
+    CHK(load_state(g, theta, &x, &h));
+    PhaseOut* d;
+    CHK(ensure(c, S_OUT, sizeof(PhaseOut), (void**)&d));
+    // bounded launches keep every kernel short
+    const int64_t cap = 16 * g->n + 4096;
+    int64_t done = 0;
+    double sum = 0.0;
+    PhaseOut o;
+    do {
+        const int64_t want = std::min(cap, max_steps - done);
+        launch_descent(*g, x, h, want, d, c->stream);
+        CHK(read_out(c, d, &o));
+        done += o.accepts;
+        sum += o.gain_sum;
+        if (o.accepts < want || want == 0) break;
+    } while (done < max_steps);
+    HIPCHK(hipMemcpyAsync(theta, x, g->n, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    *accepts = done;
+    *gain = is_spin_glass ? 2.0 * sum : sum;
     return PQI_OK;
 }
 

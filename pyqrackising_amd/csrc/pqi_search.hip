@@ -37,6 +37,8 @@ struct GView {
     const int32_t* ai;
     const WT* aw;
     int use_adj;
+    const int32_t* kadj;  // int8 weights: fixed-stride packed adjacency (or nullptr)
+    int64_t kstride;
 };
 
 template <typename WT>
@@ -49,6 +51,8 @@ static GView<WT> view(const Graph& g) {
     v.ai = g.adj_idx;
     v.aw = (const WT*)g.adj_w;
     v.use_adj = (g.use_adj || g.W == nullptr) ? 1 : 0;
+    v.kadj = g.key_adj;
+    v.kstride = g.key_adj ? g.key_adj_stride : 0;
     return v;
 }
 
@@ -153,6 +157,20 @@ __device__ __forceinline__ T wave_reduce_dpp(T v, Op op) {
     }
 }
 
+__device__ __forceinline__ int32_t max3i(int32_t a, int32_t b, int32_t c) { return max(max(a, b), c); }
+
+// Integer wave maximum; the identity (INT_MIN) as the DPP fill value lets
+// each step fold into one v_max_i32 with a DPP source.
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x114, 0xF, 0xF, false));  // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x118, 0xF, 0xF, false));  // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // First maximum over the wave: the largest value, and among the lanes holding
 // it the smallest index.  Every lane returns the winner.
 template <typename FT>
@@ -224,7 +242,8 @@ static void field_impl(const Graph& g, const uint8_t* x, void* h, hipStream_t s)
     } while (0)
 
 // Descent variant: 0 or 1 the full scan (measured faster on MI355X), 2 the
-// two-level argmax for sparse graphs.  Read once from PQI_DESCENT_VARIANT.
+// two-level argmax for sparse graphs, 3 the full scan without packed keys
+// for integer weights.  Read once from PQI_DESCENT_VARIANT.
 static int descent_variant() {
     static const int v = [] {
         const char* e = getenv("PQI_DESCENT_VARIANT");
@@ -456,6 +475,192 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
     }
 }
 
+// Integer-weight descent on packed keys.
+//   key_v = (Delta_v << (ib + 1)) | ((mask - v) << 1) | x_v,  mask = 2^ib - 1 >= n - 1.
+// The largest key carries the largest gain and, among equal gains, the lowest
+// node -- the first argmax of pyqrackising/spin_glass_solver.py:45 as one
+// 32-bit integer maximum (v_max3 over 4-wide LDS loads, then a DPP
+// reduction) -- and the spin rides in bit 0, so a flip is one read-modify-
+// write per neighbour.  The host enables this only when every row strength
+// sum_u |w_vu| is below 2^(30 - ib), so no key overflows at any point of the
+// descent.  Neighbours come from LDS when they fit there: a fixed-stride table
+// (row v at v * stride, entries (w << 24) | v', zero padded) built at upload,
+// or the CSR lists; otherwise from global adjacency lists or dense rows.
+// Same selection rule and result as the other descents.
+struct KeyLds {
+    int np;          // n rounded up to 256: key slots (padding holds INT_MIN)
+    size_t base;     // bytes of the keys alone
+    size_t fixed;    // + fixed-stride table (0 if none)
+    size_t csr;      // + CSR row pointers and entries (0 if no lists)
+};
+
+__host__ __device__ inline KeyLds key_lds_layout(int64_t n, int64_t nnz, int64_t stride, bool lists) {
+    KeyLds L;
+    L.np = (int)((n + 255) & ~(int64_t)255);
+    L.base = (size_t)L.np * 4;
+    L.fixed = stride > 0 ? L.base + (((size_t)n * (size_t)stride + 3) & ~(size_t)3) * 4 : 0;
+    L.csr = lists ? L.base + (((size_t)n + 1 + 3) & ~(size_t)3) * 4 + (size_t)nnz * 4 : 0;
+    return L;
+}
+
+// 0: unstaged, 1: fixed-stride table in LDS, 2: CSR lists in LDS
+__host__ __device__ inline int key_stage_mode(const KeyLds& L, size_t lds_bytes) {
+    if (L.fixed && L.fixed <= lds_bytes) return 1;
+    if (L.csr && L.csr <= lds_bytes) return 2;
+    return 0;
+}
+
+__device__ __forceinline__ int32_t key_of(int32_t d, int v, uint32_t x, int ib, int mask) {
+    return (int32_t)(((uint32_t)d << (ib + 1)) | ((uint32_t)(mask - v) << 1) | x);
+}
+
+// K[v] -= 2 s_k s_v w << (ib + 1), with s_v read from the key itself
+__device__ __forceinline__ void key_update(int32_t* K, int v, int sk2w, int ib) {
+    const int32_t k = K[v];
+    K[v] = (int32_t)((uint32_t)k - ((uint32_t)((k & 1) ? -sk2w : sk2w) << (ib + 1)));
+}
+
+__device__ void descent_key_body(const GView<int8_t>& g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                 PhaseOut* out, unsigned char* lds, size_t lds_bytes, int ib) {
+    const int lane = threadIdx.x;
+    const int n = (int)g.n;
+    const int mask = (1 << ib) - 1;
+    const int sh = ib + 1;
+    const KeyLds L = key_lds_layout(n, g.use_adj ? g.ap[n] : 0, g.kstride, g.use_adj != 0);
+    const int mode = max_steps > 0 ? key_stage_mode(L, lds_bytes) : 0;
+    int32_t* K = (int32_t*)lds;
+    int32_t* T = (int32_t*)(lds + L.base);  // mode 1: fixed table; mode 2: row pointers
+    int32_t* A = T + ((n + 1 + 3) & ~3);    // mode 2: entries
+    for (int v = lane; v < L.np; v += 64) {
+        if (v < n) {
+            const uint8_t xv = xg[v];
+            K[v] = key_of(xv ? -hg[v] : hg[v], v, xv, ib, mask);
+        } else {
+            K[v] = INT_MIN;
+        }
+    }
+    if (mode == 1) {
+        const int quads = (n * (int)g.kstride + 3) >> 2;  // the table is zero padded to whole quads
+        const int4* src = (const int4*)g.kadj;
+        int4* dst = (int4*)T;
+#pragma unroll 4
+        for (int q = lane; q < quads; q += 64) dst[q] = src[q];
+    } else if (mode == 2) {
+        for (int v = lane; v <= n; v += 64) T[v] = g.ap[v];
+        const int nnz = g.ap[n];
+        for (int e = lane; e < nnz; e += 64) A[e] = (int32_t)((uint32_t)(uint8_t)g.aw[e] << 24) | g.ai[e];
+    }
+    __syncthreads();
+    int64_t steps = 0, evals = 0;
+    double gsum = 0.0;
+    int32_t bv;
+    int bk;
+    const int4* K4 = (const int4*)K;
+    const int chunks = L.np >> 2;  // multiple of 64
+    const int stride = (int)g.kstride;
+    for (;;) {
+        // eight independent 16-byte loads in flight per lane
+        int32_t m0 = INT_MIN, m1 = INT_MIN, m2 = INT_MIN, m3 = INT_MIN;
+        int c = lane;
+        for (; c + 448 < chunks; c += 512) {
+            int4 a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = K4[c + 64 * u];
+            m0 = max(m0, max(max3i(a[0].x, a[0].y, a[0].z), max3i(a[0].w, a[1].x, a[1].y)));
+            m1 = max(m1, max(max3i(a[1].z, a[1].w, a[2].x), max3i(a[2].y, a[2].z, a[2].w)));
+            m2 = max(m2, max(max3i(a[3].x, a[3].y, a[3].z), max3i(a[3].w, a[4].x, a[4].y)));
+            m3 = max(m3, max(max3i(a[4].z, a[4].w, a[5].x), max3i(a[5].y, a[5].z, a[5].w)));
+            m0 = max(m0, max(max3i(a[6].x, a[6].y, a[6].z), max3i(a[6].w, a[7].x, a[7].y)));
+            m1 = max(m1, max(a[7].z, a[7].w));
+        }
+        for (; c < chunks; c += 64) {
+            const int4 a = K4[c];
+            m2 = max(m2, max(max(a.x, a.y), max(a.z, a.w)));
+        }
+        const int32_t m = wave_max_i32(max(max(m0, m1), max(m2, m3)));
+        bv = m >> sh;
+        bk = mask - ((m >> 1) & mask);
+        evals += n;
+        if (!(bv > 0) || steps >= max_steps) break;
+        const uint32_t xk = (uint32_t)m & 1u;
+        const int sk2 = xk ? -2 : 2;  // 2 s_k before the flip
+        if (mode == 1) {
+            for (int j = lane; j < stride; j += 64) {
+                const int32_t a = T[bk * stride + j];
+                if (a != 0) key_update(K, a & 0xFFFFFF, sk2 * (a >> 24), ib);
+            }
+        } else if (mode == 2) {
+            const int beg = T[bk], end = T[bk + 1];
+            for (int e = beg + lane; e < end; e += 64) {
+                const int32_t a = A[e];
+                key_update(K, a & 0xFFFFFF, sk2 * (a >> 24), ib);
+            }
+        } else if (g.use_adj) {
+            const int beg = g.ap[bk], end = g.ap[bk + 1];
+            for (int e = beg + lane; e < end; e += 64) key_update(K, g.ai[e], sk2 * (int)g.aw[e], ib);
+        } else {
+            // dense row, 4 nodes per lane per pass: W is zero beyond n and on
+            // the diagonal, and the np >= n_pad key slots cover the row
+            const char4* W4 = (const char4*)(g.W + (int64_t)bk * g.ld);
+            for (int q = lane; q < (int)(g.ld >> 2); q += 64) {
+                const char4 w = W4[q];
+                const int v = q * 4;
+                if (w.x) key_update(K, v, sk2 * w.x, ib);
+                if (w.y) key_update(K, v + 1, sk2 * w.y, ib);
+                if (w.z) key_update(K, v + 2, sk2 * w.z, ib);
+                if (w.w) key_update(K, v + 3, sk2 * w.w, ib);
+            }
+        }
+        if (lane == 0) K[bk] = key_of(-bv, bk, xk ^ 1u, ib, mask);
+        __syncthreads();
+        ++steps;
+        gsum += (double)bv;
+    }
+    if (steps) {
+        for (int v = lane; v < n; v += 64) {
+            const int32_t k = K[v];
+            const int32_t d = k >> sh;
+            hg[v] = (k & 1) ? -d : d;
+            xg[v] = (uint8_t)(k & 1);
+        }
+    }
+    if (lane == 0) {
+        out->gain = (double)bv;
+        out->k = bk;
+        out->l = -1;
+        out->accepts = steps;
+        out->evals = evals;
+        out->gain_sum = gsum;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_descent_key(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                                   PhaseOut* out, size_t lds_bytes, int ib) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    descent_key_body(g, xg, hg, max_steps, out, lds, lds_bytes, ib);
+}
+
+// Key width for the packed-key descent of g, or 0 when it does not apply.
+int key_shift(const Graph& g) {
+    if (g.kind != PQI_WEIGHTS_I8 || g.max_strength < 0 || g.n > (1 << 24)) return 0;
+    int ib = 1;
+    while ((int64_t(1) << ib) < g.n) ++ib;
+    if (ib > 29 || g.max_strength >= (int64_t(1) << (30 - ib))) return 0;
+    return ib;
+}
+
+// Dynamic LDS for the packed-key descent of g: the first staged layout that
+// fits in `limit`, else the keys alone.
+size_t key_lds_bytes(const Graph& g, size_t limit) {
+    const bool lists = g.use_adj || g.W == nullptr;
+    const KeyLds L = key_lds_layout(g.n, lists ? g.nnz : 0, g.key_adj_stride, lists);
+    switch (key_stage_mode(L, limit)) {
+        case 1: return L.fixed;
+        case 2: return L.csr;
+        default: return L.base;
+    }
+}
+
 // Sparse-graph variant with a two-level argmax: LDS keeps, for every group of
 // 64 consecutive nodes, the group's first maximum gain and its node.  A step
 // takes the first maximum over the group maxima (ties -> lowest node, since
@@ -588,6 +793,16 @@ __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, F
 template <typename WT>
 static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
+    if constexpr (sizeof(WT) == 1) {
+        static const size_t klimit = lds_limit((const void*)k_descent_key);
+        const int ib = key_shift(g);
+        const size_t kl = key_lds_bytes(g, klimit);
+        if (ib > 0 && kl <= klimit && descent_variant() != 3) {
+            hipLaunchKernelGGL(k_descent_key, dim3(1), dim3(64), kl, s, view<int8_t>(g), x, (int32_t*)h, max_steps,
+                               out, kl, ib);
+            return;
+        }
+    }
     const size_t lds = (((size_t)g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
     static const size_t limit = lds_limit((const void*)k_descent_wave<WT, FT>);
     static const size_t limit2 = lds_limit((const void*)k_descent_wave2<WT, FT>);
@@ -1097,6 +1312,8 @@ __device__ __forceinline__ GView<WT> gview_of(const BInstRaw& r) {
     v.ai = r.ai;
     v.aw = (const WT*)r.aw;
     v.use_adj = r.use_adj;
+    v.kadj = r.kadj;
+    v.kstride = r.kadj ? r.kstride : 0;
     return v;
 }
 
@@ -1106,6 +1323,19 @@ __global__ __launch_bounds__(64) void kb_descent_lds(const BInstRaw* insts, cons
     extern __shared__ __align__(16) unsigned char lds[];
     const BInstRaw& r = insts[list[blockIdx.x]];
     descent_wave_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out, lds);
+}
+
+__global__ __launch_bounds__(64) void kb_descent_key(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
+                                                    int which, size_t lds_bytes) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    descent_key_body(gview_of<int8_t>(r), which ? r.rx : r.x, (int32_t*)(which ? r.rh : r.h), max_steps, r.out, lds,
+                     lds_bytes, r.key_shift);
+}
+
+size_t b_key_lds_limit() {
+    static const size_t limit = lds_limit((const void*)kb_descent_key);
+    return limit;
 }
 
 template <typename WT, typename FT>
@@ -1233,8 +1463,13 @@ static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int6
 
 template <typename WT>
 static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                           int which, int all_sparse, hipStream_t s) {
+                           int which, int all_sparse, size_t key_lds, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
+    if (sizeof(WT) == 1 && key_lds > 0 && descent_variant() != 3) {
+        hipLaunchKernelGGL(kb_descent_key, dim3((unsigned)L), dim3(64), key_lds, s, d, list, max_steps, which,
+                           key_lds);
+        return;
+    }
     const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
     static const size_t limit = lds_limit((const void*)kb_descent_lds<WT, FT>);
     static const size_t limit2 = lds_limit((const void*)kb_descent_lds2<WT, FT>);
@@ -1311,8 +1546,8 @@ void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
     PQI_KIND_DISPATCH(kind, b_field_impl, d, list, L, max_n, s);
 }
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                      int which, int all_sparse, hipStream_t s) {
-    PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, all_sparse, s);
+                      int which, int all_sparse, size_t key_lds, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, all_sparse, key_lds, s);
 }
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s) {
     PQI_KIND_DISPATCH(kind, b_double_impl, d, list, L, which, s);

@@ -1,0 +1,154 @@
+"""Single-flip descent (the reference's single-flip phase loop,
+pyqrackising/spin_glass_solver.py:604-624) on the GPU, against the oracle's
+best-single-flip function applied step by step (small graphs) and against the
+textbook steepest-ascent rule -- first maximum of Delta_k = s_k h_k, flip while
+positive, h_v -= 2 w_vk s_k -- restated with numpy at sizes the oracle does not
+reach.  Cases cover every descent kernel: packed keys with the adjacency staged
+in LDS, packed keys over global adjacency (too large to stage), packed keys
+over dense int8 rows, the unpacked integer scan (row strengths too large for
+packed keys) and the FP64 scan for real weights."""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from pyqrackising_amd import _lib, graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def steepest(A, x):
+    """First-max steepest ascent of the cut from x; A symmetric (CSR or dense)."""
+    dense = not sp.issparse(A)
+    s = 1 - 2 * x.astype(np.int64 if A.dtype.kind in "iu" else np.float64)
+    h = A @ s
+    flips, gain = 0, 0
+    while True:
+        d = s * h
+        k = int(np.argmax(d))
+        if not d[k] > 0:
+            break
+        if dense:
+            h -= 2 * s[k] * A[k]
+        else:
+            lo, hi = A.indptr[k], A.indptr[k + 1]
+            h[A.indices[lo:hi]] -= 2 * s[k] * A.data[lo:hi]
+        s[k] = -s[k]
+        gain += d[k]
+        flips += 1
+    return ((1 - s) // 2 if A.dtype.kind in "iu" else (1 - s) / 2).astype(np.uint8), flips, gain
+
+
+def regular_int(n, d, seed, wmax=1):
+    rng = np.random.default_rng(seed)
+    lo, hi = graphs.random_regular_edges(n, d, rng)
+    w = rng.integers(1, wmax + 1, lo.size) * rng.choice([-1, 1], lo.size)
+    G = graphs.upper_csr(n, lo, hi, w.astype(np.float64), dtype=np.float32)
+    A = (G + G.T).tocsr().astype(np.int64)
+    A.sort_indices()
+    return G, A
+
+
+@pytest.mark.parametrize("n,d,wmax,seed", [(2048, 3, 1, 1), (12000, 3, 3, 2), (3000, 10, 5, 3)])
+def test_descent_sparse_int(n, d, wmax, seed):
+    log(f"descent sparse int n={n} d={d} wmax={wmax}")
+    G, A = regular_int(n, d, seed, wmax)
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = np.random.default_rng(seed + 10).integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, False)
+    xr, fr, gr = steepest(A, x0)
+    assert acc == fr and gain == gr
+    assert np.array_equal(x1, xr)
+
+
+@pytest.mark.parametrize("n,wmax,seed", [(1024, 127, 4), (8192, 127, 5), (700, 1, 6)])
+def test_descent_dense_int(n, wmax, seed):
+    """Dense int8 rows; at n=8192 the row strengths exceed the packed-key range."""
+    log(f"descent dense int n={n} wmax={wmax}")
+    rng = np.random.default_rng(seed)
+    U = np.triu(rng.integers(-wmax, wmax + 1, (n, n), dtype=np.int8), 1)
+    W = U + U.T
+    dg = _lib.DeviceGraph.dense(W)
+    x0 = rng.integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, True)
+    xr, fr, gr = steepest(W.astype(np.int64), x0)
+    assert acc == fr and gain == 2 * gr
+    assert np.array_equal(x1, xr)
+
+
+def test_descent_sparse_real():
+    log("descent sparse real")
+    rng = np.random.default_rng(7)
+    n = 5000
+    lo, hi = graphs.erdos_renyi_edges_sparse(n, 8.0 / (n - 1), rng)
+    G = graphs.upper_csr(n, lo, hi, rng.uniform(-1, 1, lo.size), dtype=np.float32)
+    A = (G + G.T).tocsr().astype(np.float64)
+    A.sort_indices()
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = rng.integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, False)
+    xr, fr, gr = steepest(A, x0)
+    assert acc == fr
+    assert np.array_equal(x1, xr)
+    assert gain == pytest.approx(gr, rel=1e-9)
+
+
+def test_descent_small_against_oracle(orc):
+    """Oracle best single flip, applied until no gain, on small int graphs."""
+    rng = np.random.default_rng(8)
+    for t in range(6):
+        n = int(rng.integers(20, 300))
+        U = np.triu(rng.integers(-3, 4, (n, n)) * (rng.random((n, n)) < 0.1), 1)
+        W = (U + U.T).astype(np.int32)
+        x = rng.integers(0, 2, n).astype(np.uint8)
+        log(f"descent vs oracle n={n}")
+        dg = _lib.DeviceGraph.dense(W)
+        x1, acc, gain = dg.single_flip_descent(x, True)
+        xo, steps, total = x.copy(), 0, 0.0
+        while True:
+            g, k = orc.single_bit_flips(xo, 1, W.astype(np.float64))
+            if not g > 0:
+                break
+            xo[k] ^= 1
+            total += g
+            steps += 1
+        assert acc == steps and gain == total
+        assert np.array_equal(x1, xo)
+
+
+def test_descent_max_steps():
+    G, A = regular_int(600, 4, 9)
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = np.random.default_rng(19).integers(0, 2, 600).astype(np.uint8)
+    x1, acc, _ = dg.single_flip_descent(x0, False, max_steps=5)
+    assert acc == 5
+    s = 1 - 2 * x0.astype(np.int64)
+    h = A @ s
+    for _ in range(5):
+        k = int(np.argmax(s * h))
+        lo, hi = A.indptr[k], A.indptr[k + 1]
+        h[A.indices[lo:hi]] -= 2 * s[k] * A.data[lo:hi]
+        s[k] = -s[k]
+    assert np.array_equal(x1, ((1 - s) // 2).astype(np.uint8))
+
+
+def test_batched_staging_mix_matches_single():
+    """One instance too large to stage its adjacency beside small ones that
+    stage theirs: every instance equals its single-instance search."""
+    log("batched staging mix")
+    Gs = [regular_int(n, 3, 40 + i)[0] for i, n in enumerate((12000, 300, 1000, 258))]
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    rng = np.random.default_rng(41)
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in Gs]
+    p = _lib.make_params(0, is_spin_glass=0, walkers=4, seed_multiple=2, gray_iterations=2000)
+    states, stats = _lib.batch_local_search(dgs, starts, p)
+    for i, dg in enumerate(dgs):
+        xs, st = dg.local_search(starts[i], p)
+        assert np.array_equal(states[i], xs), i
+        assert stats[i]["cut_value"] == st["cut_value"], i
