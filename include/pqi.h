@@ -97,7 +97,9 @@ typedef struct {
 } pqi_search_params;
 
 /* pqi_search_params.flags */
-#define PQI_FLAG_TIME_KERNELS 1 /* time the walk and seed kernels with HIP events on the context stream */
+#define PQI_FLAG_TIME_KERNELS 1 /* time the descent and main walk kernels with HIP events on the context stream */
+#define PQI_FLAG_HOST_LOOP 2    /* decide each phase on the host (one synchronisation per phase) instead of
+                                   on the device (one per loop iteration); same result */
 
 typedef struct {
     double max_energy; /* reference bookkeeping: start value plus accepted gains */
@@ -110,8 +112,9 @@ typedef struct {
     int64_t gray_evals_computed; /* of those, gains computed (the rest repeat a rejected step) */
     double seconds_single, seconds_double, seconds_seeds, seconds_walk, seconds_reheat;
     /* with PQI_FLAG_TIME_KERNELS: summed event-timed kernel durations and launch counts */
-    double walk_kernel_seconds, seed_kernel_seconds;
-    int64_t walk_launches, seed_launches;
+    double walk_kernel_seconds, descent_kernel_seconds;
+    int64_t walk_launches, descent_launches;
+    int64_t descent_evals; /* single-flip evaluations of the descents (single_evals less best-only checks) */
     /* learned phase (pqi_learned_search) */
     int64_t learned_accepts, learned_passes;
     double seconds_learned;

@@ -27,6 +27,7 @@ WEIGHTS_I8, WEIGHTS_F32, WEIGHTS_F64 = 1, 3, 4
 DOUBLE_REFERENCE, DOUBLE_EXHAUSTIVE = 0, 1
 SEEDS_AUTO, SEEDS_MFMA, SEEDS_LOWRANK = 0, 1, 2
 FLAG_TIME_KERNELS = 1
+FLAG_HOST_LOOP = 2
 ENGINE_AUTO, ENGINE_EXACT, ENGINE_BF16 = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -87,9 +88,10 @@ class SearchStats(ctypes.Structure):
         ("seconds_walk", ctypes.c_double),
         ("seconds_reheat", ctypes.c_double),
         ("walk_kernel_seconds", ctypes.c_double),
-        ("seed_kernel_seconds", ctypes.c_double),
+        ("descent_kernel_seconds", ctypes.c_double),
         ("walk_launches", ctypes.c_int64),
-        ("seed_launches", ctypes.c_int64),
+        ("descent_launches", ctypes.c_int64),
+        ("descent_evals", ctypes.c_int64),
         ("learned_accepts", ctypes.c_int64),
         ("learned_passes", ctypes.c_int64),
         ("seconds_learned", ctypes.c_double),

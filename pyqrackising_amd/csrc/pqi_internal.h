@@ -47,6 +47,16 @@ struct Buffer {
 
 }  // namespace pqi
 
+namespace pqi {
+// Captured segments of one local-search iteration, reused while every
+// pointer and size they were captured with (the key) is unchanged.
+struct LsGraphs {
+    std::vector<int64_t> key;
+    hipGraphExec_t seg[2] = {nullptr, nullptr};
+    bool failed = false;  // capture is not available: launch directly
+};
+}  // namespace pqi
+
 struct pqi_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -54,6 +64,7 @@ struct pqi_ctx {
     std::vector<pqi::Buffer> bufs;  // indexed by slot
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    pqi::LsGraphs ls;
 };
 
 struct pqi_graph : pqi::Graph {};
@@ -126,11 +137,47 @@ void launch_gather(const double* v, const int64_t* sel, int64_t k, double* out, 
 
 // Gray walk: walker w starts from seed sel[w] of (x, h) and walks; writes its
 // final state/field to wx/wh (row stride n), its total cut difference to we[w]
-// and the number of gain evaluations it computed to wevals[w].
+// and the number of gain evaluations it computed to wevals[w].  With a gate,
+// the launch does nothing when *gate != 0 at the time it runs.
 void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel,
                       const double* sel_delta, int64_t block_size, int64_t walkers,
                       int64_t thread_iterations, uint8_t* wx, void* wh, double* we, int64_t* wevals,
-                      hipStream_t s);
+                      hipStream_t s, const int32_t* gate = nullptr);
+
+// ---- device-resident control of the single-instance search loop ----------------
+// One iteration of the loop (descent, double flip, Gray seeds, Gray walk,
+// reheat) is enqueued as a whole; small control kernels read each phase's
+// result on the device, accept or reject its move and skip the rest of the
+// iteration once a move was accepted.  The host synchronises once per
+// iteration.  Every accumulation happens in the order of the reference loop.
+struct LsCtl {
+    int32_t restart;    // a move of this iteration was accepted: later phases skip
+    int32_t again;      // the descent hit its step bound: the next iteration continues it
+    int32_t done;       // the search ended
+    int32_t skip_seed;  // the best seed is not taken (gate of the one-seed walk)
+    int32_t seeds_ran, walk_ran;
+    int32_t phase, pad;
+    int64_t rounds;
+    double max_energy, scale, reheat, top;
+    int64_t single_evals, single_accepts, double_evals, double_accepts, seed_evals, seed_accepts, walk_accepts,
+        reheat_accepts, gray_evals, gray_evals_computed, descent_evals;
+    uint64_t mark;      // device wall clock at the last phase change
+    uint64_t ticks[5];  // per phase: single, double, seeds, walk, reheat
+};
+void launch_ls_begin(LsCtl* c, hipStream_t s);
+void launch_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap, hipStream_t s);
+void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, int64_t evals,
+                            hipStream_t s);
+void launch_ls_after_seeds(LsCtl* c, const double* seld, int64_t count, hipStream_t s);
+void launch_ls_adopt_seed(const Graph& g, LsCtl* c, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
+                          hipStream_t s);
+void launch_ls_after_walk(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
+                          int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
+                          void* rh, hipStream_t s);
+void launch_ls_reheat_single(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, uint8_t* rx,
+                             void* rh, hipStream_t s);
+void launch_ls_reheat_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, uint8_t* rx,
+                             void* rh, int64_t evals, hipStream_t s);
 
 // q = s^T h of the state (x, h), FP64 (exact int64 arithmetic for int8 weights).
 void launch_quad(const Graph& g, const uint8_t* x, const void* h, double* q, hipStream_t s);

@@ -44,7 +44,7 @@ int fail(int code, const std::string& msg) {
 // Scratch slots of a context.
 enum Slot {
     S_X, S_H, S_RX, S_RH, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
-    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_COUNT
+    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_COUNT
 };
 
 static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
@@ -158,6 +158,16 @@ static int classify(double maxabs, bool all_int, int dtype) {
     return PQI_WEIGHTS_F64;
 }
 
+static void ls_graphs_reset(pqi_ctx* c) {
+    for (auto& e : c->ls.seg)
+        if (e) {
+            (void)hipGraphExecDestroy(e);
+            e = nullptr;
+        }
+    c->ls.key.clear();
+    c->ls.failed = false;
+}
+
 static void free_graph(pqi_graph* g) {
     if (!g) return;
     if (g->W) (void)hipFree(g->W);
@@ -217,6 +227,7 @@ int pqi_destroy(pqi_ctx* c) {
     if (!c) return PQI_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    ls_graphs_reset(c);
     for (auto& b : c->bufs)
         if (b.p) (void)hipFree(b.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -406,6 +417,7 @@ int pqi_graph_free(pqi_graph* g) {
     if (!g) return PQI_OK;
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
+    ls_graphs_reset(g->ctx);  // captured launches may reference this graph
     free_graph(g);
     return PQI_OK;
 }
@@ -652,6 +664,28 @@ static int check_params(const pqi_graph* g, const pqi_search_params* p) {
 }
 
 // Score every Gray seed of (x, h) and select the top `walkers` (stable order).
+// Allocate every buffer gray_seeds uses (so that no allocation happens while
+// kernels are queued).
+static int gray_seeds_buffers(pqi_graph* g, const pqi_search_params* p, GrayBufs& b) {
+    pqi_ctx* c = g->ctx;
+    const int64_t nb = (g->n + 63) >> 6;
+    const int64_t count = p->walkers * p->seed_multiple * nb;
+    CHK(ensure(c, S_DELTA, count * 8, (void**)&b.delta));
+    CHK(ensure(c, S_SEL, p->walkers * 8, (void**)&b.sel));
+    CHK(ensure(c, S_SELD, p->walkers * 8, (void**)&b.seld));
+    if (g->kind == PQI_WEIGHTS_I8 && g->W != nullptr && p->seed_engine != PQI_SEEDS_LOWRANK) {
+        const int64_t wpr = g->n_pad >> 5;
+        void* t;
+        CHK(ensure(c, S_BITS, (size_t)count * wpr * 4, &t));
+        CHK(ensure(c, S_Q, count * 8, &t));
+        CHK(ensure(c, S_QB, 16, &t));
+        CHK(ensure(c, S_BATCH, batch_scratch_bytes(*g, count + 1), &t));
+    }
+    void* rank;
+    CHK(ensure(c, S_RANK, topk_scratch_bytes(count), &rank));
+    return PQI_OK;
+}
+
 static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_search_params* p, GrayBufs& b,
                       int64_t* seed_count) {
     pqi_ctx* c = g->ctx;
@@ -787,6 +821,173 @@ int pqi_gray_search(pqi_graph* g, const uint8_t* theta, const pqi_search_params*
     return sync(c);
 }
 
+// Host-driven loop: one synchronisation per phase decision (PQI_FLAG_HOST_LOOP).
+static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st);
+
+// Device-driven loop: one iteration of the reference loop is enqueued as a
+// whole and decided by control kernels (LsCtl); one synchronisation per
+// iteration.  Same moves, counters and max_energy as the host-driven loop.
+static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st) {
+    pqi_ctx* c = g->ctx;
+    const int64_t n = g->n;
+    const size_t fb = field_bytes(*g);
+    const int64_t C = n * (n - 1) / 2;
+    const int64_t G = p->walkers;
+    const int64_t T = (p->gray_iterations + G - 1) / G;
+    const int64_t dbl_evals = p->double_mode == PQI_DOUBLE_EXHAUSTIVE ? C : std::min(G, C);
+
+    uint8_t *x, *rx;
+    void *h, *rh;
+    CHK(load_state(g, theta, &x, &h));
+    PhaseOut *dout, *dout2;
+    void* dsc;
+    LsCtl* ctl;
+    CHK(ensure(c, S_RX, n, (void**)&rx));
+    CHK(ensure(c, S_RH, n * fb, &rh));
+    CHK(ensure(c, S_OUT, sizeof(PhaseOut), (void**)&dout));
+    CHK(ensure(c, S_OUT2, sizeof(PhaseOut), (void**)&dout2));
+    CHK(ensure(c, S_DBL, double_scratch_bytes(*g), &dsc));
+    CHK(ensure(c, S_CTL, sizeof(LsCtl), (void**)&ctl));
+    // every buffer of the iteration exists before the first launch (no
+    // reallocation while kernels are queued)
+    GrayBufs b;
+    int64_t count;
+    CHK(ensure(c, S_WX, (size_t)G * n, (void**)&b.wx));
+    CHK(ensure(c, S_WH, (size_t)G * n * fb, &b.wh));
+    CHK(ensure(c, S_WE, G * 8, (void**)&b.we));
+    CHK(ensure(c, S_WEV, G * 8, (void**)&b.wev));
+    CHK(gray_seeds_buffers(g, p, b));
+
+    double cut0, e0;
+    CHK(score_device(g, x, &cut0, &e0));
+    LsCtl hc;
+    std::memset(&hc, 0, sizeof(hc));
+    hc.max_energy = p->is_spin_glass ? e0 : cut0;
+    hc.scale = p->is_spin_glass ? 2.0 : 1.0;
+    hc.phase = -1;
+    HIPCHK(hipMemcpyAsync(ctl, &hc, sizeof(hc), hipMemcpyHostToDevice, c->stream));
+    // the reheat state is read by speculative launches before its first use
+    HIPCHK(hipMemcpyAsync(rx, x, n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(rh, h, n * fb, hipMemcpyDeviceToDevice, c->stream));
+
+    const bool timing = (p->flags & PQI_FLAG_TIME_KERNELS) != 0;
+    EventTimer walk_timer(timing), descent_timer(timing);
+    const int64_t cap = 16 * n + 4096;  // bounded descent launches
+    const int64_t gray_evals = G * reference_gray_steps(T, n);
+    const int64_t bs = p->walkers * p->seed_multiple;
+    // segment 0: after the descent, up to the main walk; segment 1: after it
+    auto segment = [&](int which) -> int {
+        if (which == 0) {
+            launch_ls_after_descent(ctl, dout, cap, c->stream);
+            launch_double_best(*g, x, h, G, p->double_mode, dout2, dsc, c->stream);
+            launch_ls_after_double(*g, ctl, dout2, x, h, dbl_evals, c->stream);
+            CHK(gray_seeds(g, x, h, p, b, &count));
+            launch_ls_after_seeds(ctl, b.seld, count, c->stream);
+            // the best seed as a state: a one-walker walk of zero iterations
+            launch_gray_walk(*g, x, h, b.sel, b.seld, bs, 1, 0, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->skip_seed);
+            launch_ls_adopt_seed(*g, ctl, b.wx, b.wh, x, h, c->stream);
+        } else {
+            launch_ls_after_walk(*g, ctl, b.we, b.wev, G, gray_evals, b.wx, b.wh, x, h, rx, rh, c->stream);
+            // post-reheat phase (spin_glass_solver.py:670-717)
+            launch_single_best(*g, rx, rh, dout, c->stream);
+            launch_ls_reheat_single(*g, ctl, dout, x, h, rx, rh, c->stream);
+            launch_double_best(*g, rx, rh, G, p->double_mode, dout2, dsc, c->stream);
+            launch_ls_reheat_double(*g, ctl, dout2, x, h, rx, rh, dbl_evals, c->stream);
+        }
+        return PQI_OK;
+    };
+    // captured once per (graph, parameters, buffers) and replayed per iteration
+    const std::vector<int64_t> key = {(int64_t)g, (int64_t)g->W, (int64_t)g->adj_ptr, n, G, T, bs,
+                                      p->double_mode, p->seed_engine, p->is_spin_glass, (int64_t)x, (int64_t)h,
+                                      (int64_t)rx, (int64_t)rh, (int64_t)dout, (int64_t)dout2, (int64_t)dsc,
+                                      (int64_t)ctl, (int64_t)b.wx, (int64_t)b.wh, (int64_t)b.we, (int64_t)b.wev,
+                                      (int64_t)b.sel, (int64_t)b.seld, (int64_t)b.delta,
+                                      (int64_t)c->bufs[S_BITS].p, (int64_t)c->bufs[S_Q].p,
+                                      (int64_t)c->bufs[S_QB].p, (int64_t)c->bufs[S_BATCH].p,
+                                      (int64_t)c->bufs[S_RANK].p};
+    if (c->ls.key != key) {
+        ls_graphs_reset(c);
+        for (int w = 0; w < 2 && !c->ls.failed; ++w) {
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                c->ls.failed = true;
+                break;
+            }
+            const int r = segment(w);
+            const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (r != PQI_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return r;
+            }
+            if (e != hipSuccess || !graph ||
+                hipGraphInstantiate(&c->ls.seg[w], graph, nullptr, nullptr, 0) != hipSuccess)
+                c->ls.failed = true;
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+        (void)hipGetLastError();
+        if (c->ls.failed) ls_graphs_reset(c), c->ls.failed = true;
+        c->ls.key = key;
+    }
+    auto run_segment = [&](int which) -> int {
+        if (c->ls.seg[which]) {
+            HIPCHK(hipGraphLaunch(c->ls.seg[which], c->stream));
+            return PQI_OK;
+        }
+        return segment(which);
+    };
+    for (;;) {
+        if (!hc.again && p->max_rounds > 0 && hc.rounds >= p->max_rounds) break;
+        launch_ls_begin(ctl, c->stream);
+        descent_timer.start(c->stream);
+        launch_descent(*g, x, h, cap, dout, c->stream);
+        descent_timer.stop(c->stream);
+        CHK(run_segment(0));
+        walk_timer.start(c->stream);
+        launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
+        walk_timer.stop(c->stream);
+        CHK(run_segment(1));
+        HIPCHK(hipMemcpyAsync(&hc, ctl, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
+        CHK(sync(c));
+        if (timing) {
+            st->descent_kernel_seconds += descent_timer.seconds();
+            st->descent_launches++;
+            if (hc.walk_ran) {
+                st->walk_kernel_seconds += walk_timer.seconds();
+                st->walk_launches++;
+            }
+        }
+        if (hc.done) break;
+    }
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    const double tick = 1e-3 / rate_khz;
+    st->rounds = hc.rounds;
+    st->max_energy = hc.max_energy;
+    st->single_evals = hc.single_evals;
+    st->single_accepts = hc.single_accepts;
+    st->double_evals = hc.double_evals;
+    st->double_accepts = hc.double_accepts;
+    st->seed_evals = hc.seed_evals;
+    st->seed_accepts = hc.seed_accepts;
+    st->walk_accepts = hc.walk_accepts;
+    st->reheat_accepts = hc.reheat_accepts;
+    st->gray_evals = hc.gray_evals;
+    st->gray_evals_computed = hc.gray_evals_computed;
+    st->descent_evals = hc.descent_evals;
+    st->seconds_single = hc.ticks[0] * tick;
+    st->seconds_double = hc.ticks[1] * tick;
+    st->seconds_seeds = hc.ticks[2] * tick;
+    st->seconds_walk = hc.ticks[3] * tick;
+    st->seconds_reheat = hc.ticks[4] * tick;
+    double cut, en;
+    CHK(score_device(g, x, &cut, &en));
+    st->cut_value = cut;
+    st->min_energy = p->is_spin_glass ? -en : en;
+    HIPCHK(hipMemcpyAsync(theta, x, n, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
 int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st) {
     if (!g || !theta || !st) return fail(PQI_ERR_INVALID, "null argument");
     CHK(check_params(g, p));
@@ -794,6 +995,11 @@ int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, p
     pqi_ctx* c = g->ctx;
     CHK(set_device(c));
     std::memset(st, 0, sizeof(*st));
+    return (p->flags & PQI_FLAG_HOST_LOOP) ? local_search_host(g, theta, p, st) : local_search_device(g, theta, p, st);
+}
+
+static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st) {
+    pqi_ctx* c = g->ctx;
     const int64_t n = g->n;
     const size_t fb = field_bytes(*g);
     const double scale = p->is_spin_glass ? 2.0 : 1.0;
@@ -821,7 +1027,7 @@ int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, p
     std::vector<double> we(G);
     GrayBufs b;
     const bool timing = (p->flags & PQI_FLAG_TIME_KERNELS) != 0;
-    EventTimer walk_timer(timing), seed_timer(timing);
+    EventTimer walk_timer(timing), descent_timer(timing);
     for (;;) {
         if (p->max_rounds > 0 && st->rounds >= p->max_rounds) break;
         st->rounds++;
@@ -830,8 +1036,15 @@ int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, p
         // bounded launches keep every kernel short; repeat while the bound was hit
         const int64_t cap = 16 * n + 4096;
         do {
+            descent_timer.start(c->stream);
             launch_descent(*g, x, h, cap, dout, c->stream);
+            descent_timer.stop(c->stream);
             CHK(read_out(c, dout, &o));
+            if (timing) {
+                st->descent_kernel_seconds += descent_timer.seconds();
+                st->descent_launches++;
+            }
+            st->descent_evals += o.evals;
             st->single_evals += o.evals;
             st->single_accepts += o.accepts;
             st->max_energy += scale * o.gain_sum;
@@ -854,15 +1067,11 @@ int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, p
 
         t0 = now();
         int64_t count;
-        seed_timer.start(c->stream);
         CHK(gray_seeds(g, x, h, p, b, &count));
-        seed_timer.stop(c->stream);
         st->seed_evals += count;
         double top;
         HIPCHK(hipMemcpyAsync(&top, b.seld, 8, hipMemcpyDeviceToHost, c->stream));
         CHK(sync(c));
-        st->seed_kernel_seconds += seed_timer.seconds();
-        st->seed_launches += timing ? 1 : 0;
         if (top > 0.0) {
             CHK(gray_walk(g, x, h, p, b, 1, 0));
             HIPCHK(hipMemcpyAsync(x, b.wx, n, hipMemcpyDeviceToDevice, c->stream));

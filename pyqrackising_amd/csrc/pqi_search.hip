@@ -821,8 +821,21 @@ void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, Phas
     PQI_DISPATCH(g, descent_impl, g, x, h, max_steps, out, s);
 }
 
+// Best single flip only: one scan of Delta_k = s_k h_k over the state in
+// global memory by a 1024-thread work group (no staging, no moves).
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_single_best(GView<WT> g, uint8_t* x, FT* h, PhaseOut* out) {
+    descent_body<WT, FT>(g, x, h, 0, out);
+}
+
+template <typename WT>
+static void single_best_impl(const Graph& g, const uint8_t* x, const void* h, PhaseOut* out, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_single_best<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), (uint8_t*)x, (FT*)h, out);
+}
+
 void launch_single_best(const Graph& g, const uint8_t* x, const void* h, PhaseOut* out, hipStream_t s) {
-    PQI_DISPATCH(g, descent_impl, g, (uint8_t*)x, (void*)h, 0, out, s);
+    PQI_DISPATCH(g, single_best_impl, g, x, h, out, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1273,27 +1286,288 @@ __device__ void gray_walk_body(const GView<WT>& g, const uint8_t* x0, const FT* 
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0, const FT* h0, const int64_t* sel,
                                                  const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx,
-                                                 FT* wh, double* we, int64_t* wevals, int use_lds) {
+                                                 FT* wh, double* we, int64_t* wevals, int use_lds,
+                                                 const int32_t* gate) {
     extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
     gray_walk_body<WT, FT>(g, x0, h0, sel, sel_delta, bs, T, wx, wh, we, wevals, use_lds, lds, blockIdx.x);
 }
 
 template <typename WT>
 static void walk_impl(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel, const double* sel_delta,
                       int64_t bs, int64_t walkers, int64_t T, uint8_t* wx, void* wh, double* we, int64_t* wevals,
-                      hipStream_t s) {
+                      hipStream_t s, const int32_t* gate) {
     using FT = typename FieldOf<WT>::T;
     size_t lds = ((g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
     static const size_t limit = lds_limit((const void*)k_gray_walk<WT, FT>);
     int use_lds = lds <= limit ? 1 : 0;
     hipLaunchKernelGGL((k_gray_walk<WT, FT>), dim3((unsigned)walkers), dim3(64), use_lds ? lds : 0, s, view<WT>(g),
-                       x, (const FT*)h, sel, sel_delta, bs, T, wx, (FT*)wh, we, wevals, use_lds);
+                       x, (const FT*)h, sel, sel_delta, bs, T, wx, (FT*)wh, we, wevals, use_lds, gate);
 }
 
 void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel, const double* sel_delta,
                       int64_t block_size, int64_t walkers, int64_t thread_iterations, uint8_t* wx, void* wh,
-                      double* we, int64_t* wevals, hipStream_t s) {
-    PQI_DISPATCH(g, walk_impl, g, x, h, sel, sel_delta, block_size, walkers, thread_iterations, wx, wh, we, wevals, s);
+                      double* we, int64_t* wevals, hipStream_t s, const int32_t* gate) {
+    PQI_DISPATCH(g, walk_impl, g, x, h, sel, sel_delta, block_size, walkers, thread_iterations, wx, wh, we, wevals, s,
+                 gate);
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident control of the single-instance search loop
+// (pyqrackising/spin_glass_solver.py:603-717).  Each control kernel reads the
+// previous phase's PhaseOut, applies the accepted move to (x, h) with the
+// whole work group, and updates the counters and max_energy in the order of
+// the reference loop.  Control values are read by every thread before thread
+// 0 writes any of them, so branches stay uniform.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void ls_mark(LsCtl* c, int phase) {
+    const uint64_t now = wall_clock64();
+    if (c->phase >= 0) c->ticks[c->phase] += now - c->mark;
+    c->mark = now;
+    c->phase = phase;
+}
+
+__global__ void k_ls_begin(LsCtl* c) {
+    if (!c->again) c->rounds++;
+    c->again = 0;
+    c->restart = 0;
+    c->skip_seed = 1;
+    c->seeds_ran = 0;
+    c->walk_ran = 0;
+    ls_mark(c, 0);
+}
+
+__global__ void k_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap) {
+    c->single_evals += o->evals;
+    c->descent_evals += o->evals;
+    c->single_accepts += o->accepts;
+    c->max_energy += c->scale * o->gain_sum;
+    if (o->accepts == cap) {
+        // the bound was hit: continue the descent in the next iteration (same round)
+        c->again = 1;
+        c->restart = 1;
+    } else {
+        ls_mark(c, 1);
+    }
+}
+
+template <typename FT, typename WT>
+__device__ __forceinline__ void copy_state(uint8_t* xd, FT* hd, const uint8_t* xs, const FT* hs, int64_t n) {
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+        xd[v] = xs[v];
+        hd[v] = hs[v];
+    }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_after_double(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
+                                                         FT* h, int64_t evals) {
+    if (c->restart) return;
+    const double gain = o->gain;
+    const int64_t k = o->k, l = o->l;
+    __syncthreads();
+    if (gain > 0.0) flip_body<WT, FT>(g, x, h, k, l);
+    if (threadIdx.x == 0) {
+        c->double_evals += evals;
+        if (gain > 0.0) {
+            c->max_energy += c->scale * gain;
+            c->double_accepts++;
+            c->restart = 1;
+        } else {
+            c->seeds_ran = 1;
+            ls_mark(c, 2);
+        }
+    }
+}
+
+__global__ void k_ls_after_seeds(LsCtl* c, const double* seld, int64_t count) {
+    if (c->restart) return;
+    c->seed_evals += count;
+    c->top = seld[0];
+    c->skip_seed = c->top > 0.0 ? 0 : 1;
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_adopt_seed(GView<WT> g, LsCtl* c, const uint8_t* wx, const FT* wh,
+                                                       uint8_t* x, FT* h) {
+    if (c->restart) return;
+    const int take = !c->skip_seed;
+    __syncthreads();
+    if (take) copy_state<FT, WT>(x, h, wx, wh, g.n);
+    if (threadIdx.x == 0) {
+        if (take) {
+            c->max_energy += c->scale * c->top;
+            c->seed_accepts++;
+            c->restart = 1;
+        } else {
+            c->walk_ran = 1;
+            ls_mark(c, 3);
+        }
+    }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_after_walk(GView<WT> g, LsCtl* c, const double* we, const int64_t* wev,
+                                                       int64_t walkers, int64_t gray_evals, const uint8_t* wx,
+                                                       const FT* wh, uint8_t* x, FT* h, uint8_t* rx, FT* rh) {
+    __shared__ double sv[16];
+    __shared__ int64_t sk[16];
+    __shared__ int64_t ssum[16];
+    if (c->restart) return;
+    // first maximum over the walkers (value, then lowest walker) and the
+    // number of computed evaluations
+    double bv = -INFINITY;
+    int64_t bw = LLONG_MAX, computed = 0;
+    for (int64_t w = threadIdx.x; w < walkers; w += blockDim.x) {
+        if (better(we[w], w, bv, bw)) { bv = we[w]; bw = w; }
+        computed += wev[w];
+    }
+    computed = wave_sum(computed);
+    if ((threadIdx.x & 63) == 0) ssum[threadIdx.x >> 6] = computed;
+    block_argmax(bv, bw, sv, sk);  // synchronises
+    const double walk = bv;
+    const int64_t n = g.n;
+    if (walk > 0.0) copy_state<FT, WT>(x, h, wx + bw * n, wh + bw * n, n);
+    else copy_state<FT, WT>(rx, rh, wx + bw * n, wh + bw * n, n);
+    if (threadIdx.x == 0) {
+        computed = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) computed += ssum[i];
+        c->gray_evals_computed += computed;
+        c->gray_evals += gray_evals;
+        if (walk > 0.0) {
+            c->max_energy += c->scale * walk;
+            c->walk_accepts++;
+            c->restart = 1;
+        } else {
+            c->reheat = walk;
+            ls_mark(c, 4);
+        }
+    }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_reheat_single(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
+                                                          FT* h, uint8_t* rx, FT* rh) {
+    if (c->restart) return;
+    const double gain = o->gain, reheat = c->reheat;
+    const int64_t k = o->k;
+    __syncthreads();
+    const bool take = gain + reheat > 0.0;
+    if (take) {
+        copy_state<FT, WT>(x, h, rx, rh, g.n);
+        __syncthreads();
+        flip_body<WT, FT>(g, x, h, k, -1);
+    } else if (gain > 0.0) {
+        flip_body<WT, FT>(g, rx, rh, k, -1);
+    }
+    if (threadIdx.x == 0) {
+        c->single_evals += o->evals;
+        if (take) {
+            c->max_energy += c->scale * (reheat + gain);
+            c->reheat_accepts++;
+            c->restart = 1;
+        } else if (gain > 0.0) {
+            c->reheat = reheat + gain;
+        }
+    }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_reheat_double(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
+                                                          FT* h, uint8_t* rx, FT* rh, int64_t evals) {
+    if (c->restart) return;
+    const double gain = o->gain, reheat = c->reheat;
+    const int64_t k = o->k, l = o->l;
+    __syncthreads();
+    const bool take = gain + reheat > 0.0;
+    if (take) {
+        copy_state<FT, WT>(x, h, rx, rh, g.n);
+        __syncthreads();
+        flip_body<WT, FT>(g, x, h, k, l);
+    }
+    if (threadIdx.x == 0) {
+        c->double_evals += evals;
+        if (take) {
+            c->max_energy += c->scale * (reheat + gain);
+            c->reheat_accepts++;
+            c->restart = 1;
+        } else {
+            c->done = 1;
+            ls_mark(c, -1);
+        }
+    }
+}
+
+void launch_ls_begin(LsCtl* c, hipStream_t s) { hipLaunchKernelGGL(k_ls_begin, dim3(1), dim3(1), 0, s, c); }
+
+void launch_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_ls_after_descent, dim3(1), dim3(1), 0, s, c, o, cap);
+}
+
+template <typename WT>
+static void ls_after_double_impl(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, int64_t evals,
+                                 hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_ls_after_double<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, o, x, (FT*)h, evals);
+}
+void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, int64_t evals,
+                            hipStream_t s) {
+    PQI_DISPATCH(g, ls_after_double_impl, g, c, o, x, h, evals, s);
+}
+
+void launch_ls_after_seeds(LsCtl* c, const double* seld, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_ls_after_seeds, dim3(1), dim3(1), 0, s, c, seld, count);
+}
+
+template <typename WT>
+static void ls_adopt_seed_impl(const Graph& g, LsCtl* c, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
+                               hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_ls_adopt_seed<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, wx, (const FT*)wh, x,
+                       (FT*)h);
+}
+void launch_ls_adopt_seed(const Graph& g, LsCtl* c, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
+                          hipStream_t s) {
+    PQI_DISPATCH(g, ls_adopt_seed_impl, g, c, wx, wh, x, h, s);
+}
+
+template <typename WT>
+static void ls_after_walk_impl(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
+                               int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
+                               uint8_t* rx, void* rh, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_ls_after_walk<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, we, wev, walkers,
+                       gray_evals, wx, (const FT*)wh, x, (FT*)h, rx, (FT*)rh);
+}
+void launch_ls_after_walk(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
+                          int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
+                          void* rh, hipStream_t s) {
+    PQI_DISPATCH(g, ls_after_walk_impl, g, c, we, wev, walkers, gray_evals, wx, wh, x, h, rx, rh, s);
+}
+
+template <typename WT>
+static void ls_reheat_single_impl(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, uint8_t* rx,
+                                  void* rh, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_ls_reheat_single<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, o, x, (FT*)h, rx,
+                       (FT*)rh);
+}
+void launch_ls_reheat_single(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, uint8_t* rx,
+                             void* rh, hipStream_t s) {
+    PQI_DISPATCH(g, ls_reheat_single_impl, g, c, o, x, h, rx, rh, s);
+}
+
+template <typename WT>
+static void ls_reheat_double_impl(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, uint8_t* rx,
+                                  void* rh, int64_t evals, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_ls_reheat_double<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, o, x, (FT*)h, rx,
+                       (FT*)rh, evals);
+}
+void launch_ls_reheat_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, uint8_t* rx,
+                             void* rh, int64_t evals, hipStream_t s) {
+    PQI_DISPATCH(g, ls_reheat_double_impl, g, c, o, x, h, rx, rh, evals, s);
 }
 
 // ===========================================================================

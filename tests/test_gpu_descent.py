@@ -152,3 +152,66 @@ def test_batched_staging_mix_matches_single():
         xs, st = dg.local_search(starts[i], p)
         assert np.array_equal(states[i], xs), i
         assert stats[i]["cut_value"] == st["cut_value"], i
+
+
+COUNTERS = ("max_energy", "cut_value", "min_energy", "rounds", "single_accepts", "double_accepts", "seed_accepts",
+            "walk_accepts", "reheat_accepts", "single_evals", "double_evals", "seed_evals", "gray_evals",
+            "gray_evals_computed", "descent_evals")
+
+
+def _instances():
+    rng = np.random.default_rng(50)
+    out = []
+    G, _ = regular_int(2048, 3, 51)
+    out.append(("csr 3-regular n=2048", _lib.DeviceGraph.csr(G), 2048))
+    U = np.triu(rng.integers(-2, 3, (300, 300), dtype=np.int8), 1)
+    out.append(("dense int n=300", _lib.DeviceGraph.dense(U + U.T), 300))
+    lo, hi = graphs.erdos_renyi_edges_sparse(1500, 8.0 / 1499, rng)
+    Gr = graphs.upper_csr(1500, lo, hi, rng.uniform(-1, 1, lo.size), dtype=np.float32)
+    out.append(("csr real n=1500", _lib.DeviceGraph.csr(Gr), 1500))
+    Ud = np.triu(rng.normal(size=(256, 256)), 1)
+    out.append(("dense real n=256", _lib.DeviceGraph.dense(Ud + Ud.T), 256))
+    return out
+
+
+@pytest.mark.parametrize("sg", [0, 1])
+def test_device_loop_matches_host_loop(sg):
+    """The device-decided loop takes the same moves, in the same order, as the
+    host-decided loop: identical states, counters and max_energy."""
+    rng = np.random.default_rng(60 + sg)
+    for name, dg, n in _instances():
+        for mode in (_lib.DOUBLE_REFERENCE, _lib.DOUBLE_EXHAUSTIVE):
+            if mode == _lib.DOUBLE_EXHAUSTIVE and n > 1600:
+                continue
+            log(f"device vs host loop: {name} sg={sg} double_mode={mode}")
+            x0 = rng.integers(0, 2, n).astype(np.uint8)
+            p = _lib.make_params(n, is_spin_glass=sg, walkers=8, seed_multiple=2, gray_iterations=4 * n)
+            p.double_mode = mode
+            xd, sd = dg.local_search(x0, p)
+            p.flags = _lib.FLAG_HOST_LOOP
+            xh, sh = dg.local_search(x0, p)
+            assert np.array_equal(xd, xh), name
+            for k in COUNTERS:
+                assert sd[k] == sh[k], (name, k, sd[k], sh[k])
+            # the captured iteration is replayed by later calls with the same buffers
+            p.flags = 0
+            for _ in range(2):
+                x2, s2 = dg.local_search(x0, p)
+                assert np.array_equal(x2, xh), name
+                assert all(s2[k] == sh[k] for k in COUNTERS), name
+
+
+def test_device_loop_max_rounds():
+    G, _ = regular_int(1024, 3, 70)
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = np.random.default_rng(71).integers(0, 2, 1024).astype(np.uint8)
+    for r in (1, 2, 3):
+        p = _lib.make_params(1024, is_spin_glass=1, walkers=8, seed_multiple=2, gray_iterations=2048)
+        p.max_rounds = r
+        xd, sd = dg.local_search(x0, p)
+        p.flags = _lib.FLAG_HOST_LOOP
+        xh, sh = dg.local_search(x0, p)
+        assert np.array_equal(xd, xh)
+        assert sd["rounds"] == sh["rounds"] <= r
+        for k in COUNTERS:
+            assert sd[k] == sh[k], (r, k)
