@@ -111,7 +111,9 @@ struct PhaseOut {
 };
 
 // Repeated single flips (first argmax, strictly positive gain) until none improves.
-void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s);
+// With events e0/e1, they are recorded at the start and end of the kernel itself.
+void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s,
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Best single flip only (no move applied).
 void launch_single_best(const Graph& g, const uint8_t* x, const void* h, PhaseOut* out, hipStream_t s);
 // Best double flip: reference semantics (last pair of each of thread_count threads) or exhaustive.

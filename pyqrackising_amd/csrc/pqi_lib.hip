@@ -754,6 +754,9 @@ struct EventTimer {
     void stop(hipStream_t s) {
         if (on) (void)hipEventRecord(b, s);
     }
+    // events for a kernel launch that records them itself (hipExtLaunchKernel)
+    hipEvent_t start_event() const { return on ? a : nullptr; }
+    hipEvent_t stop_event() const { return on ? b : nullptr; }
     // call after the stream has been synchronised
     double seconds() {
         float ms = 0.f;
@@ -938,9 +941,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     for (;;) {
         if (!hc.again && p->max_rounds > 0 && hc.rounds >= p->max_rounds) break;
         launch_ls_begin(ctl, c->stream);
-        descent_timer.start(c->stream);
-        launch_descent(*g, x, h, cap, dout, c->stream);
-        descent_timer.stop(c->stream);
+        launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event());
         CHK(run_segment(0));
         walk_timer.start(c->stream);
         launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
@@ -1036,9 +1037,7 @@ static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_para
         // bounded launches keep every kernel short; repeat while the bound was hit
         const int64_t cap = 16 * n + 4096;
         do {
-            descent_timer.start(c->stream);
-            launch_descent(*g, x, h, cap, dout, c->stream);
-            descent_timer.stop(c->stream);
+            launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event());
             CHK(read_out(c, dout, &o));
             if (timing) {
                 st->descent_kernel_seconds += descent_timer.seconds();

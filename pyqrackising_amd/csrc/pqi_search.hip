@@ -7,6 +7,7 @@
 // value in O(n)); a pair gains Delta_k + Delta_l - 2 w_kl s_k s_l
 // (maxcut_tfim_util.py:339-346).  Flipping k changes h_v by -2 w_vk s_k(old).
 // Fields are int32 for int8 weights (exact) and FP64 otherwise.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -790,16 +791,19 @@ __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, F
     descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds);
 }
 
+// The descent launches go through hipExtLaunchKernel so that optional start
+// and stop events time the kernel itself (no events: a plain launch).
 template <typename WT>
-static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
+static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s,
+                         hipEvent_t e0, hipEvent_t e1) {
     using FT = typename FieldOf<WT>::T;
     if constexpr (sizeof(WT) == 1) {
         static const size_t klimit = lds_limit((const void*)k_descent_key);
         const int ib = key_shift(g);
         const size_t kl = key_lds_bytes(g, klimit);
         if (ib > 0 && kl <= klimit && descent_variant() != 3) {
-            hipLaunchKernelGGL(k_descent_key, dim3(1), dim3(64), kl, s, view<int8_t>(g), x, (int32_t*)h, max_steps,
-                               out, kl, ib);
+            hipExtLaunchKernelGGL(k_descent_key, dim3(1), dim3(64), (uint32_t)kl, s, e0, e1, 0, view<int8_t>(g), x,
+                                  (int32_t*)h, max_steps, out, kl, ib);
             return;
         }
     }
@@ -808,17 +812,19 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
     static const size_t limit2 = lds_limit((const void*)k_descent_wave2<WT, FT>);
     const bool sparse = (g.use_adj || g.W == nullptr) && descent_variant() == 2;
     if (sparse && descent2_lds<FT>(g.n) <= limit2)
-        hipLaunchKernelGGL((k_descent_wave2<WT, FT>), dim3(1), dim3(64), descent2_lds<FT>(g.n), s, view<WT>(g), x,
-                           (FT*)h, max_steps, out);
+        hipExtLaunchKernelGGL(k_descent_wave2<WT, FT>, dim3(1), dim3(64), (uint32_t)descent2_lds<FT>(g.n), s, e0, e1,
+                              0, view<WT>(g), x, (FT*)h, max_steps, out);
     else if (lds <= limit)
-        hipLaunchKernelGGL((k_descent_wave<WT, FT>), dim3(1), dim3(64), lds, s, view<WT>(g), x, (FT*)h, max_steps,
-                           out);
+        hipExtLaunchKernelGGL(k_descent_wave<WT, FT>, dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, view<WT>(g), x,
+                              (FT*)h, max_steps, out);
     else
-        hipLaunchKernelGGL((k_descent<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), x, (FT*)h, max_steps, out);
+        hipExtLaunchKernelGGL(k_descent<WT, FT>, dim3(1), dim3(1024), 0, s, e0, e1, 0, view<WT>(g), x, (FT*)h,
+                              max_steps, out);
 }
 
-void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s) {
-    PQI_DISPATCH(g, descent_impl, g, x, h, max_steps, out, s);
+void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s,
+                    hipEvent_t e0, hipEvent_t e1) {
+    PQI_DISPATCH(g, descent_impl, g, x, h, max_steps, out, s, e0, e1);
 }
 
 // Best single flip only: one scan of Delta_k = s_k h_k over the state in
