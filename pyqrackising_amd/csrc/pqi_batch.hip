@@ -30,6 +30,7 @@ struct BatchInst {
     int64_t wpr;
     int64_t B;
     unsigned long long* q;
+    const int32_t* colsum;  // column sums of W (the wide tiles)
 };
 
 struct BatchTile {
@@ -47,6 +48,24 @@ __global__ __launch_bounds__(256) void k_batch_grouped(const BatchInst* __restri
         const BatchInst in = insts[tl.inst];
         tile_i8<BK>(in.W, in.n_pad, in.n_pad, in.bits, in.wpr, in.B, (int64_t)tl.mb * TILE_BM,
                     (int64_t)tl.jb * TILE_BN, in.q, As, Bs, Sb);
+    }
+}
+
+// Wide tiles (256 candidates x 256 columns, pqi_tile_i8.h), persistent over a
+// tile list laid out for the XCDs: position p is run by a work group of XCD
+// p % 8, and the candidate tiles of one (instance, column block) sit at
+// positions of one residue, so they share that block of W through one L2.
+// Empty positions (inst < 0) pad the per-XCD queues to one length.
+template <int BK>
+__global__ __launch_bounds__(256) void k_batch_wide(const BatchInst* __restrict__ insts,
+                                                    const BatchTile* __restrict__ tiles, int64_t ntiles) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BatchTile tl = tiles[t];
+        if (tl.inst < 0) continue;
+        const BatchInst in = insts[tl.inst];
+        tile_i8_wide<BK>(in.W, in.n_pad, 0, in.n_pad, in.colsum, in.bits, in.wpr, in.B, (int64_t)tl.mb * WIDE_BM,
+                         (int64_t)tl.jb * WIDE_BN, in.q, lds);
     }
 }
 
@@ -89,10 +108,13 @@ __global__ __launch_bounds__(256) void k_seg_best(const BatchInst* __restrict__ 
     }
 }
 
-// tile variants: contraction depth x persistent grid size (multiples of the CU count)
-static const int kVariantBK[] = {64, 64, 64, 128, 128, 128};
-static const int kVariantGrid[] = {1, 2, 4, 1, 2, 4};
-static const int kVariants = 6;
+// tile variants: 0-5 the 128 x 128 tile, contraction depth x persistent grid
+// size (multiples of the CU count); 6-7 the wide tile (one work group per CU)
+// with depth 128 or 64
+static const int kVariantBK[] = {64, 64, 64, 128, 128, 128, 128, 64};
+static const int kVariantGrid[] = {1, 2, 4, 1, 2, 4, 1, 1};
+static const int kVariantWide[] = {0, 0, 0, 0, 0, 0, 1, 1};
+static const int kVariants = 8;
 
 }  // namespace pqi
 
@@ -105,6 +127,8 @@ struct pqi_batch {
     int64_t total_words = 0, total_q = 0, ntiles = 0;
     pqi::BatchInst* d_insts = nullptr;
     pqi::BatchTile* d_tiles = nullptr;
+    pqi::BatchTile* d_wtiles = nullptr;  // wide-tile list (nullptr: wide tiles not usable)
+    int64_t nwtiles = 0;
     uint32_t* d_bits = nullptr;
     unsigned long long* d_q = nullptr;
     int64_t* d_best = nullptr;
@@ -121,7 +145,7 @@ static void batch_release(pqi_batch* b) {
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->d_insts, b->d_tiles, b->d_bits, b->d_q, b->d_best, b->d_bestq};
+    void* ptrs[] = {b->d_insts, b->d_tiles, b->d_wtiles, b->d_bits, b->d_q, b->d_best, b->d_bestq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -129,7 +153,23 @@ static void batch_release(pqi_batch* b) {
 
 static void enqueue_eval(pqi_batch* b, int variant, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_q, dim3((unsigned)((b->total_q + 255) / 256)), dim3(256), 0, s, b->d_q, b->total_q);
+    if (kVariantWide[variant] && !b->d_wtiles) variant = 3;  // wide tiles not usable: the nearest plain variant
     const int grid = std::max(1, b->ctx->num_cus) * kVariantGrid[variant];
+    if (kVariantWide[variant]) {
+        static const bool lds_ok =
+            hipFuncSetAttribute((const void*)k_batch_wide<128>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)wide_lds_bytes<128>()) == hipSuccess;
+        if (!lds_ok) variant = 7;  // the 64-deep wide tile fits the default LDS limit
+        const unsigned g = (unsigned)std::min<int64_t>(grid, std::max<int64_t>(1, b->nwtiles));
+        if (kVariantBK[variant] == 128)
+            hipLaunchKernelGGL(k_batch_wide<128>, dim3(g), dim3(256), wide_lds_bytes<128>(), s, b->d_insts,
+                               b->d_wtiles, b->nwtiles);
+        else
+            hipLaunchKernelGGL(k_batch_wide<64>, dim3(g), dim3(256), wide_lds_bytes<64>(), s, b->d_insts,
+                               b->d_wtiles, b->nwtiles);
+        hipLaunchKernelGGL(k_seg_best, dim3((unsigned)b->count), dim3(256), 0, s, b->d_insts, b->d_best, b->d_bestq);
+        return;
+    }
     const unsigned g = (unsigned)std::min<int64_t>(grid, std::max<int64_t>(1, b->ntiles));
     if (kVariantBK[variant] == 64)
         hipLaunchKernelGGL(k_batch_grouped<64>, dim3(g), dim3(256), 0, s, b->d_insts, b->d_tiles, b->ntiles);
@@ -200,12 +240,41 @@ int pqi_batch_create(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, int64_
     std::vector<BatchTile> sorted(tiles.size());
     for (size_t t = 0; t < order.size(); ++t) sorted[t] = tiles[order[t]];
     b->ntiles = (int64_t)sorted.size();
+    // wide tiles: groups (instance, 256-column block) of ceil(cands/256)
+    // candidate tiles, dealt longest first to the least loaded of 8 XCD queues
+    bool wide_ok = true;
+    for (int64_t i = 0; i < count && wide_ok; ++i)
+        wide_ok = graphs[i]->wsum && 256 * graphs[i]->max_strength < (int64_t(1) << 31);
+    std::vector<BatchTile> wsorted;
+    if (wide_ok) {
+        struct Group { int32_t inst, jb; int64_t cost; };
+        std::vector<Group> groups;
+        for (int64_t i = 0; i < count; ++i)
+            for (int64_t jb = 0; jb < (graphs[i]->n_pad + WIDE_BN - 1) / WIDE_BN; ++jb)
+                groups.push_back({(int32_t)i, (int32_t)jb, graphs[i]->n_pad});
+        std::stable_sort(groups.begin(), groups.end(), [](const Group& a, const Group& d) { return a.cost > d.cost; });
+        const int64_t MBw = (cands + WIDE_BM - 1) / WIDE_BM;
+        std::vector<std::vector<BatchTile>> queue(8);
+        std::vector<int64_t> load(8, 0);
+        for (const Group& gr : groups) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            for (int64_t mb = 0; mb < MBw; ++mb) queue[x].push_back({gr.inst, (int32_t)mb, gr.jb, 0});
+            load[x] += gr.cost * MBw;
+        }
+        size_t len = 0;
+        for (auto& qx : queue) len = std::max(len, qx.size());
+        wsorted.assign(len * 8, BatchTile{-1, 0, 0, 0});
+        for (int x = 0; x < 8; ++x)
+            for (size_t k = 0; k < queue[x].size(); ++k) wsorted[8 * k + x] = queue[x][k];
+        b->nwtiles = (int64_t)wsorted.size();
+    }
     auto alloc = [&](void** p, size_t bytes) {
         if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
         b->device_bytes += (int64_t)bytes;
         return true;
     };
-    if (!alloc((void**)&b->d_insts, sizeof(BatchInst) * count) ||
+    if ((wide_ok && !alloc((void**)&b->d_wtiles, sizeof(BatchTile) * wsorted.size())) ||
+        !alloc((void**)&b->d_insts, sizeof(BatchInst) * count) ||
         !alloc((void**)&b->d_tiles, sizeof(BatchTile) * sorted.size()) ||
         !alloc((void**)&b->d_bits, (size_t)b->total_words * 4) || !alloc((void**)&b->d_q, (size_t)b->total_q * 8) ||
         !alloc((void**)&b->d_best, (size_t)count * 8) || !alloc((void**)&b->d_bestq, (size_t)count * 8)) {
@@ -214,11 +283,14 @@ int pqi_batch_create(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, int64_
     }
     for (int64_t i = 0; i < count; ++i) {
         pqi_graph* g = graphs[i];
-        hin[i] = {(const int8_t*)g->W, g->n_pad, b->d_bits + b->bits_off[i], g->n_pad >> 5, cands, b->d_q + b->q_off[i]};
+        hin[i] = {(const int8_t*)g->W, g->n_pad, b->d_bits + b->bits_off[i], g->n_pad >> 5, cands, b->d_q + b->q_off[i],
+                  (const int32_t*)g->wsum};
     }
     if (hipMemcpy(b->d_insts, hin.data(), sizeof(BatchInst) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_tiles, sorted.data(), sizeof(BatchTile) * sorted.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(b->d_bits, 0, (size_t)b->total_words * 4) != hipSuccess) {
+        hipMemset(b->d_bits, 0, (size_t)b->total_words * 4) != hipSuccess ||
+        (wide_ok && hipMemcpy(b->d_wtiles, wsorted.data(), sizeof(BatchTile) * wsorted.size(),
+                              hipMemcpyHostToDevice) != hipSuccess)) {
         batch_release(b);
         return fail(PQI_ERR_HIP, "uploading the batch description failed");
     }
@@ -328,12 +400,16 @@ int pqi_batch_info(const pqi_batch* b, int64_t* ntiles, int64_t* algorithmic_ops
                    int64_t* weight_bytes, int* variant) {
     if (!b) return fail(PQI_ERR_INVALID, "null argument");
     int64_t ops = 0, exe = 0, wb = 0;
+    const bool wide = kVariantWide[b->variant] && b->d_wtiles;
     for (const pqi_graph* g : b->graphs) {
         ops += 2 * b->cands * g->n * g->n;
-        exe += 2 * b->cands * g->n_pad * g->n_pad;
+        if (wide)  // candidate and column blocks of 256, the last ones partly empty
+            exe += 2 * ((b->cands + 255) / 256 * 256) * ((g->n_pad + 255) / 256 * 256) * g->n_pad;
+        else
+            exe += 2 * b->cands * g->n_pad * g->n_pad;
         wb += g->n * g->n;
     }
-    if (ntiles) *ntiles = b->ntiles;
+    if (ntiles) *ntiles = wide ? b->nwtiles : b->ntiles;
     if (algorithmic_ops) *algorithmic_ops = ops;
     if (executed_ops) *executed_ops = exe;
     if (weight_bytes) *weight_bytes = wb;

@@ -322,6 +322,18 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             run.swap(again);
         }
         const double t_single = now_s() - t0;
+        // with targets, an instance whose descent reached its target is done
+        const std::vector<int32_t> round_set = active;
+        if (targets) {
+            std::vector<int32_t> open;
+            for (int32_t i : active)
+                if (!reached(i)) open.push_back(i);
+            active.swap(open);
+            if (active.empty()) {
+                for (int32_t i : round_set) stats[i].seconds_single += t_single / (double)round_set.size();
+                break;
+            }
+        }
 
         // double flips
         t0 = now_s();
@@ -485,8 +497,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             }
         }
         const double t_reheat = now_s() - t0;
-        const double share = 1.0 / (double)active.size();
-        for (int32_t i : active) {
+        const double share = 1.0 / (double)round_set.size();
+        for (int32_t i : round_set) {
             stats[i].seconds_single += t_single * share;
             stats[i].seconds_double += t_double * share;
             stats[i].seconds_seeds += t_seeds * share;

@@ -11,6 +11,8 @@
 //   (exact); fp32/fp64 weights: FP64 tiles (exact to FP64 rounding).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pqi_internal.h"
 #include "pqi_tile_i8.h"
 
@@ -33,6 +35,28 @@ __global__ __launch_bounds__(256) void k_batch_i8(const int8_t* __restrict__ W, 
     __shared__ uint32_t Sb[TILE_BM * (TILE_BN / 32)];
     tile_i8<64>(W, ld, n_pad, bits, wpr, B, (int64_t)blockIdx.y * TILE_BM, (int64_t)blockIdx.x * TILE_BN, q, As, Bs,
                 Sb);
+}
+
+// Wide tiles, XCD-aware order: block b runs on XCD b % 8 (round-robin
+// dispatch); the candidate tiles of one column block get the same residue and
+// consecutive slots, so they share that column block of W through one L2.
+// The grid is rounded up to whole groups of 8 column blocks; the extra blocks
+// exit at once.
+// grid: 8-column-block groups x MB candidate blocks x KS splits of K
+template <int BK>
+__global__ __launch_bounds__(256) void k_batch_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
+                                                       const int32_t* __restrict__ colsum,
+                                                       const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
+                                                       unsigned long long* __restrict__ q, int64_t MB, int64_t JB,
+                                                       int64_t KS) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t b = blockIdx.x, x = b & 7, slot = b >> 3;
+    const int64_t ks = slot % KS, t = slot / KS;
+    const int64_t jb = (t / MB) * 8 + x, mb = t % MB;
+    if (jb >= JB) return;
+    const int64_t kl = n_pad / KS;
+    tile_i8_wide<BK>(W, ld, ks * kl, (ks + 1) * kl, ks == 0 ? colsum : nullptr, bits, wpr, B, mb * WIDE_BM,
+                     jb * WIDE_BN, q, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -360,8 +384,28 @@ void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, i
     if (g.kind == PQI_WEIGHTS_I8) {
         unsigned long long* qi = (unsigned long long*)scratch;
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B);
-        dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
-        hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi);
+        if (B >= 512 && g.wsum && 256 * g.max_strength < (int64_t(1) << 31)) {
+            const int64_t MB = (B + WIDE_BM - 1) / WIDE_BM, JB = (g.n_pad + WIDE_BN - 1) / WIDE_BN;
+            // split K until there is about one work group per CU (each split >= 512 deep)
+            int64_t KS = 1;
+            const int64_t tiles = ((JB + 7) / 8) * 8 * MB, cus = g.ctx ? std::max(1, g.ctx->num_cus) : 256;
+            while (tiles * KS * 2 <= cus && g.n_pad % (128 * KS * 2) == 0 && g.n_pad / (KS * 2) >= 512) KS *= 2;
+            static const bool big = hipFuncSetAttribute((const void*)k_batch_i8_wide<128>,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                        (int)wide_lds_bytes<128>()) == hipSuccess;
+            if (big)
+                hipLaunchKernelGGL(k_batch_i8_wide<128>, dim3((unsigned)(tiles * KS)), dim3(256),
+                                   wide_lds_bytes<128>(), s, (const int8_t*)g.W, g.n_pad, g.n_pad,
+                                   (const int32_t*)g.wsum, bits, wpr, B, qi, MB, JB, KS);
+            else
+                hipLaunchKernelGGL(k_batch_i8_wide<64>, dim3((unsigned)(tiles * KS)), dim3(256), wide_lds_bytes<64>(),
+                                   s, (const int8_t*)g.W, g.n_pad, g.n_pad, (const int32_t*)g.wsum, bits, wpr, B, qi,
+                                   MB, JB, KS);
+        } else {
+            dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
+            hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr,
+                               B, qi);
+        }
         hipLaunchKernelGGL(k_i64_to_f64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B, q);
     } else {
         const int64_t nblk = (g.n + 63) / 64;

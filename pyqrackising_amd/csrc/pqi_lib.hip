@@ -176,6 +176,7 @@ static void free_graph(pqi_graph* g) {
     if (g->adj_idx) (void)hipFree(g->adj_idx);
     if (g->adj_w) (void)hipFree(g->adj_w);
     if (g->key_adj) (void)hipFree(g->key_adj);
+    if (g->wsum) (void)hipFree(g->wsum);
     delete g;
 }
 
@@ -248,6 +249,7 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
     int64_t nnz = 0;
     long double w_up_l = 0.0L;
     std::vector<double> strength(n, 0.0);  // row sums of |w|
+    std::vector<double> rowsum(n, 0.0);    // row sums of w (exact for integers)
     const int64_t TB = 64;
     for (int64_t i0 = 0; i0 < n; i0 += TB)
         for (int64_t j0 = i0; j0 < n; j0 += TB)
@@ -268,18 +270,22 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
                         w_up_l += a;
                         strength[i] += std::fabs(a);
                         strength[j] += std::fabs(a);
+                        rowsum[i] += a;
+                        rowsum[j] += a;
                     }
                 }
     pqi_graph* g = new pqi_graph();
     g->ctx = c;
     g->n = n;
-    g->kind = classify(maxabs, all_int, dtype);
+    const double max_strength = n ? *std::max_element(strength.begin(), strength.end()) : 0.0;
+    // int8 tiles sum up to 128 columns of |P| <= max strength in int32
+    g->kind = classify(maxabs, all_int && 128.0 * max_strength < 2147483648.0, dtype);
     g->n_pad = round_up(n, 128);
     g->w_up = (double)w_up_l;
     g->w_up_i = g->kind == PQI_WEIGHTS_I8 ? (int64_t)std::llround((double)w_up_l) : 0;
     g->nnz = nnz;
     g->use_adj = nnz <= n * n / 8;
-    if (g->kind == PQI_WEIGHTS_I8) g->max_strength = (int64_t)*std::max_element(strength.begin(), strength.end());
+    if (g->kind == PQI_WEIGHTS_I8) g->max_strength = (int64_t)max_strength;
     std::vector<double>().swap(strength);
     // dense copy, zero padded, in the storage type (one pitched copy)
     const size_t es = g->kind == PQI_WEIGHTS_I8 ? 1 : (g->kind == PQI_WEIGHTS_F32 ? 4 : 8);
@@ -309,6 +315,16 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
                     hipMemcpyHostToDevice) != hipSuccess)
         return bail(PQI_ERR_HIP, "uploading W failed");
     std::vector<unsigned char>().swap(conv);
+    if (g->kind == PQI_WEIGHTS_I8) {
+        const int64_t len = round_up(g->n_pad, 256);
+        std::vector<int32_t> cs(len, 0);
+        for (int64_t i = 0; i < n; ++i) cs[i] = (int32_t)std::llround(rowsum[i]);  // W symmetric
+        if (hipMalloc(&g->wsum, len * 4) != hipSuccess) return bail(PQI_ERR_OOM, "device allocation failed");
+        g->device_bytes += len * 4;
+        if (hipMemcpy(g->wsum, cs.data(), len * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(PQI_ERR_HIP, "uploading column sums failed");
+    }
+    std::vector<double>().swap(rowsum);
     if (g->kind == PQI_WEIGHTS_F32) {
         const size_t bb = (size_t)g->n_pad * (size_t)g->n_pad * 4;  // two bf16 copies
         if (hipMalloc(&g->Wbf, bb) != hipSuccess) return bail(PQI_ERR_OOM, "device allocation of bf16 W failed");

@@ -134,4 +134,201 @@ __device__ __forceinline__ void tile_i8(const int8_t* __restrict__ W, int64_t ld
     __syncthreads();  // LDS is reused by the caller's next tile
 }
 
+// ---------------------------------------------------------------------------
+// Wide tile: 256 candidates x 256 columns per work group.  With s = 1 - 2x,
+//   (S W)_bj = c_j - 2 (X W)_bj,   c_j = sum_k W_kj (column sums, kept per graph),
+// so the MFMAs run on the 0/1 candidate bytes X: three VALU operations per 4
+// bytes to unpack instead of five.  Wave w owns rows 64w..64w+63 and all 256
+// columns (2 x 8 MFMA blocks, 256 accumulators): each unpacked fragment feeds
+// 8 MFMAs, and W crosses L2 once per 256 candidates.  A wave's candidate rows
+// are its own: each lane loads the packed bit words of its rows into
+// registers one chunk ahead and unpacks its fragments there.  W, shared by
+// the four waves, goes through LDS in two stages: while the waves run chunk c
+// from one stage, each thread stores chunk c+1 (loaded one chunk earlier)
+// into the other and loads chunk c+2; one barrier per chunk.  Columns past
+// n_pad (the last tile of a row of W) read a valid row and are dropped.
+// The contraction may be split over k (the result is linear in it): each
+// split adds its own -2 (X W) part, and one of them the c_j part.
+// LDS: 2 x 256 x (BK+16) bytes of W, plus 256 x 8 words for the epilogue.
+// ---------------------------------------------------------------------------
+constexpr int WIDE_BM = 256;
+constexpr int WIDE_BN = 256;
+
+template <int BK>
+constexpr size_t wide_lds_bytes() {
+    return 2 * (size_t)WIDE_BN * (BK + 16) + (size_t)WIDE_BM * 8 * 4;
+}
+
+// 16 candidate bits (k ascending) -> one MFMA fragment of 16 bytes 0/1
+__device__ __forceinline__ v4i unpack01(uint32_t half) {
+    v4i a;
+    a[0] = (int)spread4(half & 0xFu);
+    a[1] = (int)spread4((half >> 4) & 0xFu);
+    a[2] = (int)spread4((half >> 8) & 0xFu);
+    a[3] = (int)spread4((half >> 12) & 0xFu);
+    return a;
+}
+
+template <int BK>
+__device__ __forceinline__ void wide_compute(v16i (&acc)[2][8], const uint32_t (&aw)[2][BK / 32], const int8_t* Ws,
+                                             int lc, int hi) {
+    constexpr int ROW = BK + 16;
+#pragma unroll
+    for (int t = 0; t < BK / 32; ++t) {
+        v4i av[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) av[rb] = unpack01(hi ? (aw[rb][t] >> 16) : (aw[rb][t] & 0xFFFFu));
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            const v4i bv = *(const v4i*)&Ws[(cb * 32 + lc) * ROW + 32 * t + 16 * hi];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[rb], bv, acc[rb][cb], 0, 0, 0);
+        }
+    }
+}
+
+template <int BK>
+__device__ __forceinline__ void wide_load_a(uint32_t (&aw)[2][BK / 32], const uint32_t* const (&arow)[2], int64_t k0) {
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int t = 0; t < BK / 32; ++t) aw[rb][t] = arow[rb][(k0 >> 5) + t];
+}
+
+// W loader, coalesced: BK/16 consecutive lanes read the BK bytes of one W row,
+// so a load instruction covers 256 / (BK/16) rows; pass t takes rows
+// t * 256/(BK/16) + tid / (BK/16).  wbase[t] = W row of pass t + piece offset.
+template <int BK>
+__device__ __forceinline__ void wide_load_w(v4i (&wv)[BK / 16], const int8_t* const (&wbase)[BK / 16], int64_t k0) {
+#pragma unroll
+    for (int t = 0; t < BK / 16; ++t) wv[t] = *(const v4i*)(wbase[t] + k0);
+}
+
+template <int BK>
+__device__ __forceinline__ void wide_store_w(const v4i (&wv)[BK / 16], int8_t* stage, const int (&woff)[BK / 16]) {
+#pragma unroll
+    for (int t = 0; t < BK / 16; ++t) *(v4i*)(stage + woff[t]) = wv[t];
+}
+
+// Contraction over k in [kb, ke) (a split of K = ld; multiples of BK).  The
+// c_j term of P is added when colsum is given (by one split only).
+template <int BK>
+__device__ __forceinline__ void tile_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t kb, int64_t ke,
+                                             const int32_t* __restrict__ colsum, const uint32_t* __restrict__ bits,
+                                             int64_t wpr, int64_t B, int64_t m0, int64_t j0,
+                                             unsigned long long* __restrict__ q, unsigned char* lds) {
+    constexpr int ROW = BK + 16;
+    constexpr int STAGE = WIDE_BN * ROW;
+    static_assert(BK % 32 == 0, "BK: a multiple of 32");
+    int8_t* stage0 = (int8_t*)lds;
+    int8_t* stage1 = stage0 + STAGE;
+    uint32_t* Sb = (uint32_t*)(lds + 2 * STAGE);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int hi = lane >> 5, lc = lane & 31;
+    // this lane's candidate rows (one per row block); rows past B read row 0
+    // and are dropped in the epilogue, so the loads need no branch
+    const uint32_t* arow[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+        const int64_t m = m0 + wid * 64 + rb * 32 + lc;
+        arow[rb] = bits + (m < B ? m : 0) * wpr;
+    }
+    constexpr int LPR = BK / 16;        // lanes per W row in the loader
+    constexpr int RPP = 256 / LPR;      // rows per loader pass
+    const int8_t* wbase[BK / 16];
+    int woff[BK / 16];
+#pragma unroll
+    for (int t = 0; t < BK / 16; ++t) {
+        const int r = t * RPP + tid / LPR, piece = tid % LPR;
+        const int64_t jrow = j0 + r < ld ? j0 + r : ld - 1;  // rows past n_pad: a valid row, dropped later
+        wbase[t] = W + jrow * ld + 16 * piece;
+        woff[t] = r * ROW + 16 * piece;
+    }
+
+    v16i acc[2][8];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+
+    const int64_t nc = (ke - kb) / BK;
+    auto chunk = [nc, kb](int64_t c) { return kb + (c < nc ? c : nc - 1) * BK; };  // past the end: the last
+    uint32_t a0[2][BK / 32], a1[2][BK / 32];
+    v4i w0[BK / 16], w1[BK / 16];
+    wide_load_a<BK>(a0, arow, kb);
+    wide_load_w<BK>(w0, wbase, kb);
+    wide_load_w<BK>(w1, wbase, chunk(1));
+    wide_store_w<BK>(w0, stage0, woff);
+    wide_load_w<BK>(w0, wbase, chunk(2));
+    __syncthreads();
+    int64_t c = 0;
+    for (; c + 1 < nc; c += 2) {
+        wide_load_a<BK>(a1, arow, chunk(c + 1));
+        wide_compute<BK>(acc, a0, stage0, lc, hi);
+        wide_store_w<BK>(w1, stage1, woff);
+        wide_load_w<BK>(w1, wbase, chunk(c + 3));
+        __syncthreads();
+        wide_load_a<BK>(a0, arow, chunk(c + 2));
+        wide_compute<BK>(acc, a1, stage1, lc, hi);
+        wide_store_w<BK>(w0, stage0, woff);
+        wide_load_w<BK>(w0, wbase, chunk(c + 4));
+        __syncthreads();
+    }
+    if (c < nc) wide_compute<BK>(acc, a0, stage0, lc, hi);
+
+    // candidate bits of the tile's 256 output columns (words past n_pad: 0)
+    const int64_t cwords = (ld - j0) >> 5;
+    for (int t = tid; t < WIDE_BM * 8; t += 256) {
+        const int r = t >> 3, cc = t & 7;
+        const int64_t m = m0 + r;
+        Sb[t] = (m < B && cc < cwords) ? bits[m * wpr + (j0 >> 5) + cc] : 0u;
+    }
+    __syncthreads();
+    // 32x32 accumulator layout: lane l, register r -> column l%32,
+    // row 8*(r/4) + 4*(l/32) + r%4 of the block.  P = c_j - 2 (X W).
+    int cj[8];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) cj[cb] = (colsum && j0 + cb * 32 < ld) ? colsum[j0 + cb * 32 + lc] : 0;
+    int vals[32];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            // the row's 8 words of bits (zero past n_pad, where acc is dropped)
+            const int row = wid * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            const v4i w03 = *(const v4i*)&Sb[row * 8];
+            const v4i w47 = *(const v4i*)&Sb[row * 8 + 4];
+            int v = 0;
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const uint32_t word = (uint32_t)(cb < 4 ? w03[cb] : w47[cb - 4]);
+                const int p = j0 + cb * 32 < ld ? cj[cb] - 2 * acc[rb][cb][r] : 0;
+                v += ((word >> lc) & 1u) ? -p : p;
+            }
+            vals[rb * 16 + r] = v;
+        }
+    // reduce-scatter over the 32 lanes that share rows: 32 -> 1 value
+#pragma unroll
+    for (int t = 0, half = 16; t < 5; ++t, half >>= 1) {
+        const int up = (lane >> t) & 1;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < half) {
+                const int send = up ? vals[i] : vals[i + half];
+                const int keep = up ? vals[i + half] : vals[i];
+                vals[i] = keep + __shfl_xor(send, 1 << t, 64);
+            }
+        }
+    }
+    const int qi = (lane & 1) * 16 + ((lane >> 1) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                   ((lane >> 4) & 1);
+    const int rb = qi >> 4, r = qi & 15;
+    const int64_t m = m0 + wid * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+    if (m < B) atomicAdd(&q[m], (unsigned long long)(long long)vals[0]);
+    __syncthreads();  // LDS is reused by the caller's next tile
+}
+
 }  // namespace pqi

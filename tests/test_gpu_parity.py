@@ -54,7 +54,8 @@ def ctx():
 # candidate scoring
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("n,B", [(24, 7), (100, 129), (130, 300), (257, 1000), (2048, 256)])
+@pytest.mark.parametrize("n,B", [(24, 7), (100, 129), (130, 300), (257, 1000), (2048, 256), (300, 1025),
+                                 (1000, 2500), (512, 4096)])
 @pytest.mark.parametrize("sg", [0, 1])
 def test_eval_batch_int8_mfma(ctx, orc, n, B, sg):
     log(f"eval_batch int8 n={n} B={B} sg={sg}")
@@ -68,6 +69,22 @@ def test_eval_batch_int8_mfma(ctx, orc, n, B, sg):
     obi, ovals = orc.eval_batch(W, X, sg)
     assert np.array_equal(vals, ovals)
     assert bi == obi and bv == ovals[obi]
+
+
+@pytest.mark.parametrize("n,B", [(384, 777), (129, 256)])
+def test_eval_batch_int8_full_range(ctx, orc, n, B):
+    """Dense weights over the whole int8 range: the int32 MFMA accumulation and
+    the int64 row totals stay exact."""
+    log(f"eval_batch int8 full range n={n} B={B}")
+    rng = np.random.default_rng(n * B)
+    U = np.triu(rng.integers(-127, 128, (n, n)), 1)
+    W = (U + U.T).astype(np.float64)
+    X = rng.integers(0, 2, (B, n)).astype(bool)
+    dg = _lib.DeviceGraph.dense(W.astype(np.int8))
+    bi, bv, vals = dg.eval_batch(_lib.pack_bits(X), 1)
+    obi, ovals = orc.eval_batch(W, X, 1)
+    assert np.array_equal(vals, ovals)
+    assert bi == obi
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
