@@ -30,7 +30,6 @@ struct BatchInst {
     int64_t wpr;
     int64_t B;
     unsigned long long* q;
-    const int32_t* colsum;  // column sums of W (the wide tiles)
 };
 
 struct BatchTile {
@@ -64,7 +63,7 @@ __global__ __launch_bounds__(256) void k_batch_wide(const BatchInst* __restrict_
         const BatchTile tl = tiles[t];
         if (tl.inst < 0) continue;
         const BatchInst in = insts[tl.inst];
-        tile_i8_wide<BK>(in.W, in.n_pad, 0, in.n_pad, in.colsum, in.bits, in.wpr, in.B, (int64_t)tl.mb * WIDE_BM,
+        tile_i8_wide<BK>(in.W, in.n_pad, 0, in.n_pad, in.bits, in.wpr, in.B, (int64_t)tl.mb * WIDE_BM,
                          (int64_t)tl.jb * WIDE_BN, in.q, lds);
     }
 }
@@ -244,7 +243,7 @@ int pqi_batch_create(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, int64_
     // candidate tiles, dealt longest first to the least loaded of 8 XCD queues
     bool wide_ok = true;
     for (int64_t i = 0; i < count && wide_ok; ++i)
-        wide_ok = graphs[i]->wsum && 256 * graphs[i]->max_strength < (int64_t(1) << 31);
+        wide_ok = graphs[i]->max_strength >= 0 && 256 * graphs[i]->max_strength < (int64_t(1) << 31);
     std::vector<BatchTile> wsorted;
     if (wide_ok) {
         struct Group { int32_t inst, jb; int64_t cost; };
@@ -283,8 +282,7 @@ int pqi_batch_create(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, int64_
     }
     for (int64_t i = 0; i < count; ++i) {
         pqi_graph* g = graphs[i];
-        hin[i] = {(const int8_t*)g->W, g->n_pad, b->d_bits + b->bits_off[i], g->n_pad >> 5, cands, b->d_q + b->q_off[i],
-                  (const int32_t*)g->wsum};
+        hin[i] = {(const int8_t*)g->W, g->n_pad, b->d_bits + b->bits_off[i], g->n_pad >> 5, cands, b->d_q + b->q_off[i]};
     }
     if (hipMemcpy(b->d_insts, hin.data(), sizeof(BatchInst) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_tiles, sorted.data(), sizeof(BatchTile) * sorted.size(), hipMemcpyHostToDevice) != hipSuccess ||

@@ -45,7 +45,6 @@ __global__ __launch_bounds__(256) void k_batch_i8(const int8_t* __restrict__ W, 
 // grid: 8-column-block groups x MB candidate blocks x KS splits of K
 template <int BK>
 __global__ __launch_bounds__(256) void k_batch_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
-                                                       const int32_t* __restrict__ colsum,
                                                        const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
                                                        unsigned long long* __restrict__ q, int64_t MB, int64_t JB,
                                                        int64_t KS) {
@@ -55,8 +54,7 @@ __global__ __launch_bounds__(256) void k_batch_i8_wide(const int8_t* __restrict_
     const int64_t jb = (t / MB) * 8 + x, mb = t % MB;
     if (jb >= JB) return;
     const int64_t kl = n_pad / KS;
-    tile_i8_wide<BK>(W, ld, ks * kl, (ks + 1) * kl, ks == 0 ? colsum : nullptr, bits, wpr, B, mb * WIDE_BM,
-                     jb * WIDE_BN, q, lds);
+    tile_i8_wide<BK>(W, ld, ks * kl, (ks + 1) * kl, bits, wpr, B, mb * WIDE_BM, jb * WIDE_BN, q, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -384,7 +382,7 @@ void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, i
     if (g.kind == PQI_WEIGHTS_I8) {
         unsigned long long* qi = (unsigned long long*)scratch;
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B);
-        if (B >= 512 && g.wsum && 256 * g.max_strength < (int64_t(1) << 31)) {
+        if (B >= 512 && g.max_strength >= 0 && 256 * g.max_strength < (int64_t(1) << 31)) {
             const int64_t MB = (B + WIDE_BM - 1) / WIDE_BM, JB = (g.n_pad + WIDE_BN - 1) / WIDE_BN;
             // split K until there is about one work group per CU (each split >= 512 deep)
             int64_t KS = 1;
@@ -395,12 +393,11 @@ void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, i
                                                         (int)wide_lds_bytes<128>()) == hipSuccess;
             if (big)
                 hipLaunchKernelGGL(k_batch_i8_wide<128>, dim3((unsigned)(tiles * KS)), dim3(256),
-                                   wide_lds_bytes<128>(), s, (const int8_t*)g.W, g.n_pad, g.n_pad,
-                                   (const int32_t*)g.wsum, bits, wpr, B, qi, MB, JB, KS);
+                                   wide_lds_bytes<128>(), s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi,
+                                   MB, JB, KS);
             else
                 hipLaunchKernelGGL(k_batch_i8_wide<64>, dim3((unsigned)(tiles * KS)), dim3(256), wide_lds_bytes<64>(),
-                                   s, (const int8_t*)g.W, g.n_pad, g.n_pad, (const int32_t*)g.wsum, bits, wpr, B, qi,
-                                   MB, JB, KS);
+                                   s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
         } else {
             dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
             hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr,

@@ -32,9 +32,6 @@ struct Graph {
     // n * stride is a multiple of 4 (nullptr when not built)
     int32_t* key_adj = nullptr;
     int64_t key_adj_stride = 0;
-    // dense int8 weights: column sums c_j = sum_k w_kj (int32, n_pad rounded up
-    // to 256 entries, zero padded), for the 0/1-operand MFMA tiles
-    int32_t* wsum = nullptr;
     double w_up = 0.0;           // sum_{i<j} w_ij
     int64_t w_up_i = 0;          // same, exact, for int8 weights
     bool use_adj = true;         // flips update fields through the adjacency lists

@@ -176,7 +176,6 @@ static void free_graph(pqi_graph* g) {
     if (g->adj_idx) (void)hipFree(g->adj_idx);
     if (g->adj_w) (void)hipFree(g->adj_w);
     if (g->key_adj) (void)hipFree(g->key_adj);
-    if (g->wsum) (void)hipFree(g->wsum);
     delete g;
 }
 
@@ -249,7 +248,6 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
     int64_t nnz = 0;
     long double w_up_l = 0.0L;
     std::vector<double> strength(n, 0.0);  // row sums of |w|
-    std::vector<double> rowsum(n, 0.0);    // row sums of w (exact for integers)
     const int64_t TB = 64;
     for (int64_t i0 = 0; i0 < n; i0 += TB)
         for (int64_t j0 = i0; j0 < n; j0 += TB)
@@ -270,8 +268,6 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
                         w_up_l += a;
                         strength[i] += std::fabs(a);
                         strength[j] += std::fabs(a);
-                        rowsum[i] += a;
-                        rowsum[j] += a;
                     }
                 }
     pqi_graph* g = new pqi_graph();
@@ -315,16 +311,6 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
                     hipMemcpyHostToDevice) != hipSuccess)
         return bail(PQI_ERR_HIP, "uploading W failed");
     std::vector<unsigned char>().swap(conv);
-    if (g->kind == PQI_WEIGHTS_I8) {
-        const int64_t len = round_up(g->n_pad, 256);
-        std::vector<int32_t> cs(len, 0);
-        for (int64_t i = 0; i < n; ++i) cs[i] = (int32_t)std::llround(rowsum[i]);  // W symmetric
-        if (hipMalloc(&g->wsum, len * 4) != hipSuccess) return bail(PQI_ERR_OOM, "device allocation failed");
-        g->device_bytes += len * 4;
-        if (hipMemcpy(g->wsum, cs.data(), len * 4, hipMemcpyHostToDevice) != hipSuccess)
-            return bail(PQI_ERR_HIP, "uploading column sums failed");
-    }
-    std::vector<double>().swap(rowsum);
     if (g->kind == PQI_WEIGHTS_F32) {
         const size_t bb = (size_t)g->n_pad * (size_t)g->n_pad * 4;  // two bf16 copies
         if (hipMalloc(&g->Wbf, bb) != hipSuccess) return bail(PQI_ERR_OOM, "device allocation of bf16 W failed");

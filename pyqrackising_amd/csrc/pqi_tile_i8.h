@@ -135,20 +135,17 @@ __device__ __forceinline__ void tile_i8(const int8_t* __restrict__ W, int64_t ld
 }
 
 // ---------------------------------------------------------------------------
-// Wide tile: 256 candidates x 256 columns per work group.  With s = 1 - 2x,
-//   (S W)_bj = c_j - 2 (X W)_bj,   c_j = sum_k W_kj (column sums, kept per graph),
-// so the MFMAs run on the 0/1 candidate bytes X: three VALU operations per 4
-// bytes to unpack instead of five.  Wave w owns rows 64w..64w+63 and all 256
-// columns (2 x 8 MFMA blocks, 256 accumulators): each unpacked fragment feeds
-// 8 MFMAs, and W crosses L2 once per 256 candidates.  A wave's candidate rows
-// are its own: each lane loads the packed bit words of its rows into
-// registers one chunk ahead and unpacks its fragments there.  W, shared by
-// the four waves, goes through LDS in two stages: while the waves run chunk c
-// from one stage, each thread stores chunk c+1 (loaded one chunk earlier)
-// into the other and loads chunk c+2; one barrier per chunk.  Columns past
-// n_pad (the last tile of a row of W) read a valid row and are dropped.
-// The contraction may be split over k (the result is linear in it): each
-// split adds its own -2 (X W) part, and one of them the c_j part.
+// Wide tile: 256 candidates x 256 columns per work group.  Wave w owns rows
+// 64w..64w+63 and all 256 columns (2 x 8 MFMA blocks, 256 accumulators): each
+// unpacked +-1 fragment feeds 8 MFMAs, and W crosses L2 once per 256
+// candidates.  A wave's candidate rows are its own: each lane loads the packed
+// bit words of its rows into registers one chunk ahead and unpacks its
+// fragments there.  W, shared by the four waves, goes through LDS in two
+// stages with coalesced loads: while the waves run chunk c from one stage,
+// each thread stores chunk c+1 (loaded one chunk earlier) into the other and
+// loads chunk c+2; one barrier per chunk.  Columns past n_pad (the last tile
+// of a row of W) read a valid row and are dropped.  The contraction may be
+// split over k (the result is linear in it); each split adds its own part.
 // LDS: 2 x 256 x (BK+16) bytes of W, plus 256 x 8 words for the epilogue.
 // ---------------------------------------------------------------------------
 constexpr int WIDE_BM = 256;
@@ -159,13 +156,13 @@ constexpr size_t wide_lds_bytes() {
     return 2 * (size_t)WIDE_BN * (BK + 16) + (size_t)WIDE_BM * 8 * 4;
 }
 
-// 16 candidate bits (k ascending) -> one MFMA fragment of 16 bytes 0/1
-__device__ __forceinline__ v4i unpack01(uint32_t half) {
+// 16 candidate bits (k ascending) -> one MFMA fragment of 16 +-1 int8 bytes
+__device__ __forceinline__ v4i unpack16(uint32_t half) {
     v4i a;
-    a[0] = (int)spread4(half & 0xFu);
-    a[1] = (int)spread4((half >> 4) & 0xFu);
-    a[2] = (int)spread4((half >> 8) & 0xFu);
-    a[3] = (int)spread4((half >> 12) & 0xFu);
+    a[0] = (int)to_pm1(spread4(half & 0xFu));
+    a[1] = (int)to_pm1(spread4((half >> 4) & 0xFu));
+    a[2] = (int)to_pm1(spread4((half >> 8) & 0xFu));
+    a[3] = (int)to_pm1(spread4((half >> 12) & 0xFu));
     return a;
 }
 
@@ -177,7 +174,7 @@ __device__ __forceinline__ void wide_compute(v16i (&acc)[2][8], const uint32_t (
     for (int t = 0; t < BK / 32; ++t) {
         v4i av[2];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) av[rb] = unpack01(hi ? (aw[rb][t] >> 16) : (aw[rb][t] & 0xFFFFu));
+        for (int rb = 0; rb < 2; ++rb) av[rb] = unpack16(hi ? (aw[rb][t] >> 16) : (aw[rb][t] & 0xFFFFu));
 #pragma unroll
         for (int cb = 0; cb < 8; ++cb) {
             const v4i bv = *(const v4i*)&Ws[(cb * 32 + lc) * ROW + 32 * t + 16 * hi];
@@ -211,11 +208,10 @@ __device__ __forceinline__ void wide_store_w(const v4i (&wv)[BK / 16], int8_t* s
     for (int t = 0; t < BK / 16; ++t) *(v4i*)(stage + woff[t]) = wv[t];
 }
 
-// Contraction over k in [kb, ke) (a split of K = ld; multiples of BK).  The
-// c_j term of P is added when colsum is given (by one split only).
+// Contraction over k in [kb, ke) (a split of K = ld; multiples of BK).
 template <int BK>
 __device__ __forceinline__ void tile_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t kb, int64_t ke,
-                                             const int32_t* __restrict__ colsum, const uint32_t* __restrict__ bits,
+                                             const uint32_t* __restrict__ bits,
                                              int64_t wpr, int64_t B, int64_t m0, int64_t j0,
                                              unsigned long long* __restrict__ q, unsigned char* lds) {
     constexpr int ROW = BK + 16;
@@ -289,24 +285,29 @@ __device__ __forceinline__ void tile_i8_wide(const int8_t* __restrict__ W, int64
     __syncthreads();
     // 32x32 accumulator layout: lane l, register r -> column l%32,
     // row 8*(r/4) + 4*(l/32) + r%4 of the block.  P = c_j - 2 (X W).
-    int cj[8];
+    // column blocks past n_pad are dropped (uniform per block)
 #pragma unroll
-    for (int cb = 0; cb < 8; ++cb) cj[cb] = (colsum && j0 + cb * 32 < ld) ? colsum[j0 + cb * 32 + lc] : 0;
+    for (int cb = 0; cb < 8; ++cb)
+        if (j0 + cb * 32 >= ld)
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[rb][cb][r] = 0;
+    // per element: m = -(candidate bit) by one signed bit-field extract, and
+    // s * P = (P + m) ^ m
     int vals[32];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            // the row's 8 words of bits (zero past n_pad, where acc is dropped)
             const int row = wid * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
             const v4i w03 = *(const v4i*)&Sb[row * 8];
             const v4i w47 = *(const v4i*)&Sb[row * 8 + 4];
             int v = 0;
 #pragma unroll
             for (int cb = 0; cb < 8; ++cb) {
-                const uint32_t word = (uint32_t)(cb < 4 ? w03[cb] : w47[cb - 4]);
-                const int p = j0 + cb * 32 < ld ? cj[cb] - 2 * acc[rb][cb][r] : 0;
-                v += ((word >> lc) & 1u) ? -p : p;
+                const int m = __builtin_amdgcn_sbfe(cb < 4 ? w03[cb] : w47[cb - 4], (unsigned)lc, 1u);
+                v += (acc[rb][cb][r] + m) ^ m;
             }
             vals[rb * 16 + r] = v;
         }
