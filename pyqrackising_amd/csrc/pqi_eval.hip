@@ -361,6 +361,29 @@ size_t batch_scratch_bytes(const Graph& g, int64_t B) {
     return (size_t)B * (size_t)nblk * 8;
 }
 
+void launch_batch_i8_raw(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, unsigned long long* qi,
+                         hipStream_t s) {
+    if (B >= 512 && g.max_strength >= 0 && 256 * g.max_strength < (int64_t(1) << 31)) {
+        const int64_t MB = (B + WIDE_BM - 1) / WIDE_BM, JB = (g.n_pad + WIDE_BN - 1) / WIDE_BN;
+        // split K until there is about one work group per CU (each split >= 512 deep)
+        int64_t KS = 1;
+        const int64_t tiles = ((JB + 7) / 8) * 8 * MB, cus = g.ctx ? std::max(1, g.ctx->num_cus) : 256;
+        while (tiles * KS * 2 <= cus && g.n_pad % (128 * KS * 2) == 0 && g.n_pad / (KS * 2) >= 512) KS *= 2;
+        static const bool big = hipFuncSetAttribute((const void*)k_batch_i8_wide<128>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    (int)wide_lds_bytes<128>()) == hipSuccess;
+        if (big)
+            hipLaunchKernelGGL(k_batch_i8_wide<128>, dim3((unsigned)(tiles * KS)), dim3(256), wide_lds_bytes<128>(), s,
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+        else
+            hipLaunchKernelGGL(k_batch_i8_wide<64>, dim3((unsigned)(tiles * KS)), dim3(256), wide_lds_bytes<64>(), s,
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+    } else {
+        dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
+        hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi);
+    }
+}
+
 void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, double* q, void* scratch,
                             hipStream_t s) {
     if (B <= 0) return;
@@ -382,27 +405,7 @@ void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, i
     if (g.kind == PQI_WEIGHTS_I8) {
         unsigned long long* qi = (unsigned long long*)scratch;
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B);
-        if (B >= 512 && g.max_strength >= 0 && 256 * g.max_strength < (int64_t(1) << 31)) {
-            const int64_t MB = (B + WIDE_BM - 1) / WIDE_BM, JB = (g.n_pad + WIDE_BN - 1) / WIDE_BN;
-            // split K until there is about one work group per CU (each split >= 512 deep)
-            int64_t KS = 1;
-            const int64_t tiles = ((JB + 7) / 8) * 8 * MB, cus = g.ctx ? std::max(1, g.ctx->num_cus) : 256;
-            while (tiles * KS * 2 <= cus && g.n_pad % (128 * KS * 2) == 0 && g.n_pad / (KS * 2) >= 512) KS *= 2;
-            static const bool big = hipFuncSetAttribute((const void*)k_batch_i8_wide<128>,
-                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                        (int)wide_lds_bytes<128>()) == hipSuccess;
-            if (big)
-                hipLaunchKernelGGL(k_batch_i8_wide<128>, dim3((unsigned)(tiles * KS)), dim3(256),
-                                   wide_lds_bytes<128>(), s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi,
-                                   MB, JB, KS);
-            else
-                hipLaunchKernelGGL(k_batch_i8_wide<64>, dim3((unsigned)(tiles * KS)), dim3(256), wide_lds_bytes<64>(),
-                                   s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
-        } else {
-            dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
-            hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr,
-                               B, qi);
-        }
+        launch_batch_i8_raw(g, bits, wpr, B, qi, s);
         hipLaunchKernelGGL(k_i64_to_f64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B, q);
     } else {
         const int64_t nblk = (g.n + 63) / 64;

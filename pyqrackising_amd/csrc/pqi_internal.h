@@ -92,6 +92,9 @@ void launch_score_with(const Graph& g, const uint8_t* x, int64_t count, double* 
 void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t words_per_row, int64_t B,
                             double* q, void* scratch, hipStream_t s);
 size_t batch_scratch_bytes(const Graph& g, int64_t B);
+// int8 MFMA contraction only: qi[b] += s_b^T W s_b (int64), qi zeroed by the caller.
+void launch_batch_i8_raw(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, unsigned long long* qi,
+                         hipStream_t s);
 // Split-BF16 MFMA energies for fp32 weights (needs g.Wbf); q approximate (see pqi_eval.hip).
 void launch_batch_bf16(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, double* q, void* scratch,
                        hipStream_t s);
@@ -125,7 +128,15 @@ void launch_flip(const Graph& g, uint8_t* x, void* h, int64_t k, int64_t l, hipS
 
 // Gray seeds: packed candidate rows for seeds [0, count) of theta.
 void launch_seed_bits(const uint8_t* x, int64_t n, int64_t words_per_row, int64_t block_size,
-                      int64_t count, uint32_t* bits, hipStream_t s);
+                      int64_t count, uint32_t* bits, hipStream_t s, unsigned long long* qzero = nullptr);
+// Seed cut differences from int64 quadratic forms (int8 weights): q_base = s^T h of
+// (x, h), delta[b] = (q_base - qi[b]) / 4; also clears rank[0..count).
+void launch_seed_delta_i8(const uint8_t* x, const int32_t* h, int64_t n, const unsigned long long* qi,
+                          int64_t count, double* delta, int32_t* rank, hipStream_t s);
+// Stable top-k of v by rank counting with the ranks already cleared, then
+// sel[r] = index of rank r and seld[r] = its value, r < k.
+void launch_topk_gather(const double* v, int64_t count, int64_t k, int64_t* sel, double* seld, int32_t* rank,
+                        hipStream_t s);
 // Seed cut differences from the field (low-rank formula).
 void launch_seed_lowrank(const Graph& g, const uint8_t* x, const void* h, int64_t block_size, int64_t count,
                          double* delta, hipStream_t s);

@@ -679,8 +679,6 @@ static int gray_seeds_buffers(pqi_graph* g, const pqi_search_params* p, GrayBufs
         const int64_t wpr = g->n_pad >> 5;
         void* t;
         CHK(ensure(c, S_BITS, (size_t)count * wpr * 4, &t));
-        CHK(ensure(c, S_Q, count * 8, &t));
-        CHK(ensure(c, S_QB, 16, &t));
         CHK(ensure(c, S_BATCH, batch_scratch_bytes(*g, count + 1), &t));
     }
     void* rank;
@@ -705,20 +703,21 @@ static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_s
     if (mfma) {
         const int64_t wpr = g->n_pad >> 5;
         uint32_t* bits;
-        double *q, *qb;
         void* scratch;
         CHK(ensure(c, S_BITS, (size_t)count * wpr * 4, (void**)&bits));
-        CHK(ensure(c, S_Q, count * 8, (void**)&q));
-        CHK(ensure(c, S_QB, 16, (void**)&qb));
         CHK(ensure(c, S_BATCH, batch_scratch_bytes(*g, count + 1), &scratch));
-        launch_seed_bits(x, g->n, wpr, bs, count, bits, c->stream);
-        launch_batch_quadratic(*g, bits, wpr, count, q, scratch, c->stream);
-        // q of the base state from its maintained field: q_base = s^T h
-        launch_quad(*g, x, h, qb, c->stream);
-        launch_q_to_delta(q, qb, count, b.delta, c->stream);
-    } else {
-        launch_seed_lowrank(*g, x, h, bs, count, b.delta, c->stream);
+        void* rank;
+        CHK(ensure(c, S_RANK, topk_scratch_bytes(count), &rank));
+        unsigned long long* qi = (unsigned long long*)scratch;
+        // seed rows (clearing the row totals), int8 MFMA contraction, cut
+        // differences against q_base = s^T h (clearing the ranks), top-k
+        launch_seed_bits(x, g->n, wpr, bs, count, bits, c->stream, qi);
+        launch_batch_i8_raw(*g, bits, wpr, count, qi, c->stream);
+        launch_seed_delta_i8(x, (const int32_t*)h, g->n, qi, count, b.delta, (int32_t*)rank, c->stream);
+        launch_topk_gather(b.delta, count, p->walkers, b.sel, b.seld, (int32_t*)rank, c->stream);
+        return PQI_OK;
     }
+    launch_seed_lowrank(*g, x, h, bs, count, b.delta, c->stream);
     void* rank;
     CHK(ensure(c, S_RANK, topk_scratch_bytes(count), &rank));
     launch_topk(b.delta, count, p->walkers, b.sel, rank, c->stream);
@@ -907,8 +906,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                                       (int64_t)rx, (int64_t)rh, (int64_t)dout, (int64_t)dout2, (int64_t)dsc,
                                       (int64_t)ctl, (int64_t)b.wx, (int64_t)b.wh, (int64_t)b.we, (int64_t)b.wev,
                                       (int64_t)b.sel, (int64_t)b.seld, (int64_t)b.delta,
-                                      (int64_t)c->bufs[S_BITS].p, (int64_t)c->bufs[S_Q].p,
-                                      (int64_t)c->bufs[S_QB].p, (int64_t)c->bufs[S_BATCH].p,
+                                      (int64_t)c->bufs[S_BITS].p, (int64_t)c->bufs[S_BATCH].p,
                                       (int64_t)c->bufs[S_RANK].p};
     if (c->ls.key != key) {
         ls_graphs_reset(c);

@@ -1031,31 +1031,79 @@ __device__ __forceinline__ int64_t seed_pos(int64_t n, int64_t offset, int j) {
 }
 
 __global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs,
-                                                   int64_t count, uint32_t* bits) {
+                                                   int64_t count, uint32_t* bits, unsigned long long* qzero) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * wpr) return;
     const int64_t b = t / wpr, w = t % wpr;
+    if (qzero && w == 0) qzero[b] = 0ull;  // the MFMA row totals of this seed start at zero
+    // the state's word w: 4 bytes of 0/1 -> 4 bits by one multiply, when the
+    // 32 bytes lie inside x and x is 16-byte aligned
     uint32_t word = 0;
-    for (int q = 0; q < 32; ++q) {
-        int64_t v = w * 32 + q;
-        if (v < n && x[v]) word |= 1u << q;
+    if (w * 32 + 32 <= n && ((uintptr_t)x & 15) == 0) {
+        const uint4* x4 = (const uint4*)(x + w * 32);
+        const uint4 lo = x4[0], hi = x4[1];
+        const uint32_t u[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) word |= ((u[q] * 0x01020408u) >> 24 & 0xFu) << (4 * q);
+    } else {
+        for (int q = 0; q < 32; ++q) {
+            const int64_t v = w * 32 + q;
+            if (v < n && x[v]) word |= 1u << q;
+        }
     }
     const uint64_t i = (uint64_t)(b % bs);
     const uint64_t gray = i ^ (i >> 1);
     const int64_t offset = (b / bs) << 6;
-    for (int j = 0; j < 64 && j < n; ++j) {
-        if (!((gray >> j) & 1u)) continue;
-        int64_t idx = seed_pos(n, offset, j);
-        if ((idx >> 5) == w) word ^= 1u << (idx & 31);
+    const int64_t base = n - offset - 1;  // position of gray bit 0
+    if (n >= 64 && base >= 63) {
+        // no wrap: bit j flips position base - j; with r = bit-reverse(gray),
+        // position p takes bit p + 63 - base of r
+        const uint64_t r = __brevll(gray);
+        const int64_t sh = w * 32 + 63 - base;
+        uint32_t mask = 0;
+        if (sh >= 0 && sh < 64) mask = (uint32_t)(r >> sh);
+        else if (sh < 0 && sh > -32) mask = (uint32_t)(r << (-sh));
+        word ^= mask;
+    } else {
+        for (int j = 0; j < 64 && j < n; ++j) {
+            if (!((gray >> j) & 1u)) continue;
+            const int64_t idx = seed_pos(n, offset, j);
+            if ((idx >> 5) == w) word ^= 1u << (idx & 31);
+        }
     }
     bits[b * wpr + w] = word;
 }
 
 void launch_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs, int64_t count, uint32_t* bits,
-                      hipStream_t s) {
+                      hipStream_t s, unsigned long long* qzero) {
     int64_t threads = count * wpr;
     hipLaunchKernelGGL(k_seed_bits, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, x, n, wpr, bs, count,
-                       bits);
+                       bits, qzero);
+}
+
+// Each work group recomputes q_base = s^T h (n terms, int64, exact) and
+// converts its 256 seeds; the rank counters of the following top-k start at 0.
+__global__ __launch_bounds__(256) void k_seed_delta_i8(const uint8_t* x, const int32_t* h, int64_t n,
+                                                       const unsigned long long* qi, int64_t count, double* delta,
+                                                       int32_t* rank) {
+    __shared__ long long part[4];
+    long long acc = 0;
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) acc += x[v] ? -(long long)h[v] : (long long)h[v];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    const long long qb = part[0] + part[1] + part[2] + part[3];
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < count) {
+        delta[b] = (double)(qb - (long long)qi[b]) * 0.25;
+        rank[b] = 0;
+    }
+}
+
+void launch_seed_delta_i8(const uint8_t* x, const int32_t* h, int64_t n, const unsigned long long* qi,
+                          int64_t count, double* delta, int32_t* rank, hipStream_t s) {
+    hipLaunchKernelGGL(k_seed_delta_i8, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, x, h, n, qi, count,
+                       delta, rank);
 }
 
 // Cut difference of flipping the set F from (x, h):
@@ -1170,10 +1218,29 @@ __global__ void k_zero_i32(int32_t* p, int64_t count) {
 
 size_t topk_scratch_bytes(int64_t count) { return (size_t)count * 4; }
 
+__global__ void k_rank_scatter_gather(const int32_t* rank, const double* v, int64_t count, int64_t k, int64_t* sel,
+                                      double* seld) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < count && rank[a] < k) {
+        sel[rank[a]] = a;
+        seld[rank[a]] = v[a];
+    }
+}
+
+void launch_topk_gather(const double* v, int64_t count, int64_t k, int64_t* sel, double* seld, int32_t* rank,
+                        hipStream_t s) {
+    const unsigned gx = (unsigned)((count + 255) / 256);
+    int64_t splits = (count + 63) / 64;  // about 64 comparisons per thread, at most 64 splits
+    if (splits > 64) splits = 64;
+    const int64_t span = (count + splits - 1) / splits;
+    hipLaunchKernelGGL(k_rank, dim3(gx, (unsigned)splits), dim3(256), 0, s, v, count, span, rank);
+    hipLaunchKernelGGL(k_rank_scatter_gather, dim3(gx), dim3(256), 0, s, rank, v, count, k, sel, seld);
+}
+
 void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, void* scratch, hipStream_t s) {
     int32_t* rank = (int32_t*)scratch;
     const unsigned gx = (unsigned)((count + 255) / 256);
-    int64_t splits = (count + 511) / 512;  // about 512 comparisons per thread
+    int64_t splits = (count + 63) / 64;  // about 64 comparisons per thread, at most 64 splits
     if (splits > 64) splits = 64;
     const int64_t span = (count + splits - 1) / splits;
     hipLaunchKernelGGL(k_zero_i32, dim3(gx), dim3(256), 0, s, rank, count);
