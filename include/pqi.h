@@ -160,6 +160,24 @@ PQI_API int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is
 PQI_API int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, int engine,
                               double* values, int64_t* best_index, double* best_value, double* max_rel_err);
 
+/* Candidate sampler.  Replaces sample_mag + local_repulsion_choice
+ * (pyqrackising/maxcut_tfim.py:36-75, 139-145; maxcut_tfim_util.py:690-724):
+ * shot s = shot_offset + i draws its Hamming weight from cum_prob (ncum
+ * entries, the maxcut_hamming_cdf output) and picks nodes with neighbour
+ * repulsion repulsion_base^(-w).  The random stream is counter-based in
+ * (seed, s, draw) and therefore reproducible; the reference's is not
+ * seedable.  out: shots x n bytes, 1 = selected. */
+PQI_API int pqi_sample_candidates(pqi_graph* g, const double* cum_prob, int64_t ncum, double repulsion_base,
+                                  uint64_t seed, int64_t shot_offset, int64_t shots, uint8_t* out);
+
+/* One sampling round of maxcut_tfim's accelerator path
+ * (pyqrackising/maxcut_tfim.py:148-186): shots candidates sampled, scored
+ * (cut, or energy in spin-glass mode) and reduced to the first best, all on
+ * the device; only the best bitstring (n bytes) is copied back. */
+PQI_API int pqi_sample_round(pqi_graph* g, const double* cum_prob, int64_t ncum, double repulsion_base,
+                             uint64_t seed, int64_t shot_offset, int64_t shots, int is_spin_glass,
+                             int64_t* best_index, double* best_value, uint8_t* best_bits);
+
 /* Single-flip phase.  Replaces run_single_bit_flips / run_bit_flips_opencl(False)
  * (pyqrackising/spin_glass_solver.py:37-52, 156-241; kernels.cl:100-148).
  * *gain is the best cut gain, x2 in spin-glass mode; *k its first argmax. */

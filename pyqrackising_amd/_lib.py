@@ -120,6 +120,8 @@ SIGNATURES = {
     "pqi_gray_search": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "pqi_local_search": (_int, [_vp, _vp, _vp, _vp]),
     "pqi_score": (_int, [_vp, _vp, _vp, _vp]),
+    "pqi_sample_candidates": (_int, [_vp, _vp, _i64, _dbl, ctypes.c_uint64, _i64, _i64, _vp]),
+    "pqi_sample_round": (_int, [_vp, _vp, _i64, _dbl, ctypes.c_uint64, _i64, _i64, _int, _vp, _vp, _vp]),
     "pqi_batch_create": (_int, [_vp, _vp, _i64, _i64, _vp]),
     "pqi_batch_free": (_int, [_vp]),
     "pqi_batch_set_candidates": (_int, [_vp, _vp]),
@@ -388,6 +390,27 @@ class DeviceGraph:
         with self.ctx.lock:
             check(load().pqi_score(self._h, _p(t), ctypes.byref(c), ctypes.byref(e)))
         return c.value, e.value
+
+    def sample_candidates(self, cum_prob, repulsion_base, seed, shot_offset, shots):
+        """shots x n bool candidates from the device sampler (shot indices
+        shot_offset .. shot_offset + shots - 1)."""
+        cum = np.ascontiguousarray(cum_prob, dtype=np.float64)
+        out = np.empty((int(shots), self.n), dtype=np.uint8)
+        with self.ctx.lock:
+            check(load().pqi_sample_candidates(self._h, _p(cum), cum.shape[0], float(repulsion_base),
+                                               int(seed) & (2 ** 64 - 1), int(shot_offset), int(shots), _p(out)))
+        return out.astype(bool)
+
+    def sample_round(self, cum_prob, repulsion_base, seed, shot_offset, shots, is_spin_glass):
+        """Sample, score and reduce one round on the device: (index, value, bits)."""
+        cum = np.ascontiguousarray(cum_prob, dtype=np.float64)
+        bi, bv = ctypes.c_int64(), ctypes.c_double()
+        bits = np.empty(self.n, dtype=np.uint8)
+        with self.ctx.lock:
+            check(load().pqi_sample_round(self._h, _p(cum), cum.shape[0], float(repulsion_base),
+                                          int(seed) & (2 ** 64 - 1), int(shot_offset), int(shots),
+                                          int(bool(is_spin_glass)), ctypes.byref(bi), ctypes.byref(bv), _p(bits)))
+        return bi.value, bv.value, bits.astype(bool)
 
 
 class Batch:

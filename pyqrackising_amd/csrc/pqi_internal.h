@@ -261,6 +261,11 @@ void launch_b_order(const BInstRaw* d, const int32_t* list, int64_t L, hipStream
 void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n,
                            float min_score, hipStream_t s);
 
+// candidate sampler (pqi_sampler.hip): shots rows of packed candidates from the
+// Hamming-weight CDF and the neighbour-repulsion picker, shot indices shot0..
+int launch_sampler(const Graph& g, const double* cum, int64_t ncum, double base, uint64_t seed, int64_t shot0,
+                   int64_t shots, uint32_t* bits, int64_t wpr, hipStream_t s);
+
 // network internals used by the learned search (pqi_mpnn.hip)
 float* mpnn_features(pqi_mpnn* m);
 const float* mpnn_scores(pqi_mpnn* m);

@@ -44,7 +44,7 @@ int fail(int code, const std::string& msg) {
 // Scratch slots of a context.
 enum Slot {
     S_X, S_H, S_RX, S_RH, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
-    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_COUNT
+    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_COUNT
 };
 
 static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
@@ -479,6 +479,9 @@ int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy) {
     return score_device(g, x, cut, energy);
 }
 
+static int eval_on_device(pqi_graph* g, uint32_t* dbits, int64_t B, int is_spin_glass, bool bf16, double* values,
+                          int64_t* best_index, double* best_value, double* max_rel_err);
+
 int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, int engine,
                       double* values, int64_t* best_index, double* best_value, double* max_rel_err) {
     if (!g || (!bits && B > 0)) return fail(PQI_ERR_INVALID, "null argument");
@@ -493,14 +496,22 @@ int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin
     const int64_t nw = (g->n + 31) >> 5;
     const int64_t wpr = g->n_pad >> 5;
     uint32_t* dbits;
-    double* q;
-    void* scratch;
     CHK(ensure(c, S_BITS, (size_t)B * wpr * 4, (void**)&dbits));
-    CHK(ensure(c, S_Q, (size_t)B * 8, (void**)&q));
-    CHK(ensure(c, S_BATCH, bf16 ? batch_bf16_scratch_bytes(*g, B) : batch_scratch_bytes(*g, B), &scratch));
     // padding words stay zero: padded columns of W are zero, so they add nothing
     HIPCHK(hipMemsetAsync(dbits, 0, (size_t)B * wpr * 4, c->stream));
     HIPCHK(hipMemcpy2DAsync(dbits, wpr * 4, bits, nw * 4, nw * 4, B, hipMemcpyHostToDevice, c->stream));
+    return eval_on_device(g, dbits, B, is_spin_glass, bf16, values, best_index, best_value, max_rel_err);
+}
+
+// Scores B candidate rows already on the device (n_pad/32 words per row).
+static int eval_on_device(pqi_graph* g, uint32_t* dbits, int64_t B, int is_spin_glass, bool bf16, double* values,
+                          int64_t* best_index, double* best_value, double* max_rel_err) {
+    pqi_ctx* c = g->ctx;
+    const int64_t wpr = g->n_pad >> 5;
+    double* q;
+    void* scratch;
+    CHK(ensure(c, S_Q, (size_t)B * 8, (void**)&q));
+    CHK(ensure(c, S_BATCH, bf16 ? batch_bf16_scratch_bytes(*g, B) : batch_scratch_bytes(*g, B), &scratch));
     if (bf16) launch_batch_bf16(*g, dbits, wpr, B, q, scratch, c->stream);
     else launch_batch_quadratic(*g, dbits, wpr, B, q, scratch, c->stream);
     std::vector<double> hq(B);
@@ -564,6 +575,54 @@ int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin
 int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, double* values,
                    int64_t* best_index, double* best_value) {
     return pqi_eval_batch_ex(g, bits, B, is_spin_glass, PQI_ENGINE_AUTO, values, best_index, best_value, nullptr);
+}
+
+static int sample_to_device(pqi_graph* g, const double* cum_prob, int64_t ncum, double repulsion_base, uint64_t seed,
+                            int64_t shot_offset, int64_t shots, uint32_t** dbits) {
+    if (!cum_prob || ncum < 1) return fail(PQI_ERR_INVALID, "cum_prob must have at least one entry");
+    if (shots < 1 || shot_offset < 0) return fail(PQI_ERR_INVALID, "shots must be >= 1, shot_offset >= 0");
+    if (!(repulsion_base > 0.0) || !std::isfinite(repulsion_base))
+        return fail(PQI_ERR_INVALID, "repulsion_base must be positive and finite");
+    pqi_ctx* c = g->ctx;
+    const int64_t wpr = g->n_pad >> 5;
+    double* dcum;
+    CHK(ensure(c, S_CUM, (size_t)ncum * 8, (void**)&dcum));
+    CHK(ensure(c, S_BITS, (size_t)shots * wpr * 4, (void**)dbits));
+    HIPCHK(hipMemcpyAsync(dcum, cum_prob, (size_t)ncum * 8, hipMemcpyHostToDevice, c->stream));
+    return launch_sampler(*g, dcum, ncum, repulsion_base, seed, shot_offset, shots, *dbits, wpr, c->stream);
+}
+
+int pqi_sample_candidates(pqi_graph* g, const double* cum_prob, int64_t ncum, double repulsion_base, uint64_t seed,
+                          int64_t shot_offset, int64_t shots, uint8_t* out) {
+    if (!g || !out) return fail(PQI_ERR_INVALID, "null argument");
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    uint32_t* dbits;
+    CHK(sample_to_device(g, cum_prob, ncum, repulsion_base, seed, shot_offset, shots, &dbits));
+    const int64_t wpr = g->n_pad >> 5, n = g->n;
+    std::vector<uint32_t> hb((size_t)shots * wpr);
+    HIPCHK(hipMemcpyAsync(hb.data(), dbits, hb.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    for (int64_t s2 = 0; s2 < shots; ++s2)
+        for (int64_t v = 0; v < n; ++v) out[s2 * n + v] = (uint8_t)((hb[s2 * wpr + (v >> 5)] >> (v & 31)) & 1u);
+    return PQI_OK;
+}
+
+int pqi_sample_round(pqi_graph* g, const double* cum_prob, int64_t ncum, double repulsion_base, uint64_t seed,
+                     int64_t shot_offset, int64_t shots, int is_spin_glass, int64_t* best_index, double* best_value,
+                     uint8_t* best_bits) {
+    if (!g || !best_index || !best_value || !best_bits) return fail(PQI_ERR_INVALID, "null argument");
+    pqi_ctx* c = g->ctx;
+    CHK(set_device(c));
+    uint32_t* dbits;
+    CHK(sample_to_device(g, cum_prob, ncum, repulsion_base, seed, shot_offset, shots, &dbits));
+    CHK(eval_on_device(g, dbits, shots, is_spin_glass, false, nullptr, best_index, best_value, nullptr));
+    const int64_t wpr = g->n_pad >> 5, n = g->n;
+    std::vector<uint32_t> row(wpr);
+    HIPCHK(hipMemcpyAsync(row.data(), dbits + *best_index * wpr, wpr * 4, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    for (int64_t v = 0; v < n; ++v) best_bits[v] = (uint8_t)((row[v >> 5] >> (v & 31)) & 1u);
+    return PQI_OK;
 }
 
 int pqi_single_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, int64_t* k, double* gain) {

@@ -1,14 +1,9 @@
-"""Host-side candidate sampler: the TFIM Hamming-weight prior and the
-neighbour-repulsion bit picker that produce the candidate bitstrings the GPU
-scores (pyqrackising/maxcut_tfim_util.py:545-724, maxcut_tfim.py:36-75,
-139-145, 189-209; maxcut_tfim_sparse.py:37-88, 204-224).
-
-The arithmetic is restated in float64 numpy.  Two deliberate differences from
-the reference: the random stream is a seeded numpy PCG64 generator (the
-reference draws from numba's unseedable per-thread generator, so its candidate
-streams cannot be reproduced anyway), and shots are drawn vectorised over a
-chunk of shots instead of one numba thread per shot.
-"""
+"""Host side of the candidate sampler: the TFIM Hamming-weight prior
+(pyqrackising/maxcut_tfim_util.py:545-687; maxcut_tfim.py:189-209;
+maxcut_tfim_sparse.py:204-224), restated in float64 numpy.  Its output, the
+cumulative Hamming-weight distribution, drives the device sampler
+(pqi_sample_candidates / pqi_sample_round), which draws the candidates with
+neighbour repulsion on the GPU."""
 import math
 
 import numpy as np
@@ -138,72 +133,3 @@ def maxcut_hamming_cdf(n_qubits, J_func, degrees, quality, tot_t, h_mult, omega=
     cum_prob = np.cumsum(hamming_prob[1:-1])
     cum_prob[-1] = 1.0
     return cum_prob
-
-
-# ---- sampler ----------------------------------------------------------------
-
-class RepulsionSampler:
-    """pyqrackising/maxcut_tfim.py:36-75 (local_repulsion_choice) with
-    sample_mag / bit_pick (maxcut_tfim_util.py:690-724), vectorised over shots.
-
-    Semantics kept: the first node of shot s is s mod n; a sampled m <= 1 picks
-    as the reference does (m == 1 gives one node, m == 0 gives two); a node is
-    "selected" when its weight is exactly 0 at the end (weights == 0.0)."""
-
-    def __init__(self, W, cum_prob, repulsion_base, rng, chunk=None):
-        self.W = np.asarray(W, dtype=np.float64)  # symmetric dense
-        self.n = self.W.shape[0]
-        self.cum_prob = np.asarray(cum_prob, dtype=np.float64)
-        self.base = float(repulsion_base)
-        self.rng = rng
-        self.no_repulsion = abs(1.0 - self.base) <= EPSILON
-        if not self.no_repulsion:
-            self.factor = self.base ** (-self.W)  # row k: multiplier after picking k
-        self.chunk = chunk or max(1, min(4096, (1 << 24) // max(1, self.n)))
-
-    def sample(self, shots, start=0):
-        n = self.n
-        out = np.empty((shots, n), dtype=bool)
-        for c0 in range(0, shots, self.chunk):
-            c1 = min(shots, c0 + self.chunk)
-            out[c0:c1] = self._chunk(np.arange(start + c0, start + c1))
-        return out
-
-    def _pick(self, weights):
-        p = weights.sum(axis=1) * self.rng.random(weights.shape[0])
-        cum = np.cumsum(weights, axis=1)
-        hit = p[:, None] < cum
-        node = np.argmax(hit, axis=1)
-        node[~hit.any(axis=1)] = 0
-        return node
-
-    def _update(self, weights, rows, node):
-        weights[rows, node] = 0.0
-        if not self.no_repulsion:
-            weights[rows] *= self.factor[node]
-
-    def _chunk(self, s_idx):
-        n = self.n
-        S = s_idx.size
-        m = np.searchsorted(self.cum_prob, self.rng.random(S), side="left")
-        first = s_idx % n
-        weights = np.ones((S, n), dtype=np.float64)
-        rows = np.arange(S)
-        one = m == 1
-        self._update(weights, rows, first)
-        # picks 1 .. m-2 update, pick m-1 (or 1 when m == 0) only zeroes
-        last_pick = np.maximum(m - 1, 1)
-        tmax = int(last_pick.max()) if S else 0
-        for t in range(1, tmax + 1):
-            act = (t <= last_pick) & ~one
-            if not act.any():
-                break
-            r = rows[act]
-            node = self._pick(weights[r])
-            upd = t < last_pick[r]
-            self._update(weights, r[upd], node[upd])
-            weights[r[~upd], node[~upd]] = 0.0
-        sel = weights == 0.0
-        sel[one] = False
-        sel[rows[one], first[one]] = True
-        return sel

@@ -74,7 +74,6 @@ def _sample_and_score(dg, W_dense, n, shots, quality, is_spin_glass, anneal_t, a
     J_eff, degrees = prior
     cum_prob = smp.maxcut_hamming_cdf(n, J_eff, degrees, quality, anneal_t, anneal_h)
     shots = ((max(1, shots >> 1) + 31) >> 5) << 5
-    sampler = smp.RepulsionSampler(W_dense, cum_prob, repulsion_base, rng)
     best_solution = np.zeros(n, dtype=bool)
     if is_spin_glass:
         _, e0 = dg.score(best_solution)
@@ -82,14 +81,17 @@ def _sample_and_score(dg, W_dense, n, shots, quality, is_spin_glass, anneal_t, a
     else:
         best_energy = 0.0
     improved = True
-    start = 0
+    seed = int(rng.integers(0, 2 ** 63))
+    rnd = 0
     while improved:
         improved = False
-        samples = sampler.sample(shots, start)
-        bi, bv, _ = dg.eval_batch(_lib.pack_bits(samples), is_spin_glass, want_values=False)
+        # the reference draws fresh samples for shot indices 0..shots-1 each
+        # round; here each round gets its own counter range of the stream
+        _, bv, bits = dg.sample_round(cum_prob, repulsion_base, seed + rnd, 0, shots, is_spin_glass)
+        rnd += 1
         if bv > best_energy:
             best_energy = bv
-            best_solution = samples[bi].copy()
+            best_solution = bits
             improved = True
     if is_spin_glass:
         best_energy, _ = dg.score(best_solution)

@@ -411,15 +411,18 @@ def test_c1_pipeline_matches_oracle(ctx, orc):
     n, seed = 64, 31
     W = er_weights(n, 0.5, 1001, "unit").astype(np.int32)
     bits, value, _ = maxcut_tfim(W, quality=6, seed=seed)
-    # replay (pyqrackising/maxcut_tfim.py:148-186 semantics)
+    # replay (pyqrackising/maxcut_tfim.py:148-186 semantics): the same device
+    # candidates (same seed schedule), scored and reduced by the oracle
     Wd = W.astype(np.float64)
     J, z = smp.init_J_and_z_dense(Wd)
     cum = smp.maxcut_hamming_cdf(n, J, z, 6, 8.0, 8.0)
     shots = ((max(1, (n << 6) >> 1) + 31) >> 5) << 5
-    sampler = smp.RepulsionSampler(Wd, cum, 5.0, np.random.default_rng(seed))
-    best, best_e = np.zeros(n, bool), 0.0
+    dg = _lib.DeviceGraph.dense(W)
+    s0 = int(np.random.default_rng(seed).integers(0, 2 ** 63))
+    best, best_e, rnd = np.zeros(n, bool), 0.0, 0
     while True:
-        X = sampler.sample(shots, 0)
+        X = dg.sample_candidates(cum, 5.0, s0 + rnd, 0, shots)
+        rnd += 1
         bi, vals = orc.eval_batch(Wd, X, 0)
         if vals[bi] > best_e:
             best_e, best = vals[bi], X[bi].copy()
