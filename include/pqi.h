@@ -272,6 +272,50 @@ PQI_API int pqi_learned_search(pqi_ctx* ctx, pqi_mpnn* net, pqi_graph* const* gr
 /* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
 PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);
 
+typedef struct pqi_bnb_stats {
+    int64_t nodes_explored; /* nodes popped */
+    int64_t nodes_pruned;   /* nodes with ub <= incumbent + 1e-9 */
+    int64_t batches;        /* node batches evaluated (one kernel launch each) */
+    int64_t max_stack;      /* deepest stack reached, in nodes */
+    int32_t certified;      /* 1: the stack emptied; 0: time_limit stopped the search */
+    int32_t pad;
+    double seconds;         /* wall time of the search loop */
+    double kernel_seconds;  /* summed device time of the node-evaluation kernel */
+} pqi_bnb_stats;
+
+/* Batched branch-and-bound node evaluation.  Replaces _upper_bound_batch,
+ * _eval_leaves_cut / _eval_leaves_energy and _influence_scores + argmax
+ * (pyqrackising/solve_maxcut_bnb.py:48-121, 187-188; sparse
+ * solve_maxcut_bnb_sparse.py:32-114): nodes is count x n int8 (-1 free,
+ * 0/1 fixed).  ub[k]: exact cut of the fixed pairs plus max(w, 0) for every
+ * pair with a free end; leaf[k]: the cut (or, in spin-glass mode,
+ * sum_{i<j} (x_i == x_j ? -w : w)) when node k has no free variable, NaN
+ * otherwise; branch[k]: the first free variable of largest sum |w| to the
+ * other free variables, -1 for a leaf.  kernel_seconds (optional): device
+ * time of the launch.  Dense graphs only, n <= 4096, count * n <= 2^30. */
+PQI_API int pqi_bnb_eval(pqi_graph* g, const int8_t* nodes, int64_t count, int is_spin_glass, double* ub,
+                         double* leaf, int32_t* branch, double* kernel_seconds);
+
+/* Warm-started branch and bound.  Replaces _branch_and_bound
+ * (pyqrackising/solve_maxcut_bnb.py:128-201; sparse
+ * solve_maxcut_bnb_sparse.py:119-198): each batch of up to batch_cap nodes
+ * popped from the top of the stack is evaluated as in pqi_bnb_eval; nodes
+ * whose bound is <= incumbent + 1e-9 are pruned, the first best leaf replaces
+ * the incumbent if larger, and every other node pushes its two children (the
+ * incumbent's value of the branch variable first).  The exploration order is
+ * the reference's, so with the reference's batch_cap (os.cpu_count()**2) the
+ * counts and result are the reference's.
+ * energy_bound = 0 prunes with ub in both modes, as the reference does; in
+ * spin-glass mode that bound does not bound the leaf value E = 2 cut -
+ * sum_{i<j} w, so the optimum can be pruned.  energy_bound = 1 prunes
+ * spin-glass nodes with 2 ub - sum_{i<j} w instead, which is exact.
+ * best_bits (n bytes, 0/1) and *best_value carry the incumbent in and the
+ * result out.  time_limit in seconds, negative for none; when it stops the
+ * search, stats->certified is 0.  Dense graphs only, n <= 4096,
+ * batch_cap * n <= 2^30. */
+PQI_API int pqi_bnb(pqi_graph* g, uint8_t* best_bits, double* best_value, int is_spin_glass, int energy_bound,
+                    int64_t batch_cap, double time_limit, pqi_bnb_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,22 @@ class SearchStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BnbStats(ctypes.Structure):
+    _fields_ = [
+        ("nodes_explored", ctypes.c_int64),
+        ("nodes_pruned", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("max_stack", ctypes.c_int64),
+        ("certified", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("seconds", ctypes.c_double),
+        ("kernel_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
 # name -> (restype, argtypes); every symbol include/pqi.h declares
 SIGNATURES = {
     "pqi_last_error": (ctypes.c_char_p, []),
@@ -141,6 +157,8 @@ SIGNATURES = {
     "pqi_mpnn_loss_grad": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "pqi_mpnn_train_step": (_int, [_vp, _vp, _vp, ctypes.c_float, _vp]),
     "pqi_learned_search": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, ctypes.c_float, _vp]),
+    "pqi_bnb_eval": (_int, [_vp, _vp, _i64, _int, _vp, _vp, _vp, _vp]),
+    "pqi_bnb": (_int, [_vp, _vp, _vp, _int, _int, _i64, _dbl, _vp]),
 }
 
 _lib = None
@@ -411,6 +429,32 @@ class DeviceGraph:
                                           int(seed) & (2 ** 64 - 1), int(shot_offset), int(shots),
                                           int(bool(is_spin_glass)), ctypes.byref(bi), ctypes.byref(bv), _p(bits)))
         return bi.value, bv.value, bits.astype(bool)
+
+    def bnb_eval(self, nodes, is_spin_glass, return_time=False):
+        """Bound, leaf value and branch variable of count x n nodes (pqi_bnb_eval)."""
+        F = np.ascontiguousarray(nodes, dtype=np.int8)
+        if F.ndim != 2 or F.shape[1] != self.n:
+            raise ValueError(f"nodes must have shape (count, {self.n})")
+        cnt = F.shape[0]
+        ub = np.empty(cnt, dtype=np.float64)
+        leaf = np.empty(cnt, dtype=np.float64)
+        br = np.empty(cnt, dtype=np.int32)
+        ks = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_bnb_eval(self._h, _p(F), cnt, int(bool(is_spin_glass)), _p(ub), _p(leaf), _p(br),
+                                      ctypes.byref(ks)))
+        return (ub, leaf, br, ks.value) if return_time else (ub, leaf, br)
+
+    def bnb(self, warm_bits, warm_value, is_spin_glass, batch_cap, time_limit=None, energy_bound=False):
+        """Warm-started branch and bound (pqi_bnb): (bits, value, stats)."""
+        bits = self._theta(warm_bits).copy()
+        val = ctypes.c_double(float(warm_value))
+        st = BnbStats()
+        tl = -1.0 if time_limit is None else float(time_limit)
+        with self.ctx.lock:
+            check(load().pqi_bnb(self._h, _p(bits), ctypes.byref(val), int(bool(is_spin_glass)),
+                                 int(bool(energy_bound)), int(batch_cap), tl, ctypes.byref(st)))
+        return bits.astype(bool), val.value, st.as_dict()
 
 
 class Batch:
