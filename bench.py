@@ -223,6 +223,92 @@ def sk_bf16(log):
     }
 
 
+def sampler_round(log, with_cpu):
+    """One maxcut_tfim sampling round at the same shape (random 3-regular
+    n = 2048, quality 6): roundup32((n << 6) / 2) = 65536 candidates drawn with
+    neighbour repulsion (base 5), scored and reduced to the best on the device.
+    CPU baseline: the numpy restatement of the reference sampler (vectorised
+    over shots) on a bounded sample of 512 shots."""
+    from pyqrackising_amd import _lib
+    from pyqrackising_amd import sampler as smp
+
+    n = 2048
+    W, _ = c2_instance(n)
+    Wd = W.astype(np.float64)
+    J, z = smp.init_J_and_z_dense(Wd)
+    cum = smp.maxcut_hamming_cdf(n, J, z, 6, 8.0, 8.0)
+    shots = ((((n << 6) >> 1) + 31) >> 5) << 5
+    dg = _lib.DeviceGraph.dense(W)
+    dg.sample_round(cum, 5.0, 1, 0, shots, 1)
+    reps = 3
+    device_sync()
+    t = time.perf_counter()
+    for r in range(reps):
+        _, bv, _ = dg.sample_round(cum, 5.0, 2 + r, 0, shots, 1)
+    device_sync()
+    dt = (time.perf_counter() - t) / reps
+    dg.close()
+    log("# sampler round done")
+    res = {
+        "workload": f"maxcut_tfim sampling round, random 3-regular n={n}, +-1 int8, {shots} shots, repulsion base 5",
+        "round_ms": dt * 1e3,
+        "shots_per_s": shots / dt,
+        "best_energy": bv,
+    }
+    if with_cpu:
+        from oracle.sampler_ref import RepulsionSampler
+
+        cs = 512
+        rs = RepulsionSampler(Wd, cum, 5.0, np.random.default_rng(1))
+        t = time.perf_counter()
+        rs.sample(cs)
+        ct = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": cs / ct, "unit": "shots/s", "cores": 1, "kind": "port",
+                               "sample": f"numpy restatement of the reference sampler, {cs} shots, same prior"}
+        res["gpu_over_cpu"] = res["shots_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
+def bnb_exact(log, with_cpu):
+    """Branch and bound (solve_maxcut_bnb's loop) certifying MAXCUT on random
+    +-1 graphs (ER p = 0.5) from the all-zeros incumbent, batch_cap 256: one
+    device evaluation per batch of popped nodes.  CPU baseline: the numpy
+    restatement of the reference loop on the n = 20 instance (same counts)."""
+    from pyqrackising_amd import _lib
+
+    def inst(n):
+        rng = np.random.default_rng(n)
+        U = np.triu(rng.choice([-1.0, 1.0], (n, n)) * (rng.random((n, n)) < 0.5), 1)
+        return U + U.T
+
+    res = {"workload": "MAXCUT branch and bound, ER(n, 0.5) +-1 weights, zero incumbent, batch_cap 256"}
+    for n in (20, 24):
+        W = inst(n)
+        dg = _lib.DeviceGraph.dense(W)
+        x0 = np.zeros(n, np.uint8)
+        dg.bnb(x0, 0.0, False, 256)
+        t = time.perf_counter()
+        _, v, st = dg.bnb(x0, 0.0, False, 256)
+        dt = time.perf_counter() - t
+        dg.close()
+        res[f"n{n}"] = {"seconds": dt, "optimum": v, "certified": bool(st["certified"]),
+                        "nodes_explored": st["nodes_explored"], "nodes_per_s": st["nodes_explored"] / dt,
+                        "batches": st["batches"], "kernel_seconds": st["kernel_seconds"]}
+    log("# branch and bound done")
+    if with_cpu:
+        from oracle import bnb_ref
+
+        W = inst(20)
+        t = time.perf_counter()
+        _, v, cert, ex, _ = bnb_ref.branch_and_bound(W, np.zeros(20, bool), 0.0, False, 256)
+        ct = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": ex / ct, "unit": "nodes/s", "cores": 1, "kind": "port",
+                               "sample": "numpy restatement of the reference loop, the n=20 instance",
+                               "same_result": bool(v == res["n20"]["optimum"] and ex == res["n20"]["nodes_explored"])}
+        res["gpu_over_cpu_n20"] = res["n20"]["nodes_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,8 +423,10 @@ def main():
     }
     if not args.no_extra:
         log = lambda m: print(m, file=sys.stderr, flush=True)
-        for key, fn in (("time_to_equal_cut", time_to_equal_cut), ("batched_eval", batched_eval),
-                        ("sk_bf16", sk_bf16)):
+        legs = (("time_to_equal_cut", time_to_equal_cut), ("batched_eval", batched_eval), ("sk_bf16", sk_bf16),
+                ("sampler", lambda lg: sampler_round(lg, not args.no_cpu)),
+                ("branch_and_bound", lambda lg: bnb_exact(lg, not args.no_cpu)))
+        for key, fn in legs:
             try:
                 out[key] = fn(log)
             except Exception as e:  # a failing leg is reported, not fatal to the headline

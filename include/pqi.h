@@ -272,6 +272,25 @@ PQI_API int pqi_learned_search(pqi_ctx* ctx, pqi_mpnn* net, pqi_graph* const* gr
 /* Exact re-score of one state: cut and E in FP64 (int64 for int8 weights). */
 PQI_API int pqi_score(pqi_graph* g, const uint8_t* theta, double* cut, double* energy);
 
+/* Belief-propagation marginals.  Replaces belief_propagation_marginals_dense
+ * and belief_propagation_marginals (pyqrackising/spin_glass_solver.py:311-415;
+ * spin_glass_solver_sparse.py:378-503): damped Jacobi sweeps of
+ * m(a->b) <- d m(a->b) + (1-d) atanh(clip(tanh(w_ab) tanh(h), +-(1-1e-9))), h the sum
+ * of the messages into a from its other neighbours,
+ * over the arcs of the graph (dense: |w| > 1e-12; CSR: the nonzero entries; a
+ * zero entry of the reference only ever carries zero messages from a zero start),
+ * at most max_iterations sweeps (the reference's max(1, int(bp_scale n))),
+ * stopping early at a fixed point (a sweep that changes no message, after
+ * which every sweep repeats it).  marginals (n) = tanh(sum of the messages
+ * into each node).  init (optional, init_count == number of arcs) sets the
+ * starting messages in arc order (rows ascending, neighbours ascending; arc
+ * (a, b) holds the message from b to a); NULL starts from zero as the
+ * reference does, which is a fixed point with all marginals zero.
+ * *iterations_run and *arc_count (both optional) report the sweeps run and
+ * the number of arcs. */
+PQI_API int pqi_bp_marginals(pqi_graph* g, int64_t max_iterations, double damping, const double* init,
+                             int64_t init_count, double* marginals, int64_t* iterations_run, int64_t* arc_count);
+
 typedef struct pqi_bnb_stats {
     int64_t nodes_explored; /* nodes popped */
     int64_t nodes_pruned;   /* nodes with ub <= incumbent + 1e-9 */

@@ -157,6 +157,7 @@ SIGNATURES = {
     "pqi_mpnn_loss_grad": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "pqi_mpnn_train_step": (_int, [_vp, _vp, _vp, ctypes.c_float, _vp]),
     "pqi_learned_search": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, ctypes.c_float, _vp]),
+    "pqi_bp_marginals": (_int, [_vp, _i64, _dbl, _vp, _i64, _vp, _vp, _vp]),
     "pqi_bnb_eval": (_int, [_vp, _vp, _i64, _int, _vp, _vp, _vp, _vp]),
     "pqi_bnb": (_int, [_vp, _vp, _vp, _int, _int, _i64, _dbl, _vp]),
 }
@@ -429,6 +430,20 @@ class DeviceGraph:
                                           int(seed) & (2 ** 64 - 1), int(shot_offset), int(shots),
                                           int(bool(is_spin_glass)), ctypes.byref(bi), ctypes.byref(bv), _p(bits)))
         return bi.value, bv.value, bits.astype(bool)
+
+    def bp_marginals(self, max_iterations, damping, init=None):
+        """Belief-propagation marginals (pqi_bp_marginals): (marginals,
+        sweeps run, number of arcs)."""
+        marg = np.empty(self.n, dtype=np.float64)
+        it, arcs = ctypes.c_int64(), ctypes.c_int64()
+        ip, cnt = None, 0
+        if init is not None:
+            init = np.ascontiguousarray(init, dtype=np.float64)
+            ip, cnt = _p(init), init.size
+        with self.ctx.lock:
+            check(load().pqi_bp_marginals(self._h, int(max_iterations), float(damping), ip, cnt, _p(marg),
+                                          ctypes.byref(it), ctypes.byref(arcs)))
+        return marg, it.value, arcs.value
 
     def bnb_eval(self, nodes, is_spin_glass, return_time=False):
         """Bound, leaf value and branch variable of count x n nodes (pqi_bnb_eval)."""

@@ -37,7 +37,7 @@ def init_J_and_z_sparse(G_m):
     val = data - G_min
     m = val > EPSILON
     degrees = np.bincount(rows[m], minlength=n).astype(np.int64)
-    J = np.bincount(rows[m], weights=val[m], minlength=n)
+    J = np.bincount(rows[m], weights=val[m], minlength=n).astype(np.float64)
     zval = 0.0 - G_min
     if zval > EPSILON:
         implicit = n - nnz_row

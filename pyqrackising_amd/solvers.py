@@ -183,10 +183,8 @@ def _guess_bits(best_guess, n):
 
 
 def _solve(G, sparse, quality, shots, best_guess, is_spin_glass, anneal_t, anneal_h, repulsion_base,
-           gray_iterations, gray_seed_multiple, bp_scale, seed, gray_walkers, double_flip_mode, seed_engine,
-           max_rounds, return_stats):
-    if bp_scale is not None:
-        raise NotImplementedError("belief-propagation warm start (bp_scale) is not part of this build")
+           gray_iterations, gray_seed_multiple, bp_scale, bp_damping, seed, gray_walkers, double_flip_mode,
+           seed_engine, max_rounds, return_stats):
     if sparse:
         nodes, G_m = _ingest_sparse(G)
         n = G_m.shape[0]
@@ -213,10 +211,22 @@ def _solve(G, sparse, quality, shots, best_guess, is_spin_glass, anneal_t, annea
     bitstring = _guess_bits(best_guess, n)
     cut_value = None
     if bitstring is None:
+        use_bp = bp_scale is not None and n >= threshold
+        if use_bp:
+            marginals, _, _ = dg.bp_marginals(max(1, int(bp_scale * n)), bp_damping)
+            bp_theta = marginals > 0.0
         fn = maxcut_tfim_sparse if sparse else maxcut_tfim
         bitstring, cut_value, _ = fn(G_m, quality=quality, shots=shots, is_spin_glass=is_spin_glass,
                                      anneal_t=anneal_t, anneal_h=anneal_h, repulsion_base=repulsion_base,
                                      is_nested=True, seed=seed, _dg=dg)
+        if use_bp:
+            # spin_glass_solver.py:496-523: keep the BP partition if it scores higher
+            k = 1 if is_spin_glass else 0
+            bp_energy = dg.score(bp_theta)[k]
+            sample_energy = dg.score(np.array([b == "1" for b in bitstring], dtype=bool))[k]
+            if bp_energy > sample_energy:
+                bitstring = "".join("1" if b else "0" for b in bp_theta)
+                cut_value = bp_energy if not is_spin_glass else None
     best_theta = np.array([b == "1" for b in bitstring], dtype=bool)
     if best_theta.size != n:
         raise ValueError(f"best_guess has {best_theta.size} bits, graph has {n} nodes")
@@ -249,8 +259,8 @@ def spin_glass_solver(G, quality=None, shots=None, best_guess=None, is_maxcut_gp
     """pyqrackising/spin_glass_solver.py:424-727.  Returns
     (bitstring, cut_value, (l, r), min_energy)."""
     return _solve(G, False, quality, shots, best_guess, is_spin_glass, anneal_t, anneal_h, repulsion_base,
-                  gray_iterations, gray_seed_multiple, bp_scale, seed, gray_walkers, double_flip_mode, seed_engine,
-                  max_rounds, return_stats)
+                  gray_iterations, gray_seed_multiple, bp_scale, bp_damping, seed, gray_walkers, double_flip_mode,
+                  seed_engine, max_rounds, return_stats)
 
 
 def spin_glass_solver_sparse(G, quality=None, shots=None, best_guess=None, is_maxcut_gpu=True, is_spin_glass=True,
@@ -260,5 +270,5 @@ def spin_glass_solver_sparse(G, quality=None, shots=None, best_guess=None, is_ma
     """pyqrackising/spin_glass_solver_sparse.py:517-864 (absolute-value form of
     the same search; identical moves on the symmetrised matrix)."""
     return _solve(G, True, quality, shots, best_guess, is_spin_glass, anneal_t, anneal_h, repulsion_base,
-                  gray_iterations, gray_seed_multiple, bp_scale, seed, gray_walkers, double_flip_mode, seed_engine,
-                  max_rounds, return_stats)
+                  gray_iterations, gray_seed_multiple, bp_scale, bp_damping, seed, gray_walkers, double_flip_mode,
+                  seed_engine, max_rounds, return_stats)
