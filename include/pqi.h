@@ -100,6 +100,9 @@ typedef struct {
 #define PQI_FLAG_TIME_KERNELS 1 /* time the descent and main walk kernels with HIP events on the context stream */
 #define PQI_FLAG_HOST_LOOP 2    /* decide each phase on the host (one synchronisation per phase) instead of
                                    on the device (one per loop iteration); same result */
+#define PQI_FLAG_TEST_DRIFT 4   /* test hook (real weights): perturb the maintained field by
+                                   1e-3 ((v mod 7) - 3) before the first round, which the round's exact
+                                   recompute must detect and repair */
 
 typedef struct {
     double max_energy; /* reference bookkeeping: start value plus accepted gains */
@@ -118,6 +121,10 @@ typedef struct {
     /* learned phase (pqi_learned_search) */
     int64_t learned_accepts, learned_passes;
     double seconds_learned;
+    /* real weights: the fp64 field maintained by flips is recomputed exactly from W at the start of
+     * every round and replaced; entries that had moved by more than 1e-9 max(1, |h|) are counted */
+    int64_t field_checks, field_repairs;
+    double max_field_drift;
 } pqi_search_stats;
 
 /* Thread-local message for the last failing call on this thread. */

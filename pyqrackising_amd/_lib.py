@@ -28,6 +28,7 @@ DOUBLE_REFERENCE, DOUBLE_EXHAUSTIVE = 0, 1
 SEEDS_AUTO, SEEDS_MFMA, SEEDS_LOWRANK = 0, 1, 2
 FLAG_TIME_KERNELS = 1
 FLAG_HOST_LOOP = 2
+FLAG_TEST_DRIFT = 4
 ENGINE_AUTO, ENGINE_EXACT, ENGINE_BF16 = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -95,6 +96,9 @@ class SearchStats(ctypes.Structure):
         ("learned_accepts", ctypes.c_int64),
         ("learned_passes", ctypes.c_int64),
         ("seconds_learned", ctypes.c_double),
+        ("field_checks", ctypes.c_int64),
+        ("field_repairs", ctypes.c_int64),
+        ("max_field_drift", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -302,6 +302,13 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             if (active.empty()) break;
         }
         for (int32_t i : active) stats[i].rounds++;
+        if (kind != PQI_WEIGHTS_I8) {
+            // a new round: the fp64 fields are recomputed exactly (as the
+            // single-instance search does), so drift never carries over
+            BCHK(upload_list(active));
+            launch_b_field(kind, d_inst, d_list, (int64_t)active.size(), max_n, s);
+            for (int32_t i : active) stats[i].field_checks++;
+        }
 
         // single-flip descent, bounded launches repeated while the bound is hit
         double t0 = now_s();

@@ -80,6 +80,20 @@ int fail(int code, const std::string& msg);
 
 // h = W s for the state x (uint8 per node).  FT = int32 for int8 weights, double otherwise.
 void launch_field(const Graph& g, const uint8_t* x, void* h, hipStream_t s);
+// Drift check of an incrementally maintained fp64 field: h is recomputed
+// exactly from W and x and overwritten; entries that had moved by more than
+// 1e-9 max(1, |h_exact|) are counted.  Skipped when gate is non-null and
+// gate->again is set (a descent continuing in the same round).
+struct FieldRepair {
+    unsigned long long checks;    // recomputes run
+    unsigned long long entries;   // entries beyond the tolerance
+    unsigned long long max_bits;  // largest |h - h_exact| (bits of a non-negative double)
+};
+struct LsCtl;
+void launch_field_repair(const Graph& g, const uint8_t* x, void* h, const LsCtl* gate, FieldRepair* fr,
+                         hipStream_t s);
+// Test hook: adds 1e-3 ((v mod 7) - 3) to every fp64 field entry.
+void launch_perturb_field(const Graph& g, void* h, hipStream_t s);
 
 // Exact score of the states x[b] (uint8, row stride n): out_e[b] = E(x_b) = -(1/2) s^T W s,
 // out_cut[b] = cut(x_b); FP64 (int64 for int8 weights).  hbuf holds one field.

@@ -43,7 +43,7 @@ int fail(int code, const std::string& msg) {
 
 // Scratch slots of a context.
 enum Slot {
-    S_X, S_H, S_RX, S_RH, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
+    S_X, S_H, S_RX, S_RH, S_FIX, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
     S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_COUNT
 };
 
@@ -884,6 +884,18 @@ int pqi_gray_search(pqi_graph* g, const uint8_t* theta, const pqi_search_params*
     return sync(c);
 }
 
+static int read_field_repair(pqi_ctx* c, const FieldRepair* fix, pqi_search_stats* st) {
+    FieldRepair f;
+    HIPCHK(hipMemcpyAsync(&f, fix, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    st->field_checks = (int64_t)f.checks;
+    st->field_repairs = (int64_t)f.entries;
+    double d;
+    std::memcpy(&d, &f.max_bits, sizeof(d));
+    st->max_field_drift = d;
+    return PQI_OK;
+}
+
 // Host-driven loop: one synchronisation per phase decision (PQI_FLAG_HOST_LOOP).
 static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st);
 
@@ -905,6 +917,10 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     PhaseOut *dout, *dout2;
     void* dsc;
     LsCtl* ctl;
+    FieldRepair* fix;
+    CHK(ensure(c, S_FIX, sizeof(FieldRepair), (void**)&fix));
+    HIPCHK(hipMemsetAsync(fix, 0, sizeof(FieldRepair), c->stream));
+    if (p->flags & PQI_FLAG_TEST_DRIFT) launch_perturb_field(*g, h, c->stream);
     CHK(ensure(c, S_RX, n, (void**)&rx));
     CHK(ensure(c, S_RH, n * fb, &rh));
     CHK(ensure(c, S_OUT, sizeof(PhaseOut), (void**)&dout));
@@ -999,6 +1015,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     };
     for (;;) {
         if (!hc.again && p->max_rounds > 0 && hc.rounds >= p->max_rounds) break;
+        launch_field_repair(*g, x, h, ctl, fix, c->stream);  // a new round: exact fp64 field
         launch_ls_begin(ctl, c->stream);
         launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event());
         CHK(run_segment(0));
@@ -1040,6 +1057,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     st->seconds_seeds = hc.ticks[2] * tick;
     st->seconds_walk = hc.ticks[3] * tick;
     st->seconds_reheat = hc.ticks[4] * tick;
+    CHK(read_field_repair(c, fix, st));
     double cut, en;
     CHK(score_device(g, x, &cut, &en));
     st->cut_value = cut;
@@ -1070,6 +1088,10 @@ static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_para
     uint8_t* x;
     void* h;
     CHK(load_state(g, theta, &x, &h));
+    FieldRepair* fix;
+    CHK(ensure(c, S_FIX, sizeof(FieldRepair), (void**)&fix));
+    HIPCHK(hipMemsetAsync(fix, 0, sizeof(FieldRepair), c->stream));
+    if (p->flags & PQI_FLAG_TEST_DRIFT) launch_perturb_field(*g, h, c->stream);
     uint8_t* rx;
     void* rh;
     PhaseOut* dout;
@@ -1091,6 +1113,7 @@ static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_para
     for (;;) {
         if (p->max_rounds > 0 && st->rounds >= p->max_rounds) break;
         st->rounds++;
+        launch_field_repair(*g, x, h, nullptr, fix, c->stream);  // a new round: exact fp64 field
 
         double t0 = now();
         // bounded launches keep every kernel short; repeat while the bound was hit
@@ -1203,6 +1226,7 @@ static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_para
         break;
     }
     double cut, en;
+    CHK(read_field_repair(c, fix, st));
     CHK(score_device(g, x, &cut, &en));
     st->cut_value = cut;
     st->min_energy = p->is_spin_glass ? -en : en;

@@ -222,6 +222,51 @@ __global__ __launch_bounds__(256) void k_field_adj(GView<WT> g, const uint8_t* x
     h[row] = acc;
 }
 
+// the recompute of node `row` replaces h[row]; with a FieldRepair the old
+// value is compared first
+template <typename FT>
+__device__ __forceinline__ void field_store(FT* h, int64_t row, FT exact, FieldRepair* fr) {
+    if (fr) {
+        const double d = fabs((double)h[row] - (double)exact);
+        if (d > 1e-9 * fmax(1.0, fabs((double)exact))) atomicAdd(&fr->entries, 1ull);
+        atomicMax(&fr->max_bits, (unsigned long long)__double_as_longlong(d));
+    }
+    h[row] = exact;
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void k_field_repair_dense(GView<WT> g, const uint8_t* x, FT* h, const LsCtl* gate,
+                                                            FieldRepair* fr) {
+    if (gate && gate->again) return;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&fr->checks, 1ull);
+    if (row >= g.n) return;
+    const WT* Wr = g.W + row * g.ld;
+    FT acc = 0;
+    for (int64_t u = lane; u < g.n; u += 64) acc += x[u] ? -(FT)Wr[u] : (FT)Wr[u];
+    acc = wave_sum(acc);
+    if (lane == 0) field_store(h, row, acc, fr);
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(256) void k_field_repair_adj(GView<WT> g, const uint8_t* x, FT* h, const LsCtl* gate,
+                                                          FieldRepair* fr) {
+    if (gate && gate->again) return;
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&fr->checks, 1ull);
+    if (row >= g.n) return;
+    FT acc = 0;
+    for (int32_t e = g.ap[row]; e < g.ap[row + 1]; ++e) acc += x[g.ai[e]] ? -(FT)g.aw[e] : (FT)g.aw[e];
+    field_store(h, row, acc, fr);
+}
+
+template <typename FT>
+__global__ void k_perturb_field(FT* h, int64_t n) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) h[v] += (FT)(1e-3 * (double)((v % 7) - 3));
+}
+
 template <typename WT>
 static void field_impl(const Graph& g, const uint8_t* x, void* h, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
@@ -264,6 +309,32 @@ static size_t lds_limit(const void* fn) {
 
 void launch_field(const Graph& g, const uint8_t* x, void* h, hipStream_t s) {
     PQI_DISPATCH(g, field_impl, g, x, h, s);
+}
+
+template <typename WT>
+static void field_repair_impl(const Graph& g, const uint8_t* x, void* h, const LsCtl* gate, FieldRepair* fr,
+                              hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    GView<WT> v = view<WT>(g);
+    if (g.W && !g.use_adj) {
+        hipLaunchKernelGGL((k_field_repair_dense<WT, FT>), dim3((unsigned)((g.n + 3) / 4)), dim3(256), 0, s, v, x,
+                           (FT*)h, gate, fr);
+    } else {
+        hipLaunchKernelGGL((k_field_repair_adj<WT, FT>), dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, v, x,
+                           (FT*)h, gate, fr);
+    }
+}
+
+void launch_field_repair(const Graph& g, const uint8_t* x, void* h, const LsCtl* gate, FieldRepair* fr,
+                         hipStream_t s) {
+    if (g.kind == PQI_WEIGHTS_I8) return;  // int32 fields are exact
+    PQI_DISPATCH(g, field_repair_impl, g, x, h, gate, fr, s);
+}
+
+void launch_perturb_field(const Graph& g, void* h, hipStream_t s) {
+    if (g.kind == PQI_WEIGHTS_I8) return;
+    hipLaunchKernelGGL(k_perturb_field<double>, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, (double*)h,
+                       g.n);
 }
 
 size_t field_bytes(const Graph& g) { return g.kind == PQI_WEIGHTS_I8 ? 4 : 8; }
