@@ -201,3 +201,36 @@ def test_trained_network_gain():
     untrained = sum(s["max_energy"] >= t for s, t in zip(un, targets))
     log(f"  reached the plain energy: trained {trained}/32, untrained {untrained}/32")
     assert trained >= untrained
+
+
+def test_adversarial_network_reaches_the_plain_result():
+    """Adversarial weights: the score is 10 relu(-f0), f0 = gain / strength, so
+    only spins whose flip LOSES are proposed (min_score 1), worst first.  The
+    exact gain check rejects every one of them, and the reference loop that
+    follows as fallback and judge ends exactly where the plain batched search
+    from the same starts ends (same states, same counters)."""
+    log("adversarial network")
+    rng = np.random.default_rng(83)
+    Gs, x0s = [], []
+    for t in range(6):
+        G, W = _er_int(int(rng.integers(100, 400)), 6.0, 500 + t)
+        Gs.append(G)
+        x0s.append(rng.integers(0, 2, W.shape[0]).astype(np.uint8))
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    net = _lib.Mpnn(seed=1)
+    q = unpack(np.zeros(net.n_params, np.float32))
+    q["T1"][0, 0] = -1.0  # h0 = relu(-f0)
+    q["T2"][0, 0] = 1.0
+    q["T3"][0, 0] = 1.0
+    q["TO"][0] = 10.0
+    net.set_params(np.concatenate([q[k].ravel() for k in ("T1", "B1", "T2", "B2", "T3", "B3", "TO", "BO")]))
+    net.bind(dgs)
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=600)
+    states, stats = _lib.learned_search(net, dgs, x0s, p, passes=4, min_score=1.0)
+    plain_states, plain = _lib.batch_local_search(dgs, x0s, p)
+    for i in range(len(dgs)):
+        assert stats[i]["learned_passes"] >= 1 and stats[i]["learned_accepts"] == 0, i
+        assert np.array_equal(states[i], plain_states[i]), i
+        for k in ("rounds", "single_accepts", "double_accepts", "seed_accepts", "walk_accepts", "max_energy",
+                  "min_energy", "cut_value"):
+            assert stats[i][k] == plain[i][k], (i, k)
