@@ -480,7 +480,7 @@ class Batch:
     """Candidate scoring for many dense int8 instances in one persistent,
     graph-captured launch (pqi_batch_*)."""
 
-    N_VARIANTS = 8
+    N_VARIANTS = 10
 
     def __init__(self, graphs, cands):
         self.graphs = list(graphs)

@@ -55,15 +55,15 @@ __global__ __launch_bounds__(256) void k_batch_grouped(const BatchInst* __restri
 // p % 8, and the candidate tiles of one (instance, column block) sit at
 // positions of one residue, so they share that block of W through one L2.
 // Empty positions (inst < 0) pad the per-XCD queues to one length.
-template <int BK>
-__global__ __launch_bounds__(256) void k_batch_wide(const BatchInst* __restrict__ insts,
-                                                    const BatchTile* __restrict__ tiles, int64_t ntiles) {
+template <int BK, int NW>
+__global__ __launch_bounds__(64 * NW) void k_batch_wide(const BatchInst* __restrict__ insts,
+                                                        const BatchTile* __restrict__ tiles, int64_t ntiles) {
     extern __shared__ __align__(16) unsigned char lds[];
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const BatchTile tl = tiles[t];
         if (tl.inst < 0) continue;
         const BatchInst in = insts[tl.inst];
-        tile_i8_wide<BK>(in.W, in.n_pad, 0, in.n_pad, in.bits, in.wpr, in.B, (int64_t)tl.mb * WIDE_BM,
+        tile_i8_wide<BK, NW>(in.W, in.n_pad, 0, in.n_pad, in.bits, in.wpr, in.B, (int64_t)tl.mb * WIDE_BM,
                          (int64_t)tl.jb * WIDE_BN, in.q, lds);
     }
 }
@@ -108,12 +108,12 @@ __global__ __launch_bounds__(256) void k_seg_best(const BatchInst* __restrict__ 
 }
 
 // tile variants: 0-5 the 128 x 128 tile, contraction depth x persistent grid
-// size (multiples of the CU count); 6-7 the wide tile (one work group per CU)
-// with depth 128 or 64
-static const int kVariantBK[] = {64, 64, 64, 128, 128, 128, 128, 64};
-static const int kVariantGrid[] = {1, 2, 4, 1, 2, 4, 1, 1};
-static const int kVariantWide[] = {0, 0, 0, 0, 0, 0, 1, 1};
-static const int kVariants = 8;
+// size (multiples of the CU count); 6-7 the wide tile on 4 waves (one work
+// group per CU) with depth 128 or 64; 8-9 the wide tile on 8 waves
+static const int kVariantBK[] = {64, 64, 64, 128, 128, 128, 128, 64, 128, 64};
+static const int kVariantGrid[] = {1, 2, 4, 1, 2, 4, 1, 1, 1, 1};
+static const int kVariantWide[] = {0, 0, 0, 0, 0, 0, 4, 4, 8, 8};
+static const int kVariants = 10;
 
 }  // namespace pqi
 
@@ -155,16 +155,28 @@ static void enqueue_eval(pqi_batch* b, int variant, hipStream_t s) {
     if (kVariantWide[variant] && !b->d_wtiles) variant = 3;  // wide tiles not usable: the nearest plain variant
     const int grid = std::max(1, b->ctx->num_cus) * kVariantGrid[variant];
     if (kVariantWide[variant]) {
-        static const bool lds_ok =
-            hipFuncSetAttribute((const void*)k_batch_wide<128>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)wide_lds_bytes<128>()) == hipSuccess;
-        if (!lds_ok) variant = 7;  // the 64-deep wide tile fits the default LDS limit
+        static const bool lds_ok = [] {
+            bool ok = true;
+            ok &= hipFuncSetAttribute((const void*)k_batch_wide<128, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)wide_lds_bytes<128>()) == hipSuccess;
+            ok &= hipFuncSetAttribute((const void*)k_batch_wide<128, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)wide_lds_bytes<128>()) == hipSuccess;
+            return ok;
+        }();
+        const int nw = kVariantWide[variant];
+        const int bk = lds_ok ? kVariantBK[variant] : 64;  // the 64-deep wide tile fits the default LDS limit
         const unsigned g = (unsigned)std::min<int64_t>(grid, std::max<int64_t>(1, b->nwtiles));
-        if (kVariantBK[variant] == 128)
-            hipLaunchKernelGGL(k_batch_wide<128>, dim3(g), dim3(256), wide_lds_bytes<128>(), s, b->d_insts,
+        if (bk == 128 && nw == 8)
+            hipLaunchKernelGGL((k_batch_wide<128, 8>), dim3(g), dim3(512), wide_lds_bytes<128>(), s, b->d_insts,
+                               b->d_wtiles, b->nwtiles);
+        else if (bk == 128)
+            hipLaunchKernelGGL((k_batch_wide<128, 4>), dim3(g), dim3(256), wide_lds_bytes<128>(), s, b->d_insts,
+                               b->d_wtiles, b->nwtiles);
+        else if (nw == 8)
+            hipLaunchKernelGGL((k_batch_wide<64, 8>), dim3(g), dim3(512), wide_lds_bytes<64>(), s, b->d_insts,
                                b->d_wtiles, b->nwtiles);
         else
-            hipLaunchKernelGGL(k_batch_wide<64>, dim3(g), dim3(256), wide_lds_bytes<64>(), s, b->d_insts,
+            hipLaunchKernelGGL((k_batch_wide<64, 4>), dim3(g), dim3(256), wide_lds_bytes<64>(), s, b->d_insts,
                                b->d_wtiles, b->nwtiles);
         hipLaunchKernelGGL(k_seg_best, dim3((unsigned)b->count), dim3(256), 0, s, b->d_insts, b->d_best, b->d_bestq);
         return;
@@ -316,9 +328,11 @@ int pqi_batch_set_candidates(pqi_batch* b, const uint32_t* bits) {
     for (int64_t i = 0; i < b->count; ++i) {
         pqi_graph* g = b->graphs[i];
         const int64_t nw = (g->n + 31) >> 5, wpr = g->n_pad >> 5;
-        if (hipMemcpy2DAsync(b->d_bits + b->bits_off[i], wpr * 4, src, nw * 4, nw * 4, b->cands,
-                             hipMemcpyHostToDevice, b->ctx->stream) != hipSuccess)
-            return fail(PQI_ERR_HIP, "uploading candidates failed");
+        const int r = upload_rows(b->ctx, b->d_bits + b->bits_off[i], (size_t)wpr * 4, src, (size_t)nw * 4,
+                                  (size_t)b->cands);
+        if (r != PQI_OK) return r;
+        // the staging buffer is reused by the next instance
+        if (hipStreamSynchronize(b->ctx->stream) != hipSuccess) return fail(PQI_ERR_HIP, "upload sync failed");
         src += b->cands * nw;
     }
     if (hipStreamSynchronize(b->ctx->stream) != hipSuccess) return fail(PQI_ERR_HIP, "upload sync failed");

@@ -43,8 +43,8 @@ __global__ __launch_bounds__(256) void k_batch_i8(const int8_t* __restrict__ W, 
 // The grid is rounded up to whole groups of 8 column blocks; the extra blocks
 // exit at once.
 // grid: 8-column-block groups x MB candidate blocks x KS splits of K
-template <int BK>
-__global__ __launch_bounds__(256) void k_batch_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
+template <int BK, int NW>
+__global__ __launch_bounds__(64 * NW) void k_batch_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
                                                        const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
                                                        unsigned long long* __restrict__ q, int64_t MB, int64_t JB,
                                                        int64_t KS) {
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_batch_i8_wide(const int8_t* __restrict_
     const int64_t jb = (t / MB) * 8 + x, mb = t % MB;
     if (jb >= JB) return;
     const int64_t kl = n_pad / KS;
-    tile_i8_wide<BK>(W, ld, ks * kl, (ks + 1) * kl, bits, wpr, B, mb * WIDE_BM, jb * WIDE_BN, q, lds);
+    tile_i8_wide<BK, NW>(W, ld, ks * kl, (ks + 1) * kl, bits, wpr, B, mb * WIDE_BM, jb * WIDE_BN, q, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -361,6 +361,15 @@ size_t batch_scratch_bytes(const Graph& g, int64_t B) {
     return (size_t)B * (size_t)nblk * 8;
 }
 
+// Waves per wide tile (4 or 8), from PQI_WIDE_WAVES; default 4.
+static int wide_waves() {
+    static const int v = [] {
+        const char* e = getenv("PQI_WIDE_WAVES");
+        return e && atoi(e) == 8 ? 8 : 4;
+    }();
+    return v;
+}
+
 void launch_batch_i8_raw(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, unsigned long long* qi,
                          hipStream_t s) {
     if (B >= 512 && g.max_strength >= 0 && 256 * g.max_strength < (int64_t(1) << 31)) {
@@ -369,14 +378,24 @@ void launch_batch_i8_raw(const Graph& g, const uint32_t* bits, int64_t wpr, int6
         int64_t KS = 1;
         const int64_t tiles = ((JB + 7) / 8) * 8 * MB, cus = g.ctx ? std::max(1, g.ctx->num_cus) : 256;
         while (tiles * KS * 2 <= cus && g.n_pad % (128 * KS * 2) == 0 && g.n_pad / (KS * 2) >= 512) KS *= 2;
-        static const bool big = hipFuncSetAttribute((const void*)k_batch_i8_wide<128>,
-                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                    (int)wide_lds_bytes<128>()) == hipSuccess;
-        if (big)
-            hipLaunchKernelGGL(k_batch_i8_wide<128>, dim3((unsigned)(tiles * KS)), dim3(256), wide_lds_bytes<128>(), s,
+        static const bool big =
+            hipFuncSetAttribute((const void*)k_batch_i8_wide<128, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)wide_lds_bytes<128>()) == hipSuccess &&
+            hipFuncSetAttribute((const void*)k_batch_i8_wide<128, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)wide_lds_bytes<128>()) == hipSuccess;
+        const int nw = wide_waves();
+        const dim3 grid((unsigned)(tiles * KS));
+        if (big && nw == 8)
+            hipLaunchKernelGGL((k_batch_i8_wide<128, 8>), grid, dim3(512), wide_lds_bytes<128>(), s,
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+        else if (big)
+            hipLaunchKernelGGL((k_batch_i8_wide<128, 4>), grid, dim3(256), wide_lds_bytes<128>(), s,
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+        else if (nw == 8)
+            hipLaunchKernelGGL((k_batch_i8_wide<64, 8>), grid, dim3(512), wide_lds_bytes<64>(), s,
                                (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
         else
-            hipLaunchKernelGGL(k_batch_i8_wide<64>, dim3((unsigned)(tiles * KS)), dim3(256), wide_lds_bytes<64>(), s,
+            hipLaunchKernelGGL((k_batch_i8_wide<64, 4>), grid, dim3(256), wide_lds_bytes<64>(), s,
                                (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
     } else {
         dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));

@@ -80,6 +80,12 @@ int fail(int code, const std::string& msg);
 
 // h = W s for the state x (uint8 per node).  FT = int32 for int8 weights, double otherwise.
 void launch_field(const Graph& g, const uint8_t* x, void* h, hipStream_t s);
+// Host rows of `width` bytes (contiguous) -> device rows at pitch dpitch,
+// the bytes past width zeroed: one host copy into the context's staging
+// buffer, then a restride kernel on the device (a pitched copy from pageable
+// host memory would move one row per transfer).  Enqueued on the context
+// stream; the staging buffer is reused by the next call.
+int upload_rows(pqi_ctx* c, void* dst, size_t dpitch, const void* src, size_t width, size_t rows);
 // Drift check of an incrementally maintained fp64 field: h is recomputed
 // exactly from W and x and overwritten; entries that had moved by more than
 // 1e-9 max(1, |h_exact|) are counted.  Skipped when gate is non-null and

@@ -44,7 +44,7 @@ int fail(int code, const std::string& msg) {
 // Scratch slots of a context.
 enum Slot {
     S_X, S_H, S_RX, S_RH, S_FIX, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
-    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_COUNT
+    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_STAGE, S_COUNT
 };
 
 static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
@@ -60,6 +60,46 @@ static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
         b.bytes = want;
     }
     *out = b.p;
+    return PQI_OK;
+}
+
+// dst row r = src row r (width bytes) followed by zeros up to dpitch
+__global__ void k_restride_words(uint32_t* __restrict__ dst, int64_t dwords, const uint32_t* __restrict__ src,
+                                 int64_t swords, int64_t rows) {
+    const int64_t total = dwords * rows;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / dwords, w = t - r * dwords;
+        dst[t] = w < swords ? src[r * swords + w] : 0u;
+    }
+}
+
+__global__ void k_restride_bytes(uint8_t* __restrict__ dst, int64_t dpitch, const uint8_t* __restrict__ src,
+                                 int64_t width, int64_t rows) {
+    const int64_t total = dpitch * rows;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / dpitch, b = t - r * dpitch;
+        dst[t] = b < width ? src[r * width + b] : (uint8_t)0;
+    }
+}
+
+int upload_rows(pqi_ctx* c, void* dst, size_t dpitch, const void* src, size_t width, size_t rows) {
+    if (rows == 0) return PQI_OK;
+    if (dpitch == width) {
+        HIPCHK(hipMemcpyAsync(dst, src, width * rows, hipMemcpyHostToDevice, c->stream));
+        return PQI_OK;
+    }
+    void* stage;
+    CHK(ensure(c, S_STAGE, width * rows, &stage));
+    HIPCHK(hipMemcpyAsync(stage, src, width * rows, hipMemcpyHostToDevice, c->stream));
+    const int64_t total = (int64_t)(dpitch * rows);
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+    if (dpitch % 4 == 0 && width % 4 == 0 && ((uintptr_t)dst & 3) == 0)
+        hipLaunchKernelGGL(k_restride_words, dim3(blocks), dim3(256), 0, c->stream, (uint32_t*)dst,
+                           (int64_t)(dpitch / 4), (const uint32_t*)stage, (int64_t)(width / 4), (int64_t)rows);
+    else
+        hipLaunchKernelGGL(k_restride_bytes, dim3(blocks), dim3(256), 0, c->stream, (uint8_t*)dst, (int64_t)dpitch,
+                           (const uint8_t*)stage, (int64_t)width, (int64_t)rows);
+    HIPCHK(hipGetLastError());
     return PQI_OK;
 }
 
@@ -307,8 +347,8 @@ int pqi_graph_dense(pqi_ctx* c, const void* W, int dtype, int64_t n, pqi_graph**
         }
         src = conv.data();
     }
-    if (hipMemcpy2D(g->W, (size_t)g->n_pad * es, src, (size_t)n * es, (size_t)n * es, (size_t)n,
-                    hipMemcpyHostToDevice) != hipSuccess)
+    if (upload_rows(c, g->W, (size_t)g->n_pad * es, src, (size_t)n * es, (size_t)n) != PQI_OK ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
         return bail(PQI_ERR_HIP, "uploading W failed");
     std::vector<unsigned char>().swap(conv);
     if (g->kind == PQI_WEIGHTS_F32) {
@@ -497,9 +537,8 @@ int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin
     const int64_t wpr = g->n_pad >> 5;
     uint32_t* dbits;
     CHK(ensure(c, S_BITS, (size_t)B * wpr * 4, (void**)&dbits));
-    // padding words stay zero: padded columns of W are zero, so they add nothing
-    HIPCHK(hipMemsetAsync(dbits, 0, (size_t)B * wpr * 4, c->stream));
-    HIPCHK(hipMemcpy2DAsync(dbits, wpr * 4, bits, nw * 4, nw * 4, B, hipMemcpyHostToDevice, c->stream));
+    // padding words are zero: padded columns of W are zero, so they add nothing
+    CHK(upload_rows(c, dbits, (size_t)wpr * 4, bits, (size_t)nw * 4, (size_t)B));
     return eval_on_device(g, dbits, B, is_spin_glass, bf16, values, best_index, best_value, max_rel_err);
 }
 

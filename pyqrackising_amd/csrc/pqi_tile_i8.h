@@ -166,9 +166,10 @@ __device__ __forceinline__ v4i unpack16(uint32_t half) {
     return a;
 }
 
-template <int BK>
-__device__ __forceinline__ void wide_compute(v16i (&acc)[2][8], const uint32_t (&aw)[2][BK / 32], const int8_t* Ws,
-                                             int lc, int hi) {
+// one chunk of BK: NCB column blocks starting at cb0, both row blocks
+template <int BK, int NCB>
+__device__ __forceinline__ void wide_compute(v16i (&acc)[2][NCB], const uint32_t (&aw)[2][BK / 32], const int8_t* Ws,
+                                             int lc, int hi, int cb0) {
     constexpr int ROW = BK + 16;
 #pragma unroll
     for (int t = 0; t < BK / 32; ++t) {
@@ -176,8 +177,8 @@ __device__ __forceinline__ void wide_compute(v16i (&acc)[2][8], const uint32_t (
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) av[rb] = unpack16(hi ? (aw[rb][t] >> 16) : (aw[rb][t] & 0xFFFFu));
 #pragma unroll
-        for (int cb = 0; cb < 8; ++cb) {
-            const v4i bv = *(const v4i*)&Ws[(cb * 32 + lc) * ROW + 32 * t + 16 * hi];
+        for (int cb = 0; cb < NCB; ++cb) {
+            const v4i bv = *(const v4i*)&Ws[((cb0 + cb) * 32 + lc) * ROW + 32 * t + 16 * hi];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
                 acc[rb][cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[rb], bv, acc[rb][cb], 0, 0, 0);
@@ -194,90 +195,99 @@ __device__ __forceinline__ void wide_load_a(uint32_t (&aw)[2][BK / 32], const ui
 }
 
 // W loader, coalesced: BK/16 consecutive lanes read the BK bytes of one W row,
-// so a load instruction covers 256 / (BK/16) rows; pass t takes rows
-// t * 256/(BK/16) + tid / (BK/16).  wbase[t] = W row of pass t + piece offset.
-template <int BK>
-__device__ __forceinline__ void wide_load_w(v4i (&wv)[BK / 16], const int8_t* const (&wbase)[BK / 16], int64_t k0) {
+// so a load instruction covers NT / (BK/16) rows; pass t takes rows
+// t * NT/(BK/16) + tid / (BK/16).  wbase[t] = W row of pass t + piece offset.
+template <int NP>
+__device__ __forceinline__ void wide_load_w(v4i (&wv)[NP], const int8_t* const (&wbase)[NP], int64_t k0) {
 #pragma unroll
-    for (int t = 0; t < BK / 16; ++t) wv[t] = *(const v4i*)(wbase[t] + k0);
+    for (int t = 0; t < NP; ++t) wv[t] = *(const v4i*)(wbase[t] + k0);
 }
 
-template <int BK>
-__device__ __forceinline__ void wide_store_w(const v4i (&wv)[BK / 16], int8_t* stage, const int (&woff)[BK / 16]) {
+template <int NP>
+__device__ __forceinline__ void wide_store_w(const v4i (&wv)[NP], int8_t* stage, const int (&woff)[NP]) {
 #pragma unroll
-    for (int t = 0; t < BK / 16; ++t) *(v4i*)(stage + woff[t]) = wv[t];
+    for (int t = 0; t < NP; ++t) *(v4i*)(stage + woff[t]) = wv[t];
 }
 
 // Contraction over k in [kb, ke) (a split of K = ld; multiples of BK).
-template <int BK>
+// NW = 4: wave w owns candidate rows [64w, 64w+64) and all 8 column blocks
+// (256 accumulator registers per lane, one wave per SIMD).  NW = 8: wave w
+// owns rows [64(w mod 4), +64) and column blocks [4(w/4), +4) (128 accumulator
+// registers, two waves per SIMD, so one wave's MFMAs cover the other's loads).
+template <int BK, int NW = 4>
 __device__ __forceinline__ void tile_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t kb, int64_t ke,
                                              const uint32_t* __restrict__ bits,
                                              int64_t wpr, int64_t B, int64_t m0, int64_t j0,
                                              unsigned long long* __restrict__ q, unsigned char* lds) {
     constexpr int ROW = BK + 16;
     constexpr int STAGE = WIDE_BN * ROW;
+    constexpr int NT = 64 * NW;         // threads
+    constexpr int NCB = 32 / NW;        // column blocks per wave
     static_assert(BK % 32 == 0, "BK: a multiple of 32");
+    static_assert(NW == 4 || NW == 8, "NW: 4 or 8 waves");
     int8_t* stage0 = (int8_t*)lds;
     int8_t* stage1 = stage0 + STAGE;
     uint32_t* Sb = (uint32_t*)(lds + 2 * STAGE);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int hi = lane >> 5, lc = lane & 31;
+    const int rg = wid & 3, cb0 = (wid >> 2) * NCB;
     // this lane's candidate rows (one per row block); rows past B read row 0
     // and are dropped in the epilogue, so the loads need no branch
     const uint32_t* arow[2];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-        const int64_t m = m0 + wid * 64 + rb * 32 + lc;
+        const int64_t m = m0 + rg * 64 + rb * 32 + lc;
         arow[rb] = bits + (m < B ? m : 0) * wpr;
     }
     constexpr int LPR = BK / 16;        // lanes per W row in the loader
-    constexpr int RPP = 256 / LPR;      // rows per loader pass
-    const int8_t* wbase[BK / 16];
-    int woff[BK / 16];
+    constexpr int RPP = NT / LPR;       // rows per loader pass
+    constexpr int NP = WIDE_BN / RPP;   // loader passes per chunk
+    const int8_t* wbase[NP];
+    int woff[NP];
 #pragma unroll
-    for (int t = 0; t < BK / 16; ++t) {
+    for (int t = 0; t < NP; ++t) {
         const int r = t * RPP + tid / LPR, piece = tid % LPR;
         const int64_t jrow = j0 + r < ld ? j0 + r : ld - 1;  // rows past n_pad: a valid row, dropped later
         wbase[t] = W + jrow * ld + 16 * piece;
         woff[t] = r * ROW + 16 * piece;
     }
 
-    v16i acc[2][8];
+    v16i acc[2][NCB];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 8; ++b)
+        for (int b = 0; b < NCB; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
 
     const int64_t nc = (ke - kb) / BK;
     auto chunk = [nc, kb](int64_t c) { return kb + (c < nc ? c : nc - 1) * BK; };  // past the end: the last
     uint32_t a0[2][BK / 32], a1[2][BK / 32];
-    v4i w0[BK / 16], w1[BK / 16];
+    v4i w0[NP], w1[NP];
     wide_load_a<BK>(a0, arow, kb);
-    wide_load_w<BK>(w0, wbase, kb);
-    wide_load_w<BK>(w1, wbase, chunk(1));
-    wide_store_w<BK>(w0, stage0, woff);
-    wide_load_w<BK>(w0, wbase, chunk(2));
+    wide_load_w<NP>(w0, wbase, kb);
+    wide_load_w<NP>(w1, wbase, chunk(1));
+    wide_store_w<NP>(w0, stage0, woff);
+    wide_load_w<NP>(w0, wbase, chunk(2));
     __syncthreads();
     int64_t c = 0;
     for (; c + 1 < nc; c += 2) {
         wide_load_a<BK>(a1, arow, chunk(c + 1));
-        wide_compute<BK>(acc, a0, stage0, lc, hi);
-        wide_store_w<BK>(w1, stage1, woff);
-        wide_load_w<BK>(w1, wbase, chunk(c + 3));
+        wide_compute<BK, NCB>(acc, a0, stage0, lc, hi, cb0);
+        wide_store_w<NP>(w1, stage1, woff);
+        wide_load_w<NP>(w1, wbase, chunk(c + 3));
         __syncthreads();
         wide_load_a<BK>(a0, arow, chunk(c + 2));
-        wide_compute<BK>(acc, a1, stage1, lc, hi);
-        wide_store_w<BK>(w0, stage0, woff);
-        wide_load_w<BK>(w0, wbase, chunk(c + 4));
+        wide_compute<BK, NCB>(acc, a1, stage1, lc, hi, cb0);
+        wide_store_w<NP>(w0, stage0, woff);
+        wide_load_w<NP>(w0, wbase, chunk(c + 4));
         __syncthreads();
     }
-    if (c < nc) wide_compute<BK>(acc, a0, stage0, lc, hi);
+    if (c < nc) wide_compute<BK, NCB>(acc, a0, stage0, lc, hi, cb0);
 
     // candidate bits of the tile's 256 output columns (words past n_pad: 0)
     const int64_t cwords = (ld - j0) >> 5;
-    for (int t = tid; t < WIDE_BM * 8; t += 256) {
+    for (int t = tid; t < WIDE_BM * 8; t += NT) {
         const int r = t >> 3, cc = t & 7;
         const int64_t m = m0 + r;
         Sb[t] = (m < B && cc < cwords) ? bits[m * wpr + (j0 >> 5) + cc] : 0u;
@@ -287,8 +297,8 @@ __device__ __forceinline__ void tile_i8_wide(const int8_t* __restrict__ W, int64
     // row 8*(r/4) + 4*(l/32) + r%4 of the block.  P = c_j - 2 (X W).
     // column blocks past n_pad are dropped (uniform per block)
 #pragma unroll
-    for (int cb = 0; cb < 8; ++cb)
-        if (j0 + cb * 32 >= ld)
+    for (int cb = 0; cb < NCB; ++cb)
+        if (j0 + (cb0 + cb) * 32 >= ld)
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -300,13 +310,15 @@ __device__ __forceinline__ void tile_i8_wide(const int8_t* __restrict__ W, int64
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int row = wid * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            const int row = rg * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
             const v4i w03 = *(const v4i*)&Sb[row * 8];
             const v4i w47 = *(const v4i*)&Sb[row * 8 + 4];
             int v = 0;
 #pragma unroll
-            for (int cb = 0; cb < 8; ++cb) {
-                const int m = __builtin_amdgcn_sbfe(cb < 4 ? w03[cb] : w47[cb - 4], (unsigned)lc, 1u);
+            for (int cb = 0; cb < NCB; ++cb) {
+                // this block's word of the row: block cb0 + cb (cb0 is uniform per wave)
+                const int word = NW == 4 ? (cb < 4 ? w03[cb] : w47[cb - 4]) : (cb0 == 0 ? w03[cb] : w47[cb]);
+                const int m = __builtin_amdgcn_sbfe(word, (unsigned)lc, 1u);
                 v += (acc[rb][cb][r] + m) ^ m;
             }
             vals[rb * 16 + r] = v;
@@ -327,7 +339,7 @@ __device__ __forceinline__ void tile_i8_wide(const int8_t* __restrict__ W, int64
     const int qi = (lane & 1) * 16 + ((lane >> 1) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 3) & 1) * 2 +
                    ((lane >> 4) & 1);
     const int rb = qi >> 4, r = qi & 15;
-    const int64_t m = m0 + wid * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+    const int64_t m = m0 + rg * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
     if (m < B) atomicAdd(&q[m], (unsigned long long)(long long)vals[0]);
     __syncthreads();  // LDS is reused by the caller's next tile
 }
