@@ -289,6 +289,70 @@ __global__ __launch_bounds__(256) void k_batch_real(const WT* __restrict__ W, in
     }
 }
 
+// q_c = s_c^T W s_c for a few candidates (C <= 16), real weights: W is
+// streamed once.  Block (jg, rs) covers columns [256 jg, 256 jg + 256) (lane l
+// owns 4 consecutive columns) and the rows of split rs (wave w takes every
+// fourth).  Per row i each lane forms, for every candidate, the sum of its 4
+// weights signed by the candidate's spins at those columns, and adds it
+// signed by the spin at i; the lanes' sums are reduced at the end in a fixed
+// order into part[c * nparts + block].
+constexpr int kFewMax = 16;
+
+template <typename WT>
+__global__ __launch_bounds__(256) void k_rescore_few(const WT* __restrict__ W, int64_t ld, int64_t n,
+                                                     const uint32_t* __restrict__ bits, int64_t wpr, int C,
+                                                     int64_t rows_per, double* __restrict__ part, int64_t nparts) {
+    __shared__ double red[4][kFewMax];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t j0 = (int64_t)blockIdx.x * 256 + lane * 4;
+    const bool live = j0 < ld;  // ld is a multiple of 4: all 4 columns or none
+    uint32_t nib[kFewMax];
+#pragma unroll
+    for (int c = 0; c < kFewMax; ++c)
+        nib[c] = (live && c < C) ? (bits[c * wpr + (j0 >> 5)] >> (j0 & 31)) & 0xFu : 0u;
+    double acc[kFewMax];
+#pragma unroll
+    for (int c = 0; c < kFewMax; ++c) acc[c] = 0.0;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per, r1 = r0 + rows_per < n ? r0 + rows_per : n;
+    for (int64_t i = r0 + wid; i < r1; i += 4) {
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        if (live) {
+            const WT* Wr = W + i * ld + j0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) w[u] = (double)Wr[u];
+        }
+#pragma unroll
+        for (int c = 0; c < kFewMax; ++c) {
+            if (c < C) {
+                double t = 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) t += ((nib[c] >> u) & 1u) ? -w[u] : w[u];
+                const bool si = (bits[c * wpr + (i >> 5)] >> (i & 31)) & 1u;
+                acc[c] += si ? -t : t;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kFewMax; ++c) {
+        double v = acc[c];
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+        if (lane == 0) red[wid][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < C) {
+        const int c = threadIdx.x;
+        part[c * nparts + blockIdx.y * gridDim.x + blockIdx.x] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+    }
+}
+
+// the few-candidate grid: 256-column groups x row splits, about 512 blocks
+static void few_grid(const Graph& g, int64_t* jg, int64_t* rs, int64_t* rows_per) {
+    *jg = (g.n_pad + 255) / 256;
+    *rs = std::max<int64_t>(1, std::min<int64_t>((512 + *jg - 1) / *jg, (g.n + 63) / 64));
+    *rows_per = (g.n + *rs - 1) / *rs;
+}
+
 __global__ void k_batch_sum(const double* part, int64_t nblk, int64_t B, double* q) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= B) return;
@@ -358,6 +422,11 @@ size_t batch_scratch_bytes(const Graph& g, int64_t B) {
     if (g.W == nullptr) return 8;
     if (g.kind == PQI_WEIGHTS_I8) return (size_t)B * 8;
     int64_t nblk = (g.n + 63) / 64;
+    if (B <= kFewMax) {
+        int64_t jg, rs, rp;
+        few_grid(g, &jg, &rs, &rp);
+        nblk = std::max(nblk, jg * rs);
+    }
     return (size_t)B * (size_t)nblk * 8;
 }
 
@@ -426,6 +495,18 @@ void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, i
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B);
         launch_batch_i8_raw(g, bits, wpr, B, qi, s);
         hipLaunchKernelGGL(k_i64_to_f64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B, q);
+    } else if (B <= kFewMax) {
+        int64_t jg, rs, rp;
+        few_grid(g, &jg, &rs, &rp);
+        double* part = (double*)scratch;
+        const dim3 grid((unsigned)jg, (unsigned)rs);
+        if (g.kind == PQI_WEIGHTS_F32)
+            hipLaunchKernelGGL((k_rescore_few<float>), grid, dim3(256), 0, s, (const float*)g.W, g.n_pad, g.n, bits,
+                               wpr, (int)B, rp, part, jg * rs);
+        else
+            hipLaunchKernelGGL((k_rescore_few<double>), grid, dim3(256), 0, s, (const double*)g.W, g.n_pad, g.n, bits,
+                               wpr, (int)B, rp, part, jg * rs);
+        hipLaunchKernelGGL(k_batch_sum, dim3(1), dim3(256), 0, s, part, jg * rs, B, q);
     } else {
         const int64_t nblk = (g.n + 63) / 64;
         double* part = (double*)scratch;

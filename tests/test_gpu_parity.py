@@ -119,6 +119,26 @@ def sk_matrix(n, seed):
 BF16_TOL = 2e-5
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n,B", [(100, 1), (130, 5), (257, 16), (1000, 16), (2050, 9)])
+def test_eval_batch_real_few_candidates(ctx, orc, dtype, n, B):
+    """Up to 16 candidates take the streaming FP64 re-score kernel (W read
+    once; 256-column groups, so n_pad that is not a multiple of 256 and the
+    row splits are covered): values within 1e-9 relative of the oracle."""
+    log(f"few-candidate re-score n={n} B={B} {dtype.__name__}")
+    rng = np.random.default_rng(n + B)
+    W = er_weights(n, 0.3, n + B, "real").astype(dtype)
+    X = rng.integers(0, 2, (B, n)).astype(bool)
+    dg = _lib.DeviceGraph.dense(W)
+    for sg in (0, 1):
+        bi, bv, vals = dg.eval_batch(_lib.pack_bits(X), sg, engine=_lib.ENGINE_EXACT)
+        ob, ov = orc.eval_batch(W.astype(np.float64), X, sg)
+        scale = np.abs(np.triu(W.astype(np.float64), 1)).sum()
+        np.testing.assert_allclose(vals, ov, rtol=0, atol=1e-9 * scale)
+        assert vals[bi] == bv == vals.max()
+    dg.close()
+
+
 @pytest.mark.parametrize("sg", [0, 1])
 def test_eval_batch_bf16_split(ctx, orc, sg):
     n, B = 1000, 300
