@@ -199,17 +199,191 @@ __global__ __launch_bounds__(256) void k_batch_bf16(const uint16_t* __restrict__
     if (m < B) part[m * nparts + blockIdx.x * 2 + wn] = (double)vals[0];
 }
 
+// ---------------------------------------------------------------------------
+// Wide split-BF16 tile: 256 candidates x 256 output columns per work group of
+// 8 waves (wave w: candidate rows [64 (w mod 4), +64), column blocks
+// [4 (w / 4), +4); 128 fp32 accumulators per lane, two waves per SIMD).  The
+// contraction runs over K = 2 n_pad (W_hi then W_lo, the candidate's spins
+// twice) in chunks of 64 bf16: a W_cat chunk is 128 bytes per row, the same
+// LDS layout and loaders as the 128-deep int8 wide tile.  The +-1 A
+// fragments are made in registers from the candidate bits (sign bit over
+// 1.0).  Each wave writes its rows' partial sums (its 128 columns) to
+// part[m * 2 JB + 2 jb + column half]; they are summed in a fixed order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ v8bf unpack8_bf16(uint32_t x) {
+    v4i w;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        w[u] = (int)(0x3F803F80u | ((x << (15 - 2 * u)) & 0x8000u) | ((x << (30 - 2 * u)) & 0x80000000u));
+    return __builtin_bit_cast(v8bf, w);
+}
+
+constexpr int BF_KC = 64;                  // bf16 per chunk
+constexpr int BF_ROW = 2 * BF_KC + 16;     // padded LDS row in bytes
+constexpr size_t bf16_wide_lds_bytes() { return 2 * (size_t)WIDE_BN * BF_ROW + (size_t)WIDE_BM * 8 * 4; }
+
+template <int NCB>
+__device__ __forceinline__ void wide_compute_bf16(v16f (&acc)[2][NCB], const uint32_t (&aw)[2][BF_KC / 32],
+                                                  const int8_t* Ws, int lc, int hi, int cb0) {
+#pragma unroll
+    for (int t = 0; t < BF_KC / 16; ++t) {
+        v8bf av[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) av[rb] = unpack8_bf16((aw[rb][t >> 1] >> (16 * (t & 1) + 8 * hi)) & 0xFFu);
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            const v8bf bv = *(const v8bf*)&Ws[((cb0 + cb) * 32 + lc) * BF_ROW + 32 * t + 16 * hi];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[rb], bv, acc[rb][cb], 0, 0, 0);
+        }
+    }
+}
+
+// grid: XCD-aware as k_batch_i8_wide (8-column-block groups x MB candidate blocks)
+__global__ __launch_bounds__(512) void k_batch_bf16_wide(const uint16_t* __restrict__ Wcat, int64_t n_pad,
+                                                         const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
+                                                         double* __restrict__ part, int64_t MB, int64_t JB) {
+    constexpr int NW = 8, NT = 64 * NW, NCB = 32 / NW;
+    constexpr int CB = 2 * BF_KC;                 // bytes per W_cat row per chunk
+    constexpr int STAGE = WIDE_BN * BF_ROW;
+    constexpr int LPR = CB / 16, RPP = NT / LPR, NP = WIDE_BN / RPP;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t blk = blockIdx.x, xcd = blk & 7, slot = blk >> 3;
+    const int64_t jb = (slot / MB) * 8 + xcd, mb = slot % MB;
+    if (jb >= JB) return;  // whole work group, before any barrier
+    const int64_t m0 = mb * WIDE_BM, j0 = jb * WIDE_BN;
+    int8_t* stage0 = (int8_t*)lds;
+    int8_t* stage1 = stage0 + STAGE;
+    uint32_t* Sb = (uint32_t*)(lds + 2 * STAGE);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int hi = lane >> 5, lc = lane & 31;
+    const int rg = wid & 3, ch = wid >> 2, cb0 = ch * NCB;
+    const uint32_t* arow[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+        const int64_t m = m0 + rg * 64 + rb * 32 + lc;
+        arow[rb] = bits + (m < B ? m : 0) * wpr;
+    }
+    const int64_t pitch = 4 * n_pad;  // bytes per W_cat row (2 n_pad bf16)
+    const int8_t* Wb = (const int8_t*)Wcat;
+    const int8_t* wbase[NP];
+    int woff[NP];
+#pragma unroll
+    for (int t = 0; t < NP; ++t) {
+        const int r = t * RPP + tid / LPR, piece = tid % LPR;
+        const int64_t jrow = j0 + r < n_pad ? j0 + r : n_pad - 1;  // rows past n_pad: a valid row, dropped later
+        wbase[t] = Wb + jrow * pitch + 16 * piece;
+        woff[t] = r * BF_ROW + 16 * piece;
+    }
+    v16f acc[2][NCB];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < NCB; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+
+    const int64_t nc = 2 * n_pad / BF_KC;
+    auto cl = [nc](int64_t c) { return c < nc ? c : nc - 1; };           // past the end: the last chunk
+    auto boff = [&](int64_t c) { return cl(c) * CB; };                   // byte offset in a W_cat row
+    auto kbit = [&](int64_t c) { return (cl(c) * BF_KC) % n_pad; };     // the chunk's spin index
+    uint32_t a0[2][BF_KC / 32], a1[2][BF_KC / 32];
+    v4i w0[NP], w1[NP];
+    wide_load_a<BF_KC>(a0, arow, kbit(0));
+    wide_load_w<NP>(w0, wbase, boff(0));
+    wide_load_w<NP>(w1, wbase, boff(1));
+    wide_store_w<NP>(w0, stage0, woff);
+    wide_load_w<NP>(w0, wbase, boff(2));
+    __syncthreads();
+    int64_t c = 0;
+    for (; c + 1 < nc; c += 2) {
+        wide_load_a<BF_KC>(a1, arow, kbit(c + 1));
+        wide_compute_bf16<NCB>(acc, a0, stage0, lc, hi, cb0);
+        wide_store_w<NP>(w1, stage1, woff);
+        wide_load_w<NP>(w1, wbase, boff(c + 3));
+        __syncthreads();
+        wide_load_a<BF_KC>(a0, arow, kbit(c + 2));
+        wide_compute_bf16<NCB>(acc, a1, stage1, lc, hi, cb0);
+        wide_store_w<NP>(w0, stage0, woff);
+        wide_load_w<NP>(w0, wbase, boff(c + 4));
+        __syncthreads();
+    }
+    if (c < nc) wide_compute_bf16<NCB>(acc, a0, stage0, lc, hi, cb0);
+
+    const int64_t cwords = (n_pad - j0) >> 5;
+    for (int t = tid; t < WIDE_BM * 8; t += NT) {
+        const int r = t >> 3, cc = t & 7;
+        const int64_t m = m0 + r;
+        Sb[t] = (m < B && cc < cwords) ? bits[m * wpr + (j0 >> 5) + cc] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+        if (j0 + (cb0 + cb) * 32 >= n_pad)
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[rb][cb][r] = 0.f;
+    float vals[32];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = rg * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            const v4i wv = *(const v4i*)&Sb[row * 8 + 4 * ch];  // this wave's 4 column words
+            float v = 0.f;
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const float p = acc[rb][cb][r];
+                v += ((wv[cb] >> lc) & 1) ? -p : p;
+            }
+            vals[rb * 16 + r] = v;
+        }
+#pragma unroll
+    for (int t = 0, half = 16; t < 5; ++t, half >>= 1) {
+        const int up = (lane >> t) & 1;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < half) {
+                const float send = up ? vals[i] : vals[i + half];
+                const float keep = up ? vals[i + half] : vals[i];
+                vals[i] = keep + __shfl_xor(send, 1 << t, 64);
+            }
+        }
+    }
+    const int qi = (lane & 1) * 16 + ((lane >> 1) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                   ((lane >> 4) & 1);
+    const int rb = qi >> 4, r = qi & 15;
+    const int64_t m = m0 + rg * 64 + rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+    if (m < B) part[m * (2 * JB) + 2 * jb + ch] = (double)vals[0];
+}
+
 void launch_batch_bf16(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, double* q, void* scratch,
                        hipStream_t s) {
-    const int64_t nparts = 2 * (g.n_pad / BN);
     double* part = (double*)scratch;
+    static const bool wide_ok = hipFuncSetAttribute((const void*)k_batch_bf16_wide,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    (int)bf16_wide_lds_bytes()) == hipSuccess;
+    if (B >= 512 && wide_ok) {
+        const int64_t MB = (B + WIDE_BM - 1) / WIDE_BM, JB = (g.n_pad + WIDE_BN - 1) / WIDE_BN;
+        const int64_t blocks = ((JB + 7) / 8) * 8 * MB;
+        hipLaunchKernelGGL(k_batch_bf16_wide, dim3((unsigned)blocks), dim3(512), bf16_wide_lds_bytes(), s,
+                           (const uint16_t*)g.Wbf, g.n_pad, bits, wpr, B, part, MB, JB);
+        hipLaunchKernelGGL(k_batch_sum, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, part, 2 * JB, B, q);
+        return;
+    }
+    const int64_t nparts = 2 * (g.n_pad / BN);
     dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
     hipLaunchKernelGGL(k_batch_bf16, grid, dim3(256), 0, s, (const uint16_t*)g.Wbf, g.n_pad, bits, wpr, B, part,
                        nparts);
     hipLaunchKernelGGL(k_batch_sum, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, part, nparts, B, q);
 }
 
-size_t batch_bf16_scratch_bytes(const Graph& g, int64_t B) { return (size_t)B * 2 * (g.n_pad / BN) * 8; }
+size_t batch_bf16_scratch_bytes(const Graph& g, int64_t B) {
+    const int64_t parts = std::max<int64_t>(2 * (g.n_pad / BN), 2 * ((g.n_pad + WIDE_BN - 1) / WIDE_BN));
+    return (size_t)B * parts * 8;
+}
 
 __global__ void k_gather_rows(const uint32_t* bits, int64_t wpr, const int64_t* sel, int64_t k, uint32_t* out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -140,9 +140,11 @@ def test_eval_batch_real_few_candidates(ctx, orc, dtype, n, B):
 
 
 @pytest.mark.parametrize("sg", [0, 1])
-def test_eval_batch_bf16_split(ctx, orc, sg):
-    n, B = 1000, 300
-    log(f"eval_batch bf16 split n={n} sg={sg}")
+@pytest.mark.parametrize("n,B", [(1000, 300), (1000, 777), (1100, 600)])
+def test_eval_batch_bf16_split(ctx, orc, sg, n, B):
+    """B >= 512 takes the wide 256 x 256 tile (n = 1100: n_pad 1152, a last
+    column tile half past n_pad); smaller batches the 128 x 128 tile."""
+    log(f"eval_batch bf16 split n={n} B={B} sg={sg}")
     W = sk_matrix(n, 41)
     X = np.random.default_rng(42).integers(0, 2, (B, n)).astype(bool)
     dg = _lib.DeviceGraph.dense(W)
