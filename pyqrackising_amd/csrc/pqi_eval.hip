@@ -487,22 +487,35 @@ __global__ __launch_bounds__(256) void k_rescore_few(const WT* __restrict__ W, i
     double acc[kFewMax];
 #pragma unroll
     for (int c = 0; c < kFewMax; ++c) acc[c] = 0.0;
+    // rows in groups of 32 (one spin word per candidate); wave w takes groups w, w+4, ...
     const int64_t r0 = (int64_t)blockIdx.y * rows_per, r1 = r0 + rows_per < n ? r0 + rows_per : n;
-    for (int64_t i = r0 + wid; i < r1; i += 4) {
-        double w[4] = {0.0, 0.0, 0.0, 0.0};
-        if (live) {
-            const WT* Wr = W + i * ld + j0;
+    for (int64_t g0 = r0 + 32 * wid; g0 < r1; g0 += 128) {
+        uint32_t wc[kFewMax];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) w[u] = (double)Wr[u];
-        }
+        for (int c = 0; c < kFewMax; ++c) wc[c] = c < C ? bits[c * wpr + (g0 >> 5)] : 0u;
+        const int cnt = r1 - g0 < 32 ? (int)(r1 - g0) : 32;
+        for (int k = 0; k < cnt; k += 4) {
+            // four rows' loads in flight
+            double w[4][4];
 #pragma unroll
-        for (int c = 0; c < kFewMax; ++c) {
-            if (c < C) {
-                double t = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                const bool ok = live && k + q < cnt;
+                const WT* Wr = W + (g0 + k + q) * ld + j0;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) t += ((nib[c] >> u) & 1u) ? -w[u] : w[u];
-                const bool si = (bits[c * wpr + (i >> 5)] >> (i & 31)) & 1u;
-                acc[c] += si ? -t : t;
+                for (int u = 0; u < 4; ++u) w[q][u] = ok ? (double)Wr[u] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int sh = k + q;  // rows past cnt have w = 0
+#pragma unroll
+                for (int c = 0; c < kFewMax; ++c) {
+                    if (c < C) {
+                        double t = 0.0;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) t += ((nib[c] >> u) & 1u) ? -w[q][u] : w[q][u];
+                        acc[c] += ((wc[c] >> (sh & 31)) & 1u) ? -t : t;
+                    }
+                }
             }
         }
     }
@@ -520,11 +533,13 @@ __global__ __launch_bounds__(256) void k_rescore_few(const WT* __restrict__ W, i
     }
 }
 
-// the few-candidate grid: 256-column groups x row splits, about 512 blocks
+// the few-candidate grid: 256-column groups x row splits
 static void few_grid(const Graph& g, int64_t* jg, int64_t* rs, int64_t* rows_per) {
     *jg = (g.n_pad + 255) / 256;
-    *rs = std::max<int64_t>(1, std::min<int64_t>((512 + *jg - 1) / *jg, (g.n + 63) / 64));
-    *rows_per = (g.n + *rs - 1) / *rs;
+    // about 2048 work groups (several waves per SIMD hide the row loads);
+    // splits of whole 32-row groups, at least four per split
+    *rs = std::max<int64_t>(1, std::min<int64_t>((2048 + *jg - 1) / *jg, (g.n + 127) / 128));
+    *rows_per = ((g.n + *rs - 1) / *rs + 31) / 32 * 32;
 }
 
 __global__ void k_batch_sum(const double* part, int64_t nblk, int64_t B, double* q) {
@@ -604,11 +619,12 @@ size_t batch_scratch_bytes(const Graph& g, int64_t B) {
     return (size_t)B * (size_t)nblk * 8;
 }
 
-// Waves per wide tile (4 or 8), from PQI_WIDE_WAVES; default 4.
+// Waves per wide tile: 8 (two waves per SIMD; measured faster on MI355X),
+// or 4 with PQI_WIDE_WAVES=4.
 static int wide_waves() {
     static const int v = [] {
         const char* e = getenv("PQI_WIDE_WAVES");
-        return e && atoi(e) == 8 ? 8 : 4;
+        return e && atoi(e) == 4 ? 4 : 8;
     }();
     return v;
 }
