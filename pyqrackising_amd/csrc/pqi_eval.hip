@@ -463,83 +463,79 @@ __global__ __launch_bounds__(256) void k_batch_real(const WT* __restrict__ W, in
     }
 }
 
-// q_c = s_c^T W s_c for a few candidates (C <= 16), real weights: W is
-// streamed once.  Block (jg, rs) covers columns [256 jg, 256 jg + 256) (lane l
-// owns 4 consecutive columns) and the rows of split rs (wave w takes every
-// fourth).  Per row i each lane forms, for every candidate, the sum of its 4
-// weights signed by the candidate's spins at those columns, and adds it
-// signed by the spin at i; the lanes' sums are reduced at the end in a fixed
-// order into part[c * nparts + block].
+// FP64 matrix-core re-score of a few candidates (C <= 16), real weights:
+// H = W S (rows x candidates) on v_mfma_f64_16x16x4f64, then
+// q_c = sum_i s_ci H_ic; W is streamed once.  A wave owns a block of 16 rows
+// and a split of the columns; per 16 columns each lane loads 4 consecutive
+// weights of row (lane mod 16) (one coalesced 16-byte load) and the 4 spins of
+// candidate (lane mod 16) at the same columns, and feeds 4 MFMAs
+// (A[row][k] = W, B[k][cand] = +-1, k = lane / 16).  Rows and columns up to
+// n_pad are read without a branch: W is zero there.  D layout (measured on
+// gfx950): lane l, register r -> row 4 r + l / 16, candidate l mod 16.  Each
+// wave writes one partial per candidate; k_sum_parts adds them in a fixed
+// order.
 constexpr int kFewMax = 16;
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <typename WT>
-__global__ __launch_bounds__(256) void k_rescore_few(const WT* __restrict__ W, int64_t ld, int64_t n,
-                                                     const uint32_t* __restrict__ bits, int64_t wpr, int C,
-                                                     int64_t rows_per, double* __restrict__ part, int64_t nparts) {
-    __shared__ double red[4][kFewMax];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t j0 = (int64_t)blockIdx.x * 256 + lane * 4;
-    const bool live = j0 < ld;  // ld is a multiple of 4: all 4 columns or none
-    uint32_t nib[kFewMax];
+__global__ __launch_bounds__(256) void k_rescore_mfma(const WT* __restrict__ W, int64_t ld, int64_t nrb,
+                                                      const uint32_t* __restrict__ bits, int64_t wpr, int C,
+                                                      int64_t ncs, int64_t cols_per, double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int64_t gw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gw >= nrb * ncs) return;  // whole wave; no barrier in this kernel
+    const int64_t rb = gw / ncs, cs = gw - rb * ncs;
+    const int64_t i0 = rb * 16, jb = cs * cols_per, je = jb + cols_per < ld ? jb + cols_per : ld;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int c = r16;  // this lane's candidate column of B and D
+    const bool cl = c < C;
+    const WT* Wr = W + (i0 + r16) * ld + 4 * q;
+    const uint32_t* brow = bits + (cl ? c : 0) * wpr;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t j0 = jb; j0 < je; j0 += 16) {
+        const uint32_t nib = cl ? (brow[j0 >> 5] >> ((j0 & 31) + 4 * q)) & 0xFu : 0u;
+        double a[4];
 #pragma unroll
-    for (int c = 0; c < kFewMax; ++c)
-        nib[c] = (live && c < C) ? (bits[c * wpr + (j0 >> 5)] >> (j0 & 31)) & 0xFu : 0u;
-    double acc[kFewMax];
+        for (int u = 0; u < 4; ++u) a[u] = (double)Wr[j0 + u];
 #pragma unroll
-    for (int c = 0; c < kFewMax; ++c) acc[c] = 0.0;
-    // rows in groups of 32 (one spin word per candidate); wave w takes groups w, w+4, ...
-    const int64_t r0 = (int64_t)blockIdx.y * rows_per, r1 = r0 + rows_per < n ? r0 + rows_per : n;
-    for (int64_t g0 = r0 + 32 * wid; g0 < r1; g0 += 128) {
-        uint32_t wc[kFewMax];
-#pragma unroll
-        for (int c = 0; c < kFewMax; ++c) wc[c] = c < C ? bits[c * wpr + (g0 >> 5)] : 0u;
-        const int cnt = r1 - g0 < 32 ? (int)(r1 - g0) : 32;
-        for (int k = 0; k < cnt; k += 4) {
-            // four rows' loads in flight
-            double w[4][4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool ok = live && k + q < cnt;
-                const WT* Wr = W + (g0 + k + q) * ld + j0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) w[q][u] = ok ? (double)Wr[u] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int sh = k + q;  // rows past cnt have w = 0
-#pragma unroll
-                for (int c = 0; c < kFewMax; ++c) {
-                    if (c < C) {
-                        double t = 0.0;
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) t += ((nib[c] >> u) & 1u) ? -w[q][u] : w[q][u];
-                        acc[c] += ((wc[c] >> (sh & 31)) & 1u) ? -t : t;
-                    }
-                }
-            }
+        for (int u = 0; u < 4; ++u) {
+            const double b = cl ? (((nib >> u) & 1u) ? -1.0 : 1.0) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b, acc, 0, 0, 0);
         }
     }
+    // q_c partial: the spins of candidate c at this lane's rows 4 r + q
+    const uint32_t rw = cl ? brow[i0 >> 5] >> ((i0 & 31) + q) : 0u;
+    double v = 0.0;
 #pragma unroll
-    for (int c = 0; c < kFewMax; ++c) {
-        double v = acc[c];
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-        if (lane == 0) red[wid][c] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < C) {
-        const int c = threadIdx.x;
-        part[c * nparts + blockIdx.y * gridDim.x + blockIdx.x] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
-    }
+    for (int r = 0; r < 4; ++r) v += ((rw >> (4 * r)) & 1u) ? -acc[r] : acc[r];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (lane < 16 && cl) part[c * (nrb * ncs) + gw] = v;
 }
 
-// the few-candidate grid: 256-column groups x row splits
-static void few_grid(const Graph& g, int64_t* jg, int64_t* rs, int64_t* rows_per) {
-    *jg = (g.n_pad + 255) / 256;
-    // about 2048 work groups (several waves per SIMD hide the row loads);
-    // splits of whole 32-row groups, at least four per split
-    *rs = std::max<int64_t>(1, std::min<int64_t>((2048 + *jg - 1) / *jg, (g.n + 127) / 128));
-    *rows_per = ((g.n + *rs - 1) / *rs + 31) / 32 * 32;
+// the FP64-MFMA grid: 16-row blocks x column splits (multiples of 16, at
+// least 256 columns each), about 8192 waves in all
+static void mfma_grid(const Graph& g, int64_t* nrb, int64_t* ncs, int64_t* cols_per) {
+    *nrb = (g.n + 15) / 16;
+    const int64_t want = 8192;
+    *ncs = std::max<int64_t>(1, std::min<int64_t>((want + *nrb - 1) / *nrb, g.n_pad / 256));
+    *cols_per = ((g.n_pad + *ncs - 1) / *ncs + 15) / 16 * 16;
+}
+
+// q[m] = sum of part[m * nparts + p] over p: one work group per candidate,
+// strided partial sums then a fixed-order tree (deterministic)
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int64_t nparts, double* q) {
+    __shared__ double red[256];
+    const int64_t m = blockIdx.x;
+    double s = 0.0;
+    for (int64_t p = threadIdx.x; p < nparts; p += 256) s += part[m * nparts + p];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) q[m] = red[0];
 }
 
 __global__ void k_batch_sum(const double* part, int64_t nblk, int64_t B, double* q) {
@@ -612,9 +608,9 @@ size_t batch_scratch_bytes(const Graph& g, int64_t B) {
     if (g.kind == PQI_WEIGHTS_I8) return (size_t)B * 8;
     int64_t nblk = (g.n + 63) / 64;
     if (B <= kFewMax) {
-        int64_t jg, rs, rp;
-        few_grid(g, &jg, &rs, &rp);
-        nblk = std::max(nblk, jg * rs);
+        int64_t nrb, ncs, cp;
+        mfma_grid(g, &nrb, &ncs, &cp);
+        nblk = std::max(nblk, nrb * ncs);
     }
     return (size_t)B * (size_t)nblk * 8;
 }
@@ -686,17 +682,17 @@ void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t wpr, i
         launch_batch_i8_raw(g, bits, wpr, B, qi, s);
         hipLaunchKernelGGL(k_i64_to_f64, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, qi, B, q);
     } else if (B <= kFewMax) {
-        int64_t jg, rs, rp;
-        few_grid(g, &jg, &rs, &rp);
+        int64_t nrb, ncs, cp;
+        mfma_grid(g, &nrb, &ncs, &cp);
         double* part = (double*)scratch;
-        const dim3 grid((unsigned)jg, (unsigned)rs);
+        const dim3 grid((unsigned)((nrb * ncs + 3) / 4));
         if (g.kind == PQI_WEIGHTS_F32)
-            hipLaunchKernelGGL((k_rescore_few<float>), grid, dim3(256), 0, s, (const float*)g.W, g.n_pad, g.n, bits,
-                               wpr, (int)B, rp, part, jg * rs);
+            hipLaunchKernelGGL((k_rescore_mfma<float>), grid, dim3(256), 0, s, (const float*)g.W, g.n_pad, nrb, bits,
+                               wpr, (int)B, ncs, cp, part);
         else
-            hipLaunchKernelGGL((k_rescore_few<double>), grid, dim3(256), 0, s, (const double*)g.W, g.n_pad, g.n, bits,
-                               wpr, (int)B, rp, part, jg * rs);
-        hipLaunchKernelGGL(k_batch_sum, dim3(1), dim3(256), 0, s, part, jg * rs, B, q);
+            hipLaunchKernelGGL((k_rescore_mfma<double>), grid, dim3(256), 0, s, (const double*)g.W, g.n_pad, nrb,
+                               bits, wpr, (int)B, ncs, cp, part);
+        hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)B), dim3(256), 0, s, part, nrb * ncs, q);
     } else {
         const int64_t nblk = (g.n + 63) / 64;
         double* part = (double*)scratch;
