@@ -309,6 +309,16 @@ def bnb_exact(log, with_cpu):
     return res
 
 
+# the extra legs, each (log, with_cpu) -> dict
+LEGS = {
+    "time_to_equal_cut": lambda lg, cpu: time_to_equal_cut(lg),
+    "batched_eval": lambda lg, cpu: batched_eval(lg),
+    "sk_bf16": lambda lg, cpu: sk_bf16(lg),
+    "sampler": lambda lg, cpu: sampler_round(lg, cpu),
+    "branch_and_bound": lambda lg, cpu: bnb_exact(lg, cpu),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,9 +333,15 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the batched, learned and SK legs")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
+    ap.add_argument("--only", choices=sorted(LEGS), default=None,
+                    help="run one extra leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.copy_kernel:
         copy_kernel(args.copy_kernel)
+        return
+    if args.only:
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        print(json.dumps({args.only: LEGS[args.only](log, not args.no_cpu)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -423,12 +439,9 @@ def main():
     }
     if not args.no_extra:
         log = lambda m: print(m, file=sys.stderr, flush=True)
-        legs = (("time_to_equal_cut", time_to_equal_cut), ("batched_eval", batched_eval), ("sk_bf16", sk_bf16),
-                ("sampler", lambda lg: sampler_round(lg, not args.no_cpu)),
-                ("branch_and_bound", lambda lg: bnb_exact(lg, not args.no_cpu)))
-        for key, fn in legs:
+        for key, fn in LEGS.items():
             try:
-                out[key] = fn(log)
+                out[key] = fn(log, not args.no_cpu)
             except Exception as e:  # a failing leg is reported, not fatal to the headline
                 out[key] = {"error": repr(e)}
     if not args.no_cpu:
