@@ -2045,6 +2045,8 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
     };
     for (int64_t v = t; v < r.n; v += 256) atomicAdd(&cnt[bucket(r.score[v])], 1);
     __syncthreads();
+    // bucket starts (exclusive prefix of the counts)
+    __shared__ int32_t wcnt[4][256];
     if (t == 0) {
         int32_t run = 0;
         for (int b = 0; b < 256; ++b) {
@@ -2052,7 +2054,43 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
             cnt[b] = run;
             run += c;
         }
-        for (int64_t v = 0; v < r.n; ++v) r.perm[cnt[bucket(r.score[v])]++] = (int32_t)v;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+    __syncthreads();
+    // stable scatter, 256 nodes in index order at a time: the lanes of a wave
+    // with the same bucket are found by 8 ballots (one per bucket bit) and
+    // ranked by lane; each bucket's offsets advance wave by wave
+    const int lane = t & 63, wid = t >> 6;
+    for (int64_t c0 = 0; c0 < r.n; c0 += 256) {
+        const int64_t v = c0 + t;
+        const bool ok = v < r.n;
+        const int b = ok ? bucket(r.score[v]) : 0;
+        uint64_t same = __ballot(ok);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const uint64_t m = __ballot(ok && ((b >> bit) & 1));
+            same &= ((b >> bit) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(same & ((1ull << lane) - 1ull));
+        if (ok && rank == 0) wcnt[wid][b] = __popcll(same);
+        __syncthreads();
+        {
+            int32_t run = cnt[t];  // thread t: bucket t
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int32_t c = wcnt[w][t];
+                wcnt[w][t] = run;
+                run += c;
+            }
+            cnt[t] = run;
+        }
+        __syncthreads();
+        if (ok) r.perm[wcnt[wid][b] + rank] = (int32_t)v;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+        __syncthreads();
     }
 }
 
