@@ -153,6 +153,24 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         }
         if (keyed) key_lds = staged <= limit ? staged : (base <= limit ? base : 0);
     }
+    // register-resident descent when every instance qualifies: the largest
+    // register count and table size over the instances
+    int reg_r = 0;
+    int64_t reg_stride = 0;
+    size_t reg_lds = 0;
+    if (key_lds > 0) {
+        for (int64_t i = 0; i < count; ++i) {
+            size_t t = 0;
+            const int r = reg_descent_plan(*graphs[i], &t);
+            if (!r) {
+                reg_r = 0;
+                break;
+            }
+            reg_r = std::max(reg_r, r);
+            reg_stride = std::max(reg_stride, graphs[i]->key_adj_stride);
+            reg_lds = std::max(reg_lds, t);
+        }
+    }
     const int64_t L0 = count;
     const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
                         align256(L0 * sizeof(BInstRaw));
@@ -316,7 +334,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         while (!run.empty()) {
             const int64_t cap = 16 * max_n + 4096;
             BCHK(upload_list(run));
-            launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, all_sparse, key_lds, s);
+            launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, all_sparse, key_lds, reg_r,
+                             reg_stride, reg_lds, s);
             BCHK(read_outs(run));
             std::vector<int32_t> again;
             for (size_t t = 0; t < run.size(); ++t) {
@@ -390,7 +409,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             }
             if (!seeded.empty()) {
                 BCHK(upload_list(seeded));
-                launch_b_walk(kind, d_inst, d_list, (int64_t)seeded.size(), max_n, G, 0, 1, s);
+                launch_b_walk(kind, d_inst, d_list, (int64_t)seeded.size(), max_n, G, 0, 1, reg_r, reg_stride, reg_lds,
+                              s);
                 std::vector<int32_t> zero(seeded.size(), 0);
                 BCHK(copies(seeded, zero, 0));
                 next.insert(next.end(), seeded.begin(), seeded.end());
@@ -404,7 +424,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         std::vector<int32_t> best_walker(count, 0);
         if (!rest2.empty()) {
             BCHK(upload_list(rest2));
-            launch_b_walk(kind, d_inst, d_list, (int64_t)rest2.size(), max_n, G, -1, -1, s);
+            launch_b_walk(kind, d_inst, d_list, (int64_t)rest2.size(), max_n, G, -1, -1, reg_r, reg_stride, reg_lds,
+                          s);
             std::vector<double> all_we(rest2.size() * G);
             std::vector<int64_t> all_wev(rest2.size() * G);
             for (size_t t = 0; t < rest2.size(); ++t) {
@@ -447,7 +468,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             }
             BCHK(copies(rest3, wsrc, 1));
             BCHK(upload_list(rest3));
-            launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, all_sparse, key_lds, s);
+            launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, all_sparse, key_lds, reg_r,
+                             reg_stride, reg_lds, s);
             BCHK(read_outs(rest3));
             std::vector<int32_t> take1, improve1, rest4;
             std::vector<int64_t> mv1, mvi;

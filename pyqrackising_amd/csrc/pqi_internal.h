@@ -54,6 +54,11 @@ struct LsGraphs {
     std::vector<int64_t> key;
     hipGraphExec_t seg[2] = {nullptr, nullptr};
     bool failed = false;  // capture is not available: launch directly
+    // the iteration (without the descent when kernels are timed) and the
+    // read-back of its control record into pinned host memory, as one graph
+    // (nullptr: not available)
+    hipGraphExec_t iter = nullptr;
+    void* host_ctl = nullptr;  // pinned, sizeof(LsCtl)
 };
 }  // namespace pqi
 
@@ -221,6 +226,9 @@ size_t field_bytes(const Graph& g);  // bytes per field entry (4 or 8)
 // and the dynamic LDS it uses (adjacency staged when it fits in `limit`).
 int key_shift(const Graph& g);
 size_t key_lds_bytes(const Graph& g, size_t limit);
+// Registers per lane of the register-resident packed-key descent of g (0: it
+// does not apply); *table_bytes receives the size of its neighbour table.
+int reg_descent_plan(const Graph& g, size_t* table_bytes);
 
 // ---- batched search: per-instance descriptors in device memory ----------------
 struct BInstRaw {
@@ -258,9 +266,13 @@ struct BInstRaw {
 void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
 // all_sparse: every listed instance updates through adjacency lists (enables the
 // two-level argmax variant); key_lds > 0: every instance has key_shift > 0 and
-// the packed-key descent runs with that much dynamic LDS
+// the packed-key descent runs with that much dynamic LDS; reg_r > 0: every
+// instance takes the register-resident descent with reg_r registers per lane,
+// neighbour rows of at most reg_stride entries and neighbour tables of up to
+// reg_lds bytes (staged in LDS where they fit)
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                      int which, int all_sparse, size_t key_lds, hipStream_t s);
+                      int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
+                      hipStream_t s);
 // dynamic LDS limit of the batched packed-key descent
 size_t b_key_lds_limit();
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s);
@@ -268,8 +280,10 @@ void launch_b_flip(int kind, const BInstRaw* d, const int32_t* list, int64_t L, 
                    hipStream_t s);
 void launch_b_seeds(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_seeds, int64_t max_G,
                     hipStream_t s);
+// reg_r, reg_stride, reg_lds: as for launch_b_descent (register-resident walkers when reg_r > 0)
 void launch_b_walk(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_G,
-                   int64_t T_override, int64_t G_override, hipStream_t s);
+                   int64_t T_override, int64_t G_override, int reg_r, int64_t reg_stride, size_t reg_lds,
+                   hipStream_t s);
 // src[list position] = walker index, or -1 for the reheat state; dst 0 = best, 1 = reheat
 void launch_b_copy(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, const int32_t* src,
                    int dst, hipStream_t s);

@@ -204,6 +204,10 @@ static void ls_graphs_reset(pqi_ctx* c) {
             (void)hipGraphExecDestroy(e);
             e = nullptr;
         }
+    if (c->ls.iter) {
+        (void)hipGraphExecDestroy(c->ls.iter);
+        c->ls.iter = nullptr;
+    }
     c->ls.key.clear();
     c->ls.failed = false;
 }
@@ -268,6 +272,7 @@ int pqi_destroy(pqi_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     ls_graphs_reset(c);
+    if (c->ls.host_ctl) (void)hipHostFree(c->ls.host_ctl);
     for (auto& b : c->bufs)
         if (b.p) (void)hipFree(b.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1014,8 +1019,40 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         }
         return PQI_OK;
     };
+    // The iteration is captured once per (graph, parameters, buffers) as one
+    // graph, with the read-back of the control record into pinned memory, and
+    // replayed: one launch and one synchronisation per iteration.  Events
+    // recorded inside a graph cannot be timed here, so with kernel timing the
+    // descent is launched directly with its own events and the graph holds the
+    // rest of the iteration.  Where graph capture is not available, the two
+    // segments between the directly launched descent and walk are captured
+    // instead (also selected by PQI_LS_ITER_GRAPH=0).
+    static const bool iter_graph_enabled = [] {
+        const char* e = getenv("PQI_LS_ITER_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    if (iter_graph_enabled && !c->ls.host_ctl) {
+        if (hipHostMalloc(&c->ls.host_ctl, sizeof(LsCtl), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            c->ls.host_ctl = nullptr;
+        }
+    }
+    const bool want_iter = iter_graph_enabled && c->ls.host_ctl;
+    auto head = [&](hipEvent_t e0, hipEvent_t e1) {
+        launch_field_repair(*g, x, h, ctl, fix, c->stream);  // a new round: exact fp64 field
+        launch_ls_begin(ctl, c->stream);
+        launch_descent(*g, x, h, cap, dout, c->stream, e0, e1);
+    };
+    auto tail = [&]() -> int {
+        CHK(segment(0));
+        launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
+        CHK(segment(1));
+        HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
+        return PQI_OK;
+    };
     // captured once per (graph, parameters, buffers) and replayed per iteration
-    const std::vector<int64_t> key = {(int64_t)g, (int64_t)g->W, (int64_t)g->adj_ptr, n, G, T, bs,
+    const std::vector<int64_t> key = {(int64_t)g, (int64_t)g->W, (int64_t)g->adj_ptr, n, G, T, bs, (int64_t)timing,
+                                      (int64_t)want_iter, (int64_t)fix, (int64_t)c->ls.host_ctl,
                                       p->double_mode, p->seed_engine, p->is_spin_glass, (int64_t)x, (int64_t)h,
                                       (int64_t)rx, (int64_t)rh, (int64_t)dout, (int64_t)dout2, (int64_t)dsc,
                                       (int64_t)ctl, (int64_t)b.wx, (int64_t)b.wh, (int64_t)b.we, (int64_t)b.wev,
@@ -1024,7 +1061,22 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                                       (int64_t)c->bufs[S_RANK].p};
     if (c->ls.key != key) {
         ls_graphs_reset(c);
-        for (int w = 0; w < 2 && !c->ls.failed; ++w) {
+        if (want_iter) {
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                // a failure here only means the capture is not available: the
+                // segment path below repeats the launches and reports real errors
+                if (!timing) head(nullptr, nullptr);
+                const int r = tail();
+                const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+                if (r != PQI_OK || e != hipSuccess || !graph ||
+                    hipGraphInstantiate(&c->ls.iter, graph, nullptr, nullptr, 0) != hipSuccess)
+                    c->ls.iter = nullptr;
+                if (graph) (void)hipGraphDestroy(graph);
+            }
+            (void)hipGetLastError();
+        }
+        for (int w = 0; w < 2 && !c->ls.failed && !c->ls.iter; ++w) {
             hipGraph_t graph = nullptr;
             if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 c->ls.failed = true;
@@ -1054,6 +1106,18 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     };
     for (;;) {
         if (!hc.again && p->max_rounds > 0 && hc.rounds >= p->max_rounds) break;
+        if (c->ls.iter) {
+            if (timing) head(descent_timer.start_event(), descent_timer.stop_event());
+            HIPCHK(hipGraphLaunch(c->ls.iter, c->stream));
+            CHK(sync(c));
+            std::memcpy(&hc, c->ls.host_ctl, sizeof(hc));
+            if (timing) {
+                st->descent_kernel_seconds += descent_timer.seconds();
+                st->descent_launches++;
+            }
+            if (hc.done) break;
+            continue;
+        }
         launch_field_repair(*g, x, h, ctl, fix, c->stream);  // a new round: exact fp64 field
         launch_ls_begin(ctl, c->stream);
         launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event());

@@ -287,9 +287,10 @@ static void field_impl(const Graph& g, const uint8_t* x, void* h, hipStream_t s)
         }                                                          \
     } while (0)
 
-// Descent variant: 0 or 1 the full scan (measured faster on MI355X), 2 the
-// two-level argmax for sparse graphs, 3 the full scan without packed keys
-// for integer weights.  Read once from PQI_DESCENT_VARIANT.
+// Descent variant: 0 the register-resident packed-key descent where it
+// applies, else the full scan; 1 the full scan over keys in LDS; 2 the
+// two-level argmax for sparse graphs; 3 the full scan without packed keys for
+// integer weights.  Read once from PQI_DESCENT_VARIANT.
 static int descent_variant() {
     static const int v = [] {
         const char* e = getenv("PQI_DESCENT_VARIANT");
@@ -712,6 +713,191 @@ __global__ __launch_bounds__(64) void k_descent_key(GView<int8_t> g, uint8_t* xg
     descent_key_body(g, xg, hg, max_steps, out, lds, lds_bytes, ib);
 }
 
+// Register-resident form of the packed-key descent for graphs with a
+// fixed-stride neighbour table of at most REG_NB entries per row.  Node v's key
+// lives in register v >> 6 of lane v & 63 (n <= 64 R), so a step's maximum is
+// a v_max3 tree over R registers and one DPP reduction with no LDS access on
+// the critical path.  While the reduction runs, every lane already loads the
+// neighbour row of its own local maximum; the winning lane (bk & 63) then
+// holds the row of bk in registers.  A flip updates the neighbours one at a
+// time: the register index u >> 6 is uniform (v_movrel), and only lane u & 63
+// writes.  Same keys, selection rule, counters and result as descent_key_body.
+constexpr int REG_NB = 16;
+
+template <int R, int NB>
+__device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                 PhaseOut* out, unsigned char* lds, size_t lds_bytes, int ib) {
+    const int lane = threadIdx.x;
+    const int n = (int)g.n;
+    const int mask = (1 << ib) - 1;
+    const int sh = ib + 1;
+    const int stride = (int)g.kstride;
+    // keys in banks of at most 32 registers: a uniform dynamic index into one
+    // bank is a v_movrel (a larger private array would not stay in registers)
+    constexpr int RL = R < 32 ? R : 32;
+    int32_t klo[RL], khi[R > 32 ? R - 32 : 1];
+#define KEY(r) (*((R <= 32 || (r) < 32) ? &klo[(r) < RL ? (r) : 0] : &khi[(r) >= 32 ? (r) - 32 : 0]))
+#define KEY_APPLY(slot, mine, EXPR)                       \
+    if (R <= 32 || (slot) < 32) {                         \
+        const int32_t kv = klo[slot];                     \
+        klo[slot] = (mine) ? (EXPR) : kv;                 \
+    } else {                                              \
+        const int32_t kv = khi[(slot) - 32];              \
+        khi[(slot) - 32] = (mine) ? (EXPR) : kv;          \
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int v = r * 64 + lane;
+        if (v < n) {
+            const uint8_t xv = xg[v];
+            KEY(r) = key_of(xv ? -hg[v] : hg[v], v, xv, ib, mask);
+        } else {
+            KEY(r) = INT_MIN;
+        }
+    }
+    // the table is zero padded to whole quads; rows are zero padded after their entries
+    const int quads = (n * stride + 3) >> 2;
+    const bool staged = max_steps > 0 && (size_t)quads * 16 <= lds_bytes;
+    if (staged) {
+        const int4* src = (const int4*)g.kadj;
+        int4* dst = (int4*)lds;
+#pragma unroll 4
+        for (int q = lane; q < quads; q += 64) dst[q] = src[q];
+    }
+    __syncthreads();
+    int64_t steps = 0, evals = 0;
+    double gsum = 0.0;
+    int32_t bv;
+    int bk;
+    for (;;) {
+        int32_t m0 = KEY(0), m1 = R > 1 ? KEY(1) : INT_MIN, m2 = R > 2 ? KEY(2) : INT_MIN;
+#pragma unroll
+        for (int r = 3; r + 1 < R; r += 2) {
+            if (((r - 3) >> 1) % 3 == 0) m0 = max3i(m0, KEY(r), KEY(r + 1));
+            else if (((r - 3) >> 1) % 3 == 1) m1 = max3i(m1, KEY(r), KEY(r + 1));
+            else m2 = max3i(m2, KEY(r), KEY(r + 1));
+        }
+        if constexpr (R > 3 && (R - 3) % 2 == 1) m0 = max(m0, KEY(R - 1));
+        const int32_t lm = max3i(m0, m1, m2);
+        // neighbour row of this lane's own maximum, loaded while the wave reduces
+        // (a lane holding padding only reads row 0 and never wins); all NB
+        // loads are issued without branches or waits, entries past the stride
+        // repeat entry 0 and are never used
+        int32_t nb[NB];
+        const int c = lm == INT_MIN ? 0 : mask - ((lm >> 1) & mask);
+        if (staged) {
+            const int32_t* row = (const int32_t*)lds + c * stride;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) nb[j] = row[j < stride ? j : 0];
+        } else {
+            const int32_t* row = g.kadj + (int64_t)c * stride;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) nb[j] = row[j < stride ? j : 0];
+        }
+        const int32_t m = wave_max_i32(lm);
+        bv = m >> sh;
+        bk = mask - ((m >> 1) & mask);
+        evals += n;
+        if (!(bv > 0) || steps >= max_steps) break;
+        const uint32_t xk = (uint32_t)m & 1u;
+        const int sk2 = xk ? -2 : 2;  // 2 s_k before the flip
+        const int owner = bk & 63;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j >= stride) break;
+            const int32_t a = __builtin_amdgcn_readlane(nb[j], owner);
+            if (a == 0) break;
+            const int u = a & 0xFFFFFF;
+            const int slot = u >> 6;
+            const int sk2w = sk2 * (a >> 24);
+            KEY_APPLY(slot, lane == (u & 63),
+                      (int32_t)((uint32_t)kv - ((uint32_t)((kv & 1) ? -sk2w : sk2w) << sh)));
+        }
+        const int32_t kk = key_of(-bv, bk, xk ^ 1u, ib, mask);
+        const int kslot = bk >> 6;
+        KEY_APPLY(kslot, lane == owner, kk);
+        ++steps;
+        gsum += (double)bv;
+    }
+    if (steps) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int v = r * 64 + lane;
+            if (v < n) {
+                const int32_t k = KEY(r);
+                const int32_t d = k >> sh;
+                hg[v] = (k & 1) ? -d : d;
+                xg[v] = (uint8_t)(k & 1);
+            }
+        }
+    }
+    if (lane == 0) {
+        out->gain = (double)bv;
+        out->k = bk;
+        out->l = -1;
+        out->accepts = steps;
+        out->evals = evals;
+        out->gain_sum = gsum;
+    }
+}
+#undef KEY
+#undef KEY_APPLY
+
+template <int R, int NB>
+__global__ __launch_bounds__(64) void k_descent_reg(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                                   PhaseOut* out, size_t lds_bytes, int ib) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    descent_reg_body<R, NB>(g, xg, hg, max_steps, out, lds, lds_bytes, ib);
+}
+
+// Registers per lane of the register-resident descent for n nodes with a
+// neighbour table of `stride` entries per row, or 0 when it does not apply.
+static int reg_descent_regs(int64_t n, int64_t stride) {
+    if (stride <= 0 || stride > REG_NB) return 0;
+    for (int r = 8; r <= 64; r *= 2)
+        if (n <= 64 * r) return r;
+    return 0;
+}
+
+static size_t reg_table_bytes(const Graph& g) {
+    return (size_t)((g.n * g.key_adj_stride + 3) >> 2) * 16;
+}
+
+static int reg_descent_enabled() { return descent_variant() == 0; }
+
+int reg_descent_plan(const Graph& g, size_t* table_bytes) {
+    if (!reg_descent_enabled() || key_shift(g) <= 0 || !g.key_adj) return 0;
+    const int r = reg_descent_regs(g.n, g.key_adj_stride);
+    if (r && table_bytes) *table_bytes = reg_table_bytes(g);
+    return r;
+}
+
+// neighbour-row width of the register-resident descent: the stride rounded up to 4, 8 or 16
+static int reg_nb(int64_t stride) { return stride <= 4 ? 4 : (stride <= 8 ? 8 : 16); }
+
+#define PQI_REG_NB(R, NB, KERNEL, ...)                                   \
+    switch (NB) {                                                       \
+        case 4: KERNEL<R, 4>(__VA_ARGS__); break;                       \
+        case 8: KERNEL<R, 8>(__VA_ARGS__); break;                       \
+        default: KERNEL<R, 16>(__VA_ARGS__); break;                     \
+    }
+#define PQI_REG_DISPATCH(R, NB, KERNEL, ...)                             \
+    switch (R) {                                                        \
+        case 8: PQI_REG_NB(8, NB, KERNEL, __VA_ARGS__); break;          \
+        case 16: PQI_REG_NB(16, NB, KERNEL, __VA_ARGS__); break;        \
+        case 32: PQI_REG_NB(32, NB, KERNEL, __VA_ARGS__); break;        \
+        default: PQI_REG_NB(64, NB, KERNEL, __VA_ARGS__); break;        \
+    }
+
+template <int R, int NB>
+static void launch_reg(const GView<int8_t>& v, uint8_t* x, int32_t* h, int64_t max_steps, PhaseOut* out,
+                       size_t table, int ib, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    static const size_t limit = lds_limit((const void*)k_descent_reg<R, NB>);
+    const size_t lds = table <= limit ? table : 0;
+    hipExtLaunchKernelGGL((k_descent_reg<R, NB>), dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, v, x, h,
+                          max_steps, out, lds, ib);
+}
+
 // Key width for the packed-key descent of g, or 0 when it does not apply.
 int key_shift(const Graph& g) {
     if (g.kind != PQI_WEIGHTS_I8 || g.max_strength < 0 || g.n > (1 << 24)) return 0;
@@ -869,6 +1055,13 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
                          hipEvent_t e0, hipEvent_t e1) {
     using FT = typename FieldOf<WT>::T;
     if constexpr (sizeof(WT) == 1) {
+        size_t table = 0;
+        const int regs = reg_descent_plan(g, &table);
+        if (regs) {
+            PQI_REG_DISPATCH(regs, reg_nb(g.key_adj_stride), launch_reg, view<int8_t>(g), x, (int32_t*)h, max_steps,
+                             out, table, key_shift(g), s, e0, e1);
+            return;
+        }
         static const size_t klimit = lds_limit((const void*)k_descent_key);
         const int ib = key_shift(g);
         const size_t kl = key_lds_bytes(g, klimit);
@@ -1330,6 +1523,29 @@ void launch_topk(const double* v, int64_t count, int64_t k, int64_t* sel, void* 
 // The walker's state lives in LDS when it fits, else in its global slot.
 // ---------------------------------------------------------------------------
 
+// Skipping rule of the Gray walkers.  Iteration i (i + 1 = m) evaluates the
+// flips at position ctz(m); `clean` has bit p set when every flip at position
+// p has been evaluated and rejected since the walker last changed, so those
+// iterations repeat exactly and are skipped.  Returns the smallest m >= v whose
+// position is not clean, or 0 when there is none up to T (pmax = floor(log2 T)).
+// Positions below p0 = ctz(~clean) are all clean, so the answer is m0, the
+// smallest odd multiple of 2^p0 that is >= v, unless the one multiple of 2^p0
+// below it, t = m0 - 2^p0, is >= v and sits at a position that is not clean
+// (t is an even multiple of 2^p0, so its position is above p0).
+__device__ __forceinline__ uint64_t next_unclean(uint64_t v, uint64_t clean, int pmax, uint64_t T) {
+    const uint64_t nc = ~clean;
+    if (!nc) return 0;
+    const int p0 = __builtin_ctzll(nc);
+    if (p0 > pmax) return 0;
+    const uint64_t step = 1ull << p0;
+    uint64_t m0 = (v + step - 1) & ~(step - 1);
+    if (!(m0 & step)) m0 += step;
+    uint64_t best = m0;
+    const uint64_t t = m0 - step;
+    if (t >= v && t != 0 && ((nc >> __builtin_ctzll(t)) & 1u)) best = t;
+    return best <= T ? best : 0;
+}
+
 template <typename WT, typename FT>
 __device__ void gray_walk_body(const GView<WT>& g, const uint8_t* x0, const FT* h0, const int64_t* sel,
                                const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx, FT* wh, double* we,
@@ -1373,18 +1589,9 @@ __device__ void gray_walk_body(const GView<WT>& g, const uint8_t* x0, const FT* 
     for (int64_t ci = 0; ci < T;) {
         const int p = __builtin_ctzll((unsigned long long)(ci + 1));
         if ((clean >> p) & 1u) {
-            // next iteration index whose p is not clean: i+1 = q 2^p with q odd
-            const uint64_t v = (uint64_t)ci + 1;
-            uint64_t best = ~0ull;
-            for (int pp = 0; pp <= pmax; ++pp) {
-                if ((clean >> pp) & 1u) continue;
-                uint64_t q = (v + ((1ull << pp) - 1)) >> pp;
-                q |= 1ull;
-                const uint64_t u = q << pp;
-                if (u < best) best = u;
-            }
-            if (best == ~0ull || best > (uint64_t)T) break;
-            ci = (int64_t)best - 1;
+            const uint64_t next = next_unclean((uint64_t)ci + 1, clean, pmax, (uint64_t)T);
+            if (!next) break;
+            ci = (int64_t)next - 1;
             continue;
         }
         bool changed = false;
@@ -1427,6 +1634,145 @@ __device__ void gray_walk_body(const GView<WT>& g, const uint8_t* x0, const FT* 
     }
 }
 
+// Register-resident Gray walker for int8 weights with a fixed-stride neighbour
+// table (n <= 2048, rows of at most NB entries).  Node v lives in register
+// v & 31 of lane (v >> 6) + 32 ((v >> 5) & 1): lanes 0-31 hold the low halves
+// of the 64-node blocks, lanes 32-63 the high halves, so the candidates of an
+// iteration at position p (nodes p + 64 b) are register p & 31 of the lanes of
+// one half, in block order: one uniform v_movrel read, no memory access.
+// word = (Delta << 1) | x.  A flip updates the neighbours one at a time (the
+// row comes from the table, staged in LDS when it fits).  Same iteration
+// order, acceptance rule, skipping rule, energy and evaluation count as
+// gray_walk_body.  The host enables it only when every row strength is below
+// 2^29, so Delta << 1 fits.
+__device__ __forceinline__ int walk_lane_of(int v) { return (v >> 6) | (((v >> 5) & 1) << 5); }
+
+template <int NB>
+__device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, const int32_t* h0, const int64_t* sel,
+                                   const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx, int32_t* wh,
+                                   double* we, int64_t* wevals, unsigned char* lds, size_t lds_bytes, int64_t w) {
+    const int lane = threadIdx.x;
+    const int n = (int)g.n;
+    const int nb = (n + 63) >> 6;
+    const int stride = (int)g.kstride;
+    const int base = ((lane & 31) << 6) | ((lane >> 5) << 5);  // node of register 0 in this lane
+    int32_t wr[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        const int v = base + r;
+        int32_t word = 0;
+        if ((lane & 31) < nb && v < n) {
+            const int32_t xv = x0[v];
+            word = (int32_t)((uint32_t)(xv ? -h0[v] : h0[v]) << 1) | xv;
+        }
+        wr[r] = word;
+    }
+    const int quads = (n * stride + 3) >> 2;
+    const bool staged = (size_t)quads * 16 <= lds_bytes;
+    if (staged) {
+        const int4* src = (const int4*)g.kadj;
+        int4* dst = (int4*)lds;
+#pragma unroll 4
+        for (int q = lane; q < quads; q += 64) dst[q] = src[q];
+    }
+    __syncthreads();
+    // flip node K (register K & 31 of lane walk_lane_of(K)); SK2 = 2 s_K before the flip
+#define WFLIP(K, SK2)                                                                                        \
+    {                                                                                                      \
+        const int fk = (K), fsk2 = (SK2);                                                                  \
+        const int col = lane < stride ? lane : 0;                                                          \
+        const int32_t a0 = staged ? ((const int32_t*)lds)[fk * stride + col] : g.kadj[(int64_t)fk * stride + col]; \
+        _Pragma("unroll") for (int j = 0; j < NB; ++j) {                                                   \
+            if (j >= stride) break;                                                                        \
+            const int32_t a = __builtin_amdgcn_readlane(a0, j);                                           \
+            if (a == 0) break;                                                                             \
+            const int u = a & 0xFFFFFF;                                                                    \
+            const int slot = u & 31;                                                                       \
+            const int sk2w = fsk2 * (a >> 24);                                                             \
+            const int32_t kv = wr[slot];                                                                   \
+            const int32_t nv = (int32_t)((uint32_t)kv - ((uint32_t)((kv & 1) ? -sk2w : sk2w) << 1));        \
+            wr[slot] = lane == walk_lane_of(u) ? nv : kv;                                                  \
+        }                                                                                                  \
+        const int kslot = fk & 31;                                                                         \
+        const int32_t kv = wr[kslot];                                                                      \
+        const int32_t nv = (int32_t)(((uint32_t)(-(kv >> 1)) << 1) | ((kv & 1) ^ 1));                      \
+        wr[kslot] = lane == walk_lane_of(fk) ? nv : kv;                                                    \
+    }
+
+    // seed mutation
+    const int64_t seed = sel[w];
+    const uint64_t si = (uint64_t)(seed % bs);
+    const uint64_t gray = si ^ (si >> 1);
+    const int64_t offset = (seed / bs) << 6;
+    for (int j = 0; j < 64 && j < n; ++j) {
+        if (!((gray >> j) & 1u)) continue;
+        const int k = (int)seed_pos(n, offset, j);
+        const int32_t wk = __builtin_amdgcn_readlane(wr[k & 31], walk_lane_of(k));
+        WFLIP(k, (wk & 1) ? -2 : 2);
+    }
+
+    double energy = sel_delta[w];
+    int64_t evals = 0;
+    // clean bit p: as in gray_walk_body
+    uint64_t clean = 0;
+    const int pmax = 63 - __builtin_clzll((unsigned long long)(T > 0 ? T : 1));
+    for (int64_t ci = 0; ci < T;) {
+        const int p = __builtin_ctzll((unsigned long long)(ci + 1));
+        if ((clean >> p) & 1u) {
+            const uint64_t next = next_unclean((uint64_t)ci + 1, clean, pmax, (uint64_t)T);
+            if (!next) break;
+            ci = (int64_t)next - 1;
+            continue;
+        }
+        const int ps = p & 31;
+        const bool valid = (lane >> 5) == (p >> 5) && (lane & 31) < nb && base + ps < n;
+        int32_t word = wr[ps];
+        evals += __popcll(__ballot(valid));
+        uint64_t mask = __ballot(valid && (word >> 1) > 0);
+        bool changed = false;
+        double gain_i = 0.0;
+        while (mask) {
+            const int bl = __builtin_ctzll(mask);
+            const int32_t wk = __builtin_amdgcn_readlane(word, bl);
+            gain_i += (double)(wk >> 1);
+            changed = true;
+            WFLIP(p + 64 * (bl & 31), (wk & 1) ? -2 : 2);
+            word = wr[ps];
+            mask = __ballot(valid && (word >> 1) > 0 && lane > bl);
+        }
+        energy += gain_i;
+        clean = changed ? 0ull : (clean | (1ull << p));
+        ++ci;
+    }
+#undef WFLIP
+    uint8_t* gx = wx + w * n;
+    int32_t* gh = wh + w * n;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        const int v = base + r;
+        if ((lane & 31) < nb && v < n) {
+            const int32_t word = wr[r];
+            const int32_t d = word >> 1;
+            gh[v] = (word & 1) ? -d : d;
+            gx[v] = (uint8_t)(word & 1);
+        }
+    }
+    if (lane == 0) {
+        we[w] = energy;
+        wevals[w] = evals;
+    }
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_gray_walk_reg(GView<int8_t> g, const uint8_t* x0, const int32_t* h0,
+                                                     const int64_t* sel, const double* sel_delta, int64_t bs,
+                                                     int64_t T, uint8_t* wx, int32_t* wh, double* we,
+                                                     int64_t* wevals, size_t lds_bytes, const int32_t* gate) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
+    gray_walk_reg_body<NB>(g, x0, h0, sel, sel_delta, bs, T, wx, wh, we, wevals, lds, lds_bytes, blockIdx.x);
+}
+
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0, const FT* h0, const int64_t* sel,
                                                  const double* sel_delta, int64_t bs, int64_t T, uint8_t* wx,
@@ -1437,11 +1783,35 @@ __global__ __launch_bounds__(64) void k_gray_walk(GView<WT> g, const uint8_t* x0
     gray_walk_body<WT, FT>(g, x0, h0, sel, sel_delta, bs, T, wx, wh, we, wevals, use_lds, lds, blockIdx.x);
 }
 
+template <int NB>
+static void launch_walk_reg(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel,
+                            const double* sel_delta, int64_t bs, int64_t walkers, int64_t T, uint8_t* wx, void* wh,
+                            double* we, int64_t* wevals, size_t table, hipStream_t s, const int32_t* gate) {
+    static const size_t limit = lds_limit((const void*)k_gray_walk_reg<NB>);
+    const size_t lds = table <= limit ? table : 0;
+    hipLaunchKernelGGL(k_gray_walk_reg<NB>, dim3((unsigned)walkers), dim3(64), lds, s, view<int8_t>(g), x,
+                       (const int32_t*)h, sel, sel_delta, bs, T, wx, (int32_t*)wh, we, wevals, lds, gate);
+}
+
 template <typename WT>
 static void walk_impl(const Graph& g, const uint8_t* x, const void* h, const int64_t* sel, const double* sel_delta,
                       int64_t bs, int64_t walkers, int64_t T, uint8_t* wx, void* wh, double* we, int64_t* wevals,
                       hipStream_t s, const int32_t* gate) {
     using FT = typename FieldOf<WT>::T;
+    if constexpr (sizeof(WT) == 1) {
+        size_t table = 0;
+        if (g.n <= 2048 && reg_descent_plan(g, &table)) {
+            switch (reg_nb(g.key_adj_stride)) {
+                case 4: launch_walk_reg<4>(g, x, h, sel, sel_delta, bs, walkers, T, wx, wh, we, wevals, table, s,
+                                           gate); break;
+                case 8: launch_walk_reg<8>(g, x, h, sel, sel_delta, bs, walkers, T, wx, wh, we, wevals, table, s,
+                                           gate); break;
+                default: launch_walk_reg<16>(g, x, h, sel, sel_delta, bs, walkers, T, wx, wh, we, wevals, table, s,
+                                             gate); break;
+            }
+            return;
+        }
+    }
     size_t lds = ((g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
     static const size_t limit = lds_limit((const void*)k_gray_walk<WT, FT>);
     int use_lds = lds <= limit ? 1 : 0;
@@ -1756,6 +2126,24 @@ size_t b_key_lds_limit() {
     return limit;
 }
 
+template <int R, int NB>
+__global__ __launch_bounds__(64) void kb_descent_reg(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
+                                                    int which, size_t lds_bytes) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    descent_reg_body<R, NB>(gview_of<int8_t>(r), which ? r.rx : r.x, (int32_t*)(which ? r.rh : r.h), max_steps,
+                            r.out, lds, lds_bytes, r.key_shift);
+}
+
+template <int R, int NB>
+static void launch_b_reg(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_steps, int which,
+                         size_t table, hipStream_t s) {
+    static const size_t limit = lds_limit((const void*)kb_descent_reg<R, NB>);
+    const size_t lds = table <= limit ? table : 0;
+    hipLaunchKernelGGL((kb_descent_reg<R, NB>), dim3((unsigned)L), dim3(64), lds, s, d, list, max_steps, which,
+                       lds);
+}
+
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void kb_descent_lds2(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
                                                      int which) {
@@ -1825,6 +2213,27 @@ __global__ __launch_bounds__(64) void kb_walk(const BInstRaw* insts, const int32
                            r.wev, use_lds, lds, blockIdx.x);
 }
 
+template <int NB>
+__global__ __launch_bounds__(64) void kb_walk_reg(const BInstRaw* insts, const int32_t* list, int64_t T_override,
+                                                 int64_t G_override, size_t lds_bytes) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.y]];
+    const int64_t G = G_override > 0 ? G_override : r.G;
+    if ((int64_t)blockIdx.x >= G) return;
+    const int64_t T = T_override >= 0 ? T_override : r.T;
+    gray_walk_reg_body<NB>(gview_of<int8_t>(r), r.x, (const int32_t*)r.h, r.sel, r.seld, r.bs, T, r.wx,
+                           (int32_t*)r.wh, r.we, r.wev, lds, lds_bytes, blockIdx.x);
+}
+
+template <int NB>
+static void launch_b_walk_reg(const BInstRaw* d, const int32_t* list, int64_t L, int64_t gx, int64_t T_override,
+                              int64_t G_override, size_t table, hipStream_t s) {
+    static const size_t limit = lds_limit((const void*)kb_walk_reg<NB>);
+    const size_t lds = table <= limit ? table : 0;
+    hipLaunchKernelGGL(kb_walk_reg<NB>, dim3((unsigned)gx, (unsigned)L), dim3(64), lds, s, d, list, T_override,
+                       G_override, lds);
+}
+
 // copy a state: src >= 0 -> walker src, src == -1 -> reheat state; dst 0 -> best, 1 -> reheat
 template <typename FT>
 __global__ __launch_bounds__(256) void kb_copy(const BInstRaw* insts, const int32_t* list, const int32_t* src,
@@ -1881,8 +2290,13 @@ static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int6
 
 template <typename WT>
 static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                           int which, int all_sparse, size_t key_lds, hipStream_t s) {
+                           int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
+                           hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
+    if (sizeof(WT) == 1 && reg_r > 0) {
+        PQI_REG_DISPATCH(reg_r, reg_nb(reg_stride), launch_b_reg, d, list, L, max_steps, which, reg_lds, s);
+        return;
+    }
     if (sizeof(WT) == 1 && key_lds > 0 && descent_variant() != 3) {
         hipLaunchKernelGGL(kb_descent_key, dim3((unsigned)L), dim3(64), key_lds, s, d, list, max_steps, which,
                            key_lds);
@@ -1927,12 +2341,21 @@ static void b_seeds_impl(const BInstRaw* d, const int32_t* list, int64_t L, int6
 
 template <typename WT>
 static void b_walk_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_G,
-                        int64_t T_override, int64_t G_override, hipStream_t s) {
+                        int64_t T_override, int64_t G_override, int reg_r, int64_t reg_stride, size_t reg_lds,
+                        hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
+    const int64_t gx = G_override > 0 ? G_override : max_G;
+    if (sizeof(WT) == 1 && reg_r > 0 && max_n <= 2048) {
+        switch (reg_nb(reg_stride)) {
+            case 4: launch_b_walk_reg<4>(d, list, L, gx, T_override, G_override, reg_lds, s); break;
+            case 8: launch_b_walk_reg<8>(d, list, L, gx, T_override, G_override, reg_lds, s); break;
+            default: launch_b_walk_reg<16>(d, list, L, gx, T_override, G_override, reg_lds, s); break;
+        }
+        return;
+    }
     const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
     static const size_t limit = lds_limit((const void*)kb_walk<WT, FT>);
     const int use_lds = lds <= limit ? 1 : 0;
-    const int64_t gx = G_override > 0 ? G_override : max_G;
     hipLaunchKernelGGL((kb_walk<WT, FT>), dim3((unsigned)gx, (unsigned)L), dim3(64), use_lds ? lds : 0, s, d, list,
                        T_override, G_override, use_lds);
 }
@@ -1964,8 +2387,10 @@ void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
     PQI_KIND_DISPATCH(kind, b_field_impl, d, list, L, max_n, s);
 }
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
-                      int which, int all_sparse, size_t key_lds, hipStream_t s) {
-    PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, all_sparse, key_lds, s);
+                      int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
+                      hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, all_sparse, key_lds, reg_r,
+                      reg_stride, reg_lds, s);
 }
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s) {
     PQI_KIND_DISPATCH(kind, b_double_impl, d, list, L, which, s);
@@ -1979,8 +2404,10 @@ void launch_b_seeds(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
     PQI_KIND_DISPATCH(kind, b_seeds_impl, d, list, L, max_seeds, max_G, s);
 }
 void launch_b_walk(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_G,
-                   int64_t T_override, int64_t G_override, hipStream_t s) {
-    PQI_KIND_DISPATCH(kind, b_walk_impl, d, list, L, max_n, max_G, T_override, G_override, s);
+                   int64_t T_override, int64_t G_override, int reg_r, int64_t reg_stride, size_t reg_lds,
+                   hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_walk_impl, d, list, L, max_n, max_G, T_override, G_override, reg_r, reg_stride, reg_lds,
+                      s);
 }
 void launch_b_copy(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, const int32_t* src,
                    int dst, hipStream_t s) {

@@ -82,6 +82,22 @@ def test_descent_dense_int(n, wmax, seed):
     assert np.array_equal(x1, xr)
 
 
+@pytest.mark.parametrize("n,d,wmax,seed", [(300, 3, 1, 11), (1000, 5, 2, 12), (2000, 7, 1, 13), (4096, 3, 1, 14),
+                                            (4000, 12, 1, 15), (4096, 16, 3, 16), (64, 3, 1, 17)])
+def test_descent_register_resident(n, d, wmax, seed):
+    """Packed keys held in registers (n <= 4096, neighbour rows of at most 16
+    entries): 8 to 64 registers per lane, rows of 4, 8 and 16 entries, the
+    neighbour table staged in LDS or, above 160 KiB, read from global memory."""
+    log(f"register descent n={n} d={d} wmax={wmax}")
+    G, A = regular_int(n, d, seed, wmax)
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = np.random.default_rng(seed + 10).integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, False)
+    xr, fr, gr = steepest(A, x0)
+    assert acc == fr and gain == gr
+    assert np.array_equal(x1, xr)
+
+
 def test_descent_sparse_real():
     log("descent sparse real")
     rng = np.random.default_rng(7)
@@ -152,6 +168,26 @@ def test_batched_staging_mix_matches_single():
         xs, st = dg.local_search(starts[i], p)
         assert np.array_equal(states[i], xs), i
         assert stats[i]["cut_value"] == st["cut_value"], i
+
+
+@pytest.mark.parametrize("sizes", [((300, 3), (1000, 7), (2048, 3), (4000, 12)), ((64, 3), (700, 5), (2048, 13))])
+def test_batched_register_descent_matches_single(sizes):
+    """A batch whose instances all take the register-resident descent, with
+    different sizes and row widths (one table too large to stage; with every
+    n <= 2048 the register-resident walkers too): every instance equals its
+    single-instance search, which the parity tests check against the oracle."""
+    log(f"batched register descent {sizes}")
+    Gs = [regular_int(n, d, 80 + i)[0] for i, (n, d) in enumerate(sizes)]
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    rng = np.random.default_rng(81)
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in Gs]
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=2000)
+    states, stats = _lib.batch_local_search(dgs, starts, p)
+    for i, dg in enumerate(dgs):
+        xs, st = dg.local_search(starts[i], p)
+        assert np.array_equal(states[i], xs), i
+        assert stats[i]["cut_value"] == st["cut_value"], i
+        assert stats[i]["single_accepts"] == st["single_accepts"], i
 
 
 COUNTERS = ("max_energy", "cut_value", "min_energy", "rounds", "single_accepts", "double_accepts", "seed_accepts",

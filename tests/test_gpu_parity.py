@@ -265,6 +265,57 @@ def test_gray_search(ctx, orc, engine, n):
     assert np.array_equal(state, it[bw])
 
 
+def _walk_start(orc, W, seed, count=4):
+    """A single-flip local optimum with `count` non-adjacent nodes flipped away
+    from it.  The nodes sit at block positions 0-5 (reached by a short walk) and
+    away from the nodes the Gray seeds mutate and their neighbours, so no seed
+    improves and the walkers must find the moves back."""
+    n = W.shape[0]
+    rng = np.random.default_rng(seed)
+    x = rng.integers(0, 2, n).astype(np.uint8)
+    while True:
+        g, k = orc.single_bit_flips(x, 1, W)
+        if not g > 0:
+            break
+        x[k] ^= 1
+    seeded = {(n - 1 - j - 64 * o) % n for o in range((n + 63) // 64) for j in range(4)}
+    near = set(seeded)
+    for v in seeded:
+        near.update(np.nonzero(W[v])[0].tolist())
+    chosen = []
+    for v in (int(v) for v in rng.permutation(n) if v not in near and (v & 63) < 6):
+        if len(chosen) == count:
+            break
+        if all(W[v, u] == 0 for u in chosen):
+            chosen.append(v)
+    for v in chosen:
+        x[v] ^= 1
+    return x
+
+
+@pytest.mark.parametrize("n,d", [(64, 3), (1030, 6), (2048, 3), (2048, 13), (500, 16)])
+def test_gray_search_register_walkers(ctx, orc, n, d):
+    """Walkers that climb: regular +-1 graphs with neighbour rows of 4, 8 and 16
+    entries and n <= 2048 (register-resident walkers; a partial last block at
+    n=1030).  The start is a perturbed local optimum that no seed improves, so
+    the walk phase itself must take the moves: gains and the best walker's
+    state against the oracle."""
+    log(f"gray search register walkers n={n} d={d}")
+    W = regular_pm1(n, d, seed=90 + d).astype(np.float64)
+    x = _walk_start(orc, W, 91)
+    walkers, mult, iters = 4, 1, 4000
+    dg = _lib.DeviceGraph.dense(W.astype(np.int8))
+    p = _lib.make_params(n, is_spin_glass=1, gray_iterations=iters, walkers=walkers, seed_multiple=mult,
+                         seed_engine=_lib.SEEDS_MFMA)
+    sg_gain, wg_gain, state = dg.gray_search(x, p)
+    seeds, en, idx = orc.pick_gray_seeds(x, walkers, mult, W)
+    assert en[0] <= 0 and sg_gain == 2 * en[0]
+    bw, gain, it, en2, _ = orc.gray_optimization(seeds, en, iters, walkers, 1, W)
+    assert gain > 0
+    assert wg_gain == gain
+    assert np.array_equal(state, it[bw])
+
+
 # ---------------------------------------------------------------------------
 # the whole local search
 # ---------------------------------------------------------------------------
