@@ -203,6 +203,8 @@ struct LsCtl {
     uint64_t ticks[5];  // per phase: single, double, seeds, walk, reheat
 };
 void launch_ls_begin(LsCtl* c, hipStream_t s);
+// max_energy = score[1] (E, spin glass) or score[0] (cut): the exact score of the start
+void launch_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass, hipStream_t s);
 void launch_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap, hipStream_t s);
 void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, int64_t evals,
                             hipStream_t s);

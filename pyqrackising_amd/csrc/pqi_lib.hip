@@ -273,6 +273,7 @@ int pqi_destroy(pqi_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     ls_graphs_reset(c);
     if (c->ls.host_ctl) (void)hipHostFree(c->ls.host_ctl);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     for (auto& b : c->bufs)
         if (b.p) (void)hipFree(b.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -492,6 +493,21 @@ static int load_state(pqi_graph* g, const uint8_t* theta, uint8_t** x, void** h)
         if (theta[i] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
     HIPCHK(hipMemcpyAsync(*x, theta, g->n, hipMemcpyHostToDevice, c->stream));
     launch_field(*g, *x, *h, c->stream);
+    return PQI_OK;
+}
+
+// Pinned host staging of at least `bytes` (grown on demand; freed by pqi_destroy).
+static int ensure_pinned(pqi_ctx* c, size_t bytes) {
+    if (c->pinned_bytes >= bytes) return PQI_OK;
+    if (c->pinned) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinned_bytes = 0;
+    }
+    const size_t want = std::max<size_t>(bytes, 4096);
+    HIPCHK(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+    c->pinned_bytes = want;
     return PQI_OK;
 }
 
@@ -928,6 +944,14 @@ int pqi_gray_search(pqi_graph* g, const uint8_t* theta, const pqi_search_params*
     return sync(c);
 }
 
+static void field_repair_stats(const FieldRepair& f, pqi_search_stats* st) {
+    st->field_checks = (int64_t)f.checks;
+    st->field_repairs = (int64_t)f.entries;
+    double d;
+    std::memcpy(&d, &f.max_bits, sizeof(d));
+    st->max_field_drift = d;
+}
+
 static int read_field_repair(pqi_ctx* c, const FieldRepair* fix, pqi_search_stats* st) {
     FieldRepair f;
     HIPCHK(hipMemcpyAsync(&f, fix, sizeof(f), hipMemcpyDeviceToHost, c->stream));
@@ -981,14 +1005,19 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     CHK(ensure(c, S_WEV, G * 8, (void**)&b.wev));
     CHK(gray_seeds_buffers(g, p, b));
 
-    double cut0, e0;
-    CHK(score_device(g, x, &cut0, &e0));
+    // the starting max_energy is the exact score of the start, set on the
+    // device (no synchronisation before the first iteration)
+    double* score;
+    void* hb;
+    CHK(ensure(c, S_SCORE, 16, (void**)&score));
+    CHK(ensure(c, S_HB, n * fb, &hb));
     LsCtl hc;
     std::memset(&hc, 0, sizeof(hc));
-    hc.max_energy = p->is_spin_glass ? e0 : cut0;
     hc.scale = p->is_spin_glass ? 2.0 : 1.0;
     hc.phase = -1;
     HIPCHK(hipMemcpyAsync(ctl, &hc, sizeof(hc), hipMemcpyHostToDevice, c->stream));
+    launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);
+    launch_ls_start_energy(ctl, score, p->is_spin_glass, c->stream);
     // the reheat state is read by speculative launches before its first use
     HIPCHK(hipMemcpyAsync(rx, x, n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(rh, h, n * fb, hipMemcpyDeviceToDevice, c->stream));
@@ -1160,13 +1189,24 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     st->seconds_seeds = hc.ticks[2] * tick;
     st->seconds_walk = hc.ticks[3] * tick;
     st->seconds_reheat = hc.ticks[4] * tick;
-    CHK(read_field_repair(c, fix, st));
-    double cut, en;
-    CHK(score_device(g, x, &cut, &en));
-    st->cut_value = cut;
-    st->min_energy = p->is_spin_glass ? -en : en;
-    HIPCHK(hipMemcpyAsync(theta, x, n, hipMemcpyDeviceToHost, c->stream));
-    return sync(c);
+    // field repairs, the exact score and the state come back in one
+    // synchronisation, through pinned memory
+    launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);
+    CHK(ensure_pinned(c, 64 + (size_t)n));
+    char* pb = (char*)c->pinned;
+    HIPCHK(hipMemcpyAsync(pb, fix, sizeof(FieldRepair), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(pb + 32, score, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(pb + 64, x, n, hipMemcpyDeviceToHost, c->stream));
+    CHK(sync(c));
+    FieldRepair f;
+    std::memcpy(&f, pb, sizeof(f));
+    field_repair_stats(f, st);
+    double hv[2];
+    std::memcpy(hv, pb + 32, 16);
+    st->cut_value = hv[0];
+    st->min_energy = p->is_spin_glass ? -hv[1] : hv[1];
+    std::memcpy(theta, pb + 64, (size_t)n);
+    return PQI_OK;
 }
 
 int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st) {

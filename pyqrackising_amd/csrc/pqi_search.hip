@@ -2015,6 +2015,14 @@ __global__ __launch_bounds__(1024) void k_ls_reheat_double(GView<WT> g, LsCtl* c
 
 void launch_ls_begin(LsCtl* c, hipStream_t s) { hipLaunchKernelGGL(k_ls_begin, dim3(1), dim3(1), 0, s, c); }
 
+__global__ void k_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass) {
+    c->max_energy = is_spin_glass ? score[1] : score[0];
+}
+
+void launch_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass, hipStream_t s) {
+    hipLaunchKernelGGL(k_ls_start_energy, dim3(1), dim3(1), 0, s, c, score, is_spin_glass);
+}
+
 void launch_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap, hipStream_t s) {
     hipLaunchKernelGGL(k_ls_after_descent, dim3(1), dim3(1), 0, s, c, o, cap);
 }
