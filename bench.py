@@ -9,9 +9,10 @@ blocks) scored by the int8 MFMA contraction; gray_iterations = n^2.
 Prints one JSON line:
   value = flip evaluations per second of the whole local search (single-flip
   gains, pair gains and Gray steps; each is one exact gain evaluation),
-  roofline = the dominant kernel (the single-flip descent, k_descent_key)
-  against HBM bandwidth; its duration comes from HIP events the library
-  records around each descent launch on its own stream,
+  roofline = the dominant kernel (the single-flip descent, k_descent_reg,
+  packed keys in registers) against HBM bandwidth; its duration comes from
+  HIP events the library records around each descent launch on its own
+  stream,
   cpu_baseline = the CPU oracle (a C restatement of the reference CPU path with
   the reference's O(n)-per-evaluation cost) on a bounded sample of the same
   instance, timed on this host.
@@ -28,9 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_descent_key launch on this workload, from rocprofv3 FETCH_SIZE
+# HBM bytes per k_descent_reg launch on this workload, from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE passes scaled by a plain copy of known size (see the file)
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_descent_key_c2_n2048.md")
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_descent_reg_c2_n2048.md")
 
 
 def measured_traffic():
@@ -380,7 +381,7 @@ def main():
     value = total_evals / dt
     ms_per_step = dt / args.steps * 1e3
 
-    # dominant kernel: the single-flip descent (k_descent_key).  Algorithmic
+    # dominant kernel: the single-flip descent (k_descent_reg).  Algorithmic
     # bytes per flip evaluation: the gain (4 B int32) and the spin (1 B); per
     # accepted flip, 16 B per neighbour (index and weight read, gain read and
     # written) -- degree 3 here.
@@ -424,14 +425,14 @@ def main():
                                                   "seconds_walk", "seconds_reheat")},
         },
         "roofline": {
-            "kernel": "k_descent_key",
+            "kernel": "k_descent_reg<32, 4>",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(),
-            "traffic_source": "profiles/k_descent_key_c2_n2048.md",
+            "traffic_source": "profiles/k_descent_reg_c2_n2048.md",
             "avg_launch_ms": avg_launch_s * 1e3,
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },

@@ -737,13 +737,13 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
     constexpr int RL = R < 32 ? R : 32;
     int32_t klo[RL], khi[R > 32 ? R - 32 : 1];
 #define KEY(r) (*((R <= 32 || (r) < 32) ? &klo[(r) < RL ? (r) : 0] : &khi[(r) >= 32 ? (r) - 32 : 0]))
-#define KEY_APPLY(slot, mine, EXPR)                       \
-    if (R <= 32 || (slot) < 32) {                         \
-        const int32_t kv = klo[slot];                     \
-        klo[slot] = (mine) ? (EXPR) : kv;                 \
-    } else {                                              \
-        const int32_t kv = khi[(slot) - 32];              \
-        khi[(slot) - 32] = (mine) ? (EXPR) : kv;          \
+#define KEY_APPLY(slot, mine, EXPR)                                        \
+    if (R <= 32 || (slot) < 32) {                                          \
+        const int32_t kv = klo[slot];                                      \
+        klo[slot] = kv ^ ((kv ^ (EXPR)) & -(int32_t)(mine));               \
+    } else {                                                               \
+        const int32_t kv = khi[(slot) - 32];                               \
+        khi[(slot) - 32] = kv ^ ((kv ^ (EXPR)) & -(int32_t)(mine));        \
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -765,8 +765,11 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
         for (int q = lane; q < quads; q += 64) dst[q] = src[q];
     }
     __syncthreads();
-    int64_t steps = 0, evals = 0;
-    double gsum = 0.0;
+    // step counters in 32 bits (a descent makes far fewer than 2^31 moves);
+    // every pass of the loop scans all n gains, so evals = n (steps + 1)
+    const int32_t step_cap = max_steps < INT_MAX ? (int32_t)max_steps : INT_MAX;
+    int32_t steps = 0;
+    int64_t gsum = 0;
     int32_t bv;
     int bk;
     for (;;) {
@@ -797,8 +800,7 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
         const int32_t m = wave_max_i32(lm);
         bv = m >> sh;
         bk = mask - ((m >> 1) & mask);
-        evals += n;
-        if (!(bv > 0) || steps >= max_steps) break;
+        if (!(bv > 0) || steps >= step_cap) break;
         const uint32_t xk = (uint32_t)m & 1u;
         const int sk2 = xk ? -2 : 2;  // 2 s_k before the flip
         const int owner = bk & 63;
@@ -817,7 +819,7 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
         const int kslot = bk >> 6;
         KEY_APPLY(kslot, lane == owner, kk);
         ++steps;
-        gsum += (double)bv;
+        gsum += bv;
     }
     if (steps) {
 #pragma unroll
@@ -836,8 +838,8 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
         out->k = bk;
         out->l = -1;
         out->accepts = steps;
-        out->evals = evals;
-        out->gain_sum = gsum;
+        out->evals = (int64_t)n * (steps + 1);
+        out->gain_sum = (double)gsum;
     }
 }
 #undef KEY
@@ -1677,14 +1679,14 @@ __device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, co
     }
     __syncthreads();
     // flip node K (register K & 31 of lane walk_lane_of(K)); SK2 = 2 s_K before the flip
-#define WFLIP(K, SK2)                                                                                        \
+    // apply the flip of node K whose neighbour row entry j is ROW(j) (uniform);
+    // SK2 = 2 s_K before the flip
+#define WFLIP_ROW(K, SK2, ROW)                                                                               \
     {                                                                                                      \
         const int fk = (K), fsk2 = (SK2);                                                                  \
-        const int col = lane < stride ? lane : 0;                                                          \
-        const int32_t a0 = staged ? ((const int32_t*)lds)[fk * stride + col] : g.kadj[(int64_t)fk * stride + col]; \
         _Pragma("unroll") for (int j = 0; j < NB; ++j) {                                                   \
             if (j >= stride) break;                                                                        \
-            const int32_t a = __builtin_amdgcn_readlane(a0, j);                                           \
+            const int32_t a = ROW(j);                                                                      \
             if (a == 0) break;                                                                             \
             const int u = a & 0xFFFFFF;                                                                    \
             const int slot = u & 31;                                                                       \
@@ -1698,17 +1700,37 @@ __device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, co
         const int32_t nv = (int32_t)(((uint32_t)(-(kv >> 1)) << 1) | ((kv & 1) ^ 1));                      \
         wr[kslot] = lane == walk_lane_of(fk) ? nv : kv;                                                    \
     }
+    // flip node K, its row read by lanes 0..stride-1
+#define WFLIP(K, SK2)                                                                                        \
+    {                                                                                                      \
+        const int col = lane < stride ? lane : 0;                                                          \
+        const int32_t a0 = staged ? ((const int32_t*)lds)[(K) * stride + col]                              \
+                                  : g.kadj[(int64_t)(K) * stride + col];                                   \
+        auto row_of = [&](int j) { return __builtin_amdgcn_readlane(a0, j); };                             \
+        WFLIP_ROW(K, SK2, row_of)                                                                          \
+    }
 
-    // seed mutation
+    // seed mutation: lane j loads the neighbour row of the j-th mutated node
+    // up front, so the sequential flips read their rows from registers
     const int64_t seed = sel[w];
     const uint64_t si = (uint64_t)(seed % bs);
     const uint64_t gray = si ^ (si >> 1);
     const int64_t offset = (seed / bs) << 6;
-    for (int j = 0; j < 64 && j < n; ++j) {
-        if (!((gray >> j) & 1u)) continue;
-        const int k = (int)seed_pos(n, offset, j);
-        const int32_t wk = __builtin_amdgcn_readlane(wr[k & 31], walk_lane_of(k));
-        WFLIP(k, (wk & 1) ? -2 : 2);
+    {
+        const int kj = lane < n ? (int)seed_pos(n, offset, lane) : 0;
+        int32_t rowj[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int col = j < stride ? j : 0;
+            rowj[j] = staged ? ((const int32_t*)lds)[kj * stride + col] : g.kadj[(int64_t)kj * stride + col];
+        }
+        for (int j = 0; j < 64 && j < n; ++j) {
+            if (!((gray >> j) & 1u)) continue;
+            const int k = (int)seed_pos(n, offset, j);
+            const int32_t wk = __builtin_amdgcn_readlane(wr[k & 31], walk_lane_of(k));
+            auto row_of = [&](int i) { return __builtin_amdgcn_readlane(rowj[i], j); };
+            WFLIP_ROW(k, (wk & 1) ? -2 : 2, row_of)
+        }
     }
 
     double energy = sel_delta[w];
@@ -1745,6 +1767,7 @@ __device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, co
         ++ci;
     }
 #undef WFLIP
+#undef WFLIP_ROW
     uint8_t* gx = wx + w * n;
     int32_t* gh = wh + w * n;
 #pragma unroll
