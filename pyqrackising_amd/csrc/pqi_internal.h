@@ -188,29 +188,48 @@ void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int
 // result on the device, accept or reject its move and skip the rest of the
 // iteration once a move was accepted.  The host synchronises once per
 // iteration.  Every accumulation happens in the order of the reference loop.
+// A new round starts after this iteration (gate of the field repair, which
+// the iteration graph runs at its end for the next iteration): the search
+// goes on, the descent is not being continued, and the round bound allows it.
+struct LsCtl;
+__host__ __device__ inline bool ls_new_round(const LsCtl* c);
+
 struct LsCtl {
     int32_t restart;    // a move of this iteration was accepted: later phases skip
     int32_t again;      // the descent hit its step bound: the next iteration continues it
     int32_t done;       // the search ended
-    int32_t skip_seed;  // the best seed is not taken (gate of the one-seed walk)
+    int32_t skip_seed;  // the best seed is not taken
     int32_t seeds_ran, walk_ran;
     int32_t phase, pad;
     int64_t rounds;
+    int64_t max_rounds;  // the caller's bound (0: none); set once
     double max_energy, scale, reheat, top;
     int64_t single_evals, single_accepts, double_evals, double_accepts, seed_evals, seed_accepts, walk_accepts,
         reheat_accepts, gray_evals, gray_evals_computed, descent_evals;
     uint64_t mark;      // device wall clock at the last phase change
     uint64_t ticks[5];  // per phase: single, double, seeds, walk, reheat
 };
+
+__host__ __device__ inline bool ls_new_round(const LsCtl* c) {
+    return !c->again && !c->done && !(c->max_rounds > 0 && c->rounds >= c->max_rounds);
+}
 void launch_ls_begin(LsCtl* c, hipStream_t s);
 // max_energy = score[1] (E, spin glass) or score[0] (cut): the exact score of the start
 void launch_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass, hipStream_t s);
 void launch_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap, hipStream_t s);
 void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t* x, void* h, int64_t evals,
                             hipStream_t s);
-void launch_ls_after_seeds(LsCtl* c, const double* seld, int64_t count, hipStream_t s);
-void launch_ls_adopt_seed(const Graph& g, LsCtl* c, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
-                          hipStream_t s);
+// Fused control steps (reference-mode double flips): descent bookkeeping + double flip +
+// its decision; Gray-seed decision + adoption of the best seed by in-place mutation;
+// walk decision + post-reheat single and double flips and their decisions.
+void launch_ls_descent_double(const Graph& g, LsCtl* c, const PhaseOut* dout, int64_t cap, uint8_t* x, void* h,
+                              int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s);
+void launch_ls_seed_take(const Graph& g, LsCtl* c, const double* seld, int64_t count, const int64_t* sel, int64_t bs,
+                         uint8_t* x, void* h, hipStream_t s);
+void launch_ls_walk_reheat(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
+                           int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
+                           void* rh, PhaseOut* dout, int64_t thread_count, PhaseOut* dout2, int64_t dbl_evals,
+                           hipStream_t s);
 void launch_ls_after_walk(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
                           int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
                           void* rh, hipStream_t s);

@@ -1015,6 +1015,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     std::memset(&hc, 0, sizeof(hc));
     hc.scale = p->is_spin_glass ? 2.0 : 1.0;
     hc.phase = -1;
+    hc.max_rounds = p->max_rounds;
     HIPCHK(hipMemcpyAsync(ctl, &hc, sizeof(hc), hipMemcpyHostToDevice, c->stream));
     launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);
     launch_ls_start_energy(ctl, score, p->is_spin_glass, c->stream);
@@ -1028,16 +1029,25 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     const int64_t gray_evals = G * reference_gray_steps(T, n);
     const int64_t bs = p->walkers * p->seed_multiple;
     // segment 0: after the descent, up to the main walk; segment 1: after it
+    // Control steps that sit between single-work-group phases run fused in one
+    // kernel each (a kernel boundary costs a few microseconds); the exhaustive
+    // double flip is multi-work-group and keeps its own launches.
+    const bool fused = p->double_mode == PQI_DOUBLE_REFERENCE;
     auto segment = [&](int which) -> int {
         if (which == 0) {
-            launch_ls_after_descent(ctl, dout, cap, c->stream);
-            launch_double_best(*g, x, h, G, p->double_mode, dout2, dsc, c->stream);
-            launch_ls_after_double(*g, ctl, dout2, x, h, dbl_evals, c->stream);
+            if (fused) {
+                launch_ls_descent_double(*g, ctl, dout, cap, x, h, G, dout2, dbl_evals, c->stream);
+            } else {
+                launch_ls_after_descent(ctl, dout, cap, c->stream);
+                launch_double_best(*g, x, h, G, p->double_mode, dout2, dsc, c->stream);
+                launch_ls_after_double(*g, ctl, dout2, x, h, dbl_evals, c->stream);
+            }
             CHK(gray_seeds(g, x, h, p, b, &count));
-            launch_ls_after_seeds(ctl, b.seld, count, c->stream);
-            // the best seed as a state: a one-walker walk of zero iterations
-            launch_gray_walk(*g, x, h, b.sel, b.seld, bs, 1, 0, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->skip_seed);
-            launch_ls_adopt_seed(*g, ctl, b.wx, b.wh, x, h, c->stream);
+            // the best seed, if it improves, is applied to the state
+            launch_ls_seed_take(*g, ctl, b.seld, count, b.sel, bs, x, h, c->stream);
+        } else if (fused) {
+            launch_ls_walk_reheat(*g, ctl, b.we, b.wev, G, gray_evals, b.wx, b.wh, x, h, rx, rh, dout, G, dout2,
+                                  dbl_evals, c->stream);
         } else {
             launch_ls_after_walk(*g, ctl, b.we, b.wev, G, gray_evals, b.wx, b.wh, x, h, rx, rh, c->stream);
             // post-reheat phase (spin_glass_solver.py:670-717)
@@ -1067,16 +1077,20 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         }
     }
     const bool want_iter = iter_graph_enabled && c->ls.host_ctl;
-    auto head = [&](hipEvent_t e0, hipEvent_t e1) {
-        launch_field_repair(*g, x, h, ctl, fix, c->stream);  // a new round: exact fp64 field
+    // the start of an iteration: the exact fp64 field of a new round and the
+    // iteration's control reset.  In the graph it closes the previous iteration,
+    // after the control record has been copied for the host, so that the next
+    // iteration starts with the descent
+    auto prologue = [&]() {
+        launch_field_repair(*g, x, h, ctl, fix, c->stream);
         launch_ls_begin(ctl, c->stream);
-        launch_descent(*g, x, h, cap, dout, c->stream, e0, e1);
     };
     auto tail = [&]() -> int {
         CHK(segment(0));
         launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
         CHK(segment(1));
         HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
+        prologue();
         return PQI_OK;
     };
     // captured once per (graph, parameters, buffers) and replayed per iteration
@@ -1095,7 +1109,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
             if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                 // a failure here only means the capture is not available: the
                 // segment path below repeats the launches and reports real errors
-                if (!timing) head(nullptr, nullptr);
+                if (!timing) launch_descent(*g, x, h, cap, dout, c->stream);
                 const int r = tail();
                 const hipError_t e = hipStreamEndCapture(c->stream, &graph);
                 if (r != PQI_OK || e != hipSuccess || !graph ||
@@ -1133,10 +1147,12 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         }
         return segment(which);
     };
+    if (c->ls.iter) prologue();
     for (;;) {
         if (!hc.again && p->max_rounds > 0 && hc.rounds >= p->max_rounds) break;
         if (c->ls.iter) {
-            if (timing) head(descent_timer.start_event(), descent_timer.stop_event());
+            if (timing)
+                launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event());
             HIPCHK(hipGraphLaunch(c->ls.iter, c->stream));
             CHK(sync(c));
             std::memcpy(&hc, c->ls.host_ctl, sizeof(hc));

@@ -237,7 +237,7 @@ __device__ __forceinline__ void field_store(FT* h, int64_t row, FT exact, FieldR
 template <typename WT, typename FT>
 __global__ __launch_bounds__(256) void k_field_repair_dense(GView<WT> g, const uint8_t* x, FT* h, const LsCtl* gate,
                                                             FieldRepair* fr) {
-    if (gate && gate->again) return;
+    if (gate && !ls_new_round(gate)) return;
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&fr->checks, 1ull);
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_field_repair_dense(GView<WT> g, const u
 template <typename WT, typename FT>
 __global__ __launch_bounds__(256) void k_field_repair_adj(GView<WT> g, const uint8_t* x, FT* h, const LsCtl* gate,
                                                           FieldRepair* fr) {
-    if (gate && gate->again) return;
+    if (gate && !ls_new_round(gate)) return;
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&fr->checks, 1ull);
     if (row >= g.n) return;
@@ -1875,7 +1875,20 @@ __global__ void k_ls_begin(LsCtl* c) {
     ls_mark(c, 0);
 }
 
-__global__ void k_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap) {
+// Control steps as work-group bodies: thread 0 writes LsCtl; the bodies that
+// apply a move use the whole work group.  The kernels below run one body, or
+// several in the iteration's order (fused, one kernel boundary instead of
+// several); between bodies the group re-reads `restart` after a barrier, so
+// every thread takes the same branch.
+__device__ __forceinline__ bool ls_restarted(const LsCtl* c) {
+    __shared__ int r;
+    __syncthreads();
+    if (threadIdx.x == 0) r = c->restart;
+    __syncthreads();
+    return r != 0;
+}
+
+__device__ __forceinline__ void ls_after_descent_body(LsCtl* c, const PhaseOut* o, int64_t cap) {
     c->single_evals += o->evals;
     c->descent_evals += o->evals;
     c->single_accepts += o->accepts;
@@ -1889,6 +1902,8 @@ __global__ void k_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap) {
     }
 }
 
+__global__ void k_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap) { ls_after_descent_body(c, o, cap); }
+
 template <typename FT, typename WT>
 __device__ __forceinline__ void copy_state(uint8_t* xd, FT* hd, const uint8_t* xs, const FT* hs, int64_t n) {
     for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
@@ -1898,9 +1913,8 @@ __device__ __forceinline__ void copy_state(uint8_t* xd, FT* hd, const uint8_t* x
 }
 
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_ls_after_double(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
-                                                         FT* h, int64_t evals) {
-    if (c->restart) return;
+__device__ void ls_after_double_body(const GView<WT>& g, LsCtl* c, const PhaseOut* o, uint8_t* x, FT* h,
+                                     int64_t evals) {
     const double gain = o->gain;
     const int64_t k = o->k, l = o->l;
     __syncthreads();
@@ -1918,20 +1932,56 @@ __global__ __launch_bounds__(1024) void k_ls_after_double(GView<WT> g, LsCtl* c,
     }
 }
 
-__global__ void k_ls_after_seeds(LsCtl* c, const double* seld, int64_t count) {
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_after_double(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
+                                                         FT* h, int64_t evals) {
     if (c->restart) return;
+    ls_after_double_body<WT, FT>(g, c, o, x, h, evals);
+}
+
+// After the descent: its bookkeeping, the reference-mode double flip
+// (spin_glass_solver.py:55-92) and its decision, in one kernel.
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_descent_double(GView<WT> g, LsCtl* c, const PhaseOut* dout,
+                                                           int64_t cap, uint8_t* x, FT* h, int64_t thread_count,
+                                                           PhaseOut* dout2, int64_t evals) {
+    if (threadIdx.x == 0) ls_after_descent_body(c, dout, cap);
+    if (ls_restarted(c)) return;
+    double_ref_body<WT, FT>(g, x, h, thread_count, dout2);
+    __syncthreads();
+    ls_after_double_body<WT, FT>(g, c, dout2, x, h, evals);
+}
+
+__device__ __forceinline__ void ls_after_seeds_body(LsCtl* c, const double* seld, int64_t count) {
     c->seed_evals += count;
     c->top = seld[0];
     c->skip_seed = c->top > 0.0 ? 0 : 1;
 }
 
+// The Gray-seed decision and, when the best seed improves, its mutation applied
+// to (x, h) in place: the same flips, in the same order and with the same
+// field arithmetic, as the one-walker, zero-iteration walk followed by the
+// copy of its state (gray_mutation, maxcut_tfim_util.py:740-749).
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_ls_adopt_seed(GView<WT> g, LsCtl* c, const uint8_t* wx, const FT* wh,
-                                                       uint8_t* x, FT* h) {
-    if (c->restart) return;
-    const int take = !c->skip_seed;
+__global__ __launch_bounds__(1024) void k_ls_seed_take(GView<WT> g, LsCtl* c, const double* seld, int64_t count,
+                                                      const int64_t* sel, int64_t bs, uint8_t* x, FT* h) {
+    __shared__ int take;
+    if (threadIdx.x == 0 && !c->restart) ls_after_seeds_body(c, seld, count);
+    if (ls_restarted(c)) return;
+    if (threadIdx.x == 0) take = !c->skip_seed;
     __syncthreads();
-    if (take) copy_state<FT, WT>(x, h, wx, wh, g.n);
+    if (take) {
+        const int64_t n = g.n;
+        const int64_t seed = sel[0];
+        const uint64_t si = (uint64_t)(seed % bs);
+        const uint64_t gray = si ^ (si >> 1);
+        const int64_t offset = (seed / bs) << 6;
+        for (int j = 0; j < 64 && j < n; ++j) {
+            if (!((gray >> j) & 1u)) continue;
+            flip_body<WT, FT>(g, x, h, seed_pos(n, offset, j), -1);
+            __syncthreads();
+        }
+    }
     if (threadIdx.x == 0) {
         if (take) {
             c->max_energy += c->scale * c->top;
@@ -1945,13 +1995,12 @@ __global__ __launch_bounds__(1024) void k_ls_adopt_seed(GView<WT> g, LsCtl* c, c
 }
 
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_ls_after_walk(GView<WT> g, LsCtl* c, const double* we, const int64_t* wev,
-                                                       int64_t walkers, int64_t gray_evals, const uint8_t* wx,
-                                                       const FT* wh, uint8_t* x, FT* h, uint8_t* rx, FT* rh) {
+__device__ void ls_after_walk_body(const GView<WT>& g, LsCtl* c, const double* we, const int64_t* wev,
+                                   int64_t walkers, int64_t gray_evals, const uint8_t* wx, const FT* wh, uint8_t* x,
+                                   FT* h, uint8_t* rx, FT* rh) {
     __shared__ double sv[16];
     __shared__ int64_t sk[16];
     __shared__ int64_t ssum[16];
-    if (c->restart) return;
     // first maximum over the walkers (value, then lowest walker) and the
     // number of computed evaluations
     double bv = -INFINITY;
@@ -1984,9 +2033,16 @@ __global__ __launch_bounds__(1024) void k_ls_after_walk(GView<WT> g, LsCtl* c, c
 }
 
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_ls_reheat_single(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
-                                                          FT* h, uint8_t* rx, FT* rh) {
+__global__ __launch_bounds__(1024) void k_ls_after_walk(GView<WT> g, LsCtl* c, const double* we, const int64_t* wev,
+                                                       int64_t walkers, int64_t gray_evals, const uint8_t* wx,
+                                                       const FT* wh, uint8_t* x, FT* h, uint8_t* rx, FT* rh) {
     if (c->restart) return;
+    ls_after_walk_body<WT, FT>(g, c, we, wev, walkers, gray_evals, wx, wh, x, h, rx, rh);
+}
+
+template <typename WT, typename FT>
+__device__ void ls_reheat_single_body(const GView<WT>& g, LsCtl* c, const PhaseOut* o, uint8_t* x, FT* h,
+                                      uint8_t* rx, FT* rh) {
     const double gain = o->gain, reheat = c->reheat;
     const int64_t k = o->k;
     __syncthreads();
@@ -2011,9 +2067,15 @@ __global__ __launch_bounds__(1024) void k_ls_reheat_single(GView<WT> g, LsCtl* c
 }
 
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_ls_reheat_double(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
-                                                          FT* h, uint8_t* rx, FT* rh, int64_t evals) {
+__global__ __launch_bounds__(1024) void k_ls_reheat_single(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
+                                                          FT* h, uint8_t* rx, FT* rh) {
     if (c->restart) return;
+    ls_reheat_single_body<WT, FT>(g, c, o, x, h, rx, rh);
+}
+
+template <typename WT, typename FT>
+__device__ void ls_reheat_double_body(const GView<WT>& g, LsCtl* c, const PhaseOut* o, uint8_t* x, FT* h,
+                                      uint8_t* rx, FT* rh, int64_t evals) {
     const double gain = o->gain, reheat = c->reheat;
     const int64_t k = o->k, l = o->l;
     __syncthreads();
@@ -2034,6 +2096,34 @@ __global__ __launch_bounds__(1024) void k_ls_reheat_double(GView<WT> g, LsCtl* c
             ls_mark(c, -1);
         }
     }
+}
+
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_reheat_double(GView<WT> g, LsCtl* c, const PhaseOut* o, uint8_t* x,
+                                                          FT* h, uint8_t* rx, FT* rh, int64_t evals) {
+    if (c->restart) return;
+    ls_reheat_double_body<WT, FT>(g, c, o, x, h, rx, rh, evals);
+}
+
+// After the main walk, in one kernel: its decision, then the post-reheat
+// phase (spin_glass_solver.py:670-717) with the best single flip and the
+// reference-mode double flip of the reheat state and their decisions.
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_walk_reheat(GView<WT> g, LsCtl* c, const double* we, const int64_t* wev,
+                                                        int64_t walkers, int64_t gray_evals, const uint8_t* wx,
+                                                        const FT* wh, uint8_t* x, FT* h, uint8_t* rx, FT* rh,
+                                                        PhaseOut* dout, int64_t thread_count, PhaseOut* dout2,
+                                                        int64_t dbl_evals) {
+    if (ls_restarted(c)) return;
+    ls_after_walk_body<WT, FT>(g, c, we, wev, walkers, gray_evals, wx, wh, x, h, rx, rh);
+    if (ls_restarted(c)) return;
+    descent_body<WT, FT>(g, rx, rh, 0, dout);  // best single flip of the reheat state, no move
+    __syncthreads();
+    ls_reheat_single_body<WT, FT>(g, c, dout, x, h, rx, rh);
+    if (ls_restarted(c)) return;
+    double_ref_body<WT, FT>(g, rx, rh, thread_count, dout2);
+    __syncthreads();
+    ls_reheat_double_body<WT, FT>(g, c, dout2, x, h, rx, rh, dbl_evals);
 }
 
 void launch_ls_begin(LsCtl* c, hipStream_t s) { hipLaunchKernelGGL(k_ls_begin, dim3(1), dim3(1), 0, s, c); }
@@ -2061,20 +2151,45 @@ void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t
     PQI_DISPATCH(g, ls_after_double_impl, g, c, o, x, h, evals, s);
 }
 
-void launch_ls_after_seeds(LsCtl* c, const double* seld, int64_t count, hipStream_t s) {
-    hipLaunchKernelGGL(k_ls_after_seeds, dim3(1), dim3(1), 0, s, c, seld, count);
+template <typename WT>
+static void ls_descent_double_impl(const Graph& g, LsCtl* c, const PhaseOut* dout, int64_t cap, uint8_t* x, void* h,
+                                   int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_ls_descent_double<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, dout, cap, x,
+                       (FT*)h, thread_count, dout2, evals);
+}
+void launch_ls_descent_double(const Graph& g, LsCtl* c, const PhaseOut* dout, int64_t cap, uint8_t* x, void* h,
+                              int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s) {
+    PQI_DISPATCH(g, ls_descent_double_impl, g, c, dout, cap, x, h, thread_count, dout2, evals, s);
 }
 
 template <typename WT>
-static void ls_adopt_seed_impl(const Graph& g, LsCtl* c, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
-                               hipStream_t s) {
+static void ls_seed_take_impl(const Graph& g, LsCtl* c, const double* seld, int64_t count, const int64_t* sel,
+                              int64_t bs, uint8_t* x, void* h, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
-    hipLaunchKernelGGL((k_ls_adopt_seed<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, wx, (const FT*)wh, x,
+    hipLaunchKernelGGL((k_ls_seed_take<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, seld, count, sel, bs, x,
                        (FT*)h);
 }
-void launch_ls_adopt_seed(const Graph& g, LsCtl* c, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
-                          hipStream_t s) {
-    PQI_DISPATCH(g, ls_adopt_seed_impl, g, c, wx, wh, x, h, s);
+void launch_ls_seed_take(const Graph& g, LsCtl* c, const double* seld, int64_t count, const int64_t* sel, int64_t bs,
+                         uint8_t* x, void* h, hipStream_t s) {
+    PQI_DISPATCH(g, ls_seed_take_impl, g, c, seld, count, sel, bs, x, h, s);
+}
+
+template <typename WT>
+static void ls_walk_reheat_impl(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
+                                int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
+                                uint8_t* rx, void* rh, PhaseOut* dout, int64_t thread_count, PhaseOut* dout2,
+                                int64_t dbl_evals, hipStream_t s) {
+    using FT = typename FieldOf<WT>::T;
+    hipLaunchKernelGGL((k_ls_walk_reheat<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, we, wev, walkers,
+                       gray_evals, wx, (const FT*)wh, x, (FT*)h, rx, (FT*)rh, dout, thread_count, dout2, dbl_evals);
+}
+void launch_ls_walk_reheat(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
+                           int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
+                           void* rh, PhaseOut* dout, int64_t thread_count, PhaseOut* dout2, int64_t dbl_evals,
+                           hipStream_t s) {
+    PQI_DISPATCH(g, ls_walk_reheat_impl, g, c, we, wev, walkers, gray_evals, wx, wh, x, h, rx, rh, dout, thread_count,
+                 dout2, dbl_evals, s);
 }
 
 template <typename WT>
