@@ -807,15 +807,19 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             if (j >= stride) break;
+            // a zero padding entry (w = 0) leaves node 0's key unchanged: no test
             const int32_t a = __builtin_amdgcn_readlane(nb[j], owner);
-            if (a == 0) break;
             const int u = a & 0xFFFFFF;
             const int slot = u >> 6;
             const int sk2w = sk2 * (a >> 24);
+            // key -= (s_v * 2 s_k w) << sh with s_v = -1 when bit 0 is set:
+            // (t ^ m) - m negates t exactly when m = -(bit 0) is -1
             KEY_APPLY(slot, lane == (u & 63),
-                      (int32_t)((uint32_t)kv - ((uint32_t)((kv & 1) ? -sk2w : sk2w) << sh)));
+                      (int32_t)((uint32_t)kv - ((uint32_t)((sk2w ^ __builtin_amdgcn_sbfe(kv, 0u, 1u)) -
+                                                            __builtin_amdgcn_sbfe(kv, 0u, 1u)) << sh)));
         }
-        const int32_t kk = key_of(-bv, bk, xk ^ 1u, ib, mask);
+        // the flipped node: gain negated, spin bit toggled, node bits kept
+        const int32_t kk = (int32_t)(((uint32_t)(-bv) << sh) | (((uint32_t)m & ((1u << sh) - 1u)) ^ 1u));
         const int kslot = bk >> 6;
         KEY_APPLY(kslot, lane == owner, kk);
         ++steps;
@@ -1686,13 +1690,13 @@ __device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, co
         const int fk = (K), fsk2 = (SK2);                                                                  \
         _Pragma("unroll") for (int j = 0; j < NB; ++j) {                                                   \
             if (j >= stride) break;                                                                        \
-            const int32_t a = ROW(j);                                                                      \
-            if (a == 0) break;                                                                             \
+            const int32_t a = ROW(j); /* a zero padding entry (w = 0) changes nothing */                   \
             const int u = a & 0xFFFFFF;                                                                    \
             const int slot = u & 31;                                                                       \
             const int sk2w = fsk2 * (a >> 24);                                                             \
             const int32_t kv = wr[slot];                                                                   \
-            const int32_t nv = (int32_t)((uint32_t)kv - ((uint32_t)((kv & 1) ? -sk2w : sk2w) << 1));        \
+            const int32_t sm = __builtin_amdgcn_sbfe(kv, 0u, 1u);                                          \
+            const int32_t nv = (int32_t)((uint32_t)kv - ((uint32_t)((sk2w ^ sm) - sm) << 1));               \
             wr[slot] = lane == walk_lane_of(u) ? nv : kv;                                                  \
         }                                                                                                  \
         const int kslot = fk & 31;                                                                         \
