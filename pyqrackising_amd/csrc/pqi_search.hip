@@ -745,6 +745,9 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
         const int32_t kv = khi[(slot) - 32];                               \
         khi[(slot) - 32] = kv ^ ((kv ^ (EXPR)) & -(int32_t)(mine));        \
     }
+    // padding slots hold the lowest key of node 0: below every real key (the
+    // row strengths keep real gains above -2^(30 - ib)) and decoded as node 0
+    const int32_t pad = (int32_t)(0x80000000u | ((uint32_t)mask << 1));
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int v = r * 64 + lane;
@@ -752,10 +755,13 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
             const uint8_t xv = xg[v];
             KEY(r) = key_of(xv ? -hg[v] : hg[v], v, xv, ib, mask);
         } else {
-            KEY(r) = INT_MIN;
+            KEY(r) = pad;
         }
     }
-    // the table is zero padded to whole quads; rows are zero padded after their entries
+    // the neighbour table, zero padded to whole quads (rows zero padded after
+    // their entries), is staged in LDS (the host admits only tables that fit;
+    // with max_steps = 0 nothing is staged and rows come from global memory)
+    __builtin_assume(stride >= 1);
     const int quads = (n * stride + 3) >> 2;
     const bool staged = max_steps > 0 && (size_t)quads * 16 <= lds_bytes;
     if (staged) {
@@ -785,9 +791,11 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
         // neighbour row of this lane's own maximum, loaded while the wave reduces
         // (a lane holding padding only reads row 0 and never wins); all NB
         // loads are issued without branches or waits, entries past the stride
-        // repeat entry 0 and are never used
+        // repeat entry 0 and are never used.  node = mask - bits = ~bits & mask
+        // (the two load paths also keep the loads ahead of the reduction; with
+        // max_steps = 0 the rows read are never used)
         int32_t nb[NB];
-        const int c = lm == INT_MIN ? 0 : mask - ((lm >> 1) & mask);
+        const int c = (int)__builtin_amdgcn_ubfe(~(uint32_t)lm, 1u, (uint32_t)ib);
         if (staged) {
             const int32_t* row = (const int32_t*)lds + c * stride;
 #pragma unroll
@@ -871,10 +879,21 @@ static size_t reg_table_bytes(const Graph& g) {
 
 static int reg_descent_enabled() { return descent_variant() == 0; }
 
+template <int R, int NB>
+__global__ __launch_bounds__(64) void k_descent_reg(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                                   PhaseOut* out, size_t lds_bytes, int ib);
+
+// LDS available to the register-resident kernels (their neighbour table must fit)
+static size_t reg_lds_cap() {
+    static const size_t cap = lds_limit((const void*)k_descent_reg<64, 16>);
+    return cap;
+}
+
 int reg_descent_plan(const Graph& g, size_t* table_bytes) {
     if (!reg_descent_enabled() || key_shift(g) <= 0 || !g.key_adj) return 0;
     const int r = reg_descent_regs(g.n, g.key_adj_stride);
-    if (r && table_bytes) *table_bytes = reg_table_bytes(g);
+    if (!r || reg_table_bytes(g) > reg_lds_cap()) return 0;
+    if (table_bytes) *table_bytes = reg_table_bytes(g);
     return r;
 }
 
@@ -899,9 +918,9 @@ template <int R, int NB>
 static void launch_reg(const GView<int8_t>& v, uint8_t* x, int32_t* h, int64_t max_steps, PhaseOut* out,
                        size_t table, int ib, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     static const size_t limit = lds_limit((const void*)k_descent_reg<R, NB>);
-    const size_t lds = table <= limit ? table : 0;
-    hipExtLaunchKernelGGL((k_descent_reg<R, NB>), dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, v, x, h,
-                          max_steps, out, lds, ib);
+    (void)limit;  // the plan admits only tables within it
+    hipExtLaunchKernelGGL((k_descent_reg<R, NB>), dim3(1), dim3(64), (uint32_t)table, s, e0, e1, 0, v, x, h,
+                          max_steps, out, table, ib);
 }
 
 // Key width for the packed-key descent of g, or 0 when it does not apply.
@@ -2289,9 +2308,9 @@ template <int R, int NB>
 static void launch_b_reg(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_steps, int which,
                          size_t table, hipStream_t s) {
     static const size_t limit = lds_limit((const void*)kb_descent_reg<R, NB>);
-    const size_t lds = table <= limit ? table : 0;
-    hipLaunchKernelGGL((kb_descent_reg<R, NB>), dim3((unsigned)L), dim3(64), lds, s, d, list, max_steps, which,
-                       lds);
+    (void)limit;  // the plan admits only tables within it
+    hipLaunchKernelGGL((kb_descent_reg<R, NB>), dim3((unsigned)L), dim3(64), table, s, d, list, max_steps, which,
+                       table);
 }
 
 template <typename WT, typename FT>

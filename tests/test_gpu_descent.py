@@ -86,8 +86,9 @@ def test_descent_dense_int(n, wmax, seed):
                                             (4000, 12, 1, 15), (4096, 16, 3, 16), (64, 3, 1, 17)])
 def test_descent_register_resident(n, d, wmax, seed):
     """Packed keys held in registers (n <= 4096, neighbour rows of at most 16
-    entries): 8 to 64 registers per lane, rows of 4, 8 and 16 entries, the
-    neighbour table staged in LDS or, above 160 KiB, read from global memory."""
+    entries, the neighbour table in LDS): 8 to 64 registers per lane, rows of
+    4, 8 and 16 entries; tables above 160 KiB (n=4000, d=12 and n=4096, d=16)
+    take the LDS-key descent instead."""
     log(f"register descent n={n} d={d} wmax={wmax}")
     G, A = regular_int(n, d, seed, wmax)
     dg = _lib.DeviceGraph.csr(G)
@@ -172,9 +173,9 @@ def test_batched_staging_mix_matches_single():
 
 @pytest.mark.parametrize("sizes", [((300, 3), (1000, 7), (2048, 3), (4000, 12)), ((64, 3), (700, 5), (2048, 13))])
 def test_batched_register_descent_matches_single(sizes):
-    """A batch whose instances all take the register-resident descent, with
-    different sizes and row widths (one table too large to stage; with every
-    n <= 2048 the register-resident walkers too): every instance equals its
+    """Batches of different sizes and row widths: with one table too large for
+    LDS the batch takes the LDS-key descent; with every n <= 2048 the
+    register-resident descent and walkers.  Every instance equals its
     single-instance search, which the parity tests check against the oracle."""
     log(f"batched register descent {sizes}")
     Gs = [regular_int(n, d, 80 + i)[0] for i, (n, d) in enumerate(sizes)]
