@@ -67,7 +67,8 @@ struct pqi_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     std::vector<pqi::Buffer> bufs;  // indexed by slot
-    void* pinned = nullptr;
+    void* pinned = nullptr;      // pinned host staging (grown on demand)
+    void* pinned_dev = nullptr;  // its device-visible address
     size_t pinned_bytes = 0;
     pqi::LsGraphs ls;
 };
@@ -214,6 +215,20 @@ __host__ __device__ inline bool ls_new_round(const LsCtl* c) {
     return !c->again && !c->done && !(c->max_rounds > 0 && c->rounds >= c->max_rounds);
 }
 void launch_ls_begin(LsCtl* c, hipStream_t s);
+// What the host reads after each iteration of the int8 fast path (pinned host memory).
+struct LsReport {
+    LsCtl ctl;
+    FieldRepair fix;
+    double cut, energy;  // exact score of the state after the iteration
+};
+// int8 weights with adjacency lists: the whole start of a local search in one launch
+// (theta_dev: the caller's 0/1 bytes in pinned host memory, device-visible address)
+void launch_ls_start_i8(const Graph& g, const uint8_t* theta_dev, uint8_t* x, void* h, uint8_t* rx, void* rh,
+                        LsCtl* c, FieldRepair* fix, double scale, int64_t max_rounds, int is_spin_glass,
+                        hipStream_t s);
+// int8 weights: control record, repair counters, exact score and state into a pinned report
+void launch_ls_report_i8(const Graph& g, const uint8_t* x, const void* h, const LsCtl* c, const FieldRepair* fix,
+                         LsReport* rep_dev, uint8_t* x_out_dev, hipStream_t s);
 // max_energy = score[1] (E, spin glass) or score[0] (cut): the exact score of the start
 void launch_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass, hipStream_t s);
 void launch_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap, hipStream_t s);

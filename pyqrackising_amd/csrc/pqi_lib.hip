@@ -503,11 +503,13 @@ static int ensure_pinned(pqi_ctx* c, size_t bytes) {
         HIPCHK(hipStreamSynchronize(c->stream));
         (void)hipHostFree(c->pinned);
         c->pinned = nullptr;
+        c->pinned_dev = nullptr;
         c->pinned_bytes = 0;
     }
     const size_t want = std::max<size_t>(bytes, 4096);
     HIPCHK(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
     c->pinned_bytes = want;
+    HIPCHK(hipHostGetDevicePointer(&c->pinned_dev, c->pinned, 0));
     return PQI_OK;
 }
 
@@ -979,15 +981,41 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     const int64_t T = (p->gray_iterations + G - 1) / G;
     const int64_t dbl_evals = p->double_mode == PQI_DOUBLE_EXHAUSTIVE ? C : std::min(G, C);
 
+    static const bool iter_graph_enabled = [] {
+        const char* e = getenv("PQI_LS_ITER_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    // int8 weights with adjacency lists: the start is one launch reading the
+    // caller's state from pinned memory, and every iteration ends with a report
+    // (control record, repair counters, exact score, state) written into pinned
+    // memory, so the search needs no copy before the first descent and nothing
+    // after the last synchronisation
+    const bool fast = iter_graph_enabled && g->kind == PQI_WEIGHTS_I8 && g->adj_ptr &&
+                      (g->use_adj || g->W == nullptr) && n <= 65536 && !(p->flags & PQI_FLAG_TEST_DRIFT);
+    const size_t off_rep = ((size_t)n + 255) & ~(size_t)255;
+    const size_t off_xout = off_rep + ((sizeof(LsReport) + 255) & ~(size_t)255);
     uint8_t *x, *rx;
     void *h, *rh;
-    CHK(load_state(g, theta, &x, &h));
+    if (fast) {
+        for (int64_t i = 0; i < n; ++i)
+            if (theta[i] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
+        CHK(ensure_pinned(c, off_xout + (size_t)n));
+        std::memcpy(c->pinned, theta, (size_t)n);
+        CHK(ensure(c, S_X, n, (void**)&x));
+        CHK(ensure(c, S_H, n * fb, &h));
+    } else {
+        CHK(load_state(g, theta, &x, &h));
+    }
+    LsReport* rep = fast ? (LsReport*)((char*)c->pinned + off_rep) : nullptr;
+    LsReport* rep_dev = fast ? (LsReport*)((char*)c->pinned_dev + off_rep) : nullptr;
+    uint8_t* xout = fast ? (uint8_t*)c->pinned + off_xout : nullptr;
+    uint8_t* xout_dev = fast ? (uint8_t*)c->pinned_dev + off_xout : nullptr;
     PhaseOut *dout, *dout2;
     void* dsc;
     LsCtl* ctl;
     FieldRepair* fix;
     CHK(ensure(c, S_FIX, sizeof(FieldRepair), (void**)&fix));
-    HIPCHK(hipMemsetAsync(fix, 0, sizeof(FieldRepair), c->stream));
+    if (!fast) HIPCHK(hipMemsetAsync(fix, 0, sizeof(FieldRepair), c->stream));
     if (p->flags & PQI_FLAG_TEST_DRIFT) launch_perturb_field(*g, h, c->stream);
     CHK(ensure(c, S_RX, n, (void**)&rx));
     CHK(ensure(c, S_RH, n * fb, &rh));
@@ -1016,12 +1044,14 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     hc.scale = p->is_spin_glass ? 2.0 : 1.0;
     hc.phase = -1;
     hc.max_rounds = p->max_rounds;
-    HIPCHK(hipMemcpyAsync(ctl, &hc, sizeof(hc), hipMemcpyHostToDevice, c->stream));
-    launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);
-    launch_ls_start_energy(ctl, score, p->is_spin_glass, c->stream);
-    // the reheat state is read by speculative launches before its first use
-    HIPCHK(hipMemcpyAsync(rx, x, n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(rh, h, n * fb, hipMemcpyDeviceToDevice, c->stream));
+    if (!fast) {
+        HIPCHK(hipMemcpyAsync(ctl, &hc, sizeof(hc), hipMemcpyHostToDevice, c->stream));
+        launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);
+        launch_ls_start_energy(ctl, score, p->is_spin_glass, c->stream);
+        // the reheat state is read by speculative launches before its first use
+        HIPCHK(hipMemcpyAsync(rx, x, n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(rh, h, n * fb, hipMemcpyDeviceToDevice, c->stream));
+    }
 
     const bool timing = (p->flags & PQI_FLAG_TIME_KERNELS) != 0;
     EventTimer walk_timer(timing), descent_timer(timing);
@@ -1066,17 +1096,13 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     // rest of the iteration.  Where graph capture is not available, the two
     // segments between the directly launched descent and walk are captured
     // instead (also selected by PQI_LS_ITER_GRAPH=0).
-    static const bool iter_graph_enabled = [] {
-        const char* e = getenv("PQI_LS_ITER_GRAPH");
-        return !(e && e[0] == '0');
-    }();
-    if (iter_graph_enabled && !c->ls.host_ctl) {
+    if (iter_graph_enabled && !fast && !c->ls.host_ctl) {
         if (hipHostMalloc(&c->ls.host_ctl, sizeof(LsCtl), hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
             c->ls.host_ctl = nullptr;
         }
     }
-    const bool want_iter = iter_graph_enabled && c->ls.host_ctl;
+    const bool want_iter = iter_graph_enabled && (fast || c->ls.host_ctl);
     // the start of an iteration: the exact fp64 field of a new round and the
     // iteration's control reset.  In the graph it closes the previous iteration,
     // after the control record has been copied for the host, so that the next
@@ -1089,13 +1115,15 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         CHK(segment(0));
         launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
         CHK(segment(1));
-        HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
+        if (fast) launch_ls_report_i8(*g, x, h, ctl, fix, rep_dev, xout_dev, c->stream);
+        else HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
         prologue();
         return PQI_OK;
     };
     // captured once per (graph, parameters, buffers) and replayed per iteration
     const std::vector<int64_t> key = {(int64_t)g, (int64_t)g->W, (int64_t)g->adj_ptr, n, G, T, bs, (int64_t)timing,
-                                      (int64_t)want_iter, (int64_t)fix, (int64_t)c->ls.host_ctl,
+                                      (int64_t)want_iter, (int64_t)fix, (int64_t)c->ls.host_ctl, (int64_t)fast,
+                                      (int64_t)rep_dev, (int64_t)xout_dev,
                                       p->double_mode, p->seed_engine, p->is_spin_glass, (int64_t)x, (int64_t)h,
                                       (int64_t)rx, (int64_t)rh, (int64_t)dout, (int64_t)dout2, (int64_t)dsc,
                                       (int64_t)ctl, (int64_t)b.wx, (int64_t)b.wh, (int64_t)b.we, (int64_t)b.wev,
@@ -1147,15 +1175,22 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         }
         return segment(which);
     };
-    if (c->ls.iter) prologue();
+    if (fast) {
+        launch_ls_start_i8(*g, (const uint8_t*)c->pinned_dev, x, h, rx, rh, ctl, fix, hc.scale, p->max_rounds,
+                           p->is_spin_glass, c->stream);
+    } else if (c->ls.iter) {
+        prologue();
+    }
     for (;;) {
         if (!hc.again && p->max_rounds > 0 && hc.rounds >= p->max_rounds) break;
-        if (c->ls.iter) {
-            if (timing)
+        if (c->ls.iter || fast) {
+            // (without the captured graph, the fast path launches the same sequence directly)
+            if (timing || !c->ls.iter)
                 launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event());
-            HIPCHK(hipGraphLaunch(c->ls.iter, c->stream));
+            if (c->ls.iter) HIPCHK(hipGraphLaunch(c->ls.iter, c->stream));
+            else CHK(tail());
             CHK(sync(c));
-            std::memcpy(&hc, c->ls.host_ctl, sizeof(hc));
+            std::memcpy(&hc, fast ? (const void*)&rep->ctl : c->ls.host_ctl, sizeof(hc));
             if (timing) {
                 st->descent_kernel_seconds += descent_timer.seconds();
                 st->descent_launches++;
@@ -1205,6 +1240,14 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     st->seconds_seeds = hc.ticks[2] * tick;
     st->seconds_walk = hc.ticks[3] * tick;
     st->seconds_reheat = hc.ticks[4] * tick;
+    if (fast) {
+        // the last iteration's report holds everything
+        field_repair_stats(rep->fix, st);
+        st->cut_value = rep->cut;
+        st->min_energy = p->is_spin_glass ? -rep->energy : rep->energy;
+        std::memcpy(theta, xout, (size_t)n);
+        return PQI_OK;
+    }
     // field repairs, the exact score and the state come back in one
     // synchronisation, through pinned memory
     launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);

@@ -2155,6 +2155,97 @@ __global__ void k_ls_start_energy(LsCtl* c, const double* score, int is_spin_gla
     c->max_energy = is_spin_glass ? score[1] : score[0];
 }
 
+// Start of a local search over int8 weights with adjacency lists, in one work
+// group: the state from the caller's 0/1 bytes (read from pinned host memory),
+// the exact int32 field h = W s (k_field_adj's arithmetic), the exact score
+// (k_score's), the reheat copy, the control record (max_energy = the start's
+// score, then the first round's reset as k_ls_begin makes it) and cleared
+// field-repair counters.  It replaces the copy, field, score, initialisation
+// and reset launches of the general start.
+__global__ __launch_bounds__(1024) void k_ls_start_i8(GView<int8_t> g, const uint8_t* theta, uint8_t* x, int32_t* h,
+                                                     uint8_t* rx, int32_t* rh, LsCtl* c, FieldRepair* fix,
+                                                     double scale, int64_t max_rounds, double w_up, int64_t w_up_i,
+                                                     int is_spin_glass) {
+    __shared__ long long part[16];
+    const int64_t n = g.n;
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+        const uint8_t xv = theta[v];
+        x[v] = xv;
+        rx[v] = xv;
+    }
+    __syncthreads();
+    long long q = 0;
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+        int32_t acc = 0;
+        for (int32_t e = g.ap[v]; e < g.ap[v + 1]; ++e) acc += x[g.ai[e]] ? -(int32_t)g.aw[e] : (int32_t)g.aw[e];
+        h[v] = acc;
+        rh[v] = acc;
+        q += x[v] ? -(long long)acc : (long long)acc;
+    }
+    q = wave_sum(q);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long qs = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) qs += part[i];
+        (void)w_up;
+        const double cut = (double)((2 * w_up_i - qs) / 4), energy = (double)(-qs / 2);
+        LsCtl z = {};
+        z.scale = scale;
+        z.phase = -1;
+        z.max_rounds = max_rounds;
+        z.max_energy = is_spin_glass ? energy : cut;
+        // the first round's reset (k_ls_begin)
+        z.rounds = 1;
+        z.skip_seed = 1;
+        *c = z;
+        ls_mark(c, 0);
+        fix->checks = 0;
+        fix->entries = 0;
+        fix->max_bits = 0;
+    }
+}
+
+void launch_ls_start_i8(const Graph& g, const uint8_t* theta_dev, uint8_t* x, void* h, uint8_t* rx, void* rh,
+                        LsCtl* c, FieldRepair* fix, double scale, int64_t max_rounds, int is_spin_glass,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_ls_start_i8, dim3(1), dim3(1024), 0, s, view<int8_t>(g), theta_dev, x, (int32_t*)h, rx,
+                       (int32_t*)rh, c, fix, scale, max_rounds, g.w_up, g.w_up_i, is_spin_glass);
+}
+
+// End of an iteration over int8 weights: the control record, the field-repair
+// counters, the exact score of the state (k_score's arithmetic on the exact
+// int32 field) and the state, written by one work group into a report in
+// pinned host memory that the host reads after its one synchronisation.
+__global__ __launch_bounds__(1024) void k_ls_report_i8(const uint8_t* x, const int32_t* h, int64_t n, int64_t w_up_i,
+                                                      const LsCtl* c, const FieldRepair* fix, LsReport* rep,
+                                                      uint8_t* x_out) {
+    __shared__ long long part[16];
+    long long q = 0;
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+        const uint8_t xv = x[v];
+        x_out[v] = xv;
+        q += xv ? -(long long)h[v] : (long long)h[v];
+    }
+    q = wave_sum(q);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long qs = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) qs += part[i];
+        rep->cut = (double)((2 * w_up_i - qs) / 4);
+        rep->energy = (double)(-qs / 2);
+        rep->ctl = *c;
+        rep->fix = *fix;
+    }
+}
+
+void launch_ls_report_i8(const Graph& g, const uint8_t* x, const void* h, const LsCtl* c, const FieldRepair* fix,
+                         LsReport* rep_dev, uint8_t* x_out_dev, hipStream_t s) {
+    hipLaunchKernelGGL(k_ls_report_i8, dim3(1), dim3(1024), 0, s, x, (const int32_t*)h, g.n, g.w_up_i, c, fix,
+                       rep_dev, x_out_dev);
+}
+
 void launch_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass, hipStream_t s) {
     hipLaunchKernelGGL(k_ls_start_energy, dim3(1), dim3(1), 0, s, c, score, is_spin_glass);
 }
