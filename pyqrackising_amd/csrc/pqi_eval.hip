@@ -29,7 +29,8 @@ constexpr int BN = TILE_BN;  // output columns per work group
 // q[m] += sum over this block's 128 columns j of s_mj (S W)_mj.  q is zeroed by the caller.
 __global__ __launch_bounds__(256) void k_batch_i8(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
                                                   const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
-                                                  unsigned long long* __restrict__ q) {
+                                                  unsigned long long* __restrict__ q, const int32_t* gate) {
+    if (gate && *gate) return;
     __shared__ __align__(16) int8_t As[TILE_BM * (64 + 16)];
     __shared__ __align__(16) int8_t Bs[TILE_BN * (64 + 16)];
     __shared__ uint32_t Sb[TILE_BM * (TILE_BN / 32)];
@@ -47,8 +48,9 @@ template <int BK, int NW>
 __global__ __launch_bounds__(64 * NW) void k_batch_i8_wide(const int8_t* __restrict__ W, int64_t ld, int64_t n_pad,
                                                        const uint32_t* __restrict__ bits, int64_t wpr, int64_t B,
                                                        unsigned long long* __restrict__ q, int64_t MB, int64_t JB,
-                                                       int64_t KS) {
+                                                       int64_t KS, const int32_t* gate) {
     extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
     const int64_t b = blockIdx.x, x = b & 7, slot = b >> 3;
     const int64_t ks = slot % KS, t = slot / KS;
     const int64_t jb = (t / MB) * 8 + x, mb = t % MB;
@@ -626,7 +628,7 @@ static int wide_waves() {
 }
 
 void launch_batch_i8_raw(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, unsigned long long* qi,
-                         hipStream_t s) {
+                         hipStream_t s, const int32_t* gate) {
     if (B >= 512 && g.max_strength >= 0 && 256 * g.max_strength < (int64_t(1) << 31)) {
         const int64_t MB = (B + WIDE_BM - 1) / WIDE_BM, JB = (g.n_pad + WIDE_BN - 1) / WIDE_BN;
         // split K until there is about one work group per CU (each split >= 512 deep)
@@ -642,19 +644,20 @@ void launch_batch_i8_raw(const Graph& g, const uint32_t* bits, int64_t wpr, int6
         const dim3 grid((unsigned)(tiles * KS));
         if (big && nw == 8)
             hipLaunchKernelGGL((k_batch_i8_wide<128, 8>), grid, dim3(512), wide_lds_bytes<128>(), s,
-                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS, gate);
         else if (big)
             hipLaunchKernelGGL((k_batch_i8_wide<128, 4>), grid, dim3(256), wide_lds_bytes<128>(), s,
-                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS, gate);
         else if (nw == 8)
             hipLaunchKernelGGL((k_batch_i8_wide<64, 8>), grid, dim3(512), wide_lds_bytes<64>(), s,
-                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS, gate);
         else
             hipLaunchKernelGGL((k_batch_i8_wide<64, 4>), grid, dim3(256), wide_lds_bytes<64>(), s,
-                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS);
+                               (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi, MB, JB, KS, gate);
     } else {
         dim3 grid((unsigned)(g.n_pad / BN), (unsigned)((B + BM - 1) / BM));
-        hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi);
+        hipLaunchKernelGGL(k_batch_i8, grid, dim3(256), 0, s, (const int8_t*)g.W, g.n_pad, g.n_pad, bits, wpr, B, qi,
+                           gate);
     }
 }
 

@@ -45,6 +45,13 @@ struct Buffer {
     size_t bytes = 0;
 };
 
+// Pinned host memory with its device-visible address.
+struct Pinned {
+    void* p = nullptr;
+    void* dev = nullptr;
+    size_t bytes = 0;
+};
+
 }  // namespace pqi
 
 namespace pqi {
@@ -58,7 +65,9 @@ struct LsGraphs {
     // read-back of its control record into pinned host memory, as one graph
     // (nullptr: not available)
     hipGraphExec_t iter = nullptr;
-    void* host_ctl = nullptr;  // pinned, sizeof(LsCtl)
+    hipGraphExec_t iter2 = nullptr;  // int8 fast path: the same iteration reporting into the second slot
+    void* host_ctl = nullptr;        // pinned, sizeof(LsCtl)
+    hipEvent_t done_ev[2] = {nullptr, nullptr};  // int8 fast path: end of the iteration of each slot
 };
 }  // namespace pqi
 
@@ -67,9 +76,11 @@ struct pqi_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     std::vector<pqi::Buffer> bufs;  // indexed by slot
-    void* pinned = nullptr;      // pinned host staging (grown on demand)
-    void* pinned_dev = nullptr;  // its device-visible address
-    size_t pinned_bytes = 0;
+    // pinned host staging, grown on demand: [0] inputs and read-backs, [1] the
+    // local search's iteration reports (kept apart so that a report still
+    // being written by the previous call's last, idle iteration never lands on
+    // the next call's input)
+    pqi::Pinned pin[2];
     pqi::LsGraphs ls;
 };
 
@@ -120,7 +131,7 @@ void launch_batch_quadratic(const Graph& g, const uint32_t* bits, int64_t words_
 size_t batch_scratch_bytes(const Graph& g, int64_t B);
 // int8 MFMA contraction only: qi[b] += s_b^T W s_b (int64), qi zeroed by the caller.
 void launch_batch_i8_raw(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, unsigned long long* qi,
-                         hipStream_t s);
+                         hipStream_t s, const int32_t* gate = nullptr);
 // Split-BF16 MFMA energies for fp32 weights (needs g.Wbf); q approximate (see pqi_eval.hip).
 void launch_batch_bf16(const Graph& g, const uint32_t* bits, int64_t wpr, int64_t B, double* q, void* scratch,
                        hipStream_t s);
@@ -141,8 +152,9 @@ struct PhaseOut {
 
 // Repeated single flips (first argmax, strictly positive gain) until none improves.
 // With events e0/e1, they are recorded at the start and end of the kernel itself.
+// With a gate, the launch does nothing while *gate is nonzero.
 void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s,
-                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const int32_t* gate = nullptr);
 // Best single flip only (no move applied).
 void launch_single_best(const Graph& g, const uint8_t* x, const void* h, PhaseOut* out, hipStream_t s);
 // Best double flip: reference semantics (last pair of each of thread_count threads) or exhaustive.
@@ -154,7 +166,8 @@ void launch_flip(const Graph& g, uint8_t* x, void* h, int64_t k, int64_t l, hipS
 
 // Gray seeds: packed candidate rows for seeds [0, count) of theta.
 void launch_seed_bits(const uint8_t* x, int64_t n, int64_t words_per_row, int64_t block_size,
-                      int64_t count, uint32_t* bits, hipStream_t s, unsigned long long* qzero = nullptr);
+                      int64_t count, uint32_t* bits, hipStream_t s, unsigned long long* qzero = nullptr,
+                      const int32_t* gate = nullptr);
 // Seed cut differences from int64 quadratic forms (int8 weights): q_base = s^T h of
 // (x, h), delta[b] = (q_base - qi[b]) / 4; also clears rank[0..count).
 void launch_seed_delta_i8(const uint8_t* x, const int32_t* h, int64_t n, const unsigned long long* qi,
@@ -194,6 +207,7 @@ void launch_gray_walk(const Graph& g, const uint8_t* x, const void* h, const int
 // goes on, the descent is not being continued, and the round bound allows it.
 struct LsCtl;
 __host__ __device__ inline bool ls_new_round(const LsCtl* c);
+__host__ __device__ inline bool ls_stopped(const LsCtl* c);
 
 struct LsCtl {
     int32_t restart;    // a move of this iteration was accepted: later phases skip
@@ -211,9 +225,12 @@ struct LsCtl {
     uint64_t ticks[5];  // per phase: single, double, seeds, walk, reheat
 };
 
-__host__ __device__ inline bool ls_new_round(const LsCtl* c) {
-    return !c->again && !c->done && !(c->max_rounds > 0 && c->rounds >= c->max_rounds);
+// The search has stopped: done, or the round bound reached outside a continued descent.
+__host__ __device__ inline bool ls_stopped(const LsCtl* c) {
+    return c->done || (!c->again && c->max_rounds > 0 && c->rounds >= c->max_rounds);
 }
+
+__host__ __device__ inline bool ls_new_round(const LsCtl* c) { return !c->again && !ls_stopped(c); }
 void launch_ls_begin(LsCtl* c, hipStream_t s);
 // What the host reads after each iteration of the int8 fast path (pinned host memory).
 struct LsReport {

@@ -52,6 +52,9 @@ static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
     Buffer& b = c->bufs[slot];
     if (bytes == 0) bytes = 16;
     if (b.bytes < bytes) {
+        // queued work may still use the old buffer (a local search returns with
+        // its last, idle iteration still on the stream)
+        if (b.p) HIPCHK(hipStreamSynchronize(c->stream));
         if (b.p) HIPCHK(hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
@@ -199,15 +202,17 @@ static int classify(double maxabs, bool all_int, int dtype) {
 }
 
 static void ls_graphs_reset(pqi_ctx* c) {
+    (void)hipStreamSynchronize(c->stream);  // a replay may still be queued
     for (auto& e : c->ls.seg)
         if (e) {
             (void)hipGraphExecDestroy(e);
             e = nullptr;
         }
-    if (c->ls.iter) {
-        (void)hipGraphExecDestroy(c->ls.iter);
-        c->ls.iter = nullptr;
-    }
+    for (hipGraphExec_t* e : {&c->ls.iter, &c->ls.iter2})
+        if (*e) {
+            (void)hipGraphExecDestroy(*e);
+            *e = nullptr;
+        }
     c->ls.key.clear();
     c->ls.failed = false;
 }
@@ -273,7 +278,10 @@ int pqi_destroy(pqi_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     ls_graphs_reset(c);
     if (c->ls.host_ctl) (void)hipHostFree(c->ls.host_ctl);
-    if (c->pinned) (void)hipHostFree(c->pinned);
+    for (auto& e : c->ls.done_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& pb : c->pin)
+        if (pb.p) (void)hipHostFree(pb.p);
     for (auto& b : c->bufs)
         if (b.p) (void)hipFree(b.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -497,19 +505,18 @@ static int load_state(pqi_graph* g, const uint8_t* theta, uint8_t** x, void** h)
 }
 
 // Pinned host staging of at least `bytes` (grown on demand; freed by pqi_destroy).
-static int ensure_pinned(pqi_ctx* c, size_t bytes) {
-    if (c->pinned_bytes >= bytes) return PQI_OK;
-    if (c->pinned) {
+static int ensure_pinned(pqi_ctx* c, int which, size_t bytes) {
+    Pinned& pb = c->pin[which];
+    if (pb.bytes >= bytes) return PQI_OK;
+    if (pb.p) {
         HIPCHK(hipStreamSynchronize(c->stream));
-        (void)hipHostFree(c->pinned);
-        c->pinned = nullptr;
-        c->pinned_dev = nullptr;
-        c->pinned_bytes = 0;
+        (void)hipHostFree(pb.p);
+        pb = Pinned{};
     }
     const size_t want = std::max<size_t>(bytes, 4096);
-    HIPCHK(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
-    c->pinned_bytes = want;
-    HIPCHK(hipHostGetDevicePointer(&c->pinned_dev, c->pinned, 0));
+    HIPCHK(hipHostMalloc(&pb.p, want, hipHostMallocDefault));
+    pb.bytes = want;
+    HIPCHK(hipHostGetDevicePointer(&pb.dev, pb.p, 0));
     return PQI_OK;
 }
 
@@ -807,8 +814,11 @@ static int gray_seeds_buffers(pqi_graph* g, const pqi_search_params* p, GrayBufs
     return PQI_OK;
 }
 
+// Gray seeds of the state: their gains and the top `walkers` (sel, seld).  With
+// a gate, the seed rows and the int8 MFMA contraction do nothing while *gate is
+// nonzero (their results are then not used).
 static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_search_params* p, GrayBufs& b,
-                      int64_t* seed_count) {
+                      int64_t* seed_count, const int32_t* gate = nullptr) {
     pqi_ctx* c = g->ctx;
     const int64_t nb = (g->n + 63) >> 6;
     const int64_t bs = p->walkers * p->seed_multiple;
@@ -832,8 +842,8 @@ static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_s
         unsigned long long* qi = (unsigned long long*)scratch;
         // seed rows (clearing the row totals), int8 MFMA contraction, cut
         // differences against q_base = s^T h (clearing the ranks), top-k
-        launch_seed_bits(x, g->n, wpr, bs, count, bits, c->stream, qi);
-        launch_batch_i8_raw(*g, bits, wpr, count, qi, c->stream);
+        launch_seed_bits(x, g->n, wpr, bs, count, bits, c->stream, qi, gate);
+        launch_batch_i8_raw(*g, bits, wpr, count, qi, c->stream, gate);
         launch_seed_delta_i8(x, (const int32_t*)h, g->n, qi, count, b.delta, (int32_t*)rank, c->stream);
         launch_topk_gather(b.delta, count, p->walkers, b.sel, b.seld, (int32_t*)rank, c->stream);
         return PQI_OK;
@@ -966,6 +976,32 @@ static int read_field_repair(pqi_ctx* c, const FieldRepair* fix, pqi_search_stat
     return PQI_OK;
 }
 
+// Counters and phase clocks of a device-decided search from its control record.
+static void fill_search_stats(const pqi_ctx* c, const LsCtl& hc, pqi_search_stats* st) {
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    const double tick = 1e-3 / rate_khz;
+    st->rounds = hc.rounds;
+    st->max_energy = hc.max_energy;
+    st->single_evals = hc.single_evals;
+    st->single_accepts = hc.single_accepts;
+    st->double_evals = hc.double_evals;
+    st->double_accepts = hc.double_accepts;
+    st->seed_evals = hc.seed_evals;
+    st->seed_accepts = hc.seed_accepts;
+    st->walk_accepts = hc.walk_accepts;
+    st->reheat_accepts = hc.reheat_accepts;
+    st->gray_evals = hc.gray_evals;
+    st->gray_evals_computed = hc.gray_evals_computed;
+    st->descent_evals = hc.descent_evals;
+    st->seconds_single = hc.ticks[0] * tick;
+    st->seconds_double = hc.ticks[1] * tick;
+    st->seconds_seeds = hc.ticks[2] * tick;
+    st->seconds_walk = hc.ticks[3] * tick;
+    st->seconds_reheat = hc.ticks[4] * tick;
+}
+
 // Host-driven loop: one synchronisation per phase decision (PQI_FLAG_HOST_LOOP).
 static int local_search_host(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, pqi_search_stats* st);
 
@@ -992,24 +1028,35 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     // after the last synchronisation
     const bool fast = iter_graph_enabled && g->kind == PQI_WEIGHTS_I8 && g->adj_ptr &&
                       (g->use_adj || g->W == nullptr) && n <= 65536 && !(p->flags & PQI_FLAG_TEST_DRIFT);
-    const size_t off_rep = ((size_t)n + 255) & ~(size_t)255;
-    const size_t off_xout = off_rep + ((sizeof(LsReport) + 255) & ~(size_t)255);
+    // report buffer layout: two slots of (report, state)
+    const size_t slot_bytes = ((sizeof(LsReport) + 255) & ~(size_t)255) + (((size_t)n + 255) & ~(size_t)255);
+    const size_t off_rep[2] = {0, slot_bytes};
+    const size_t off_xout[2] = {off_rep[0] + ((sizeof(LsReport) + 255) & ~(size_t)255),
+                                off_rep[1] + ((sizeof(LsReport) + 255) & ~(size_t)255)};
     uint8_t *x, *rx;
     void *h, *rh;
     if (fast) {
         for (int64_t i = 0; i < n; ++i)
             if (theta[i] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
-        CHK(ensure_pinned(c, off_xout + (size_t)n));
-        std::memcpy(c->pinned, theta, (size_t)n);
+        CHK(ensure_pinned(c, 0, (size_t)n));
+        CHK(ensure_pinned(c, 1, off_xout[1] + (size_t)n));
+        std::memcpy(c->pin[0].p, theta, (size_t)n);
         CHK(ensure(c, S_X, n, (void**)&x));
         CHK(ensure(c, S_H, n * fb, &h));
     } else {
         CHK(load_state(g, theta, &x, &h));
     }
-    LsReport* rep = fast ? (LsReport*)((char*)c->pinned + off_rep) : nullptr;
-    LsReport* rep_dev = fast ? (LsReport*)((char*)c->pinned_dev + off_rep) : nullptr;
-    uint8_t* xout = fast ? (uint8_t*)c->pinned + off_xout : nullptr;
-    uint8_t* xout_dev = fast ? (uint8_t*)c->pinned_dev + off_xout : nullptr;
+    LsReport* rep[2] = {nullptr, nullptr};
+    LsReport* rep_dev[2] = {nullptr, nullptr};
+    uint8_t* xout[2] = {nullptr, nullptr};
+    uint8_t* xout_dev[2] = {nullptr, nullptr};
+    if (fast)
+        for (int k = 0; k < 2; ++k) {
+            rep[k] = (LsReport*)((char*)c->pin[1].p + off_rep[k]);
+            rep_dev[k] = (LsReport*)((char*)c->pin[1].dev + off_rep[k]);
+            xout[k] = (uint8_t*)c->pin[1].p + off_xout[k];
+            xout_dev[k] = (uint8_t*)c->pin[1].dev + off_xout[k];
+        }
     PhaseOut *dout, *dout2;
     void* dsc;
     LsCtl* ctl;
@@ -1072,7 +1119,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                 launch_double_best(*g, x, h, G, p->double_mode, dout2, dsc, c->stream);
                 launch_ls_after_double(*g, ctl, dout2, x, h, dbl_evals, c->stream);
             }
-            CHK(gray_seeds(g, x, h, p, b, &count));
+            CHK(gray_seeds(g, x, h, p, b, &count, &ctl->restart));
             // the best seed, if it improves, is applied to the state
             launch_ls_seed_take(*g, ctl, b.seld, count, b.sel, bs, x, h, c->stream);
         } else if (fused) {
@@ -1111,11 +1158,11 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         launch_field_repair(*g, x, h, ctl, fix, c->stream);
         launch_ls_begin(ctl, c->stream);
     };
-    auto tail = [&]() -> int {
+    auto tail = [&](int slot) -> int {
         CHK(segment(0));
         launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
         CHK(segment(1));
-        if (fast) launch_ls_report_i8(*g, x, h, ctl, fix, rep_dev, xout_dev, c->stream);
+        if (fast) launch_ls_report_i8(*g, x, h, ctl, fix, rep_dev[slot], xout_dev[slot], c->stream);
         else HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
         prologue();
         return PQI_OK;
@@ -1123,7 +1170,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     // captured once per (graph, parameters, buffers) and replayed per iteration
     const std::vector<int64_t> key = {(int64_t)g, (int64_t)g->W, (int64_t)g->adj_ptr, n, G, T, bs, (int64_t)timing,
                                       (int64_t)want_iter, (int64_t)fix, (int64_t)c->ls.host_ctl, (int64_t)fast,
-                                      (int64_t)rep_dev, (int64_t)xout_dev,
+                                      (int64_t)rep_dev[0], (int64_t)rep_dev[1],
                                       p->double_mode, p->seed_engine, p->is_spin_glass, (int64_t)x, (int64_t)h,
                                       (int64_t)rx, (int64_t)rh, (int64_t)dout, (int64_t)dout2, (int64_t)dsc,
                                       (int64_t)ctl, (int64_t)b.wx, (int64_t)b.wh, (int64_t)b.we, (int64_t)b.wev,
@@ -1132,21 +1179,24 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                                       (int64_t)c->bufs[S_RANK].p};
     if (c->ls.key != key) {
         ls_graphs_reset(c);
-        if (want_iter) {
+        // the fast path captures the iteration once per report slot
+        for (int slot = 0; want_iter && slot < (fast ? 2 : 1); ++slot) {
+            hipGraphExec_t& exec = slot ? c->ls.iter2 : c->ls.iter;
             hipGraph_t graph = nullptr;
             if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                 // a failure here only means the capture is not available: the
-                // segment path below repeats the launches and reports real errors
-                if (!timing) launch_descent(*g, x, h, cap, dout, c->stream);
-                const int r = tail();
+                // direct launches below repeat the sequence and report real errors
+                if (!timing) launch_descent(*g, x, h, cap, dout, c->stream, nullptr, nullptr, &ctl->restart);
+                const int r = tail(slot);
                 const hipError_t e = hipStreamEndCapture(c->stream, &graph);
                 if (r != PQI_OK || e != hipSuccess || !graph ||
-                    hipGraphInstantiate(&c->ls.iter, graph, nullptr, nullptr, 0) != hipSuccess)
-                    c->ls.iter = nullptr;
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                    exec = nullptr;
                 if (graph) (void)hipGraphDestroy(graph);
             }
             (void)hipGetLastError();
         }
+        if (fast && !(c->ls.iter && c->ls.iter2)) ls_graphs_reset(c);  // both slots or neither
         for (int w = 0; w < 2 && !c->ls.failed && !c->ls.iter; ++w) {
             hipGraph_t graph = nullptr;
             if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -1176,21 +1226,58 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         return segment(which);
     };
     if (fast) {
-        launch_ls_start_i8(*g, (const uint8_t*)c->pinned_dev, x, h, rx, rh, ctl, fix, hc.scale, p->max_rounds,
+        // Iteration k + 1 is queued before the host reads iteration k's report
+        // (slot k mod 2), so the GPU never waits for the host between
+        // iterations.  Once the search has stopped, the iteration reset leaves
+        // `restart` set and every phase of the queued iteration does nothing; it
+        // finishes on the stream after this call returns.
+        for (auto& e : c->ls.done_ev)
+            if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        EventTimer dt1(timing);
+        EventTimer* dtimer[2] = {&descent_timer, &dt1};
+        auto enqueue = [&](int slot) -> int {
+            hipGraphExec_t exec = slot ? c->ls.iter2 : c->ls.iter;
+            if (timing || !exec)
+                launch_descent(*g, x, h, cap, dout, c->stream, dtimer[slot]->start_event(),
+                               dtimer[slot]->stop_event(), &ctl->restart);
+            if (exec) HIPCHK(hipGraphLaunch(exec, c->stream));
+            else CHK(tail(slot));
+            HIPCHK(hipEventRecord(c->ls.done_ev[slot], c->stream));
+            return PQI_OK;
+        };
+        launch_ls_start_i8(*g, (const uint8_t*)c->pin[0].dev, x, h, rx, rh, ctl, fix, hc.scale, p->max_rounds,
                            p->is_spin_glass, c->stream);
-    } else if (c->ls.iter) {
-        prologue();
+        CHK(enqueue(0));
+        int slot = 0;
+        for (;;) {
+            CHK(enqueue(slot ^ 1));
+            HIPCHK(hipEventSynchronize(c->ls.done_ev[slot]));
+            std::memcpy(&hc, &rep[slot]->ctl, sizeof(hc));
+            if (timing) {
+                st->descent_kernel_seconds += dtimer[slot]->seconds();
+                st->descent_launches++;
+            }
+            if (ls_stopped(&hc)) break;
+            slot ^= 1;
+        }
+        const LsReport& r = *rep[slot];
+        fill_search_stats(c, hc, st);
+        field_repair_stats(r.fix, st);
+        st->cut_value = r.cut;
+        st->min_energy = p->is_spin_glass ? -r.energy : r.energy;
+        std::memcpy(theta, xout[slot], (size_t)n);
+        return PQI_OK;
     }
+    if (c->ls.iter) prologue();
     for (;;) {
         if (!hc.again && p->max_rounds > 0 && hc.rounds >= p->max_rounds) break;
-        if (c->ls.iter || fast) {
-            // (without the captured graph, the fast path launches the same sequence directly)
-            if (timing || !c->ls.iter)
-                launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event());
-            if (c->ls.iter) HIPCHK(hipGraphLaunch(c->ls.iter, c->stream));
-            else CHK(tail());
+        if (c->ls.iter) {
+            if (timing)
+                launch_descent(*g, x, h, cap, dout, c->stream, descent_timer.start_event(), descent_timer.stop_event(),
+                               &ctl->restart);
+            HIPCHK(hipGraphLaunch(c->ls.iter, c->stream));
             CHK(sync(c));
-            std::memcpy(&hc, fast ? (const void*)&rep->ctl : c->ls.host_ctl, sizeof(hc));
+            std::memcpy(&hc, c->ls.host_ctl, sizeof(hc));
             if (timing) {
                 st->descent_kernel_seconds += descent_timer.seconds();
                 st->descent_launches++;
@@ -1218,41 +1305,12 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         }
         if (hc.done) break;
     }
-    int rate_khz = 0;
-    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || rate_khz <= 0)
-        rate_khz = 100000;
-    const double tick = 1e-3 / rate_khz;
-    st->rounds = hc.rounds;
-    st->max_energy = hc.max_energy;
-    st->single_evals = hc.single_evals;
-    st->single_accepts = hc.single_accepts;
-    st->double_evals = hc.double_evals;
-    st->double_accepts = hc.double_accepts;
-    st->seed_evals = hc.seed_evals;
-    st->seed_accepts = hc.seed_accepts;
-    st->walk_accepts = hc.walk_accepts;
-    st->reheat_accepts = hc.reheat_accepts;
-    st->gray_evals = hc.gray_evals;
-    st->gray_evals_computed = hc.gray_evals_computed;
-    st->descent_evals = hc.descent_evals;
-    st->seconds_single = hc.ticks[0] * tick;
-    st->seconds_double = hc.ticks[1] * tick;
-    st->seconds_seeds = hc.ticks[2] * tick;
-    st->seconds_walk = hc.ticks[3] * tick;
-    st->seconds_reheat = hc.ticks[4] * tick;
-    if (fast) {
-        // the last iteration's report holds everything
-        field_repair_stats(rep->fix, st);
-        st->cut_value = rep->cut;
-        st->min_energy = p->is_spin_glass ? -rep->energy : rep->energy;
-        std::memcpy(theta, xout, (size_t)n);
-        return PQI_OK;
-    }
+    fill_search_stats(c, hc, st);
     // field repairs, the exact score and the state come back in one
     // synchronisation, through pinned memory
     launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);
-    CHK(ensure_pinned(c, 64 + (size_t)n));
-    char* pb = (char*)c->pinned;
+    CHK(ensure_pinned(c, 0, 64 + (size_t)n));
+    char* pb = (char*)c->pin[0].p;
     HIPCHK(hipMemcpyAsync(pb, fix, sizeof(FieldRepair), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(pb + 32, score, 16, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(pb + 64, x, n, hipMemcpyDeviceToHost, c->stream));

@@ -462,7 +462,8 @@ __device__ void descent_body(const GView<WT>& g, uint8_t* x, FT* h, int64_t max_
 
 template <typename WT, typename FT>
 __global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h, int64_t max_steps,
-                                                 PhaseOut* out) {
+                                                 PhaseOut* out, const int32_t* gate) {
+    if (gate && *gate) return;
     descent_body<WT, FT>(g, x, h, max_steps, out);
 }
 
@@ -708,8 +709,9 @@ __device__ void descent_key_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
 }
 
 __global__ __launch_bounds__(64) void k_descent_key(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
-                                                   PhaseOut* out, size_t lds_bytes, int ib) {
+                                                   PhaseOut* out, size_t lds_bytes, int ib, const int32_t* gate) {
     extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
     descent_key_body(g, xg, hg, max_steps, out, lds, lds_bytes, ib);
 }
 
@@ -859,8 +861,9 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
 
 template <int R, int NB>
 __global__ __launch_bounds__(64) void k_descent_reg(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
-                                                   PhaseOut* out, size_t lds_bytes, int ib) {
+                                                   PhaseOut* out, size_t lds_bytes, int ib, const int32_t* gate) {
     extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
     descent_reg_body<R, NB>(g, xg, hg, max_steps, out, lds, lds_bytes, ib);
 }
 
@@ -881,7 +884,7 @@ static int reg_descent_enabled() { return descent_variant() == 0; }
 
 template <int R, int NB>
 __global__ __launch_bounds__(64) void k_descent_reg(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
-                                                   PhaseOut* out, size_t lds_bytes, int ib);
+                                                   PhaseOut* out, size_t lds_bytes, int ib, const int32_t* gate);
 
 // LDS available to the register-resident kernels (their neighbour table must fit)
 static size_t reg_lds_cap() {
@@ -916,11 +919,11 @@ static int reg_nb(int64_t stride) { return stride <= 4 ? 4 : (stride <= 8 ? 8 : 
 
 template <int R, int NB>
 static void launch_reg(const GView<int8_t>& v, uint8_t* x, int32_t* h, int64_t max_steps, PhaseOut* out,
-                       size_t table, int ib, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                       size_t table, int ib, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const int32_t* gate) {
     static const size_t limit = lds_limit((const void*)k_descent_reg<R, NB>);
     (void)limit;  // the plan admits only tables within it
     hipExtLaunchKernelGGL((k_descent_reg<R, NB>), dim3(1), dim3(64), (uint32_t)table, s, e0, e1, 0, v, x, h,
-                          max_steps, out, table, ib);
+                          max_steps, out, table, ib, gate);
 }
 
 // Key width for the packed-key descent of g, or 0 when it does not apply.
@@ -1054,8 +1057,9 @@ __device__ void descent_wave2_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
 
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void k_descent_wave2(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
-                                                     PhaseOut* out) {
+                                                     PhaseOut* out, const int32_t* gate) {
     extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
     descent_wave2_body<WT, FT>(g, xg, hg, max_steps, out, lds);
 }
 
@@ -1068,8 +1072,9 @@ static size_t descent2_lds(int64_t n) {
 
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
-                                                    PhaseOut* out) {
+                                                    PhaseOut* out, const int32_t* gate) {
     extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
     descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds);
 }
 
@@ -1077,14 +1082,14 @@ __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, F
 // and stop events time the kernel itself (no events: a plain launch).
 template <typename WT>
 static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s,
-                         hipEvent_t e0, hipEvent_t e1) {
+                         hipEvent_t e0, hipEvent_t e1, const int32_t* gate) {
     using FT = typename FieldOf<WT>::T;
     if constexpr (sizeof(WT) == 1) {
         size_t table = 0;
         const int regs = reg_descent_plan(g, &table);
         if (regs) {
             PQI_REG_DISPATCH(regs, reg_nb(g.key_adj_stride), launch_reg, view<int8_t>(g), x, (int32_t*)h, max_steps,
-                             out, table, key_shift(g), s, e0, e1);
+                             out, table, key_shift(g), s, e0, e1, gate);
             return;
         }
         static const size_t klimit = lds_limit((const void*)k_descent_key);
@@ -1092,7 +1097,7 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
         const size_t kl = key_lds_bytes(g, klimit);
         if (ib > 0 && kl <= klimit && descent_variant() != 3) {
             hipExtLaunchKernelGGL(k_descent_key, dim3(1), dim3(64), (uint32_t)kl, s, e0, e1, 0, view<int8_t>(g), x,
-                                  (int32_t*)h, max_steps, out, kl, ib);
+                                  (int32_t*)h, max_steps, out, kl, ib, gate);
             return;
         }
     }
@@ -1102,18 +1107,18 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
     const bool sparse = (g.use_adj || g.W == nullptr) && descent_variant() == 2;
     if (sparse && descent2_lds<FT>(g.n) <= limit2)
         hipExtLaunchKernelGGL(k_descent_wave2<WT, FT>, dim3(1), dim3(64), (uint32_t)descent2_lds<FT>(g.n), s, e0, e1,
-                              0, view<WT>(g), x, (FT*)h, max_steps, out);
+                              0, view<WT>(g), x, (FT*)h, max_steps, out, gate);
     else if (lds <= limit)
         hipExtLaunchKernelGGL(k_descent_wave<WT, FT>, dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, view<WT>(g), x,
-                              (FT*)h, max_steps, out);
+                              (FT*)h, max_steps, out, gate);
     else
         hipExtLaunchKernelGGL(k_descent<WT, FT>, dim3(1), dim3(1024), 0, s, e0, e1, 0, view<WT>(g), x, (FT*)h,
-                              max_steps, out);
+                              max_steps, out, gate);
 }
 
 void launch_descent(const Graph& g, uint8_t* x, void* h, int64_t max_steps, PhaseOut* out, hipStream_t s,
-                    hipEvent_t e0, hipEvent_t e1) {
-    PQI_DISPATCH(g, descent_impl, g, x, h, max_steps, out, s, e0, e1);
+                    hipEvent_t e0, hipEvent_t e1, const int32_t* gate) {
+    PQI_DISPATCH(g, descent_impl, g, x, h, max_steps, out, s, e0, e1, gate);
 }
 
 // Best single flip only: one scan of Delta_k = s_k h_k over the state in
@@ -1320,7 +1325,9 @@ __device__ __forceinline__ int64_t seed_pos(int64_t n, int64_t offset, int j) {
 }
 
 __global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs,
-                                                   int64_t count, uint32_t* bits, unsigned long long* qzero) {
+                                                   int64_t count, uint32_t* bits, unsigned long long* qzero,
+                                                   const int32_t* gate) {
+    if (gate && *gate) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * wpr) return;
     const int64_t b = t / wpr, w = t % wpr;
@@ -1364,10 +1371,10 @@ __global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, 
 }
 
 void launch_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs, int64_t count, uint32_t* bits,
-                      hipStream_t s, unsigned long long* qzero) {
+                      hipStream_t s, unsigned long long* qzero, const int32_t* gate) {
     int64_t threads = count * wpr;
     hipLaunchKernelGGL(k_seed_bits, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, x, n, wpr, bs, count,
-                       bits, qzero);
+                       bits, qzero, gate);
 }
 
 // Each work group recomputes q_base = s^T h (n terms, int64, exact) and
@@ -1888,7 +1895,14 @@ __device__ __forceinline__ void ls_mark(LsCtl* c, int phase) {
     c->phase = phase;
 }
 
+// The reset of an iteration.  When the search has stopped (done, or the round
+// bound reached outside a continued descent), `restart` stays set instead, so
+// every phase of a further iteration already queued is skipped.
 __global__ void k_ls_begin(LsCtl* c) {
+    if (ls_stopped(c)) {
+        c->restart = 1;
+        return;
+    }
     if (!c->again) c->rounds++;
     c->again = 0;
     c->restart = 0;
@@ -1925,7 +1939,10 @@ __device__ __forceinline__ void ls_after_descent_body(LsCtl* c, const PhaseOut* 
     }
 }
 
-__global__ void k_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap) { ls_after_descent_body(c, o, cap); }
+__global__ void k_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap) {
+    if (c->restart) return;  // the search stopped: the descent was gated off
+    ls_after_descent_body(c, o, cap);
+}
 
 template <typename FT, typename WT>
 __device__ __forceinline__ void copy_state(uint8_t* xd, FT* hd, const uint8_t* xs, const FT* hs, int64_t n) {
@@ -1968,6 +1985,7 @@ template <typename WT, typename FT>
 __global__ __launch_bounds__(1024) void k_ls_descent_double(GView<WT> g, LsCtl* c, const PhaseOut* dout,
                                                            int64_t cap, uint8_t* x, FT* h, int64_t thread_count,
                                                            PhaseOut* dout2, int64_t evals) {
+    if (ls_restarted(c)) return;  // the search stopped: the descent was gated off
     if (threadIdx.x == 0) ls_after_descent_body(c, dout, cap);
     if (ls_restarted(c)) return;
     double_ref_body<WT, FT>(g, x, h, thread_count, dout2);

@@ -191,6 +191,29 @@ def test_batched_register_descent_matches_single(sizes):
         assert stats[i]["single_accepts"] == st["single_accepts"], i
 
 
+def test_consecutive_searches_of_different_sizes():
+    """A local search returns while its last (idle) iteration may still be on
+    the stream.  Searches of different sizes back to back on one context --
+    larger after smaller, with the pinned staging already large -- give the
+    host-decided loop's states and counters."""
+    log("consecutive searches of different sizes")
+    ns = (3000, 700, 2048, 64, 1500)
+    Gs = [regular_int(n, 3 + i % 4, 90 + i)[0] for i, n in enumerate(ns)]
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    rng = np.random.default_rng(91)
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in Gs]
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=2000)
+    ph = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=2000)
+    ph.flags = _lib.FLAG_HOST_LOOP
+    ref = [dg.local_search(x0, ph) for dg, x0 in zip(dgs, starts)]
+    for rep in range(2):
+        for i, (dg, x0) in enumerate(zip(dgs, starts)):
+            xd, sd = dg.local_search(x0, p)
+            assert np.array_equal(xd, ref[i][0]), (rep, i)
+            for k in ("cut_value", "max_energy", "rounds", "single_accepts", "walk_accepts"):
+                assert sd[k] == ref[i][1][k], (rep, i, k)
+
+
 COUNTERS = ("max_energy", "cut_value", "min_energy", "rounds", "single_accepts", "double_accepts", "seed_accepts",
             "walk_accepts", "reheat_accepts", "single_evals", "double_evals", "seed_evals", "gray_evals",
             "gray_evals_computed", "descent_evals")
