@@ -165,9 +165,14 @@ size_t double_scratch_bytes(const Graph& g);
 void launch_flip(const Graph& g, uint8_t* x, void* h, int64_t k, int64_t l, hipStream_t s);
 
 // Gray seeds: packed candidate rows for seeds [0, count) of theta.
+// qzero / rzero (optional): row totals and rank counts set to zero
 void launch_seed_bits(const uint8_t* x, int64_t n, int64_t words_per_row, int64_t block_size,
                       int64_t count, uint32_t* bits, hipStream_t s, unsigned long long* qzero = nullptr,
-                      const int32_t* gate = nullptr);
+                      const int32_t* gate = nullptr, int32_t* rzero = nullptr);
+// int8 MFMA seeds: stable top k by row total (smallest q first = largest cut difference,
+// ties by index) into sel (rank counts zeroed beforehand)
+void launch_topk_q(const unsigned long long* qi, int64_t count, int64_t k, int32_t* rank, int64_t* sel,
+                   hipStream_t s);
 // Seed cut differences from int64 quadratic forms (int8 weights): q_base = s^T h of
 // (x, h), delta[b] = (q_base - qi[b]) / 4; also clears rank[0..count).
 void launch_seed_delta_i8(const uint8_t* x, const int32_t* h, int64_t n, const unsigned long long* qi,
@@ -256,8 +261,9 @@ void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t
 // walk decision + post-reheat single and double flips and their decisions.
 void launch_ls_descent_double(const Graph& g, LsCtl* c, const PhaseOut* dout, int64_t cap, uint8_t* x, void* h,
                               int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s);
-void launch_ls_seed_take(const Graph& g, LsCtl* c, const double* seld, int64_t count, const int64_t* sel, int64_t bs,
-                         uint8_t* x, void* h, hipStream_t s);
+// With qi (int8 MFMA seeds), the selected seeds' cut differences are computed here into seld.
+void launch_ls_seed_take(const Graph& g, LsCtl* c, double* seld, int64_t count, const int64_t* sel, int64_t bs,
+                         uint8_t* x, void* h, hipStream_t s, const unsigned long long* qi = nullptr, int64_t G = 0);
 void launch_ls_walk_reheat(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
                            int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
                            void* rh, PhaseOut* dout, int64_t thread_count, PhaseOut* dout2, int64_t dbl_evals,

@@ -816,9 +816,12 @@ static int gray_seeds_buffers(pqi_graph* g, const pqi_search_params* p, GrayBufs
 
 // Gray seeds of the state: their gains and the top `walkers` (sel, seld).  With
 // a gate, the seed rows and the int8 MFMA contraction do nothing while *gate is
-// nonzero (their results are then not used).
+// nonzero (their results are then not used).  With qi_out, the int8 MFMA engine
+// selects straight from the row totals, fills sel only and returns the totals
+// (the caller computes seld; nullptr when that engine does not apply).
 static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_search_params* p, GrayBufs& b,
-                      int64_t* seed_count, const int32_t* gate = nullptr) {
+                      int64_t* seed_count, const int32_t* gate = nullptr, unsigned long long** qi_out = nullptr) {
+    if (qi_out) *qi_out = nullptr;
     pqi_ctx* c = g->ctx;
     const int64_t nb = (g->n + 63) >> 6;
     const int64_t bs = p->walkers * p->seed_multiple;
@@ -840,6 +843,15 @@ static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_s
         void* rank;
         CHK(ensure(c, S_RANK, topk_scratch_bytes(count), &rank));
         unsigned long long* qi = (unsigned long long*)scratch;
+        if (qi_out) {
+            // seed rows (clearing the row totals and ranks), int8 MFMA
+            // contraction, top-k by row total
+            launch_seed_bits(x, g->n, wpr, bs, count, bits, c->stream, qi, gate, (int32_t*)rank);
+            launch_batch_i8_raw(*g, bits, wpr, count, qi, c->stream, gate);
+            launch_topk_q(qi, count, p->walkers, (int32_t*)rank, b.sel, c->stream);
+            *qi_out = qi;
+            return PQI_OK;
+        }
         // seed rows (clearing the row totals), int8 MFMA contraction, cut
         // differences against q_base = s^T h (clearing the ranks), top-k
         launch_seed_bits(x, g->n, wpr, bs, count, bits, c->stream, qi, gate);
@@ -1119,9 +1131,10 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                 launch_double_best(*g, x, h, G, p->double_mode, dout2, dsc, c->stream);
                 launch_ls_after_double(*g, ctl, dout2, x, h, dbl_evals, c->stream);
             }
-            CHK(gray_seeds(g, x, h, p, b, &count, &ctl->restart));
+            unsigned long long* qi = nullptr;
+            CHK(gray_seeds(g, x, h, p, b, &count, &ctl->restart, &qi));
             // the best seed, if it improves, is applied to the state
-            launch_ls_seed_take(*g, ctl, b.seld, count, b.sel, bs, x, h, c->stream);
+            launch_ls_seed_take(*g, ctl, b.seld, count, b.sel, bs, x, h, c->stream, qi, G);
         } else if (fused) {
             launch_ls_walk_reheat(*g, ctl, b.we, b.wev, G, gray_evals, b.wx, b.wh, x, h, rx, rh, dout, G, dout2,
                                   dbl_evals, c->stream);

@@ -1326,12 +1326,16 @@ __device__ __forceinline__ int64_t seed_pos(int64_t n, int64_t offset, int j) {
 
 __global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs,
                                                    int64_t count, uint32_t* bits, unsigned long long* qzero,
-                                                   const int32_t* gate) {
+                                                   int32_t* rzero, const int32_t* gate) {
     if (gate && *gate) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * wpr) return;
     const int64_t b = t / wpr, w = t % wpr;
-    if (qzero && w == 0) qzero[b] = 0ull;  // the MFMA row totals of this seed start at zero
+    if (w == 0) {
+        // this seed's MFMA row total and rank count start at zero
+        if (qzero) qzero[b] = 0ull;
+        if (rzero) rzero[b] = 0;
+    }
     // the state's word w: 4 bytes of 0/1 -> 4 bits by one multiply, when the
     // 32 bytes lie inside x and x is 16-byte aligned
     uint32_t word = 0;
@@ -1371,10 +1375,49 @@ __global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, 
 }
 
 void launch_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs, int64_t count, uint32_t* bits,
-                      hipStream_t s, unsigned long long* qzero, const int32_t* gate) {
+                      hipStream_t s, unsigned long long* qzero, const int32_t* gate, int32_t* rzero) {
     int64_t threads = count * wpr;
     hipLaunchKernelGGL(k_seed_bits, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, x, n, wpr, bs, count,
-                       bits, qzero, gate);
+                       bits, qzero, rzero, gate);
+}
+
+// Stable top k of the int8 MFMA seeds straight from their row totals: a larger
+// cut difference (q_base - q_b) / 4 is a smaller q_b, so seed a's rank counts
+// the seeds with a smaller q, or an equal q and a lower index -- the order of
+// k_rank over the cut differences.  Comparisons are split over gridDim.y work
+// groups (integer atomics on counts zeroed by k_seed_bits).
+__global__ __launch_bounds__(256) void k_rank_q(const unsigned long long* qi, int64_t count, int64_t span,
+                                               int32_t* rank) {
+    __shared__ long long tile[256];
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long qa = a < count ? (long long)qi[a] : 0;
+    const int64_t lo = (int64_t)blockIdx.y * span;
+    const int64_t hi = lo + span < count ? lo + span : count;
+    int32_t r = 0;
+    for (int64_t t0 = lo; t0 < hi; t0 += 256) {
+        __syncthreads();
+        if (t0 + threadIdx.x < hi) tile[threadIdx.x] = (long long)qi[t0 + threadIdx.x];
+        __syncthreads();
+        const int lim = (int)(hi - t0 < 256 ? hi - t0 : 256);
+        for (int q = 0; q < lim; ++q) {
+            const long long qb = tile[q];
+            const int64_t b = t0 + q;
+            r += (qb < qa || (qb == qa && b < a)) ? 1 : 0;
+        }
+    }
+    if (a < count && r) atomicAdd(&rank[a], r);
+}
+
+__global__ void k_rank_scatter(const int32_t* rank, int64_t count, int64_t k, int64_t* sel);
+
+void launch_topk_q(const unsigned long long* qi, int64_t count, int64_t k, int32_t* rank, int64_t* sel,
+                   hipStream_t s) {
+    const unsigned gx = (unsigned)((count + 255) / 256);
+    int64_t splits = (count + 63) / 64;  // about 64 comparisons per thread, at most 64 splits
+    if (splits > 64) splits = 64;
+    const int64_t span = (count + splits - 1) / splits;
+    hipLaunchKernelGGL(k_rank_q, dim3(gx, (unsigned)splits), dim3(256), 0, s, qi, count, span, rank);
+    hipLaunchKernelGGL(k_rank_scatter, dim3(gx), dim3(256), 0, s, rank, count, k, sel);
 }
 
 // Each work group recomputes q_base = s^T h (n terms, int64, exact) and
@@ -2004,9 +2047,25 @@ __device__ __forceinline__ void ls_after_seeds_body(LsCtl* c, const double* seld
 // field arithmetic, as the one-walker, zero-iteration walk followed by the
 // copy of its state (gray_mutation, maxcut_tfim_util.py:740-749).
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_ls_seed_take(GView<WT> g, LsCtl* c, const double* seld, int64_t count,
-                                                      const int64_t* sel, int64_t bs, uint8_t* x, FT* h) {
+__global__ __launch_bounds__(1024) void k_ls_seed_take(GView<WT> g, LsCtl* c, double* seld, int64_t count,
+                                                      const int64_t* sel, int64_t bs, uint8_t* x, FT* h,
+                                                      const unsigned long long* qi, int64_t G) {
     __shared__ int take;
+    __shared__ long long part[16];
+    if (qi) {
+        // int8 MFMA seeds: the selected seeds' cut differences (q_base - q_b) / 4,
+        // q_base = s^T h of the state (k_seed_delta_i8's arithmetic)
+        if (ls_restarted(c)) return;
+        long long acc = 0;
+        for (int64_t v = threadIdx.x; v < g.n; v += blockDim.x) acc += x[v] ? -(long long)h[v] : (long long)h[v];
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        long long qb = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) qb += part[w];
+        for (int64_t r = threadIdx.x; r < G; r += blockDim.x) seld[r] = (double)(qb - (long long)qi[sel[r]]) * 0.25;
+        __syncthreads();
+    }
     if (threadIdx.x == 0 && !c->restart) ls_after_seeds_body(c, seld, count);
     if (ls_restarted(c)) return;
     if (threadIdx.x == 0) take = !c->skip_seed;
@@ -2296,15 +2355,15 @@ void launch_ls_descent_double(const Graph& g, LsCtl* c, const PhaseOut* dout, in
 }
 
 template <typename WT>
-static void ls_seed_take_impl(const Graph& g, LsCtl* c, const double* seld, int64_t count, const int64_t* sel,
-                              int64_t bs, uint8_t* x, void* h, hipStream_t s) {
+static void ls_seed_take_impl(const Graph& g, LsCtl* c, double* seld, int64_t count, const int64_t* sel, int64_t bs,
+                              uint8_t* x, void* h, const unsigned long long* qi, int64_t G, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
     hipLaunchKernelGGL((k_ls_seed_take<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, seld, count, sel, bs, x,
-                       (FT*)h);
+                       (FT*)h, qi, G);
 }
-void launch_ls_seed_take(const Graph& g, LsCtl* c, const double* seld, int64_t count, const int64_t* sel, int64_t bs,
-                         uint8_t* x, void* h, hipStream_t s) {
-    PQI_DISPATCH(g, ls_seed_take_impl, g, c, seld, count, sel, bs, x, h, s);
+void launch_ls_seed_take(const Graph& g, LsCtl* c, double* seld, int64_t count, const int64_t* sel, int64_t bs,
+                         uint8_t* x, void* h, hipStream_t s, const unsigned long long* qi, int64_t G) {
+    PQI_DISPATCH(g, ls_seed_take_impl, g, c, seld, count, sel, bs, x, h, qi, G, s);
 }
 
 template <typename WT>
