@@ -67,7 +67,13 @@ struct LsGraphs {
     hipGraphExec_t iter = nullptr;
     hipGraphExec_t iter2 = nullptr;  // int8 fast path: the same iteration reporting into the second slot
     void* host_ctl = nullptr;        // pinned, sizeof(LsCtl)
-    hipEvent_t done_ev[2] = {nullptr, nullptr};  // int8 fast path: end of the iteration of each slot
+    // int8 fast path: iteration reports are numbered by a device counter
+    // (seq_dev, incremented by every report kernel); seq_host counts the
+    // iterations queued, and is re-read from the device after a call that
+    // did not finish normally (seq_dirty)
+    unsigned long long* seq_dev = nullptr;
+    unsigned long long seq_host = 0;
+    bool seq_dirty = true;
 };
 }  // namespace pqi
 
@@ -242,15 +248,17 @@ struct LsReport {
     LsCtl ctl;
     FieldRepair fix;
     double cut, energy;  // exact score of the state after the iteration
+    unsigned long long seq;  // written last: the number of the report
 };
 // int8 weights with adjacency lists: the whole start of a local search in one launch
 // (theta_dev: the caller's 0/1 bytes in pinned host memory, device-visible address)
 void launch_ls_start_i8(const Graph& g, const uint8_t* theta_dev, uint8_t* x, void* h, uint8_t* rx, void* rh,
                         LsCtl* c, FieldRepair* fix, double scale, int64_t max_rounds, int is_spin_glass,
                         hipStream_t s);
-// int8 weights: control record, repair counters, exact score and state into a pinned report
-void launch_ls_report_i8(const Graph& g, const uint8_t* x, const void* h, const LsCtl* c, const FieldRepair* fix,
-                         LsReport* rep_dev, uint8_t* x_out_dev, hipStream_t s);
+// int8 weights: control record, repair counters, exact score and state into a
+// pinned report numbered from *seq, then the next iteration's control reset
+void launch_ls_report_i8(const Graph& g, const uint8_t* x, const void* h, LsCtl* c, const FieldRepair* fix,
+                         LsReport* rep_dev, uint8_t* x_out_dev, unsigned long long* seq, hipStream_t s);
 // max_energy = score[1] (E, spin glass) or score[0] (cut): the exact score of the start
 void launch_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass, hipStream_t s);
 void launch_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap, hipStream_t s);

@@ -278,8 +278,7 @@ int pqi_destroy(pqi_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     ls_graphs_reset(c);
     if (c->ls.host_ctl) (void)hipHostFree(c->ls.host_ctl);
-    for (auto& e : c->ls.done_ev)
-        if (e) (void)hipEventDestroy(e);
+    if (c->ls.seq_dev) (void)hipFree(c->ls.seq_dev);
     for (auto& pb : c->pin)
         if (pb.p) (void)hipHostFree(pb.p);
     for (auto& b : c->bufs)
@@ -514,10 +513,32 @@ static int ensure_pinned(pqi_ctx* c, int which, size_t bytes) {
         pb = Pinned{};
     }
     const size_t want = std::max<size_t>(bytes, 4096);
-    HIPCHK(hipHostMalloc(&pb.p, want, hipHostMallocDefault));
+    // coherent: the local search's reports are read while kernels still run
+    HIPCHK(hipHostMalloc(&pb.p, want, hipHostMallocCoherent));
+    std::memset(pb.p, 0, want);
     pb.bytes = want;
     HIPCHK(hipHostGetDevicePointer(&pb.dev, pb.p, 0));
     return PQI_OK;
+}
+
+// Waits until the local search report in pinned memory carries the number
+// `want` (written last by the report kernel, after a system-scope fence).  The
+// stream is queried now and then so that a failed or drained stream ends the
+// wait with an error instead of a hang.
+static int wait_report(pqi_ctx* c, const LsReport* r, unsigned long long want) {
+    for (uint32_t spins = 1;; ++spins) {
+        if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) == want) return PQI_OK;
+        if ((spins & 0x3FFF) == 0) {
+            const hipError_t e = hipStreamQuery(c->stream);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) == want) return PQI_OK;
+                return fail(PQI_ERR_HIP, "local search: an iteration report is missing");
+            }
+            if (e != hipErrorNotReady) return fail(PQI_ERR_HIP, std::string("local search: ") + hipGetErrorString(e));
+            (void)hipGetLastError();  // "not ready" is no error of the calls that follow
+        }
+        __builtin_ia32_pause();
+    }
 }
 
 static int score_device(pqi_graph* g, const uint8_t* x, double* cut, double* en) {
@@ -1052,6 +1073,17 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
             if (theta[i] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
         CHK(ensure_pinned(c, 0, (size_t)n));
         CHK(ensure_pinned(c, 1, off_xout[1] + (size_t)n));
+        if (!c->ls.seq_dev) {
+            HIPCHK(hipMalloc((void**)&c->ls.seq_dev, sizeof(unsigned long long)));
+            HIPCHK(hipMemsetAsync(c->ls.seq_dev, 0, sizeof(unsigned long long), c->stream));
+            c->ls.seq_host = 0;
+            c->ls.seq_dirty = false;
+        } else if (c->ls.seq_dirty) {
+            // a previous call ended early: take the count from the device
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipMemcpy(&c->ls.seq_host, c->ls.seq_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            c->ls.seq_dirty = false;
+        }
         std::memcpy(c->pin[0].p, theta, (size_t)n);
         CHK(ensure(c, S_X, n, (void**)&x));
         CHK(ensure(c, S_H, n * fb, &h));
@@ -1175,15 +1207,20 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         CHK(segment(0));
         launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
         CHK(segment(1));
-        if (fast) launch_ls_report_i8(*g, x, h, ctl, fix, rep_dev[slot], xout_dev[slot], c->stream);
-        else HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
+        if (fast) {
+            // the report kernel also applies the next iteration's control reset
+            // (int32 fields are exact: no field repair)
+            launch_ls_report_i8(*g, x, h, ctl, fix, rep_dev[slot], xout_dev[slot], c->ls.seq_dev, c->stream);
+            return PQI_OK;
+        }
+        HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
         prologue();
         return PQI_OK;
     };
     // captured once per (graph, parameters, buffers) and replayed per iteration
     const std::vector<int64_t> key = {(int64_t)g, (int64_t)g->W, (int64_t)g->adj_ptr, n, G, T, bs, (int64_t)timing,
                                       (int64_t)want_iter, (int64_t)fix, (int64_t)c->ls.host_ctl, (int64_t)fast,
-                                      (int64_t)rep_dev[0], (int64_t)rep_dev[1],
+                                      (int64_t)rep_dev[0], (int64_t)rep_dev[1], (int64_t)c->ls.seq_dev,
                                       p->double_mode, p->seed_engine, p->is_spin_glass, (int64_t)x, (int64_t)h,
                                       (int64_t)rx, (int64_t)rh, (int64_t)dout, (int64_t)dout2, (int64_t)dsc,
                                       (int64_t)ctl, (int64_t)b.wx, (int64_t)b.wh, (int64_t)b.we, (int64_t)b.wev,
@@ -1243,11 +1280,13 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         // (slot k mod 2), so the GPU never waits for the host between
         // iterations.  Once the search has stopped, the iteration reset leaves
         // `restart` set and every phase of the queued iteration does nothing; it
-        // finishes on the stream after this call returns.
-        for (auto& e : c->ls.done_ev)
-            if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        // finishes on the stream after this call returns.  The host waits for
+        // a report by polling its number in pinned memory: no event or marker
+        // sits between two iterations on the stream.
         EventTimer dt1(timing);
         EventTimer* dtimer[2] = {&descent_timer, &dt1};
+        unsigned long long want[2] = {0, 0};
+        c->ls.seq_dirty = true;
         auto enqueue = [&](int slot) -> int {
             hipGraphExec_t exec = slot ? c->ls.iter2 : c->ls.iter;
             if (timing || !exec)
@@ -1255,7 +1294,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                                dtimer[slot]->stop_event(), &ctl->restart);
             if (exec) HIPCHK(hipGraphLaunch(exec, c->stream));
             else CHK(tail(slot));
-            HIPCHK(hipEventRecord(c->ls.done_ev[slot], c->stream));
+            want[slot] = ++c->ls.seq_host;
             return PQI_OK;
         };
         launch_ls_start_i8(*g, (const uint8_t*)c->pin[0].dev, x, h, rx, rh, ctl, fix, hc.scale, p->max_rounds,
@@ -1264,7 +1303,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         int slot = 0;
         for (;;) {
             CHK(enqueue(slot ^ 1));
-            HIPCHK(hipEventSynchronize(c->ls.done_ev[slot]));
+            CHK(wait_report(c, rep[slot], want[slot]));
             std::memcpy(&hc, &rep[slot]->ctl, sizeof(hc));
             if (timing) {
                 st->descent_kernel_seconds += dtimer[slot]->seconds();
@@ -1273,6 +1312,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
             if (ls_stopped(&hc)) break;
             slot ^= 1;
         }
+        c->ls.seq_dirty = false;
         const LsReport& r = *rep[slot];
         fill_search_stats(c, hc, st);
         field_repair_stats(r.fix, st);

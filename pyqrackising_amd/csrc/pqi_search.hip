@@ -1941,7 +1941,7 @@ __device__ __forceinline__ void ls_mark(LsCtl* c, int phase) {
 // The reset of an iteration.  When the search has stopped (done, or the round
 // bound reached outside a continued descent), `restart` stays set instead, so
 // every phase of a further iteration already queued is skipped.
-__global__ void k_ls_begin(LsCtl* c) {
+__device__ __forceinline__ void ls_begin_body(LsCtl* c) {
     if (ls_stopped(c)) {
         c->restart = 1;
         return;
@@ -1954,6 +1954,7 @@ __global__ void k_ls_begin(LsCtl* c) {
     c->walk_ran = 0;
     ls_mark(c, 0);
 }
+__global__ void k_ls_begin(LsCtl* c) { ls_begin_body(c); }
 
 // Control steps as work-group bodies: thread 0 writes LsCtl; the bodies that
 // apply a move use the whole work group.  The kernels below run one body, or
@@ -2290,37 +2291,56 @@ void launch_ls_start_i8(const Graph& g, const uint8_t* theta_dev, uint8_t* x, vo
                        (int32_t*)rh, c, fix, scale, max_rounds, g.w_up, g.w_up_i, is_spin_glass);
 }
 
-// End of an iteration over int8 weights: the control record, the field-repair
-// counters, the exact score of the state (k_score's arithmetic on the exact
-// int32 field) and the state, written by one work group into a report in
-// pinned host memory that the host reads after its one synchronisation.
+// End of an iteration of the int8 fast path, in one work group: the control
+// record, the repair counters, the exact score of the state (k_score's
+// arithmetic on the exact int32 field) and the state are written into a report
+// in pinned host memory, then the next iteration's control reset (k_ls_begin)
+// is applied.  The report is published by its sequence number: wave 0 writes
+// the state and the report, then a system-scope fence, then `seq` (one more
+// than the previous report's, counted in device memory), so a host that reads
+// the expected number reads a whole report.
 __global__ __launch_bounds__(1024) void k_ls_report_i8(const uint8_t* x, const int32_t* h, int64_t n, int64_t w_up_i,
-                                                      const LsCtl* c, const FieldRepair* fix, LsReport* rep,
-                                                      uint8_t* x_out) {
+                                                      LsCtl* c, const FieldRepair* fix, LsReport* rep,
+                                                      uint8_t* x_out, unsigned long long* seq) {
     __shared__ long long part[16];
     long long q = 0;
     for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
         const uint8_t xv = x[v];
-        x_out[v] = xv;
         q += xv ? -(long long)h[v] : (long long)h[v];
+    }
+    if (threadIdx.x < 64) {
+        // 16-byte pieces, then the ragged end byte by byte (x and x_out are 256-byte aligned)
+        const int64_t nv = n >> 4;
+        for (int64_t k = threadIdx.x; k < nv; k += 64) ((uint4*)x_out)[k] = ((const uint4*)x)[k];
+        for (int64_t v = (nv << 4) + threadIdx.x; v < n; v += 64) x_out[v] = x[v];
     }
     q = wave_sum(q);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = q;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        long long qs = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) qs += part[i];
-        rep->cut = (double)((2 * w_up_i - qs) / 4);
-        rep->energy = (double)(-qs / 2);
-        rep->ctl = *c;
-        rep->fix = *fix;
+    if (threadIdx.x < 64) {
+        if (threadIdx.x == 0) {
+            long long qs = 0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) qs += part[i];
+            rep->cut = (double)((2 * w_up_i - qs) / 4);
+            rep->energy = (double)(-qs / 2);
+            rep->ctl = *c;
+            rep->fix = *fix;
+        }
+        // every store of wave 0 (the state and the report) before the number
+        __threadfence_system();
+        if (threadIdx.x == 0) {
+            const unsigned long long k = *seq + 1;
+            *seq = k;
+            __hip_atomic_store(&rep->seq, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            ls_begin_body(c);
+        }
     }
 }
 
-void launch_ls_report_i8(const Graph& g, const uint8_t* x, const void* h, const LsCtl* c, const FieldRepair* fix,
-                         LsReport* rep_dev, uint8_t* x_out_dev, hipStream_t s) {
+void launch_ls_report_i8(const Graph& g, const uint8_t* x, const void* h, LsCtl* c, const FieldRepair* fix,
+                         LsReport* rep_dev, uint8_t* x_out_dev, unsigned long long* seq, hipStream_t s) {
     hipLaunchKernelGGL(k_ls_report_i8, dim3(1), dim3(1024), 0, s, x, (const int32_t*)h, g.n, g.w_up_i, c, fix,
-                       rep_dev, x_out_dev);
+                       rep_dev, x_out_dev, seq);
 }
 
 void launch_ls_start_energy(LsCtl* c, const double* score, int is_spin_glass, hipStream_t s) {
