@@ -179,6 +179,8 @@ void launch_seed_bits(const uint8_t* x, int64_t n, int64_t words_per_row, int64_
 // ties by index) into sel (rank counts zeroed beforehand)
 void launch_topk_q(const unsigned long long* qi, int64_t count, int64_t k, int32_t* rank, int64_t* sel,
                    hipStream_t s);
+// the ranking step alone (the top k then taken by k_ls_seed_take)
+void launch_rank_q(const unsigned long long* qi, int64_t count, int32_t* rank, hipStream_t s);
 // Seed cut differences from int64 quadratic forms (int8 weights): q_base = s^T h of
 // (x, h), delta[b] = (q_base - qi[b]) / 4; also clears rank[0..count).
 void launch_seed_delta_i8(const uint8_t* x, const int32_t* h, int64_t n, const unsigned long long* qi,
@@ -267,15 +269,35 @@ void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t
 // Fused control steps (reference-mode double flips): descent bookkeeping + double flip +
 // its decision; Gray-seed decision + adoption of the best seed by in-place mutation;
 // walk decision + post-reheat single and double flips and their decisions.
+// Gray-seed rows written by the extra work groups of k_ls_descent_double
+// (count = 0: none); qzero and rzero are the row totals and rank counts to clear.
+struct SeedRows {
+    int64_t wpr = 0, bs = 1, count = 0;
+    uint32_t* bits = nullptr;
+    unsigned long long* qzero = nullptr;
+    int32_t* rzero = nullptr;
+};
 void launch_ls_descent_double(const Graph& g, LsCtl* c, const PhaseOut* dout, int64_t cap, uint8_t* x, void* h,
-                              int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s);
-// With qi (int8 MFMA seeds), the selected seeds' cut differences are computed here into seld.
-void launch_ls_seed_take(const Graph& g, LsCtl* c, double* seld, int64_t count, const int64_t* sel, int64_t bs,
-                         uint8_t* x, void* h, hipStream_t s, const unsigned long long* qi = nullptr, int64_t G = 0);
+                              int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s,
+                              const SeedRows* sr = nullptr);
+// With qi (int8 MFMA seeds), the selected seeds' cut differences are computed here into seld;
+// with rank as well, the top G by rank are first placed into sel.
+void launch_ls_seed_take(const Graph& g, LsCtl* c, double* seld, int64_t count, int64_t* sel, int64_t bs,
+                         uint8_t* x, void* h, hipStream_t s, const unsigned long long* qi = nullptr, int64_t G = 0,
+                         const int32_t* rank = nullptr);
+// Where the int8 fast path's report goes (rep = nullptr: no report).
+struct LsReportTo {
+    LsReport* rep = nullptr;
+    uint8_t* x_out = nullptr;
+    unsigned long long* seq = nullptr;
+    const FieldRepair* fix = nullptr;
+    int64_t w_up_i = 0;
+};
+// With a report (int8 weights), the iteration's report and the next control reset follow.
 void launch_ls_walk_reheat(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
                            int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
                            void* rh, PhaseOut* dout, int64_t thread_count, PhaseOut* dout2, int64_t dbl_evals,
-                           hipStream_t s);
+                           hipStream_t s, const LsReportTo* rt = nullptr);
 void launch_ls_after_walk(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
                           int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
                           void* rh, hipStream_t s);

@@ -889,6 +889,34 @@ static int gray_seeds(pqi_graph* g, const uint8_t* x, const void* h, const pqi_s
     return PQI_OK;
 }
 
+// The device loop's fused Gray seeds (reference-mode double flips, int8 MFMA
+// seeds): the seed rows are written by k_ls_descent_double's extra work groups,
+// then the MFMA contraction and the ranking run, and k_ls_seed_take takes the
+// top G.  Fills the rows and returns true when this applies; no launches.
+static bool fused_seed_plan(pqi_graph* g, const pqi_search_params* p, SeedRows* sr, unsigned long long** qi,
+                            int32_t** rank) {
+    pqi_ctx* c = g->ctx;
+    const bool mfma = g->kind == PQI_WEIGHTS_I8 && g->W != nullptr &&
+                      (p->seed_engine == PQI_SEEDS_MFMA || p->seed_engine == PQI_SEEDS_AUTO);
+    if (!mfma || p->double_mode != PQI_DOUBLE_REFERENCE) return false;
+    const int64_t nb = (g->n + 63) >> 6;
+    sr->bs = p->walkers * p->seed_multiple;
+    sr->count = sr->bs * nb;
+    sr->wpr = g->n_pad >> 5;
+    void *bits, *scratch, *rk;
+    // the buffers exist already (gray_seeds_buffers): these calls only look them up
+    if (ensure(c, S_BITS, (size_t)sr->count * sr->wpr * 4, &bits) != PQI_OK ||
+        ensure(c, S_BATCH, batch_scratch_bytes(*g, sr->count + 1), &scratch) != PQI_OK ||
+        ensure(c, S_RANK, topk_scratch_bytes(sr->count), &rk) != PQI_OK)
+        return false;
+    sr->bits = (uint32_t*)bits;
+    sr->qzero = (unsigned long long*)scratch;
+    sr->rzero = (int32_t*)rk;
+    *qi = sr->qzero;
+    *rank = sr->rzero;
+    return true;
+}
+
 static int gray_walk(pqi_graph* g, const uint8_t* x, const void* h, const pqi_search_params* p, GrayBufs& b,
                      int64_t walkers, int64_t T) {
     pqi_ctx* c = g->ctx;
@@ -1154,22 +1182,37 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     // kernel each (a kernel boundary costs a few microseconds); the exhaustive
     // double flip is multi-work-group and keeps its own launches.
     const bool fused = p->double_mode == PQI_DOUBLE_REFERENCE;
-    auto segment = [&](int which) -> int {
+    // int8 MFMA seeds with the fused control kernels: the seed rows are written
+    // alongside the double flip, and the top G are taken by the seed decision
+    SeedRows sr;
+    unsigned long long* fqi = nullptr;
+    int32_t* frank = nullptr;
+    const bool fuse_seeds = fused && fused_seed_plan(g, p, &sr, &fqi, &frank);
+    auto segment = [&](int which, const LsReportTo* rt) -> int {
         if (which == 0) {
             if (fused) {
-                launch_ls_descent_double(*g, ctl, dout, cap, x, h, G, dout2, dbl_evals, c->stream);
+                launch_ls_descent_double(*g, ctl, dout, cap, x, h, G, dout2, dbl_evals, c->stream,
+                                         fuse_seeds ? &sr : nullptr);
             } else {
                 launch_ls_after_descent(ctl, dout, cap, c->stream);
                 launch_double_best(*g, x, h, G, p->double_mode, dout2, dsc, c->stream);
                 launch_ls_after_double(*g, ctl, dout2, x, h, dbl_evals, c->stream);
             }
             unsigned long long* qi = nullptr;
-            CHK(gray_seeds(g, x, h, p, b, &count, &ctl->restart, &qi));
+            if (fuse_seeds) {
+                launch_batch_i8_raw(*g, sr.bits, sr.wpr, sr.count, fqi, c->stream, &ctl->restart);
+                launch_rank_q(fqi, sr.count, frank, c->stream);
+                qi = fqi;
+                count = sr.count;
+            } else {
+                CHK(gray_seeds(g, x, h, p, b, &count, &ctl->restart, &qi));
+            }
             // the best seed, if it improves, is applied to the state
-            launch_ls_seed_take(*g, ctl, b.seld, count, b.sel, bs, x, h, c->stream, qi, G);
+            launch_ls_seed_take(*g, ctl, b.seld, count, b.sel, bs, x, h, c->stream, qi, G,
+                                fuse_seeds ? frank : nullptr);
         } else if (fused) {
             launch_ls_walk_reheat(*g, ctl, b.we, b.wev, G, gray_evals, b.wx, b.wh, x, h, rx, rh, dout, G, dout2,
-                                  dbl_evals, c->stream);
+                                  dbl_evals, c->stream, rt);
         } else {
             launch_ls_after_walk(*g, ctl, b.we, b.wev, G, gray_evals, b.wx, b.wh, x, h, rx, rh, c->stream);
             // post-reheat phase (spin_glass_solver.py:670-717)
@@ -1204,15 +1247,19 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         launch_ls_begin(ctl, c->stream);
     };
     auto tail = [&](int slot) -> int {
-        CHK(segment(0));
+        CHK(segment(0, nullptr));
         launch_gray_walk(*g, x, h, b.sel, b.seld, bs, G, T, b.wx, b.wh, b.we, b.wev, c->stream, &ctl->restart);
-        CHK(segment(1));
         if (fast) {
-            // the report kernel also applies the next iteration's control reset
-            // (int32 fields are exact: no field repair)
-            launch_ls_report_i8(*g, x, h, ctl, fix, rep_dev[slot], xout_dev[slot], c->ls.seq_dev, c->stream);
+            // the report (in the last fused kernel, or a kernel of its own) also
+            // applies the next iteration's control reset (int32 fields are
+            // exact: no field repair)
+            const LsReportTo rt{rep_dev[slot], xout_dev[slot], c->ls.seq_dev, fix, g->w_up_i};
+            CHK(segment(1, fused ? &rt : nullptr));
+            if (!fused)
+                launch_ls_report_i8(*g, x, h, ctl, fix, rep_dev[slot], xout_dev[slot], c->ls.seq_dev, c->stream);
             return PQI_OK;
         }
+        CHK(segment(1, nullptr));
         HIPCHK(hipMemcpyAsync(c->ls.host_ctl, ctl, sizeof(LsCtl), hipMemcpyDeviceToHost, c->stream));
         prologue();
         return PQI_OK;
@@ -1253,7 +1300,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                 c->ls.failed = true;
                 break;
             }
-            const int r = segment(w);
+            const int r = segment(w, nullptr);
             const hipError_t e = hipStreamEndCapture(c->stream, &graph);
             if (r != PQI_OK) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1273,7 +1320,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
             HIPCHK(hipGraphLaunch(c->ls.seg[which], c->stream));
             return PQI_OK;
         }
-        return segment(which);
+        return segment(which, nullptr);
     };
     if (fast) {
         // Iteration k + 1 is queued before the host reads iteration k's report

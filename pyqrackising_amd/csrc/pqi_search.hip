@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 #include <cstdlib>
 #include <cmath>
 
@@ -1324,11 +1325,10 @@ __device__ __forceinline__ int64_t seed_pos(int64_t n, int64_t offset, int j) {
     return idx;
 }
 
-__global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs,
-                                                   int64_t count, uint32_t* bits, unsigned long long* qzero,
-                                                   int32_t* rzero, const int32_t* gate) {
-    if (gate && *gate) return;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Word w of seed row b, t = b * wpr + w (one thread per word).
+__device__ __forceinline__ void seed_bits_body(int64_t t, const uint8_t* x, int64_t n, int64_t wpr, int64_t bs,
+                                               int64_t count, uint32_t* bits, unsigned long long* qzero,
+                                               int32_t* rzero) {
     if (t >= count * wpr) return;
     const int64_t b = t / wpr, w = t % wpr;
     if (w == 0) {
@@ -1374,6 +1374,13 @@ __global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, 
     bits[b * wpr + w] = word;
 }
 
+__global__ __launch_bounds__(256) void k_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs,
+                                                   int64_t count, uint32_t* bits, unsigned long long* qzero,
+                                                   int32_t* rzero, const int32_t* gate) {
+    if (gate && *gate) return;
+    seed_bits_body((int64_t)blockIdx.x * blockDim.x + threadIdx.x, x, n, wpr, bs, count, bits, qzero, rzero);
+}
+
 void launch_seed_bits(const uint8_t* x, int64_t n, int64_t wpr, int64_t bs, int64_t count, uint32_t* bits,
                       hipStream_t s, unsigned long long* qzero, const int32_t* gate, int32_t* rzero) {
     int64_t threads = count * wpr;
@@ -1410,14 +1417,18 @@ __global__ __launch_bounds__(256) void k_rank_q(const unsigned long long* qi, in
 
 __global__ void k_rank_scatter(const int32_t* rank, int64_t count, int64_t k, int64_t* sel);
 
-void launch_topk_q(const unsigned long long* qi, int64_t count, int64_t k, int32_t* rank, int64_t* sel,
-                   hipStream_t s) {
+void launch_rank_q(const unsigned long long* qi, int64_t count, int32_t* rank, hipStream_t s) {
     const unsigned gx = (unsigned)((count + 255) / 256);
     int64_t splits = (count + 63) / 64;  // about 64 comparisons per thread, at most 64 splits
     if (splits > 64) splits = 64;
     const int64_t span = (count + splits - 1) / splits;
     hipLaunchKernelGGL(k_rank_q, dim3(gx, (unsigned)splits), dim3(256), 0, s, qi, count, span, rank);
-    hipLaunchKernelGGL(k_rank_scatter, dim3(gx), dim3(256), 0, s, rank, count, k, sel);
+}
+
+void launch_topk_q(const unsigned long long* qi, int64_t count, int64_t k, int32_t* rank, int64_t* sel,
+                   hipStream_t s) {
+    launch_rank_q(qi, count, rank, s);
+    hipLaunchKernelGGL(k_rank_scatter, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, rank, count, k, sel);
 }
 
 // Each work group recomputes q_base = s^T h (n terms, int64, exact) and
@@ -2024,11 +2035,22 @@ __global__ __launch_bounds__(1024) void k_ls_after_double(GView<WT> g, LsCtl* c,
 }
 
 // After the descent: its bookkeeping, the reference-mode double flip
-// (spin_glass_solver.py:55-92) and its decision, in one kernel.
+// (spin_glass_solver.py:55-92) and its decision, in work group 0.  With seed
+// rows (int8 MFMA seeds), the other work groups write the Gray-seed rows of the
+// state meanwhile.  The rows are used only when the double flip is rejected,
+// which leaves the state as the descent left it; when it is accepted (or the
+// descent restarts the iteration) they are not used, and these groups may skip
+// them as soon as they see `restart` set.
 template <typename WT, typename FT>
 __global__ __launch_bounds__(1024) void k_ls_descent_double(GView<WT> g, LsCtl* c, const PhaseOut* dout,
                                                            int64_t cap, uint8_t* x, FT* h, int64_t thread_count,
-                                                           PhaseOut* dout2, int64_t evals) {
+                                                           PhaseOut* dout2, int64_t evals, SeedRows sr) {
+    if (blockIdx.x > 0) {
+        if (*(volatile int32_t*)&c->restart) return;
+        seed_bits_body((int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x, x, g.n, sr.wpr, sr.bs, sr.count, sr.bits,
+                       sr.qzero, sr.rzero);
+        return;
+    }
     if (ls_restarted(c)) return;  // the search stopped: the descent was gated off
     if (threadIdx.x == 0) ls_after_descent_body(c, dout, cap);
     if (ls_restarted(c)) return;
@@ -2046,17 +2068,26 @@ __device__ __forceinline__ void ls_after_seeds_body(LsCtl* c, const double* seld
 // The Gray-seed decision and, when the best seed improves, its mutation applied
 // to (x, h) in place: the same flips, in the same order and with the same
 // field arithmetic, as the one-walker, zero-iteration walk followed by the
-// copy of its state (gray_mutation, maxcut_tfim_util.py:740-749).
+// copy of its state (gray_mutation, maxcut_tfim_util.py:740-749).  With rank
+// (the seeds' stable ranks), the top G are first placed into sel here
+// (k_rank_scatter's step).
 template <typename WT, typename FT>
 __global__ __launch_bounds__(1024) void k_ls_seed_take(GView<WT> g, LsCtl* c, double* seld, int64_t count,
-                                                      const int64_t* sel, int64_t bs, uint8_t* x, FT* h,
-                                                      const unsigned long long* qi, int64_t G) {
+                                                      int64_t* sel, int64_t bs, uint8_t* x, FT* h,
+                                                      const unsigned long long* qi, int64_t G, const int32_t* rank) {
     __shared__ int take;
     __shared__ long long part[16];
     if (qi) {
         // int8 MFMA seeds: the selected seeds' cut differences (q_base - q_b) / 4,
         // q_base = s^T h of the state (k_seed_delta_i8's arithmetic)
         if (ls_restarted(c)) return;
+        if (rank) {
+            for (int64_t a = threadIdx.x; a < count; a += blockDim.x) {
+                const int32_t r = rank[a];
+                if (r >= 0 && r < G) sel[r] = a;
+            }
+            __syncthreads();
+        }
         long long acc = 0;
         for (int64_t v = threadIdx.x; v < g.n; v += blockDim.x) acc += x[v] ? -(long long)h[v] : (long long)h[v];
         acc = wave_sum(acc);
@@ -2210,11 +2241,10 @@ __global__ __launch_bounds__(1024) void k_ls_reheat_double(GView<WT> g, LsCtl* c
 // phase (spin_glass_solver.py:670-717) with the best single flip and the
 // reference-mode double flip of the reheat state and their decisions.
 template <typename WT, typename FT>
-__global__ __launch_bounds__(1024) void k_ls_walk_reheat(GView<WT> g, LsCtl* c, const double* we, const int64_t* wev,
-                                                        int64_t walkers, int64_t gray_evals, const uint8_t* wx,
-                                                        const FT* wh, uint8_t* x, FT* h, uint8_t* rx, FT* rh,
-                                                        PhaseOut* dout, int64_t thread_count, PhaseOut* dout2,
-                                                        int64_t dbl_evals) {
+__device__ void ls_walk_reheat_body(const GView<WT>& g, LsCtl* c, const double* we, const int64_t* wev,
+                                    int64_t walkers, int64_t gray_evals, const uint8_t* wx, const FT* wh, uint8_t* x,
+                                    FT* h, uint8_t* rx, FT* rh, PhaseOut* dout, int64_t thread_count,
+                                    PhaseOut* dout2, int64_t dbl_evals) {
     if (ls_restarted(c)) return;
     ls_after_walk_body<WT, FT>(g, c, we, wev, walkers, gray_evals, wx, wh, x, h, rx, rh);
     if (ls_restarted(c)) return;
@@ -2225,6 +2255,28 @@ __global__ __launch_bounds__(1024) void k_ls_walk_reheat(GView<WT> g, LsCtl* c, 
     double_ref_body<WT, FT>(g, rx, rh, thread_count, dout2);
     __syncthreads();
     ls_reheat_double_body<WT, FT>(g, c, dout2, x, h, rx, rh, dbl_evals);
+}
+
+__device__ void ls_report_body(const uint8_t* x, const int32_t* h, int64_t n, int64_t w_up_i, LsCtl* c,
+                               const FieldRepair* fix, LsReport* rep, uint8_t* x_out, unsigned long long* seq);
+
+// With a report (int8 weights only), the iteration's report and the next
+// iteration's control reset (k_ls_report_i8's body) follow in the same kernel,
+// whether or not the iteration had already restarted.
+template <typename WT, typename FT>
+__global__ __launch_bounds__(1024) void k_ls_walk_reheat(GView<WT> g, LsCtl* c, const double* we, const int64_t* wev,
+                                                        int64_t walkers, int64_t gray_evals, const uint8_t* wx,
+                                                        const FT* wh, uint8_t* x, FT* h, uint8_t* rx, FT* rh,
+                                                        PhaseOut* dout, int64_t thread_count, PhaseOut* dout2,
+                                                        int64_t dbl_evals, LsReportTo rt) {
+    ls_walk_reheat_body<WT, FT>(g, c, we, wev, walkers, gray_evals, wx, wh, x, h, rx, rh, dout, thread_count, dout2,
+                                dbl_evals);
+    if constexpr (std::is_same<FT, int32_t>::value) {
+        if (rt.rep) {
+            __syncthreads();
+            ls_report_body(x, h, g.n, rt.w_up_i, c, rt.fix, rt.rep, rt.x_out, rt.seq);
+        }
+    }
 }
 
 void launch_ls_begin(LsCtl* c, hipStream_t s) { hipLaunchKernelGGL(k_ls_begin, dim3(1), dim3(1), 0, s, c); }
@@ -2299,9 +2351,8 @@ void launch_ls_start_i8(const Graph& g, const uint8_t* theta_dev, uint8_t* x, vo
 // the state and the report, then a system-scope fence, then `seq` (one more
 // than the previous report's, counted in device memory), so a host that reads
 // the expected number reads a whole report.
-__global__ __launch_bounds__(1024) void k_ls_report_i8(const uint8_t* x, const int32_t* h, int64_t n, int64_t w_up_i,
-                                                      LsCtl* c, const FieldRepair* fix, LsReport* rep,
-                                                      uint8_t* x_out, unsigned long long* seq) {
+__device__ void ls_report_body(const uint8_t* x, const int32_t* h, int64_t n, int64_t w_up_i, LsCtl* c,
+                               const FieldRepair* fix, LsReport* rep, uint8_t* x_out, unsigned long long* seq) {
     __shared__ long long part[16];
     long long q = 0;
     for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
@@ -2337,6 +2388,12 @@ __global__ __launch_bounds__(1024) void k_ls_report_i8(const uint8_t* x, const i
     }
 }
 
+__global__ __launch_bounds__(1024) void k_ls_report_i8(const uint8_t* x, const int32_t* h, int64_t n, int64_t w_up_i,
+                                                      LsCtl* c, const FieldRepair* fix, LsReport* rep,
+                                                      uint8_t* x_out, unsigned long long* seq) {
+    ls_report_body(x, h, n, w_up_i, c, fix, rep, x_out, seq);
+}
+
 void launch_ls_report_i8(const Graph& g, const uint8_t* x, const void* h, LsCtl* c, const FieldRepair* fix,
                          LsReport* rep_dev, uint8_t* x_out_dev, unsigned long long* seq, hipStream_t s) {
     hipLaunchKernelGGL(k_ls_report_i8, dim3(1), dim3(1024), 0, s, x, (const int32_t*)h, g.n, g.w_up_i, c, fix,
@@ -2364,43 +2421,52 @@ void launch_ls_after_double(const Graph& g, LsCtl* c, const PhaseOut* o, uint8_t
 
 template <typename WT>
 static void ls_descent_double_impl(const Graph& g, LsCtl* c, const PhaseOut* dout, int64_t cap, uint8_t* x, void* h,
-                                   int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s) {
+                                   int64_t thread_count, PhaseOut* dout2, int64_t evals, const SeedRows* sr,
+                                   hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
-    hipLaunchKernelGGL((k_ls_descent_double<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, dout, cap, x,
-                       (FT*)h, thread_count, dout2, evals);
+    const SeedRows rows = sr ? *sr : SeedRows{};
+    const unsigned blocks = 1u + (unsigned)((rows.count * rows.wpr + 1023) / 1024);
+    hipLaunchKernelGGL((k_ls_descent_double<WT, FT>), dim3(blocks), dim3(1024), 0, s, view<WT>(g), c, dout, cap, x,
+                       (FT*)h, thread_count, dout2, evals, rows);
 }
 void launch_ls_descent_double(const Graph& g, LsCtl* c, const PhaseOut* dout, int64_t cap, uint8_t* x, void* h,
-                              int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s) {
-    PQI_DISPATCH(g, ls_descent_double_impl, g, c, dout, cap, x, h, thread_count, dout2, evals, s);
+                              int64_t thread_count, PhaseOut* dout2, int64_t evals, hipStream_t s,
+                              const SeedRows* sr) {
+    PQI_DISPATCH(g, ls_descent_double_impl, g, c, dout, cap, x, h, thread_count, dout2, evals, sr, s);
 }
 
 template <typename WT>
-static void ls_seed_take_impl(const Graph& g, LsCtl* c, double* seld, int64_t count, const int64_t* sel, int64_t bs,
-                              uint8_t* x, void* h, const unsigned long long* qi, int64_t G, hipStream_t s) {
+static void ls_seed_take_impl(const Graph& g, LsCtl* c, double* seld, int64_t count, int64_t* sel, int64_t bs,
+                              uint8_t* x, void* h, const unsigned long long* qi, int64_t G, const int32_t* rank,
+                              hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
     hipLaunchKernelGGL((k_ls_seed_take<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, seld, count, sel, bs, x,
-                       (FT*)h, qi, G);
+                       (FT*)h, qi, G, rank);
 }
-void launch_ls_seed_take(const Graph& g, LsCtl* c, double* seld, int64_t count, const int64_t* sel, int64_t bs,
-                         uint8_t* x, void* h, hipStream_t s, const unsigned long long* qi, int64_t G) {
-    PQI_DISPATCH(g, ls_seed_take_impl, g, c, seld, count, sel, bs, x, h, qi, G, s);
+void launch_ls_seed_take(const Graph& g, LsCtl* c, double* seld, int64_t count, int64_t* sel, int64_t bs,
+                         uint8_t* x, void* h, hipStream_t s, const unsigned long long* qi, int64_t G,
+                         const int32_t* rank) {
+    PQI_DISPATCH(g, ls_seed_take_impl, g, c, seld, count, sel, bs, x, h, qi, G, rank, s);
 }
 
 template <typename WT>
 static void ls_walk_reheat_impl(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
                                 int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h,
                                 uint8_t* rx, void* rh, PhaseOut* dout, int64_t thread_count, PhaseOut* dout2,
-                                int64_t dbl_evals, hipStream_t s) {
+                                int64_t dbl_evals, const LsReportTo* rt, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
+    LsReportTo to{};
+    if (rt && std::is_same<FT, int32_t>::value) to = *rt;
     hipLaunchKernelGGL((k_ls_walk_reheat<WT, FT>), dim3(1), dim3(1024), 0, s, view<WT>(g), c, we, wev, walkers,
-                       gray_evals, wx, (const FT*)wh, x, (FT*)h, rx, (FT*)rh, dout, thread_count, dout2, dbl_evals);
+                       gray_evals, wx, (const FT*)wh, x, (FT*)h, rx, (FT*)rh, dout, thread_count, dout2, dbl_evals,
+                       to);
 }
 void launch_ls_walk_reheat(const Graph& g, LsCtl* c, const double* we, const int64_t* wev, int64_t walkers,
                            int64_t gray_evals, const uint8_t* wx, const void* wh, uint8_t* x, void* h, uint8_t* rx,
                            void* rh, PhaseOut* dout, int64_t thread_count, PhaseOut* dout2, int64_t dbl_evals,
-                           hipStream_t s) {
+                           hipStream_t s, const LsReportTo* rt) {
     PQI_DISPATCH(g, ls_walk_reheat_impl, g, c, we, wev, walkers, gray_evals, wx, wh, x, h, rx, rh, dout, thread_count,
-                 dout2, dbl_evals, s);
+                 dout2, dbl_evals, rt, s);
 }
 
 template <typename WT>
