@@ -1265,7 +1265,16 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
         return PQI_OK;
     };
     // captured once per (graph, parameters, buffers) and replayed per iteration
+    // The fast path launches its kernels directly: measured on MI355X, two
+    // back-to-back iteration graphs leave about 8 us between them, direct
+    // launches queued one iteration ahead none.  PQI_LS_FAST_GRAPH=1 captures
+    // and replays the iterations as graphs instead.
+    const bool fast_graphs = [] {
+        const char* e = getenv("PQI_LS_FAST_GRAPH");
+        return e && e[0] == '1';
+    }();
     const std::vector<int64_t> key = {(int64_t)g, (int64_t)g->W, (int64_t)g->adj_ptr, n, G, T, bs, (int64_t)timing,
+                                      (int64_t)fast_graphs,
                                       (int64_t)want_iter, (int64_t)fix, (int64_t)c->ls.host_ctl, (int64_t)fast,
                                       (int64_t)rep_dev[0], (int64_t)rep_dev[1], (int64_t)c->ls.seq_dev,
                                       p->double_mode, p->seed_engine, p->is_spin_glass, (int64_t)x, (int64_t)h,
@@ -1277,7 +1286,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     if (c->ls.key != key) {
         ls_graphs_reset(c);
         // the fast path captures the iteration once per report slot
-        for (int slot = 0; want_iter && slot < (fast ? 2 : 1); ++slot) {
+        for (int slot = 0; want_iter && (fast_graphs || !fast) && slot < (fast ? 2 : 1); ++slot) {
             hipGraphExec_t& exec = slot ? c->ls.iter2 : c->ls.iter;
             hipGraph_t graph = nullptr;
             if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
@@ -1294,7 +1303,8 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
             (void)hipGetLastError();
         }
         if (fast && !(c->ls.iter && c->ls.iter2)) ls_graphs_reset(c);  // both slots or neither
-        for (int w = 0; w < 2 && !c->ls.failed && !c->ls.iter; ++w) {
+        // (the fast path launches directly what it has not captured)
+        for (int w = 0; w < 2 && !fast && !c->ls.failed && !c->ls.iter; ++w) {
             hipGraph_t graph = nullptr;
             if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 c->ls.failed = true;

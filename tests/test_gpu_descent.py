@@ -214,6 +214,33 @@ def test_consecutive_searches_of_different_sizes():
                 assert sd[k] == ref[i][1][k], (rep, i, k)
 
 
+@pytest.mark.parametrize("graphs_on", ["0", "1"])
+def test_fast_path_direct_and_graph_launches(graphs_on, monkeypatch):
+    """The int8 fast path (direct launches by default, iteration graphs with
+    PQI_LS_FAST_GRAPH=1): a dense 3-regular instance with MFMA-scored Gray
+    seeds and a CSR instance, three searches each, give the host-decided
+    loop's states and counters."""
+    log(f"fast path PQI_LS_FAST_GRAPH={graphs_on}")
+    monkeypatch.setenv("PQI_LS_FAST_GRAPH", graphs_on)
+    G1, A1 = regular_int(1024, 3, 95)
+    W1 = A1.toarray().astype(np.int8)
+    G2, _ = regular_int(1500, 5, 96)
+    rng = np.random.default_rng(97)
+    for name, dg, n in (("dense n=1024", _lib.DeviceGraph.dense(W1), 1024),
+                        ("csr n=1500", _lib.DeviceGraph.csr(G2), 1500)):
+        p = _lib.make_params(n, is_spin_glass=1, walkers=16, seed_multiple=2, gray_iterations=n * 4)
+        ph = _lib.make_params(n, is_spin_glass=1, walkers=16, seed_multiple=2, gray_iterations=n * 4)
+        ph.flags = _lib.FLAG_HOST_LOOP
+        for rep in range(3):
+            x0 = rng.integers(0, 2, n).astype(np.uint8)
+            xr, sr = dg.local_search(x0, ph)
+            xd, sd = dg.local_search(x0, p)
+            assert np.array_equal(xd, xr), (name, rep)
+            for k in ("cut_value", "max_energy", "rounds", "single_accepts", "double_accepts", "seed_accepts",
+                      "walk_accepts", "reheat_accepts", "gray_evals_computed"):
+                assert sd[k] == sr[k], (name, rep, k)
+
+
 COUNTERS = ("max_energy", "cut_value", "min_energy", "rounds", "single_accepts", "double_accepts", "seed_accepts",
             "walk_accepts", "reheat_accepts", "single_evals", "double_evals", "seed_evals", "gray_evals",
             "gray_evals_computed", "descent_evals")
