@@ -930,12 +930,17 @@ static int gray_walk(pqi_graph* g, const uint8_t* x, const void* h, const pqi_se
     return PQI_OK;
 }
 
-// Optional event timing of one kernel group on the context stream.
+// Optional event timing of one kernel group on the context stream.  The
+// events only time: they are created without the system-scope fence (no cache
+// writeback and invalidation when they are recorded); the host reads them
+// after it has seen later work of the stream complete.
 struct EventTimer {
     bool on = false;
     hipEvent_t a = nullptr, b = nullptr;
     explicit EventTimer(bool enable) : on(enable) {
-        if (on && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) on = false;
+        if (on && (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess ||
+                   hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess))
+            on = false;
     }
     ~EventTimer() {
         if (a) (void)hipEventDestroy(a);
@@ -950,10 +955,13 @@ struct EventTimer {
     // events for a kernel launch that records them itself (hipExtLaunchKernel)
     hipEvent_t start_event() const { return on ? a : nullptr; }
     hipEvent_t stop_event() const { return on ? b : nullptr; }
-    // call after the stream has been synchronised
+    // call once later work of the stream is known to be complete (the stop
+    // event is then recorded; waiting on it returns at once)
     double seconds() {
         float ms = 0.f;
-        if (on && hipEventElapsedTime(&ms, a, b) == hipSuccess) return ms * 1e-3;
+        if (!on) return 0.0;
+        if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) return ms * 1e-3;
+        (void)hipGetLastError();
         return 0.0;
     }
 };
