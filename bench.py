@@ -324,7 +324,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--walkers", type=int, default=128)
     ap.add_argument("--seed-multiple", type=int, default=1)
