@@ -1742,16 +1742,36 @@ __device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, co
     const int nb = (n + 63) >> 6;
     const int stride = (int)g.kstride;
     const int base = ((lane & 31) << 6) | ((lane >> 5) << 5);  // node of register 0 in this lane
+    // a lane's registers hold 32 consecutive nodes: whole blocks are read in
+    // 16-byte pieces when the state is 16-byte aligned
+    const bool full = (lane & 31) < nb && base + 32 <= n;
     int32_t wr[32];
+    if (full && (((uintptr_t)x0 | (uintptr_t)h0) & 15) == 0) {
+        const uint4* xs = (const uint4*)(x0 + base);
+        const int4* hs = (const int4*)(h0 + base);
+        const uint4 xa = xs[0], xb = xs[1];
+        int4 hq[8];
 #pragma unroll
-    for (int r = 0; r < 32; ++r) {
-        const int v = base + r;
-        int32_t word = 0;
-        if ((lane & 31) < nb && v < n) {
-            const int32_t xv = x0[v];
-            word = (int32_t)((uint32_t)(xv ? -h0[v] : h0[v]) << 1) | xv;
+        for (int q = 0; q < 8; ++q) hq[q] = hs[q];
+        const uint32_t xw[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const int32_t xv = (int32_t)((xw[r >> 2] >> (8 * (r & 3))) & 0xFFu);
+            const int4 hp = hq[r >> 2];
+            const int32_t hv = (r & 3) == 0 ? hp.x : (r & 3) == 1 ? hp.y : (r & 3) == 2 ? hp.z : hp.w;
+            wr[r] = (int32_t)((uint32_t)(xv ? -hv : hv) << 1) | xv;
         }
-        wr[r] = word;
+    } else {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const int v = base + r;
+            int32_t word = 0;
+            if ((lane & 31) < nb && v < n) {
+                const int32_t xv = x0[v];
+                word = (int32_t)((uint32_t)(xv ? -h0[v] : h0[v]) << 1) | xv;
+            }
+            wr[r] = word;
+        }
     }
     const int quads = (n * stride + 3) >> 2;
     const bool staged = (size_t)quads * 16 <= lds_bytes;
@@ -1854,14 +1874,32 @@ __device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, co
 #undef WFLIP_ROW
     uint8_t* gx = wx + w * n;
     int32_t* gh = wh + w * n;
+    if (full && (((uintptr_t)gx | (uintptr_t)gh) & 15) == 0) {
+        int32_t hv[32];
+        uint32_t xw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int r = 0; r < 32; ++r) {
-        const int v = base + r;
-        if ((lane & 31) < nb && v < n) {
+        for (int r = 0; r < 32; ++r) {
             const int32_t word = wr[r];
             const int32_t d = word >> 1;
-            gh[v] = (word & 1) ? -d : d;
-            gx[v] = (uint8_t)(word & 1);
+            hv[r] = (word & 1) ? -d : d;
+            xw[r >> 2] |= (uint32_t)(word & 1) << (8 * (r & 3));
+        }
+        int4* hs = (int4*)(gh + base);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) hs[q] = make_int4(hv[4 * q], hv[4 * q + 1], hv[4 * q + 2], hv[4 * q + 3]);
+        uint4* xs = (uint4*)(gx + base);
+        xs[0] = make_uint4(xw[0], xw[1], xw[2], xw[3]);
+        xs[1] = make_uint4(xw[4], xw[5], xw[6], xw[7]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const int v = base + r;
+            if ((lane & 31) < nb && v < n) {
+                const int32_t word = wr[r];
+                const int32_t d = word >> 1;
+                gh[v] = (word & 1) ? -d : d;
+                gx[v] = (uint8_t)(word & 1);
+            }
         }
     }
     if (lane == 0) {
