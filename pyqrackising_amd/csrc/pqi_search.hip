@@ -618,7 +618,7 @@ __device__ void descent_key_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
         const int quads = (n * (int)g.kstride + 3) >> 2;  // the table is zero padded to whole quads
         const int4* src = (const int4*)g.kadj;
         int4* dst = (int4*)T;
-#pragma unroll 4
+#pragma unroll 12  // 12 loads in flight per lane: a 24 KiB table in two rounds
         for (int q = lane; q < quads; q += 64) dst[q] = src[q];
     } else if (mode == 2) {
         for (int v = lane; v <= n; v += 64) T[v] = g.ap[v];
@@ -770,7 +770,7 @@ __device__ void descent_reg_body(const GView<int8_t>& g, uint8_t* xg, int32_t* h
     if (staged) {
         const int4* src = (const int4*)g.kadj;
         int4* dst = (int4*)lds;
-#pragma unroll 4
+#pragma unroll 12  // 12 loads in flight per lane: a 24 KiB table in two rounds
         for (int q = lane; q < quads; q += 64) dst[q] = src[q];
     }
     __syncthreads();
@@ -1778,7 +1778,7 @@ __device__ void gray_walk_reg_body(const GView<int8_t>& g, const uint8_t* x0, co
     if (staged) {
         const int4* src = (const int4*)g.kadj;
         int4* dst = (int4*)lds;
-#pragma unroll 4
+#pragma unroll 12  // 12 loads in flight per lane: a 24 KiB table in two rounds
         for (int q = lane; q < quads; q += 64) dst[q] = src[q];
     }
     __syncthreads();
