@@ -2038,7 +2038,9 @@ __global__ void k_ls_after_descent(LsCtl* c, const PhaseOut* o, int64_t cap) {
 }
 
 template <typename FT, typename WT>
-__device__ __forceinline__ void copy_state(uint8_t* xd, FT* hd, const uint8_t* xs, const FT* hs, int64_t n) {
+__device__ __forceinline__ void copy_state(uint8_t* __restrict__ xd, FT* __restrict__ hd,
+                                           const uint8_t* __restrict__ xs, const FT* __restrict__ hs, int64_t n) {
+#pragma unroll 4
     for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
         xd[v] = xs[v];
         hd[v] = hs[v];
@@ -2336,10 +2338,19 @@ __global__ __launch_bounds__(1024) void k_ls_start_i8(GView<int8_t> g, const uin
                                                      int is_spin_glass) {
     __shared__ long long part[16];
     const int64_t n = g.n;
-    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
-        const uint8_t xv = theta[v];
-        x[v] = xv;
-        rx[v] = xv;
+    if ((n & 15) == 0 && (((uintptr_t)theta | (uintptr_t)x | (uintptr_t)rx) & 15) == 0) {
+        // 16-byte pieces: one round of reads from host memory
+        for (int64_t k = threadIdx.x; k < (n >> 4); k += blockDim.x) {
+            const uint4 t = ((const uint4*)theta)[k];
+            ((uint4*)x)[k] = t;
+            ((uint4*)rx)[k] = t;
+        }
+    } else {
+        for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+            const uint8_t xv = theta[v];
+            x[v] = xv;
+            rx[v] = xv;
+        }
     }
     __syncthreads();
     long long q = 0;
