@@ -167,6 +167,19 @@ PQI_API int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is
 PQI_API int pqi_eval_batch_ex(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass, int engine,
                               double* values, int64_t* best_index, double* best_value, double* max_rel_err);
 
+/* pqi_eval_batch_ex with PQI_ENGINE_EXACT that allocates no device memory.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Staging, row totals, partial sums, values and the argmax live
+ * in one static device array of 32 MiB per GPU, held from the first context
+ * on that GPU until the process ends.  A request whose layout needs more
+ * (about B * n / 4 bytes for dense graphs, more for real weights) fails with
+ * PQI_ERR_INVALID before any device work.  Results equal pqi_eval_batch_ex
+ * with the exact engine.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other contexts, other threads) race. */
+PQI_API int pqi_eval_batch_noalloc(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass,
+                                   double* values, int64_t* best_index, double* best_value);
+
 /* Candidate sampler.  Replaces sample_mag + local_repulsion_choice
  * (pyqrackising/maxcut_tfim.py:36-75, 139-145; maxcut_tfim_util.py:690-724):
  * shot s = shot_offset + i draws its Hamming weight from cum_prob (ncum

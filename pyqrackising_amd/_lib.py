@@ -134,6 +134,7 @@ SIGNATURES = {
     "pqi_graph_get_info": (_int, [_vp, _vp]),
     "pqi_eval_batch": (_int, [_vp, _vp, _i64, _int, _vp, _vp, _vp]),
     "pqi_eval_batch_ex": (_int, [_vp, _vp, _i64, _int, _int, _vp, _vp, _vp, _vp]),
+    "pqi_eval_batch_noalloc": (_int, [_vp, _vp, _i64, _int, _vp, _vp, _vp]),
     "pqi_single_flip_best": (_int, [_vp, _vp, _int, _vp, _vp]),
     "pqi_single_flip_descent": (_int, [_vp, _vp, _int, _i64, _vp, _vp]),
     "pqi_double_flip_best": (_int, [_vp, _vp, _int, _i64, _int, _vp, _vp, _vp]),
@@ -360,6 +361,22 @@ class DeviceGraph:
                                            ctypes.byref(err)))
         if return_error:
             return bi.value, bv.value, vals, err.value
+        return bi.value, bv.value, vals
+
+    def eval_batch_noalloc(self, bits, is_spin_glass, want_values=True):
+        """pqi_eval_batch_noalloc: the exact engine, with no device allocation
+        (a static 32 MiB scratch array per GPU)."""
+        bits = np.ascontiguousarray(bits, dtype=np.uint32)
+        B = bits.shape[0]
+        if bits.ndim != 2 or bits.shape[1] != (self.n + 31) >> 5:
+            raise ValueError("bits must be B x ceil(n/32) uint32")
+        vals = np.empty(B, dtype=np.float64) if want_values else None
+        bi = ctypes.c_int64()
+        bv = ctypes.c_double()
+        with self.ctx.lock:
+            check(load().pqi_eval_batch_noalloc(self._h, _p(bits), B, int(bool(is_spin_glass)),
+                                                _p(vals) if want_values else None, ctypes.byref(bi),
+                                                ctypes.byref(bv)))
         return bi.value, bv.value, vals
 
     def single_flip_best(self, theta, is_spin_glass):

@@ -109,6 +109,12 @@ void launch_field(const Graph& g, const uint8_t* x, void* h, hipStream_t s);
 // host memory would move one row per transfer).  Enqueued on the context
 // stream; the staging buffer is reused by the next call.
 int upload_rows(pqi_ctx* c, void* dst, size_t dpitch, const void* src, size_t width, size_t rows);
+// dst row r (dwords words) = src row r (swords <= dwords words) followed by zeros.
+void launch_restride_words(uint32_t* dst, int64_t dwords, const uint32_t* src, int64_t swords, int64_t rows,
+                           hipStream_t s);
+// Takes the per-device address of the no-allocation entry point's scratch
+// array (pqi_noalloc.hip); called by pqi_create with the device current.
+int noalloc_init(int device);
 // Drift check of an incrementally maintained fp64 field: h is recomputed
 // exactly from W and x and overwritten; entries that had moved by more than
 // 1e-9 max(1, |h_exact|) are counted.  Skipped when gate is non-null and

@@ -85,6 +85,14 @@ __global__ void k_restride_bytes(uint8_t* __restrict__ dst, int64_t dpitch, cons
     }
 }
 
+void launch_restride_words(uint32_t* dst, int64_t dwords, const uint32_t* src, int64_t swords, int64_t rows,
+                           hipStream_t s) {
+    const int64_t total = dwords * rows;
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+    if (total > 0)
+        hipLaunchKernelGGL(k_restride_words, dim3(blocks), dim3(256), 0, s, dst, dwords, src, swords, rows);
+}
+
 int upload_rows(pqi_ctx* c, void* dst, size_t dpitch, const void* src, size_t width, size_t rows) {
     if (rows == 0) return PQI_OK;
     if (dpitch == width) {
@@ -268,6 +276,13 @@ int pqi_create(int device, pqi_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     c->bufs.resize(S_COUNT);
+    // the device address of pqi_eval_batch_noalloc's scratch array on this device
+    const int ir = noalloc_init(device);
+    if (ir != PQI_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return ir;
+    }
     *out = c;
     return PQI_OK;
 }
