@@ -56,7 +56,6 @@ using namespace pqi;
         hipError_t _e = (expr);                                                     \
         if (_e != hipSuccess) {                                                     \
             std::string _m = std::string(#expr) + ": " + hipGetErrorString(_e);     \
-            if (arena) (void)hipFree(arena);                                        \
             return fail(_e == hipErrorOutOfMemory ? PQI_ERR_OOM : PQI_ERR_HIP, _m); \
         }                                                                           \
     } while (0)
@@ -95,8 +94,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
 
     // per-instance sizes and arena layout
     std::vector<BInstRaw> h_inst(count);
-    std::vector<size_t> off_x(count);
-    size_t total = 0;
+    std::vector<size_t> off_x(count), xoff(count);
+    size_t total = 0, xbytes = 0;
     int64_t max_n = 0, max_seeds = 0;
     for (int64_t i = 0; i < count; ++i) {
         const pqi_graph* g = graphs[i];
@@ -122,9 +121,11 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         max_n = std::max(max_n, n);
         max_seeds = std::max(max_seeds, seeds);
         off_x[i] = total;
-        total += align256(n) * 2 + align256(n * fb) * 2 + align256(G * n) + align256(G * n * fb) + align256(G * 8) * 4 +
-                 align256(seeds * 8) + align256(seeds * 4) + align256(sizeof(PhaseOut));
-        if (net) total += align256(n * 4) + align256(n);
+        total += align256(n * fb) * 2 + align256(n) + align256(G * n) + align256(G * n * fb) + align256(G * 8) +
+                 align256(seeds * 8) + align256(seeds * 4);
+        if (net) total += align256(n * 4);
+        xoff[i] = xbytes;  // best states, contiguous over the instances (one copy in, one out)
+        xbytes += align256(n);
     }
     if (net) {
         int64_t off = 0;
@@ -172,10 +173,41 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         }
     }
     const int64_t L0 = count;
+    // shared records, one per instance in instance order, so that a phase's
+    // results come back in one copy: phase records, top seed gains and walker
+    // totals (G per instance), ages of the learned phase
+    const size_t rec_out = align256(L0 * sizeof(PhaseOut)), rec_g = align256(L0 * G * 8);
+    const size_t recs = xbytes + rec_out + 3 * rec_g + (net ? xbytes : 0);
     const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
                         align256(L0 * sizeof(BInstRaw));
-    BCHK(hipMalloc(&arena, total + misc));
+    {
+        const int e = batch_arena(c, total + recs + misc, &arena);
+        if (e != PQI_OK) return e;
+    }
+    // pinned host side: the states, then the records, then a bump area for
+    // the lists and moves uploaded between two synchronisations
+    const size_t bump_cap = align256(L0 * 16) * 6;
+    {
+        const int e = ensure_pinned(c, 2, xbytes + rec_out + 3 * rec_g + align256(L0 * 8) + bump_cap);
+        if (e != PQI_OK) return e;
+    }
+    char* pin = (char*)c->pin[2].p;
+    uint8_t* pin_x = (uint8_t*)pin;
+    PhaseOut* pin_out = (PhaseOut*)(pin + xbytes);
+    double* pin_seld = (double*)(pin + xbytes + rec_out);
+    double* pin_we = (double*)(pin + xbytes + rec_out + rec_g);
+    int64_t* pin_wev = (int64_t*)(pin + xbytes + rec_out + 2 * rec_g);
+    double* pin_q = (double*)(pin + xbytes + rec_out + 3 * rec_g);
+    char* pin_bump = pin + xbytes + rec_out + 3 * rec_g + align256(L0 * 8);
+    size_t bump_used = 0;
     char* a = (char*)arena;
+    char* rec = a + total;
+    uint8_t* d_xall = (uint8_t*)rec;
+    PhaseOut* d_out = (PhaseOut*)(rec + xbytes);
+    double* d_seld = (double*)(rec + xbytes + rec_out);
+    double* d_we = (double*)(rec + xbytes + rec_out + rec_g);
+    int64_t* d_wev = (int64_t*)(rec + xbytes + rec_out + 2 * rec_g);
+    uint8_t* d_age = net ? (uint8_t*)(rec + xbytes + rec_out + 3 * rec_g) : nullptr;
     for (int64_t i = 0; i < count; ++i) {
         BInstRaw& r = h_inst[i];
         const int64_t n = r.n;
@@ -185,27 +217,27 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             q += align256(bytes);
             return t;
         };
-        r.x = (uint8_t*)take(n);
+        r.x = d_xall + xoff[i];
         r.h = take(n * fb);
         r.rx = (uint8_t*)take(n);
         r.rh = take(n * fb);
         r.wx = (uint8_t*)take(G * n);
         r.wh = take(G * n * fb);
-        r.we = (double*)take(G * 8);
-        r.wev = (int64_t*)take(G * 8);
+        r.we = d_we + i * G;
+        r.wev = d_wev + i * G;
         r.sel = (int64_t*)take(G * 8);
-        r.seld = (double*)take(G * 8);
+        r.seld = d_seld + i * G;
         r.delta = (double*)take(r.seed_count * 8);
         r.rank = (int32_t*)take(r.seed_count * 4);
-        r.out = (PhaseOut*)take(sizeof(PhaseOut));
+        r.out = d_out + i;
         if (net) {
             r.perm = (int32_t*)take(n * 4);
-            r.age = (uint8_t*)take(n);
+            r.age = d_age + xoff[i];
             r.feat = mpnn_features(net) + mpnn_node_offset(net, i) * 4;
             r.score = mpnn_scores(net) + mpnn_node_offset(net, i);
         }
     }
-    char* m = a + total;
+    char* m = a + total + recs;
     int32_t* d_list = (int32_t*)m;
     m += align256(L0 * 4);
     int64_t* d_moves = (int64_t*)m;
@@ -216,8 +248,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     m += align256(L0 * 8);
     BInstRaw* d_inst = (BInstRaw*)m;
     BCHK(hipMemcpyAsync(d_inst, h_inst.data(), count * sizeof(BInstRaw), hipMemcpyHostToDevice, s));
-    for (int64_t i = 0; i < count; ++i)
-        BCHK(hipMemcpyAsync(h_inst[i].x, thetas[i], h_inst[i].n, hipMemcpyHostToDevice, s));
+    for (int64_t i = 0; i < count; ++i) std::memcpy(pin_x + xoff[i], thetas[i], h_inst[i].n);
+    BCHK(hipMemcpyAsync(d_xall, pin_x, xbytes, hipMemcpyHostToDevice, s));
 
     std::vector<int32_t> list(count);
     for (int64_t i = 0; i < count; ++i) list[i] = (int32_t)i;
@@ -227,25 +259,41 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     std::vector<double> qv(count);
     std::memset(stats, 0, sizeof(pqi_search_stats) * count);
 
-    auto upload_list = [&](const std::vector<int32_t>& l) -> hipError_t {
-        return hipMemcpyAsync(d_list, l.data(), l.size() * 4, hipMemcpyHostToDevice, s);
+    // Host -> device uploads go through the pinned bump area, so they are
+    // asynchronous; the area is reused after each synchronisation (and, should
+    // it fill up, after an extra one).  Device -> host results of a phase come
+    // back in one copy of the instance-ordered records.
+    auto sync = [&]() -> hipError_t {
+        bump_used = 0;
+        return hipStreamSynchronize(s);
     };
-    auto read_outs = [&](const std::vector<int32_t>& l) -> hipError_t {
-        for (size_t t = 0; t < l.size(); ++t) {
-            hipError_t e = hipMemcpyAsync(&outs[t], h_inst[l[t]].out, sizeof(PhaseOut), hipMemcpyDeviceToHost, s);
+    auto upload = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        const size_t need = align256(bytes);
+        if (bump_used + need > bump_cap) {
+            const hipError_t e = sync();
             if (e != hipSuccess) return e;
         }
-        return hipStreamSynchronize(s);
+        char* hp = pin_bump + bump_used;
+        bump_used += need;
+        std::memcpy(hp, src, bytes);
+        return hipMemcpyAsync(dst, hp, bytes, hipMemcpyHostToDevice, s);
+    };
+    auto upload_list = [&](const std::vector<int32_t>& l) -> hipError_t { return upload(d_list, l.data(), l.size() * 4); };
+    auto read_outs = [&](const std::vector<int32_t>& l) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(pin_out, d_out, count * sizeof(PhaseOut), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = sync();
+        for (size_t t = 0; t < l.size() && e == hipSuccess; ++t) outs[t] = pin_out[l[t]];
+        return e;
     };
     auto flips = [&](const std::vector<int32_t>& l, const std::vector<int64_t>& mv, int which) -> hipError_t {
         hipError_t e = upload_list(l);
-        if (e == hipSuccess) e = hipMemcpyAsync(d_moves, mv.data(), l.size() * 16, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = upload(d_moves, mv.data(), l.size() * 16);
         if (e == hipSuccess) launch_b_flip(kind, d_inst, d_list, (int64_t)l.size(), d_moves, which, s);
         return e;
     };
     auto copies = [&](const std::vector<int32_t>& l, const std::vector<int32_t>& sr, int dst) -> hipError_t {
         hipError_t e = upload_list(l);
-        if (e == hipSuccess) e = hipMemcpyAsync(d_src, sr.data(), l.size() * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = upload(d_src, sr.data(), l.size() * 4);
         if (e == hipSuccess) launch_b_copy(kind, d_inst, d_list, (int64_t)l.size(), max_n, d_src, dst, s);
         return e;
     };
@@ -253,8 +301,9 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         hipError_t e = upload_list(l);
         if (e != hipSuccess) return e;
         launch_b_quad(kind, d_inst, d_list, (int64_t)l.size(), d_q, s);
-        e = hipMemcpyAsync(qv.data(), d_q, l.size() * 8, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        e = hipMemcpyAsync(pin_q, d_q, l.size() * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = sync();
+        if (e == hipSuccess) std::memcpy(qv.data(), pin_q, l.size() * 8);
         return e;
     };
 
@@ -275,10 +324,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     // learned phase
     if (net && passes > 0) {
         const double t0 = now_s();
-        std::vector<uint8_t> zeros(max_n, 64);  // every node starts "long ago flipped"
-        for (int64_t i = 0; i < count; ++i)
-            BCHK(hipMemcpyAsync(h_inst[i].age, zeros.data(), h_inst[i].n, hipMemcpyHostToDevice, s));
-        BCHK(hipStreamSynchronize(s));
+        BCHK(hipMemsetAsync(d_age, 64, xbytes, s));  // every node starts "long ago flipped"
         std::vector<int32_t> run;
         for (int32_t i : list)
             if (!reached(i)) run.push_back(i);
@@ -392,10 +438,10 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         if (!rest.empty()) {
             BCHK(upload_list(rest));
             launch_b_seeds(kind, d_inst, d_list, (int64_t)rest.size(), max_seeds, G, s);
+            BCHK(hipMemcpyAsync(pin_seld, d_seld, (size_t)count * G * 8, hipMemcpyDeviceToHost, s));
+            BCHK(sync());
             std::vector<double> top(rest.size());
-            for (size_t t = 0; t < rest.size(); ++t)
-                BCHK(hipMemcpyAsync(&top[t], h_inst[rest[t]].seld, 8, hipMemcpyDeviceToHost, s));
-            BCHK(hipStreamSynchronize(s));
+            for (size_t t = 0; t < rest.size(); ++t) top[t] = pin_seld[(size_t)rest[t] * G];
             for (size_t t = 0; t < rest.size(); ++t) {
                 const int32_t i = rest[t];
                 stats[i].seed_evals += h_inst[i].seed_count;
@@ -426,13 +472,15 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             BCHK(upload_list(rest2));
             launch_b_walk(kind, d_inst, d_list, (int64_t)rest2.size(), max_n, G, -1, -1, reg_r, reg_stride, reg_lds,
                           s);
+            BCHK(hipMemcpyAsync(pin_we, d_we, (size_t)count * G * 8, hipMemcpyDeviceToHost, s));
+            BCHK(hipMemcpyAsync(pin_wev, d_wev, (size_t)count * G * 8, hipMemcpyDeviceToHost, s));
+            BCHK(sync());
             std::vector<double> all_we(rest2.size() * G);
             std::vector<int64_t> all_wev(rest2.size() * G);
             for (size_t t = 0; t < rest2.size(); ++t) {
-                BCHK(hipMemcpyAsync(&all_we[t * G], h_inst[rest2[t]].we, G * 8, hipMemcpyDeviceToHost, s));
-                BCHK(hipMemcpyAsync(&all_wev[t * G], h_inst[rest2[t]].wev, G * 8, hipMemcpyDeviceToHost, s));
+                std::memcpy(&all_we[t * G], pin_we + (size_t)rest2[t] * G, G * 8);
+                std::memcpy(&all_wev[t * G], pin_wev + (size_t)rest2[t] * G, G * 8);
             }
-            BCHK(hipStreamSynchronize(s));
             std::vector<int32_t> wsrc;
             for (size_t t = 0; t < rest2.size(); ++t) {
                 const int32_t i = rest2[t];
@@ -552,10 +600,10 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         const double en = kind == PQI_WEIGHTS_I8 ? (double)(-(long long)qv[i] / 2) : -qv[i] / 2.0;
         stats[i].cut_value = cut;
         stats[i].min_energy = p->is_spin_glass ? -en : en;
-        BCHK(hipMemcpyAsync(thetas[i], h_inst[i].x, h_inst[i].n, hipMemcpyDeviceToHost, s));
     }
-    BCHK(hipStreamSynchronize(s));
-    (void)hipFree(arena);
+    BCHK(hipMemcpyAsync(pin_x, d_xall, xbytes, hipMemcpyDeviceToHost, s));
+    BCHK(sync());
+    for (int64_t i = 0; i < count; ++i) std::memcpy(thetas[i], pin_x + xoff[i], h_inst[i].n);
     return PQI_OK;
 }
 

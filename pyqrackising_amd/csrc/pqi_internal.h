@@ -86,7 +86,8 @@ struct pqi_ctx {
     // local search's iteration reports (kept apart so that a report still
     // being written by the previous call's last, idle iteration never lands on
     // the next call's input)
-    pqi::Pinned pin[2];
+    // [2] the batched search's state copies, phase records and lists
+    pqi::Pinned pin[3];
     pqi::LsGraphs ls;
 };
 
@@ -115,6 +116,11 @@ void launch_restride_words(uint32_t* dst, int64_t dwords, const uint32_t* src, i
 // Takes the per-device address of the no-allocation entry point's scratch
 // array (pqi_noalloc.hip); called by pqi_create with the device current.
 int noalloc_init(int device);
+// Coherent pinned host staging c->pin[which] of at least `bytes`, grown on
+// demand (the stream is synchronised before an old buffer is freed).
+int ensure_pinned(pqi_ctx* c, int which, size_t bytes);
+// The batched search's device arena, kept by the context and grown on demand.
+int batch_arena(pqi_ctx* c, size_t bytes, void** out);
 // Drift check of an incrementally maintained fp64 field: h is recomputed
 // exactly from W and x and overwritten; entries that had moved by more than
 // 1e-9 max(1, |h_exact|) are counted.  Skipped when gate is non-null and

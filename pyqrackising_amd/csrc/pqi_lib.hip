@@ -44,7 +44,7 @@ int fail(int code, const std::string& msg) {
 // Scratch slots of a context.
 enum Slot {
     S_X, S_H, S_RX, S_RH, S_FIX, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
-    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_STAGE, S_COUNT
+    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_STAGE, S_BARENA, S_COUNT
 };
 
 static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
@@ -65,6 +65,8 @@ static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
     *out = b.p;
     return PQI_OK;
 }
+
+int batch_arena(pqi_ctx* c, size_t bytes, void** out) { return ensure(c, S_BARENA, bytes, out); }
 
 // dst row r = src row r (width bytes) followed by zeros up to dpitch
 __global__ void k_restride_words(uint32_t* __restrict__ dst, int64_t dwords, const uint32_t* __restrict__ src,
@@ -519,7 +521,7 @@ static int load_state(pqi_graph* g, const uint8_t* theta, uint8_t** x, void** h)
 }
 
 // Pinned host staging of at least `bytes` (grown on demand; freed by pqi_destroy).
-static int ensure_pinned(pqi_ctx* c, int which, size_t bytes) {
+static int pin_ensure(pqi_ctx* c, int which, size_t bytes) {
     Pinned& pb = c->pin[which];
     if (pb.bytes >= bytes) return PQI_OK;
     if (pb.p) {
@@ -803,6 +805,10 @@ int pqi_double_flip_best(pqi_graph* g, const uint8_t* theta, int is_spin_glass, 
 }
 
 }  // extern "C"
+
+namespace pqi {
+int ensure_pinned(pqi_ctx* c, int which, size_t bytes) { return pin_ensure(c, which, bytes); }
+}  // namespace pqi
 
 namespace pqi {
 
@@ -1122,8 +1128,8 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     if (fast) {
         for (int64_t i = 0; i < n; ++i)
             if (theta[i] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
-        CHK(ensure_pinned(c, 0, (size_t)n));
-        CHK(ensure_pinned(c, 1, off_xout[1] + (size_t)n));
+        CHK(pin_ensure(c, 0, (size_t)n));
+        CHK(pin_ensure(c, 1, off_xout[1] + (size_t)n));
         if (!c->ls.seq_dev) {
             HIPCHK(hipMalloc((void**)&c->ls.seq_dev, sizeof(unsigned long long)));
             HIPCHK(hipMemsetAsync(c->ls.seq_dev, 0, sizeof(unsigned long long), c->stream));
@@ -1442,7 +1448,7 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
     // field repairs, the exact score and the state come back in one
     // synchronisation, through pinned memory
     launch_score_with(*g, x, 1, score, score + 1, hb, c->stream);
-    CHK(ensure_pinned(c, 0, 64 + (size_t)n));
+    CHK(pin_ensure(c, 0, 64 + (size_t)n));
     char* pb = (char*)c->pin[0].p;
     HIPCHK(hipMemcpyAsync(pb, fix, sizeof(FieldRepair), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(pb + 32, score, 16, hipMemcpyDeviceToHost, c->stream));

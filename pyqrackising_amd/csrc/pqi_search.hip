@@ -2667,6 +2667,73 @@ __global__ void kb_rank_scatter(const BInstRaw* insts, const int32_t* list) {
     if (a < r.seed_count && r.rank[a] < r.G) r.sel[r.rank[a]] = a;
 }
 
+// Stable top G of the seed gains, one 1024-thread work group per instance:
+// G rounds of a block first-argmax (largest gain, then lowest index), each
+// taking the winner out.  The winners come in rank order, the order of the
+// rank counting below (descending gain, ties by ascending index), so sel and
+// seld are the same.  Thread t owns seeds t, t + 1024, ... (at most 64 of
+// them: seed_count <= 65536), remembers the ones taken in a bit mask, and
+// rescans its own seeds only when it supplied the winner.
+constexpr int64_t kTopkMaxSeeds = 64 * 1024;
+
+__global__ __launch_bounds__(1024) void kb_topk(const BInstRaw* insts, const int32_t* list) {
+    __shared__ double sv[16];
+    __shared__ int32_t sk[16];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int count = (int)r.seed_count;
+    uint64_t taken = 0;
+    double bv;
+    int bk;
+    auto rescan = [&]() {
+        bv = -INFINITY;
+        bk = INT_MAX;
+        for (int j = 0; t + j * 1024 < count; ++j) {
+            const int a = t + j * 1024;
+            const double v = r.delta[a];
+            if (!((taken >> j) & 1u) && (v > bv || bk == INT_MAX)) {
+                bv = v;
+                bk = a;
+            }
+        }
+    };
+    rescan();
+    for (int64_t i = 0; i < r.G; ++i) {
+        double v = bv;
+        int k = bk;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(v, off, 64);
+            const int ok = __shfl_xor(k, off, 64);
+            if (ov > v || (ov == v && ok < k)) {
+                v = ov;
+                k = ok;
+            }
+        }
+        if (lane == 0) {
+            sv[wid] = v;
+            sk[wid] = k;
+        }
+        __syncthreads();
+        v = sv[0];
+        k = sk[0];
+        for (int w = 1; w < 16; ++w)
+            if (sv[w] > v || (sv[w] == v && sk[w] < k)) {
+                v = sv[w];
+                k = sk[w];
+            }
+        __syncthreads();
+        if (t == 0) {
+            r.sel[i] = k;
+            r.seld[i] = v;
+        }
+        if (k != INT_MAX && (k & 1023) == t) {
+            taken |= 1ull << (k >> 10);
+            rescan();
+        }
+    }
+}
+
 __global__ void kb_gather(const BInstRaw* insts, const int32_t* list) {
     const BInstRaw& r = insts[list[blockIdx.y]];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2805,6 +2872,10 @@ static void b_seeds_impl(const BInstRaw* d, const int32_t* list, int64_t L, int6
     using FT = typename FieldOf<WT>::T;
     const unsigned gx = (unsigned)((max_seeds + 255) / 256);
     hipLaunchKernelGGL((kb_seed_lowrank<WT, FT>), dim3(gx, (unsigned)L), dim3(256), 0, s, d, list);
+    if (max_seeds <= kTopkMaxSeeds) {
+        hipLaunchKernelGGL(kb_topk, dim3((unsigned)L), dim3(1024), 0, s, d, list);
+        return;
+    }
     int64_t splits = std::min<int64_t>(64, (max_seeds + 511) / 512);
     hipLaunchKernelGGL(kb_rank, dim3(gx, (unsigned)splits, (unsigned)L), dim3(256), 0, s, d, list, splits);
     hipLaunchKernelGGL(kb_rank_scatter, dim3(gx, (unsigned)L), dim3(256), 0, s, d, list);
