@@ -125,6 +125,8 @@ typedef struct {
      * every round and replaced; entries that had moved by more than 1e-9 max(1, |h|) are counted */
     int64_t field_checks, field_repairs;
     double max_field_drift;
+    /* learned phase: the smallest cut gain (unscaled) of an accepted learned move; +inf when none */
+    double learned_min_gain;
 } pqi_search_stats;
 
 /* Thread-local message for the last failing call on this thread. */
@@ -284,7 +286,8 @@ PQI_API int pqi_mpnn_train_step(pqi_mpnn* m, const float* feats, const float* la
  * spins whose score (a logit) exceeds min_score, then
  * the loop of pqi_batch_local_search.  The network must be bound to exactly
  * these graphs, in this order.  targets (optional, count values: energy in
- * spin-glass mode, cut otherwise): an instance stops once it reaches its own. */
+ * spin-glass mode, cut otherwise): an instance stops once it reaches its own.
+ * p->max_rounds < 0 stops after the learned phase (no round of the loop). */
 PQI_API int pqi_learned_search(pqi_ctx* ctx, pqi_mpnn* net, pqi_graph* const* graphs, int64_t count,
                                uint8_t* const* thetas, const pqi_search_params* p, const double* targets,
                                int64_t passes, float min_score, pqi_search_stats* stats);

@@ -90,7 +90,7 @@ def lib():
             "orc_sparse_gray_optimization": (_i64, [_vp, _vp, _i64, _i64, _int, _vp, _vp, _vp, _i64, _vp, _vp]),
             "orc_sparse_local_search": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp]),
             "orc_exact": (_dbl, [_vp, _i64, _int, _vp]),
-            "orc_first_improvement": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp, _vp]),
+            "orc_first_improvement": (_i64, [_vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -309,5 +309,6 @@ def first_improvement(W, x, order, max_passes):
     o = np.ascontiguousarray(order, dtype=np.int64)
     p = ctypes.c_int64()
     g = ctypes.c_double()
-    f = lib().orc_first_improvement(_p(W), W.shape[0], _p(xs), _p(o), int(max_passes), ctypes.byref(p), ctypes.byref(g))
+    f = lib().orc_first_improvement(_p(W), W.shape[0], _p(xs), _p(o), int(o.size), int(max_passes), ctypes.byref(p),
+                                    ctypes.byref(g))
     return xs, int(f), p.value, g.value

@@ -761,18 +761,19 @@ ORC_API double orc_exact(const double* W, int64_t n, int is_spin_glass, uint8_t*
 
 /* ------------------------------------------------------------------------- */
 /* First-improvement passes in a given node order (the learned phase with a   */
-/* fixed order; no counterpart in the reference): visit nodes in `order`,     */
+/* fixed order; no counterpart in the reference): visit the `count` nodes   */
+/* of `order` in turn,                                                        */
 /* flip a node iff its cut gain is positive at that moment; repeat passes     */
 /* until one flips nothing or max_passes is reached.  Returns total flips.    */
 /* ------------------------------------------------------------------------- */
 ORC_API int64_t orc_first_improvement(const double* W, int64_t n, uint8_t* x, const int64_t* order,
-                                      int64_t max_passes, int64_t* passes_out, double* gain_out) {
+                                      int64_t count, int64_t max_passes, int64_t* passes_out, double* gain_out) {
     int64_t flips = 0, passes = 0;
     double gain = 0.0;
     while (passes < max_passes) {
         ++passes;
         int64_t f = 0;
-        for (int64_t t = 0; t < n; ++t) {
+        for (int64_t t = 0; t < count; ++t) {
             const int64_t v = order[t];
             x[v] ^= 1u;
             const double d = orc_cut_diff(v, x, W, n);

@@ -99,6 +99,7 @@ class SearchStats(ctypes.Structure):
         ("field_checks", ctypes.c_int64),
         ("field_repairs", ctypes.c_int64),
         ("max_field_drift", ctypes.c_double),
+        ("learned_min_gain", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -74,6 +74,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     if (!c || !graphs || !thetas || !p || !stats) return fail(PQI_ERR_INVALID, "null argument");
     if (count < 1) return fail(PQI_ERR_INVALID, "count must be >= 1");
     if (p->walkers < 1 || p->seed_multiple < 1) return fail(PQI_ERR_INVALID, "walkers and seed_multiple must be >= 1");
+    if (p->max_rounds < 0 && !net) return fail(PQI_ERR_INVALID, "max_rounds must be >= 0");
     if (p->double_mode != PQI_DOUBLE_REFERENCE)
         return fail(PQI_ERR_UNSUPPORTED, "the batched search implements the reference double-flip mode");
     const int kind = graphs[0]->kind;
@@ -258,6 +259,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     std::vector<int32_t> src(count);
     std::vector<double> qv(count);
     std::memset(stats, 0, sizeof(pqi_search_stats) * count);
+    for (int64_t i = 0; i < count; ++i) stats[i].learned_min_gain = INFINITY;
 
     // Host -> device uploads go through the pinned bump area, so they are
     // asynchronous; the area is reused after each synchronisation (and, should
@@ -341,6 +343,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
                 pqi_search_stats& st = stats[run[t]];
                 st.learned_accepts += outs[t].accepts;
                 st.learned_passes++;
+                if (outs[t].accepts > 0) st.learned_min_gain = std::min(st.learned_min_gain, outs[t].gain);
                 st.max_energy += scale * outs[t].gain_sum;
                 if (outs[t].accepts > 0 && !reached(run[t])) again.push_back(run[t]);
             }
@@ -352,7 +355,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
 
     std::vector<int32_t> active;
     for (int32_t i : list)
-        if (!reached(i)) active.push_back(i);
+        if (!reached(i) && !(net && p->max_rounds < 0)) active.push_back(i);
     std::vector<double> walk_gain(count), reheat(count);
     std::vector<double> we(G);
     std::vector<int64_t> wev(G);

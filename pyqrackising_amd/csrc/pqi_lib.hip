@@ -1472,6 +1472,7 @@ int pqi_local_search(pqi_graph* g, uint8_t* theta, const pqi_search_params* p, p
     pqi_ctx* c = g->ctx;
     CHK(set_device(c));
     std::memset(st, 0, sizeof(*st));
+    st->learned_min_gain = INFINITY;
     return (p->flags & PQI_FLAG_HOST_LOOP) ? local_search_host(g, theta, p, st) : local_search_device(g, theta, p, st);
 }
 

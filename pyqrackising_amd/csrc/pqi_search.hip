@@ -3089,7 +3089,7 @@ __global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, con
     __syncthreads();
     auto gain = [&](int64_t v) -> FT { return use_lds ? D[v] : (x[v] ? -hg[v] : hg[v]); };
     int64_t flips = 0;
-    double gsum = 0.0;
+    double gsum = 0.0, gmin = INFINITY;
     for (int64_t c0 = 0; c0 < n; c0 += 64) {
         const int64_t v = c0 + lane < n ? r.perm[c0 + lane] : 0;
         const bool valid = c0 + lane < n && r.score[v] > min_score;
@@ -3100,6 +3100,7 @@ __global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, con
             const int64_t k = __shfl(v, bl, 64);
             const FT dd = __shfl(d, bl, 64);
             gsum += (double)dd;
+            gmin = fmin(gmin, (double)dd);
             ++flips;
             const int sk = x[k] ? -1 : 1;
             __syncthreads();
@@ -3136,7 +3137,7 @@ __global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, con
         r.out->accepts = flips;
         r.out->gain_sum = gsum;
         r.out->evals = n;
-        r.out->gain = 0.0;
+        r.out->gain = gmin;  // the smallest accepted gain (+inf: none)
         r.out->k = -1;
         r.out->l = -1;
     }

@@ -51,12 +51,14 @@ def _exact(dg, n, is_spin_glass):
     """pyqrackising/maxcut_tfim.py:288-339 on the GPU: all 2^n masks scored in
     one batch; strict '>' from max_cut = -1 keeps the first maximum.  The
     spin-glass variant maximises sum_{i<j} w_ij (+1 same side, -1 otherwise),
-    which is -E (pyqrackising/maxcut_tfim.py:328-329)."""
+    which is -E (pyqrackising/maxcut_tfim.py:328-329).  Every mask is scored
+    by the exact engine (FP64 for real weights): the approximate split-BF16
+    values would decide the argmax of -E, which is never re-scored."""
     masks = np.arange(1 << n, dtype=np.uint32).reshape(-1, 1)
     best_mask, best_val = 0, -1.0
     chunk = 1 << 20
     for c0 in range(0, masks.shape[0], chunk):
-        _, _, vals = dg.eval_batch(masks[c0:c0 + chunk], is_spin_glass)
+        _, _, vals = dg.eval_batch(masks[c0:c0 + chunk], is_spin_glass, engine=_lib.ENGINE_EXACT)
         if is_spin_glass:
             vals = -vals
         i = int(np.argmax(vals))

@@ -174,35 +174,6 @@ def test_learned_search_moves_are_exact(orc):
         assert g <= 0
 
 
-def test_trained_network_gain():
-    """What training buys: the same learned search (confident proposals only,
-    four passes, exact gains) with the trained network and with the untrained
-    one, on held-out Erdos-Renyi graphs with U[-1, 1] couplings; the measure is
-    how many instances reach the plain batched search's final energy."""
-    from pyqrackising_amd import learned
-
-    log("trained vs untrained network")
-    P = _lib.make_params(0, is_spin_glass=1, walkers=16, seed_multiple=8)
-    P.gray_iterations = -1
-    solver = learned.LearnedSolver(seed=11)
-    losses = solver.fit(learned.regular_training_graphs(32, seed=3001, n_hi=1536), P, seed=5, steps=250)
-    assert losses[-1] < 0.6 * losses[0]
-    G5 = learned.er_graphs(32, 2048, 8.0, seed=5006)
-    dgs = [_lib.DeviceGraph.csr(G) for G in G5]
-    rng = np.random.default_rng(78)
-    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in G5]
-    _, plain = _lib.batch_local_search(dgs, starts, P)
-    targets = np.array([s["max_energy"] for s in plain])
-    solver.bind(dgs)
-    _, tr = solver.solve(dgs, starts, P, targets=targets, passes=4, min_score=1.0, bind=False)
-    trained = sum(s["max_energy"] >= t for s, t in zip(tr, targets))
-    solver.net.set_params(_lib.Mpnn(seed=11).params())
-    _, un = solver.solve(dgs, starts, P, targets=targets, passes=4, min_score=1.0, bind=False)
-    untrained = sum(s["max_energy"] >= t for s, t in zip(un, targets))
-    log(f"  reached the plain energy: trained {trained}/32, untrained {untrained}/32")
-    assert trained >= untrained
-
-
 def test_adversarial_network_reaches_the_plain_result():
     """Adversarial weights: the score is 10 relu(-f0), f0 = gain / strength, so
     only spins whose flip LOSES are proposed (min_score 1), worst first.  The

@@ -12,6 +12,7 @@ import os
 import networkx as nx
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from pyqrackising_amd import graphs
 
@@ -373,3 +374,146 @@ def test_first_improvement_passes(orc):
             y = z
     x1, _, p1, _ = orc.first_improvement(W, x, order, 1)
     assert p1 == 1 and np.array_equal(x1, y)
+
+
+def test_benchmark_configuration_record():
+    """tests/golden/c2_bench_config.json (the oracle's search at the bench
+    configuration, recorded by tests/golden/make_c2_bench_golden.py): the
+    instance is the one bench.c2_instance() builds, and the recorded state's
+    cut and energy, computed here in closed form (cut = sum over cut edges,
+    E = 2 cut - W_up), are the recorded values."""
+    import hashlib
+    import json
+    import os
+
+    from bench import c2_instance
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "tests", "golden", "c2_bench_config.json")) as f:
+        gold = json.load(f)
+    W, x0 = c2_instance()
+    n = W.shape[0]
+    digest = hashlib.sha256(np.ascontiguousarray(W, np.int8).tobytes() + x0.astype(np.uint8).tobytes())
+    assert digest.hexdigest() == gold["instance_sha256"]
+    x = np.unpackbits(np.frombuffer(bytes.fromhex(gold["state_packed_hex"]), np.uint8), bitorder="little")[:n]
+    Wd = W.astype(np.float64)
+    iu, ju = np.triu_indices(n, 1)
+    w = Wd[iu, ju]
+    cut = float(w[x[iu] != x[ju]].sum())
+    r = gold["result"]
+    assert r["cut_value"] == cut
+    assert r["min_energy"] == -(2 * cut - w.sum()) and r["max_energy"] == -r["min_energy"]
+    # the run accepted moves in the walk phase, and every phase ran
+    assert r["single_accepts"] > 0 and r["walk_accepts"] > 0 and r["seed_evals"] > 0
+
+
+def _trace_cases():
+    import json
+    import os
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "tests", "golden", "loop_traces.json")) as f:
+        return json.load(f)["cases"]
+
+
+def _bits(s):
+    return np.array([c == "1" for c in s], np.uint8)
+
+
+def _dense_from_edges(n, edges):
+    W = np.zeros((n, n))
+    for u, v, w in edges:
+        W[u, v] = W[v, u] = w
+    return W
+
+
+def _closed_cut(W, x):
+    iu, ju = np.triu_indices(W.shape[0], 1)
+    return float(W[iu, ju][x[iu] != x[ju]].sum())
+
+
+@pytest.mark.parametrize("case", _trace_cases(), ids=lambda c: c["name"][:1] + ("sg" if c["is_spin_glass"] else ""))
+def test_loop_traces_derived_by_hand(orc, case):
+    """tests/golden/loop_traces.json: tiny graphs on which the reference loop
+    (pyqrackising/spin_glass_solver.py:603-717) takes, in order, a reference-
+    mode double flip after the singles stall (D2 last pairs), a Gray-seed
+    accept, a walk accept whose total starts from the seed's difference, and a
+    walk that must reject zero-gain steps.  Each case's moves, phase counters,
+    bookkeeping and final state were worked out by hand (the derivation is in
+    the file).  Here every derived step is re-checked against the closed-form
+    cut, the per-phase oracle functions against the derived intermediate
+    results, and both oracle loops (dense and sparse) against the final
+    outcome."""
+    n, sg = case["n"], case["is_spin_glass"]
+    W = _dense_from_edges(n, case["edges"])
+    x = _bits(case["x0"])
+    cut = _closed_cut(W, x)
+    for st in case["steps"]:
+        y = x.copy()
+        y[st["flip"]] ^= 1
+        assert _closed_cut(W, y) - cut == st["gain"] and _closed_cut(W, y) == st["cut"], st
+        x, cut = y, _closed_cut(W, y)
+    ex = case["expect"]
+    assert np.array_equal(x, _bits(ex["x"])) and cut == ex["cut_value"]
+    scale = 2 if sg else 1
+    for chk in case["checks"]:
+        xc = _bits(chk["x"])
+        e = chk["expect"]
+        if chk["fn"] == "single_bit_flips":
+            assert orc.single_bit_flips(xc, sg, W) == (scale * e["gain"], e["k"])
+        elif chk["fn"] == "double_bit_flips":
+            assert orc.double_bit_flips(xc, sg, W, 2, mode=0) == (e["gain"], e["k"], e["l"])
+        elif chk["fn"] == "pick_gray_seeds":
+            seeds, en, idx, alle = orc.pick_gray_seeds(xc, 2, 2, W, want_all=True)
+            assert idx.tolist() == e["order"] and en.tolist() == e["energies"] and alle.tolist() == e["all"]
+        else:
+            seeds, en, idx = orc.pick_gray_seeds(xc, 2, 2, W)
+            bw, g, it, tot, _ = orc.gray_optimization(seeds, en, 4, 2, sg, W)
+            assert (bw, g, tot.tolist()) == (e["best"], scale * e["gain"], e["totals"])
+            assert np.array_equal(it[bw], _bits(e["state"]))
+    x0 = _bits(case["x0"])
+    G = sp.csr_matrix(np.triu(W, 1))
+    for name, (xf, r) in (("dense", orc.dense_local_search(W, x0, is_spin_glass=sg, walkers=2, seed_multiple=2,
+                                                           gray_iterations=4)),
+                          ("sparse", orc.sparse_local_search(G, x0, is_spin_glass=sg, walkers=2, seed_multiple=2,
+                                                             gray_iterations=4))):
+        assert np.array_equal(xf, _bits(ex["x"])), name
+        for k in ("single_accepts", "double_accepts", "seed_accepts", "walk_accepts", "reheat_accepts",
+                  "max_energy", "cut_value"):
+            assert r[k] == ex[k], (name, k)
+        if "min_energy" in ex:
+            assert r["min_energy"] == ex["min_energy"], name
+
+
+@pytest.mark.parametrize("kind", ["int", "real"])
+def test_post_reheat_branch_never_accepts(orc, kind):
+    """The post-reheat phase (pyqrackising/spin_glass_solver.py:670-717;
+    spin_glass_solver_sparse.py:802-854) cannot accept a move once the walkers
+    start from their seeds' differences (the zero-initialised accumulator).
+    Seed 0 is gray_mutation(0, theta) = theta itself, with difference 0, the
+    largest any seed can have when the seed phase was rejected, and the lowest
+    index, so walker 0 starts at theta with total 0.  It only keeps flips of
+    positive gain, so if the walk is rejected walker 0 kept none: it is still
+    at theta with total 0, every other total is <= 0, and the first argmax
+    hands theta itself, with reheat energy 0, to the reheat phase, whose
+    single and double flips on theta were rejected at the start of the same
+    iteration.  (In the reference the dense accumulator is np.empty, so only
+    uninitialised memory can make the branch fire.)  Checked here on random
+    small instances for both loops."""
+    rng = np.random.default_rng(5 if kind == "int" else 6)
+    walks = 0
+    for t in range(40):
+        n = int(rng.integers(6, 24))
+        U = np.triu(rng.random((n, n)) < 0.4, 1)
+        w = rng.integers(-3, 4, (n, n)) if kind == "int" else rng.uniform(-1, 1, (n, n))
+        W = np.where(U, w, 0).astype(np.float64)
+        W = W + W.T
+        x0 = rng.integers(0, 2, n).astype(np.uint8)
+        G = int(rng.integers(1, 5))
+        kw = dict(is_spin_glass=int(t % 2), walkers=G, seed_multiple=int(rng.integers(1, 4)),
+                  gray_iterations=int(rng.integers(1, 200)))
+        _, r = orc.dense_local_search(W, x0, **kw)
+        _, rs = orc.sparse_local_search(sp.csr_matrix(np.triu(W, 1)), x0, **kw)
+        assert r["reheat_accepts"] == 0 and rs["reheat_accepts"] == 0
+        walks += r["walk_accepts"]
+    assert walks > 0  # the walks did accept moves on these instances
