@@ -1,21 +1,30 @@
-"""Benchmark: the spin_glass_solver local search (single flips, double flips,
-Gray seeds, Gray walkers, reheat) on one MI355X.
+"""Benchmark: time-to-equal-cut and flip evaluations per second, the learned
+and batched search against the plain HIP search, on one MI355X.
 
-Workload (one step = one whole local search from a fixed start, until no
-phase improves): random 3-regular graph, n = 2048, couplings uniform +-1
-(int8), spins +-1; 4096 Gray seeds per Gray phase (block_size 128 x 32
-blocks) scored by the int8 MFMA contraction; gray_iterations = n^2.
+Headline workload (one step = one solve of the whole batch): 64 Erdos-Renyi
+graphs, n = 8192, mean degree 8, couplings U[-1, 1] stored as fp32, CSR,
+spin-glass mode, from fixed random starts, gray_iterations = n^2, 16 walkers
+x 8.  The target of every instance is the final energy of the plain GPU
+search (the reference loop, one instance at a time) from the same start.  A
+step runs the learned+batched solver (network-ordered first-improvement
+passes with exact gains, then the reference loop, all instances per launch,
+each stopping at its target); instances that end short of their target are
+solved again from their start by the plain batched search, so every instance
+reaches its target inside the step.
 
 Prints one JSON line:
-  value = flip evaluations per second of the whole local search (single-flip
-  gains, pair gains and Gray steps; each is one exact gain evaluation),
-  roofline = the dominant kernel (the single-flip descent, k_descent_reg,
-  packed keys in registers) against HBM bandwidth; its duration comes from
-  HIP events the library records around each descent launch on its own
-  stream,
-  cpu_baseline = the CPU oracle (a C restatement of the reference CPU path with
-  the reference's O(n)-per-evaluation cost) on a bounded sample of the same
-  instance, timed on this host.
+  value = instances brought to their target per second by the learned+batched
+  solver (64 / seconds per step); the plain sequential and plain batched
+  times and both speed-ups are reported beside it,
+  roofline = the step's dominant kernel, the FP64 CSR descent, against HBM
+  bandwidth, its duration from HIP events the library records around each
+  descent launch on its own stream,
+  cpu_baseline = the oracle (a C restatement of the reference's sparse CPU
+  loop, O(nnz) per candidate) on a bounded sample of the same workload, timed
+  on this host,
+  c2_local_search = the single-instance dense search (random 3-regular
+  n = 2048, +-1 int8, 4096 Gray seeds per phase on int8 MFMA): ms per search
+  and computed flip evaluations per second, with its own roofline.
 """
 import argparse
 import json
@@ -29,15 +38,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_descent_reg launch on this workload, from rocprofv3 FETCH_SIZE
-# and WRITE_SIZE passes scaled by a plain copy of known size (see the file)
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_descent_reg_c2_n2048.md")
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+# HBM bytes per launch of each roofline kernel on its workload, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy of known size (see
+# the files)
+C5_TRAFFIC_FILE = os.path.join(PROFILES, "kb_descent_lds_c5_64x8192.md")
+C2_TRAFFIC_FILE = os.path.join(PROFILES, "k_descent_reg_c2_n2048.md")
 
 
-def measured_traffic():
-    """Bytes per launch recorded in the committed profile summary, or None."""
+def measured_traffic(path):
+    """Bytes per launch recorded in a committed profile summary, or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -81,81 +93,272 @@ def c2_instance(n=2048, seed=1002):
     return W, x0
 
 
-def cpu_baseline(W, x0, walkers, seed_multiple, gray_iterations):
-    """Bounded sample: the oracle's local search on the same instance and start
-    with the Gray phase capped; flip evaluations per second."""
+def c5_workload():
+    """The headline batch: 64 x ER(8192, mean degree 8), U[-1, 1] fp32, and
+    its random starts."""
+    from pyqrackising_amd import learned
+
+    G5 = learned.er_graphs(64, 8192, 8.0, seed=5005)
+    rng = np.random.default_rng(77)
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in G5]
+    return G5, starts
+
+
+def c5_params(n, flags=0):
+    from pyqrackising_amd import _lib
+
+    p = _lib.make_params(n, is_spin_glass=1, walkers=16, seed_multiple=8, flags=flags)
+    if n == 0:
+        p.gray_iterations = -1  # n^2 per instance
+    return p
+
+
+def c5_cpu_baseline(G, x0, iterations_needed, log):
+    """Bounded sample: the oracle's sparse loop (the reference's CPU
+    complexity: every candidate of a phase re-scored over all stored edges) on
+    instance 0 of the batch, 3 loop iterations timed and extrapolated to the
+    iterations the plain search needs on that instance.  Later iterations
+    (Gray seeds and walkers) cost more than these first single-flip ones, so
+    the extrapolated time is a lower bound and the rate an upper bound."""
     from oracle import oracle as orc
 
-    threads = os.cpu_count() or 1
+    n = G.shape[0]
     t = time.perf_counter()
-    _, r = orc.dense_local_search(W.astype(np.float64), x0, is_spin_glass=1, walkers=walkers,
-                                  seed_multiple=seed_multiple, gray_iterations=gray_iterations)
+    _, r = orc.sparse_local_search(G, x0, is_spin_glass=1, walkers=16, seed_multiple=8, gray_iterations=n * n,
+                                   max_rounds=3)
     dt = time.perf_counter() - t
-    evals = r["single_evals"] + r["double_evals"] + r["gray_evals"]  # the oracle computes every step
+    per_iter = dt / max(1, r["rounds"])
+    est = per_iter * iterations_needed
+    log("# CPU oracle sample done")
     return {
-        "value": evals / dt,
-        "unit": "flip evals/s",
-        "cores": threads,
+        "value": 1.0 / est,
+        "unit": "instances/s",
+        "cores": os.cpu_count() or 1,
         "kind": "port",
-        "sample": (f"oracle dense_local_search on the same instance and start, gray_iterations={gray_iterations} "
-                   f"(GPU run: n^2), walkers={walkers}, seed_multiple={seed_multiple}; {evals} flip evals in "
-                   f"{dt:.2f} s; final cut {r['cut_value']:.0f}"),
-        "seconds": dt,
+        "sample": (f"oracle sparse_local_search on instance 0 (n={n}), {r['rounds']} loop iterations in {dt:.2f} s; "
+                   f"the plain search of that instance takes {iterations_needed} iterations: >= {est:.0f} s per "
+                   f"instance (a lower bound: the Gray-phase iterations cost more)"),
     }
 
 
-def time_to_equal_cut(log):
-    """64 Erdos-Renyi graphs, n = 8192, mean degree 8, couplings U[-1, 1] (fp32),
-    spin-glass mode, from fixed random starts: the plain HIP local search one
-    instance at a time (the reference loop per instance) against the batched
-    search (all instances per launch, identical moves) and the learned+batched
-    search (network-ordered proposals with exact gains, stopping at the plain
-    search's final energy; instances that end short are re-solved by the
-    batched plain search from their start, so every instance reaches it)."""
+def headline_c5(steps, warmup, log, with_cpu):
+    """Time-to-equal-cut on the headline batch: plain sequential (the reference
+    loop per instance, one call each), plain batched (all instances per
+    launch, identical moves), plain batched stopping at the targets, and the
+    timed learned+batched steps."""
     from pyqrackising_amd import _lib, learned
 
-    P = _lib.make_params(0, is_spin_glass=1, walkers=16, seed_multiple=8)
-    P.gray_iterations = -1
+    P = c5_params(0)
+    Pt = c5_params(0, _lib.FLAG_TIME_KERNELS)
     solver = learned.LearnedSolver(seed=11)
     t = time.perf_counter()
     solver.fit(learned.regular_training_graphs(48, seed=3001), P, seed=5, steps=300, lr=2e-3)
     train_s = time.perf_counter() - t
-    G5 = learned.er_graphs(64, 8192, 8.0, seed=5005)
+    log(f"# network trained in {train_s:.2f} s")
+    G5, starts = c5_workload()
     dgs = [_lib.DeviceGraph.csr(G) for G in G5]
-    rng = np.random.default_rng(77)
-    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in G5]
     solver.bind(dgs)
-    # warm up every path once
-    dgs[0].local_search(starts[0], _lib.make_params(8192, is_spin_glass=1, walkers=16, seed_multiple=8))
+    # warm every path once
+    dgs[0].local_search(starts[0], c5_params(8192))
     _lib.batch_local_search(dgs[:2], starts[:2], P)
     t = time.perf_counter()
-    plain = [dg.local_search(x, _lib.make_params(dg.n, is_spin_glass=1, walkers=16, seed_multiple=8))
-             for dg, x in zip(dgs, starts)]
-    t_plain = time.perf_counter() - t
+    plain = [dg.local_search(x, c5_params(dg.n)) for dg, x in zip(dgs, starts)]
+    t_seq = time.perf_counter() - t
     targets = np.array([st["max_energy"] for _, st in plain])
-    log("# time-to-equal-cut: plain sequential done")
-    t = time.perf_counter()
-    bs, _ = _lib.batch_local_search(dgs, starts, P)
-    t_batch = time.perf_counter() - t
+    log("# plain sequential done")
+    times = []
+    for _ in range(3):
+        device_sync()
+        t = time.perf_counter()
+        bs, _ = _lib.batch_local_search(dgs, starts, P)
+        device_sync()
+        times.append(time.perf_counter() - t)
+    t_batch = float(np.median(times))
     identical = all(np.array_equal(a, b[0]) for a, b in zip(bs, plain))
     t = time.perf_counter()
-    _, lst = solver.solve(dgs, starts, P, targets=targets, passes=4, min_score=1.0, bind=False)
-    short = [i for i, s in enumerate(lst) if s["max_energy"] < targets[i]]
-    if short:
-        _lib.batch_local_search([dgs[i] for i in short], [starts[i] for i in short], P)
-    t_learned = time.perf_counter() - t
-    return {
-        "workload": "64 x ER(n=8192, mean degree 8), U[-1,1] fp32 couplings, spin glass, gray_iterations=n^2, "
-                    "walkers=16, seed_multiple=8",
-        "plain_sequential_s": t_plain,
-        "batched_s": t_batch,
-        "batched_identical_to_plain": identical,
-        "learned_batched_s": t_learned,
-        "learned_reached_without_fallback": 64 - len(short),
-        "speedup_batched": t_plain / t_batch,
-        "speedup_learned_batched": t_plain / t_learned,
-        "network_training_s": train_s,
+    _lib.learned_search(solver.net, dgs, starts, P, targets=targets, passes=0)
+    t_batch_targets = time.perf_counter() - t
+    log("# plain batched done")
+
+    def step():
+        _, lst = solver.solve(dgs, starts, Pt, targets=targets, passes=4, min_score=1.0, bind=False)
+        short = [i for i, st in enumerate(lst) if st["max_energy"] < targets[i]]
+        fb = []
+        if short:
+            _, fb = _lib.batch_local_search([dgs[i] for i in short], [starts[i] for i in short], Pt)
+        return lst, fb, short
+
+    for w in range(warmup):
+        step()
+        log(f"# warmup {w} done")
+    recs = []
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        recs.append(step())
+    device_sync()
+    dt = time.perf_counter() - t0
+    for k, (lst, fb, short) in enumerate(recs):
+        log(f"# step {k}: {64 - len(short)} of 64 reached in the learned search, {len(short)} by the fallback")
+    ms_step = dt / steps * 1e3
+    # the dominant kernel: the FP64 CSR descent (kb_descent_lds), event-timed
+    desc_s = sum(r[0][0]["descent_kernel_seconds"] + (r[1][0]["descent_kernel_seconds"] if r[1] else 0.0)
+                 for r in recs)
+    desc_launches = sum(r[0][0]["descent_launches"] + (r[1][0]["descent_launches"] if r[1] else 0) for r in recs)
+    flips = sum(sum(st["single_accepts"] for st in r[0]) + sum(st["single_accepts"] for st in r[1]) for r in recs)
+    pairs = sum(r[0][0]["descent_instance_launches"] + (r[1][0]["descent_instance_launches"] if r[1] else 0)
+                for r in recs)
+    writes = sum(r[0][0]["descent_instance_writes"] + (r[1][0]["descent_instance_writes"] if r[1] else 0)
+                 for r in recs)
+    mean_deg = float(np.mean([2.0 * G.nnz / G.shape[0] for G in G5]))
+    avg_launch_s = desc_s / max(1, desc_launches)
+    # algorithmic bytes: an accepted CSR flip reads its neighbours' indices and
+    # weights and reads and writes their fp64 gains (16 B per neighbour), and
+    # every (instance, launch) reads the instance's spins and gains (9 B per
+    # node) and writes them back when it moved
+    n5 = 8192
+    bytes_per_launch = (flips * 16.0 * mean_deg + (pairs + writes) * 9.0 * n5) / max(1, desc_launches)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    lst = recs[-1][0]
+    out = {
+        "value": 64.0 / (dt / steps),
+        "ms_per_step": ms_step,
+        "config": {
+            "workload": ("64 x Erdos-Renyi(n=8192, mean degree 8), U[-1,1] fp32 CSR, spin glass, learned+batched "
+                         "search to the plain search's final energy per instance (gray_iterations=n^2, "
+                         "16 walkers x 8, 4 learned passes, network-ordered proposals with exact gains)"),
+            "instances": 64, "n": 8192, "mean_degree": mean_deg,
+            "time_to_equal_cut_s": dt / steps,
+            "plain_sequential_s": t_seq,
+            "plain_batched_s": t_batch,
+            "plain_batched_identical_to_sequential": identical,
+            "plain_batched_stopping_at_targets_s": t_batch_targets,
+            "speedup_vs_plain_sequential": t_seq / (dt / steps),
+            "speedup_vs_plain_batched": t_batch / (dt / steps),
+            "reached_in_learned_search": 64 - len(recs[-1][2]),
+            "learned_moves_per_step": int(sum(st["learned_accepts"] for st in lst)),
+            "network_training_s": train_s,
+        },
+        "roofline": {
+            "kernel": "kb_descent_lds<float, double>",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(C5_TRAFFIC_FILE),
+            "traffic_source": "profiles/kb_descent_lds_c5_64x8192.md",
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "launches_per_step": desc_launches / steps,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": None,
     }
+    if with_cpu:
+        st0 = plain[0][1]
+        iters = (st0["single_accepts"] + st0["double_accepts"] + st0["seed_accepts"] + st0["walk_accepts"] +
+                 st0["reheat_accepts"] + 1)
+        out["cpu_baseline"] = c5_cpu_baseline(G5[0], starts[0], iters, log)
+    return out
+
+
+def c2_local_search(steps, warmup, log, with_cpu, cpu_full, walkers=128, seed_multiple=1):
+    """The single-instance dense search: random 3-regular n = 2048, +-1 int8,
+    4096 Gray seeds per phase (128 walkers x 1) on int8 MFMA, gray_iterations =
+    n^2, from a fixed start; one step = one whole search."""
+    from pyqrackising_amd import _lib
+
+    W, x0 = c2_instance()
+    n = W.shape[0]
+    gi = n * n
+    dg = _lib.DeviceGraph.dense(W)
+    params = _lib.make_params(n, is_spin_glass=1, gray_iterations=gi, walkers=walkers, seed_multiple=seed_multiple,
+                              seed_engine=_lib.SEEDS_MFMA, flags=_lib.FLAG_TIME_KERNELS)
+    for _ in range(warmup):
+        dg.local_search(x0, params)
+    stats = []
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x, st = dg.local_search(x0, params)
+        stats.append(st)
+    device_sync()
+    dt = time.perf_counter() - t0
+    st = stats[-1]
+    # computed gain evaluations only; Gray steps that repeat an already
+    # rejected step of an unchanged walker are resolved without computing
+    evals = st["single_evals"] + st["double_evals"] + st["gray_evals_computed"]
+    total_evals = sum(s["single_evals"] + s["double_evals"] + s["gray_evals_computed"] for s in stats)
+    total_ref = sum(s["single_evals"] + s["double_evals"] + s["gray_evals"] for s in stats)
+    # dominant kernel: the single-flip descent (k_descent_reg); algorithmic
+    # bytes per flip evaluation: the gain (4 B int32) and the spin (1 B); per
+    # accepted flip, 16 B per neighbour (index and weight read, gain read and
+    # written) -- degree 3 here
+    desc_s = sum(s["descent_kernel_seconds"] for s in stats)
+    desc_launches = sum(s["descent_launches"] for s in stats)
+    desc_evals = sum(s["descent_evals"] for s in stats)
+    desc_flips = sum(s["single_accepts"] for s in stats)
+    avg_launch_s = desc_s / max(1, desc_launches)
+    bytes_per_launch = (5.0 * desc_evals + 16.0 * 3 * desc_flips) / max(1, desc_launches)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    out = {
+        "workload": (f"spin_glass_solver local search, random 3-regular n={n}, +-1 int8 couplings, "
+                     f"{walkers * seed_multiple * ((n + 63) // 64)} Gray seeds per phase on int8 MFMA, "
+                     f"walkers={walkers}, seed_multiple={seed_multiple}, gray_iterations={gi}"),
+        "ms_per_search": dt / steps * 1e3,
+        "flip_evals_per_s": total_evals / dt,
+        "flip_evals_per_search": evals,
+        "reference_flip_steps_per_s": total_ref / dt,
+        "seed_candidates_per_search": st["seed_evals"],
+        "final_cut": st["cut_value"],
+        "min_energy": st["min_energy"],
+        "phase_seconds": {k: st[k] for k in ("seconds_single", "seconds_double", "seconds_seeds", "seconds_walk",
+                                              "seconds_reheat")},
+        "roofline": {
+            "kernel": "k_descent_reg<32, 4>",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(C2_TRAFFIC_FILE),
+            "traffic_source": "profiles/k_descent_reg_c2_n2048.md",
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+    }
+    log("# C2 local search done")
+    if with_cpu:
+        from oracle import oracle as orc
+
+        threads = os.cpu_count() or 1
+        if cpu_full:
+            # identical parameters, the whole search (minutes of CPU time)
+            t = time.perf_counter()
+            xc, r = orc.dense_local_search(W.astype(np.float64), x0, is_spin_glass=1, walkers=walkers,
+                                           seed_multiple=seed_multiple, gray_iterations=gi)
+            ct = time.perf_counter() - t
+            cev = r["single_evals"] + r["double_evals"] + r["gray_evals"]
+            out["cpu_same_parameters"] = {
+                "seconds": ct, "cores": threads, "kind": "port", "same_state": bool(np.array_equal(xc, x)),
+                "final_cut": r["cut_value"], "time_to_cut_ratio": ct / (dt / steps),
+                "flip_evals_per_s": cev / ct,
+            }
+        else:
+            # bounded sample: the same parameters, the first 64 loop iterations
+            # (single-flip phases, O(n) per gain as in the reference)
+            t = time.perf_counter()
+            _, r = orc.dense_local_search(W.astype(np.float64), x0, is_spin_glass=1, walkers=walkers,
+                                          seed_multiple=seed_multiple, gray_iterations=gi, max_rounds=64)
+            ct = time.perf_counter() - t
+            cev = r["single_evals"] + r["double_evals"] + r["gray_evals"]
+            out["cpu_sample"] = {
+                "flip_evals_per_s": cev / ct, "cores": threads, "kind": "port",
+                "sample": f"oracle dense_local_search, same instance and parameters, first 64 loop iterations "
+                          f"({cev} flip evals in {ct:.2f} s)",
+            }
+        log("# C2 CPU oracle done")
+    return out
 
 
 def batched_eval(log):
@@ -219,6 +422,10 @@ def sk_bf16(log):
         "workload": "SK n=16384 fp32, 1024 candidates, split hi/lo BF16 MFMA + FP64 re-score of the best 16",
         "call_ms": dt * 1e3,
         "algorithmic_flop": 2.0 * B * n * n,
+        # the split runs the hi and the lo product: the MFMA executes twice the
+        # algorithmic work; the BF16 kernel alone takes most of the call
+        "bf16_frac_of_2500_tflops_algorithmic_call": 2.0 * B * n * n / (dt * 1e12) / 2500.0,
+        "bf16_frac_of_2500_tflops_executed_call": 4.0 * B * n * n / (dt * 1e12) / 2500.0,
         "best_energy_fp64": bv,
         "bf16_max_rel_err_top16": err,
     }
@@ -312,7 +519,6 @@ def bnb_exact(log, with_cpu):
 
 # the extra legs, each (log, with_cpu) -> dict
 LEGS = {
-    "time_to_equal_cut": lambda lg, cpu: time_to_equal_cut(lg),
     "batched_eval": lambda lg, cpu: batched_eval(lg),
     "sk_bf16": lambda lg, cpu: sk_bf16(lg),
     "sampler": lambda lg, cpu: sampler_round(lg, cpu),
@@ -323,133 +529,63 @@ LEGS = {
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--walkers", type=int, default=128)
-    ap.add_argument("--seed-multiple", type=int, default=1)
-    ap.add_argument("--gray-iterations", type=int, default=None)
-    ap.add_argument("--cpu-gray-iterations", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--c2-steps", type=int, default=30)
+    ap.add_argument("--c2-warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the batched, learned and SK legs")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="run the CPU oracle on the whole C2 search with identical parameters (minutes)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C3, C4, sampler and branch-and-bound legs")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
-    ap.add_argument("--only", choices=sorted(LEGS), default=None,
-                    help="run one extra leg alone and print its JSON (for profiling)")
+    ap.add_argument("--only", choices=sorted(LEGS) + ["c2", "c5"], default=None,
+                    help="run one leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.copy_kernel:
         copy_kernel(args.copy_kernel)
         return
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU (the GPU boxes here have one); --gpus must be 1"}))
+        sys.exit(2)
+    if args.only == "c5":
+        print(json.dumps({"c5": headline_c5(args.steps, args.warmup, log, False)}))
+        return
+    if args.only == "c2":
+        print(json.dumps({"c2": c2_local_search(args.c2_steps, args.c2_warmup, log, False, False)}))
+        return
     if args.only:
-        log = lambda m: print(m, file=sys.stderr, flush=True)
         print(json.dumps({args.only: LEGS[args.only](log, not args.no_cpu)}))
         return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
-        sys.exit(2)
 
-    from pyqrackising_amd import _lib
-
-    n = args.n
-    W, x0 = c2_instance(n)
-    gi = n * n if args.gray_iterations is None else args.gray_iterations
-    dg = _lib.DeviceGraph.dense(W)
-    params = _lib.make_params(n, is_spin_glass=1, gray_iterations=gi, walkers=args.walkers,
-                              seed_multiple=args.seed_multiple, seed_engine=_lib.SEEDS_MFMA,
-                              flags=_lib.FLAG_TIME_KERNELS)
-    for i in range(args.warmup):
-        dg.local_search(x0, params)
-        print(f"# warmup {i} done", file=sys.stderr, flush=True)
-
-    stats = []
-    device_sync()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        _, st = dg.local_search(x0, params)
-        stats.append(st)
-    device_sync()
-    dt = time.perf_counter() - t0
-    for i, st in enumerate(stats):
-        print(f"# step {i}: cut={st['cut_value']:.0f} rounds={st['rounds']}", file=sys.stderr, flush=True)
-
-    st = stats[-1]
-    # computed gain evaluations only; Gray steps that repeat an already rejected
-    # step of an unchanged walker are resolved without computing and not counted
-    evals = st["single_evals"] + st["double_evals"] + st["gray_evals_computed"]
-    total_evals = sum(s["single_evals"] + s["double_evals"] + s["gray_evals_computed"] for s in stats)
-    total_ref = sum(s["single_evals"] + s["double_evals"] + s["gray_evals"] for s in stats)
-    value = total_evals / dt
-    ms_per_step = dt / args.steps * 1e3
-
-    # dominant kernel: the single-flip descent (k_descent_reg).  Algorithmic
-    # bytes per flip evaluation: the gain (4 B int32) and the spin (1 B); per
-    # accepted flip, 16 B per neighbour (index and weight read, gain read and
-    # written) -- degree 3 here.
-    desc_s = sum(s["descent_kernel_seconds"] for s in stats)
-    desc_launches = sum(s["descent_launches"] for s in stats)
-    desc_evals = sum(s["descent_evals"] for s in stats)
-    desc_flips = sum(s["single_accepts"] for s in stats)
-    avg_launch_s = desc_s / max(1, desc_launches)
-    bytes_per_launch = (5.0 * desc_evals + 16.0 * 3 * desc_flips) / max(1, desc_launches)
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-
+    h = headline_c5(args.steps, args.warmup, log, not args.no_cpu)
     out = {
         "metric": "time-to-equal-cut & flip evals/s, learned+batched vs plain HIP, 1 MI355X",
-        "value": value,
-        "unit": "flip evals/s",
+        "value": h["value"],
+        "unit": "instances/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i8",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": (f"spin_glass_solver local search, random 3-regular n={n}, +-1 int8 couplings, "
-                         f"{args.walkers * args.seed_multiple * ((n + 63) // 64)} Gray seeds per phase on int8 MFMA, "
-                         f"walkers={args.walkers}, gray_iterations={gi}"),
-            "n": n,
-            "walkers": args.walkers,
-            "seed_multiple": args.seed_multiple,
-            "gray_iterations": gi,
-            "time_to_cut_s": ms_per_step / 1e3,
-            "final_cut": st["cut_value"],
-            "min_energy": st["min_energy"],
-            "flip_evals_per_step": evals,
-            "reference_flip_steps_per_step": st["single_evals"] + st["double_evals"] + st["gray_evals"],
-            "reference_flip_steps_per_s": total_ref / dt,
-            "seed_candidates_per_step": st["seed_evals"],
-            "phase_seconds": {k: st[k] for k in ("seconds_single", "seconds_double", "seconds_seeds",
-                                                  "seconds_walk", "seconds_reheat")},
-        },
-        "roofline": {
-            "kernel": "k_descent_reg<32, 4>",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(),
-            "traffic_source": "profiles/k_descent_reg_c2_n2048.md",
-            "avg_launch_ms": avg_launch_s * 1e3,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-        },
-        "cpu_baseline": None,
+        "config": h["config"],
+        "roofline": h["roofline"],
+        "cpu_baseline": h["cpu_baseline"],
     }
+    out["c2_local_search"] = c2_local_search(args.c2_steps, args.c2_warmup, log, not args.no_cpu, args.cpu_full)
     if not args.no_extra:
-        log = lambda m: print(m, file=sys.stderr, flush=True)
         for key, fn in LEGS.items():
             try:
                 out[key] = fn(log, not args.no_cpu)
             except Exception as e:  # a failing leg is reported, not fatal to the headline
                 out[key] = {"error": repr(e)}
-    if not args.no_cpu:
-        cgi = n * n if args.cpu_gray_iterations is None else args.cpu_gray_iterations
-        cb = cpu_baseline(W, x0, 16, 8, cgi)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        out["gpu_over_cpu"] = value / cb["value"]
+    if out["cpu_baseline"]:
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 
 

@@ -97,7 +97,9 @@ typedef struct {
 } pqi_search_params;
 
 /* pqi_search_params.flags */
-#define PQI_FLAG_TIME_KERNELS 1 /* time the descent and main walk kernels with HIP events on the context stream */
+#define PQI_FLAG_TIME_KERNELS 1 /* time the descent and main walk kernels with HIP events on the context stream
+                                   (batched and learned searches: the descent launches of the whole call, the
+                                   same totals in every instance record) */
 #define PQI_FLAG_HOST_LOOP 2    /* decide each phase on the host (one synchronisation per phase) instead of
                                    on the device (one per loop iteration); same result */
 #define PQI_FLAG_TEST_DRIFT 4   /* test hook (real weights): perturb the maintained field by
@@ -127,6 +129,9 @@ typedef struct {
     double max_field_drift;
     /* learned phase: the smallest cut gain (unscaled) of an accepted learned move; +inf when none */
     double learned_min_gain;
+    /* batched and learned searches: (instance, descent launch) pairs of the whole call, and those of them
+     * whose descent moved (and wrote its state back); the same totals in every instance record */
+    int64_t descent_instance_launches, descent_instance_writes;
 } pqi_search_stats;
 
 /* Thread-local message for the last failing call on this thread. */

@@ -100,6 +100,8 @@ class SearchStats(ctypes.Structure):
         ("field_repairs", ctypes.c_int64),
         ("max_field_drift", ctypes.c_double),
         ("learned_min_gain", ctypes.c_double),
+        ("descent_instance_launches", ctypes.c_int64),
+        ("descent_instance_writes", ctypes.c_int64),
     ]
 
     def as_dict(self):

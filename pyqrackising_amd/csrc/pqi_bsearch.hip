@@ -353,6 +353,29 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         for (int64_t i = 0; i < count; ++i) stats[i].seconds_learned += dt / (double)count;
     }
 
+    // with PQI_FLAG_TIME_KERNELS, every descent launch of the call is timed by
+    // events it records itself (its duration, not the host's)
+    const bool timing = (p->flags & PQI_FLAG_TIME_KERNELS) != 0;
+    hipEvent_t dev0 = nullptr, dev1 = nullptr;
+    if (timing && (hipEventCreateWithFlags(&dev0, hipEventDisableSystemFence) != hipSuccess ||
+                   hipEventCreateWithFlags(&dev1, hipEventDisableSystemFence) != hipSuccess))
+        return fail(PQI_ERR_HIP, "event creation failed");
+    struct EvFree {
+        hipEvent_t a, b;
+        ~EvFree() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev_free{dev0, dev1};
+    double desc_s = 0.0;
+    int64_t desc_launches = 0, desc_pairs = 0, desc_writes = 0;
+    auto descent_time = [&]() {
+        float ms = 0.f;
+        if (timing && hipEventElapsedTime(&ms, dev0, dev1) == hipSuccess) {
+            desc_s += ms * 1e-3;
+            ++desc_launches;
+        }
+    };
     std::vector<int32_t> active;
     for (int32_t i : list)
         if (!reached(i) && !(net && p->max_rounds < 0)) active.push_back(i);
@@ -384,10 +407,13 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             const int64_t cap = 16 * max_n + 4096;
             BCHK(upload_list(run));
             launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, all_sparse, key_lds, reg_r,
-                             reg_stride, reg_lds, s);
+                             reg_stride, reg_lds, s, dev0, dev1);
             BCHK(read_outs(run));
+            descent_time();
+            desc_pairs += (int64_t)run.size();
             std::vector<int32_t> again;
             for (size_t t = 0; t < run.size(); ++t) {
+                if (outs[t].accepts > 0) ++desc_writes;
                 pqi_search_stats& st = stats[run[t]];
                 st.single_evals += outs[t].evals;
                 st.single_accepts += outs[t].accepts;
@@ -520,8 +546,10 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             BCHK(copies(rest3, wsrc, 1));
             BCHK(upload_list(rest3));
             launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, all_sparse, key_lds, reg_r,
-                             reg_stride, reg_lds, s);
+                             reg_stride, reg_lds, s, dev0, dev1);
             BCHK(read_outs(rest3));
+            descent_time();
+            desc_pairs += (int64_t)rest3.size();  // best-only descents: no move, no write
             std::vector<int32_t> take1, improve1, rest4;
             std::vector<int64_t> mv1, mvi;
             for (size_t t = 0; t < rest3.size(); ++t) {
@@ -603,6 +631,12 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         const double en = kind == PQI_WEIGHTS_I8 ? (double)(-(long long)qv[i] / 2) : -qv[i] / 2.0;
         stats[i].cut_value = cut;
         stats[i].min_energy = p->is_spin_glass ? -en : en;
+    }
+    for (int64_t i = 0; i < count; ++i) {
+        stats[i].descent_kernel_seconds = desc_s;  // the call's descent launches (all instances together)
+        stats[i].descent_launches = desc_launches;
+        stats[i].descent_instance_launches = desc_pairs;
+        stats[i].descent_instance_writes = desc_writes;
     }
     BCHK(hipMemcpyAsync(pin_x, d_xall, xbytes, hipMemcpyDeviceToHost, s));
     BCHK(sync());

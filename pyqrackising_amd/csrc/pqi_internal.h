@@ -373,7 +373,8 @@ void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
 // reg_lds bytes (staged in LDS where they fit)
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
                       int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
-                      hipStream_t s);
+                      hipStream_t s,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // dynamic LDS limit of the batched packed-key descent
 size_t b_key_lds_limit();
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s);

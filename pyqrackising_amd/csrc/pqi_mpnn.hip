@@ -138,6 +138,195 @@ __global__ __launch_bounds__(256) void k_linear_bf16(const float* __restrict__ H
         }
 }
 
+// ---------------------------------------------------------------------------
+// Inference forward in BF16 mode (the learned search and pqi_mpnn_forward):
+// one kernel per layer, activations stored as BF16, nothing kept for a
+// backward pass.  Layer 1 (4 inputs) runs one thread per node; layers 2 and 3
+// stage, per 128-row tile, each row's own activation and its neighbour sum
+// (accumulated in FP32 from the BF16 neighbour rows, 8 threads per row, one
+// 16-byte load per neighbour each) into LDS as BF16, then run the same
+// 32x32x16 BF16 MFMA tile as k_linear_bf16; bias and relu are applied in the
+// epilogue, and the last layer folds the output score (FP32) instead of
+// writing its activations.  Tiles are dealt so that each XCD walks a
+// contiguous run of rows (block b runs on XCD b mod 8): the rows of one graph
+// share an XCD, so their neighbours' activations are read from that XCD's L2.
+// ---------------------------------------------------------------------------
+
+// tile of block b when the 8 XCDs each take a contiguous run of the tiles
+// (the grid is 8 ceil(ntiles / 8) blocks; tiles >= ntiles are idle)
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t ntiles) {
+    const int64_t per = (ntiles + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
+__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
+
+// Theta_2 and Theta_3 to BF16, transposed: Tb[l][j][k] (column j, K contiguous)
+__global__ void k_theta_bf16(const float* __restrict__ P, uint16_t* __restrict__ Tb) {
+    constexpr int KH = 2 * HID * HID;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * KH) return;
+    const int l = t / KH, r = t % KH;
+    const int k = r / HID, j = r % HID;
+    Tb[l * KH + j * (2 * HID) + k] = f2bf(P[(l == 0 ? OFF_T2 : OFF_T3) + r]);
+}
+
+// layer 1: H1 = relu([X | A X] Theta_1 + b_1), one thread per node, BF16 out
+__global__ __launch_bounds__(256) void k_infer_l1(const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                                  const float* __restrict__ w, const float* __restrict__ X,
+                                                  const float* __restrict__ P, int64_t N, int64_t ntiles,
+                                                  uint16_t* __restrict__ H1) {
+    const int64_t t = xcd_tile(blockIdx.x, ntiles);
+    if (t >= ntiles) return;
+    const int64_t v = t * 256 + threadIdx.x;
+    if (v >= N) return;
+    const float4 x = *(const float4*)(X + v * F0);
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+    for (int32_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+        const float we = w[e];
+        const float4 xu = *(const float4*)(X + (int64_t)idx[e] * F0);
+        m0 += we * xu.x;
+        m1 += we * xu.y;
+        m2 += we * xu.z;
+        m3 += we * xu.w;
+    }
+    const float* T1 = P + OFF_T1;
+    const float* b1 = P + OFF_B1;
+    uint4 out[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        uint32_t pk[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            uint32_t two = 0;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int j = q * 8 + h * 2 + u;
+                float z = b1[j];
+                z = fmaf(x.x, T1[0 * HID + j], z);
+                z = fmaf(x.y, T1[1 * HID + j], z);
+                z = fmaf(x.z, T1[2 * HID + j], z);
+                z = fmaf(x.w, T1[3 * HID + j], z);
+                z = fmaf(m0, T1[4 * HID + j], z);
+                z = fmaf(m1, T1[5 * HID + j], z);
+                z = fmaf(m2, T1[6 * HID + j], z);
+                z = fmaf(m3, T1[7 * HID + j], z);
+                two |= (uint32_t)f2bf(z > 0.f ? z : 0.f) << (16 * u);
+            }
+            pk[h] = two;
+        }
+        out[q] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+    }
+    uint4* dst = (uint4*)(H1 + v * HID);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dst[q] = out[q];
+}
+
+// layers 2 and 3: relu([H | A H] Theta + b) on BF16 MFMA; LAST: score = . to + bo
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_infer_layer(const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                                     const float* __restrict__ w, const uint16_t* __restrict__ Hin,
+                                                     const uint16_t* __restrict__ Tb, const float* __restrict__ bias,
+                                                     const float* __restrict__ P, int64_t N, int64_t ntiles,
+                                                     uint16_t* __restrict__ Hout, float* __restrict__ score) {
+    constexpr int K = 2 * HID;
+    constexpr int ROW = K * 2 + 16;  // bytes per LDS row of the operand tiles
+    __shared__ __align__(16) unsigned char As[128 * ROW];
+    __shared__ __align__(16) unsigned char Bs[HID * ROW];
+    const int64_t t = xcd_tile(blockIdx.x, ntiles);
+    if (t >= ntiles) return;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t r0 = t * 128;
+    // Theta (BF16, column j with K contiguous): 64 rows of 256 bytes
+    for (int c = tid; c < HID * (K / 8); c += 256) {
+        const int j = c / (K / 8), q = c % (K / 8);
+        *(uint4*)&Bs[j * ROW + q * 16] = *(const uint4*)(Tb + (int64_t)j * K + q * 8);
+    }
+    // own activations: columns 0 .. 63
+    for (int c = tid; c < 128 * 8; c += 256) {
+        const int r = c >> 3, q = c & 7;
+        const int64_t v = r0 + r;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (v < N) val = *(const uint4*)(Hin + v * HID + q * 8);
+        *(uint4*)&As[r * ROW + q * 16] = val;
+    }
+    // neighbour sums: columns 64 .. 127, 8 threads per row, 8 features each
+    for (int r = tid >> 3; r < 128; r += 32) {
+        const int q = tid & 7;
+        const int64_t v = r0 + r;
+        float acc[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc[i] = 0.f;
+        if (v < N) {
+            for (int32_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+                const float we = w[e];
+                const uint4 hv = *(const uint4*)(Hin + (int64_t)idx[e] * HID + q * 8);
+                const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    acc[2 * i] += we * bf2f(hw[i] & 0xFFFFu);
+                    acc[2 * i + 1] += we * bf2f(hw[i] >> 16);
+                }
+            }
+        }
+        uint32_t pk[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pk[i] = (uint32_t)f2bf(acc[2 * i]) | ((uint32_t)f2bf(acc[2 * i + 1]) << 16);
+        *(uint4*)&As[r * ROW + 128 + q * 16] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+    }
+    __syncthreads();
+    v16f acc[2];
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[nj][r] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < K; ks += 16) {
+        const v8bf a = *(const v8bf*)&As[(wid * 32 + (lane & 31)) * ROW + (ks + 8 * (lane >> 5)) * 2];
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj) {
+            const v8bf bb = *(const v8bf*)&Bs[(nj * 32 + (lane & 31)) * ROW + (ks + 8 * (lane >> 5)) * 2];
+            acc[nj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[nj], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // the operand tile is reused for the output
+    if constexpr (LAST) {
+        float* Of = (float*)As;  // 128 x 64 FP32 activations
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = wid * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = nj * 32 + (lane & 31);
+                const float z = acc[nj][r] + bias[col];
+                Of[row * HID + col] = z > 0.f ? z : 0.f;
+            }
+        __syncthreads();
+        if (tid < 128 && r0 + tid < N) {
+            float sc = 0.f;
+            for (int j = 0; j < HID; ++j) sc = fmaf(Of[tid * HID + j], P[OFF_TO + j], sc);
+            score[r0 + tid] = sc + P[OFF_BO];
+        }
+    } else {
+        uint16_t* Ob = (uint16_t*)As;  // 128 x 64 BF16 activations
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = wid * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = nj * 32 + (lane & 31);
+                const float z = acc[nj][r] + bias[col];
+                Ob[row * HID + col] = f2bf(z > 0.f ? z : 0.f);
+            }
+        __syncthreads();
+        for (int c = tid; c < 128 * 8; c += 256) {
+            const int r = c >> 3, q = c & 7;
+            const int64_t v = r0 + r;
+            if (v < N) *(uint4*)(Hout + v * HID + q * 8) = *(const uint4*)&Ob[r * HID + q * 8];
+        }
+    }
+}
+
 // score = H3 . to + bo
 __global__ __launch_bounds__(256) void k_out(const float* __restrict__ H, const float* __restrict__ P, int64_t N,
                                              float* __restrict__ score) {
@@ -332,6 +521,28 @@ static void forward(pqi_mpnn* m, hipStream_t s) {
     hipLaunchKernelGGL(k_out, dim3(g4), dim3(256), 0, s, m->H3, m->P, N, m->score);
 }
 
+// Scores for the bound batch from m->X0 (BF16 mode: the fused inference
+// kernels, BF16 activations in the first halves of H1 and H2, Theta_2/3 as
+// BF16 in the gradient-partial buffer, which only a backward pass uses; FP32
+// mode: the training forward).
+static void forward_infer(pqi_mpnn* m, hipStream_t s) {
+    if (m->precision != 1) {
+        forward(m, s);
+        return;
+    }
+    const int64_t N = m->N;
+    uint16_t* Tb = (uint16_t*)m->part;
+    hipLaunchKernelGGL(k_theta_bf16, dim3((unsigned)((4 * HID * HID + 255) / 256)), dim3(256), 0, s, m->P, Tb);
+    const int64_t t1 = (N + 255) / 256, tl = (N + 127) / 128;
+    hipLaunchKernelGGL(k_infer_l1, dim3((unsigned)(8 * ((t1 + 7) / 8))), dim3(256), 0, s, m->ptr, m->idx, m->w, m->X0,
+                       m->P, N, t1, (uint16_t*)m->H1);
+    hipLaunchKernelGGL(k_infer_layer<false>, dim3((unsigned)(8 * ((tl + 7) / 8))), dim3(256), 0, s, m->ptr, m->idx,
+                       m->w, (const uint16_t*)m->H1, Tb, m->P + OFF_B2, m->P, N, tl, (uint16_t*)m->H2, nullptr);
+    hipLaunchKernelGGL(k_infer_layer<true>, dim3((unsigned)(8 * ((tl + 7) / 8))), dim3(256), 0, s, m->ptr, m->idx,
+                       m->w, (const uint16_t*)m->H2, Tb + 2 * HID * HID, m->P + OFF_B3, m->P, N, tl, nullptr,
+                       m->score);
+}
+
 // loss and gradient into m->G; returns the loss through *loss (after sync)
 static int backward(pqi_mpnn* m, hipStream_t s, double* loss) {
     const int64_t N = m->N;
@@ -390,7 +601,7 @@ int64_t mpnn_node_offset(pqi_mpnn* m, int64_t i) {
     return i < (int64_t)m->node_off.size() ? m->node_off[i] : -1;
 }
 int64_t mpnn_total_nodes(pqi_mpnn* m) { return m->N; }
-void mpnn_forward_device(pqi_mpnn* m, hipStream_t s) { forward(m, s); }
+void mpnn_forward_device(pqi_mpnn* m, hipStream_t s) { forward_infer(m, s); }
 }  // namespace pqi
 
 using namespace pqi;
@@ -555,7 +766,7 @@ int pqi_mpnn_forward(pqi_mpnn* m, const float* feats, float* scores) {
     if (hipSetDevice(m->ctx->device) != hipSuccess) return fail(PQI_ERR_HIP, "hipSetDevice failed");
     int r = upload_inputs(m, feats, nullptr);
     if (r != PQI_OK) return r;
-    forward(m, m->ctx->stream);
+    forward_infer(m, m->ctx->stream);
     if (hipMemcpyAsync(scores, m->score, m->N * 4, hipMemcpyDeviceToHost, m->ctx->stream) != hipSuccess ||
         hipStreamSynchronize(m->ctx->stream) != hipSuccess)
         return fail(PQI_ERR_HIP, "MPNN forward failed");

@@ -474,16 +474,22 @@ __global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h
 // No work-group barriers, 32-bit indices; same selection rule and result.
 template <typename WT, typename FT>
 __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64_t max_steps, PhaseOut* out,
-                                  unsigned char* lds) {
+                                  unsigned char* lds, int stage_ap = 0) {
     const int lane = threadIdx.x;
     const int n = (int)g.n;
     FT* D = (FT*)lds;
     uint8_t* x = lds + (((size_t)n * sizeof(FT) + 15) & ~(size_t)15);
+    // adjacency row pointers staged after x (stage_ap: the caller sized LDS
+    // for them), so a flip waits for one global access (its row), not two
+    int32_t* AP = (int32_t*)(lds + (((size_t)n * sizeof(FT) + 15) & ~(size_t)15) + (((size_t)n + 15) & ~(size_t)15));
+    const int32_t* ap = stage_ap ? AP : g.ap;
     for (int v = lane; v < n; v += 64) {
         const uint8_t xv = xg[v];
         x[v] = xv;
         D[v] = xv ? -hg[v] : hg[v];
     }
+    if (stage_ap)
+        for (int v = lane; v <= n; v += 64) AP[v] = g.ap[v];
     __syncthreads();
     int64_t steps = 0, evals = 0;
     double gsum = 0.0;
@@ -511,7 +517,7 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
         const int sk = x[bk] ? -1 : 1;  // s_k before the flip
         __syncthreads();
         if (g.use_adj) {
-            for (int32_t e = g.ap[bk] + lane; e < g.ap[bk + 1]; e += 64) {
+            for (int32_t e = ap[bk] + lane; e < ap[bk + 1]; e += 64) {
                 const int v = g.ai[e];
                 const int sv = x[v] ? -1 : 1;
                 D[v] -= (FT)(2 * sk * sv) * (FT)g.aw[e];
@@ -1073,10 +1079,17 @@ static size_t descent2_lds(int64_t n) {
 
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
-                                                    PhaseOut* out, const int32_t* gate) {
+                                                    PhaseOut* out, const int32_t* gate, int stage_ap) {
     extern __shared__ __align__(16) unsigned char lds[];
     if (gate && *gate) return;
-    descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds);
+    descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds, stage_ap);
+}
+
+// LDS of descent_wave_body: gains and spins, plus the row pointers when staged
+template <typename FT>
+static size_t wave_lds(int64_t n, bool stage_ap) {
+    return (((size_t)n * sizeof(FT) + 15) & ~(size_t)15) + (((size_t)n + 15) & ~(size_t)15) +
+           (stage_ap ? (size_t)(n + 1) * 4 : 0);
 }
 
 // The descent launches go through hipExtLaunchKernel so that optional start
@@ -1102,16 +1115,17 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
             return;
         }
     }
-    const size_t lds = (((size_t)g.n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)g.n;
     static const size_t limit = lds_limit((const void*)k_descent_wave<WT, FT>);
     static const size_t limit2 = lds_limit((const void*)k_descent_wave2<WT, FT>);
     const bool sparse = (g.use_adj || g.W == nullptr) && descent_variant() == 2;
+    const bool stage_ap = g.use_adj && descent_variant() != 7 && wave_lds<FT>(g.n, true) <= limit;
+    const size_t lds = wave_lds<FT>(g.n, stage_ap);
     if (sparse && descent2_lds<FT>(g.n) <= limit2)
         hipExtLaunchKernelGGL(k_descent_wave2<WT, FT>, dim3(1), dim3(64), (uint32_t)descent2_lds<FT>(g.n), s, e0, e1,
                               0, view<WT>(g), x, (FT*)h, max_steps, out, gate);
     else if (lds <= limit)
         hipExtLaunchKernelGGL(k_descent_wave<WT, FT>, dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, view<WT>(g), x,
-                              (FT*)h, max_steps, out, gate);
+                              (FT*)h, max_steps, out, gate, stage_ap ? 1 : 0);
     else
         hipExtLaunchKernelGGL(k_descent<WT, FT>, dim3(1), dim3(1024), 0, s, e0, e1, 0, view<WT>(g), x, (FT*)h,
                               max_steps, out, gate);
@@ -2579,10 +2593,11 @@ __device__ __forceinline__ GView<WT> gview_of(const BInstRaw& r) {
 
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void kb_descent_lds(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
-                                                    int which) {
+                                                    int which, int stage_ap) {
     extern __shared__ __align__(16) unsigned char lds[];
     const BInstRaw& r = insts[list[blockIdx.x]];
-    descent_wave_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out, lds);
+    descent_wave_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out, lds,
+                              stage_ap && r.use_adj);
 }
 
 __global__ __launch_bounds__(64) void kb_descent_key(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
@@ -2609,11 +2624,11 @@ __global__ __launch_bounds__(64) void kb_descent_reg(const BInstRaw* insts, cons
 
 template <int R, int NB>
 static void launch_b_reg(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_steps, int which,
-                         size_t table, hipStream_t s) {
+                         size_t table, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     static const size_t limit = lds_limit((const void*)kb_descent_reg<R, NB>);
     (void)limit;  // the plan admits only tables within it
-    hipLaunchKernelGGL((kb_descent_reg<R, NB>), dim3((unsigned)L), dim3(64), table, s, d, list, max_steps, which,
-                       table);
+    hipExtLaunchKernelGGL((kb_descent_reg<R, NB>), dim3((unsigned)L), dim3(64), table, s, e0, e1, 0, d, list,
+                          max_steps, which, table);
 }
 
 template <typename WT, typename FT>
@@ -2830,27 +2845,30 @@ static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int6
 template <typename WT>
 static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
                            int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
-                           hipStream_t s) {
+                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     using FT = typename FieldOf<WT>::T;
     if (sizeof(WT) == 1 && reg_r > 0) {
-        PQI_REG_DISPATCH(reg_r, reg_nb(reg_stride), launch_b_reg, d, list, L, max_steps, which, reg_lds, s);
+        PQI_REG_DISPATCH(reg_r, reg_nb(reg_stride), launch_b_reg, d, list, L, max_steps, which, reg_lds, s, e0, e1);
         return;
     }
     if (sizeof(WT) == 1 && key_lds > 0 && descent_variant() != 3) {
-        hipLaunchKernelGGL(kb_descent_key, dim3((unsigned)L), dim3(64), key_lds, s, d, list, max_steps, which,
-                           key_lds);
+        hipExtLaunchKernelGGL(kb_descent_key, dim3((unsigned)L), dim3(64), key_lds, s, e0, e1, 0, d, list, max_steps,
+                              which, key_lds);
         return;
     }
-    const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
     static const size_t limit = lds_limit((const void*)kb_descent_lds<WT, FT>);
     static const size_t limit2 = lds_limit((const void*)kb_descent_lds2<WT, FT>);
+    const bool stage_ap = descent_variant() != 7 && wave_lds<FT>(max_n, true) <= limit;
+    const size_t lds = wave_lds<FT>(max_n, stage_ap);
     if (all_sparse && descent_variant() == 2 && descent2_lds<FT>(max_n) <= limit2)
-        hipLaunchKernelGGL((kb_descent_lds2<WT, FT>), dim3((unsigned)L), dim3(64), descent2_lds<FT>(max_n), s, d, list,
-                           max_steps, which);
+        hipExtLaunchKernelGGL((kb_descent_lds2<WT, FT>), dim3((unsigned)L), dim3(64), descent2_lds<FT>(max_n), s, e0,
+                              e1, 0, d, list, max_steps, which);
     else if (lds <= limit)
-        hipLaunchKernelGGL((kb_descent_lds<WT, FT>), dim3((unsigned)L), dim3(64), lds, s, d, list, max_steps, which);
+        hipExtLaunchKernelGGL((kb_descent_lds<WT, FT>), dim3((unsigned)L), dim3(64), lds, s, e0, e1, 0, d, list,
+                              max_steps, which, stage_ap ? 1 : 0);
     else
-        hipLaunchKernelGGL((kb_descent_glb<WT, FT>), dim3((unsigned)L), dim3(1024), 0, s, d, list, max_steps, which);
+        hipExtLaunchKernelGGL((kb_descent_glb<WT, FT>), dim3((unsigned)L), dim3(1024), 0, s, e0, e1, 0, d, list,
+                              max_steps, which);
 }
 
 template <typename WT>
@@ -2931,9 +2949,9 @@ void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
 }
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
                       int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
-                      hipStream_t s) {
+                      hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, all_sparse, key_lds, reg_r,
-                      reg_stride, reg_lds, s);
+                      reg_stride, reg_lds, s, e0, e1);
 }
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s) {
     PQI_KIND_DISPATCH(kind, b_double_impl, d, list, L, which, s);

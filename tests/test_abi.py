@@ -32,7 +32,7 @@ def test_structs_match_header_sizes():
     # field order/size of the three ABI structs (all 8-byte aligned members)
     assert ctypes.sizeof(_lib.GraphInfo) == 40
     assert ctypes.sizeof(_lib.SearchParams) == 48
-    assert ctypes.sizeof(_lib.SearchStats) == 8 * 31
+    assert ctypes.sizeof(_lib.SearchStats) == 8 * 33
 
 
 def test_version_and_error_plumbing():
