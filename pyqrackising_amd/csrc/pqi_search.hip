@@ -556,6 +556,262 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
     }
 }
 
+// Sparse-graph descent in rounds: the same moves as descent_wave_body with
+// a full scan only once per round.  A scan leaves each lane its RD_M best
+// (gain, node) keys -- the round's candidates, whose neighbour rows are then
+// fetched into LDS in one batch of parallel loads -- and the threshold T, the
+// largest RD_M-th key over the lanes.  Every node outside the tracked set
+// (candidates plus "extras") has a key <= T: an untouched node is below its
+// lane's RD_M-th key, and a neighbour whose gain a flip raises above T joins
+// the extras.  So while the best tracked key is above T it is the first
+// maximum over all n nodes, exactly the move of the full scan; the round ends
+// when it is not (or the extras are full) and the next scan starts.  Keys
+// order by gain, then lower node first, as in pyqrackising/spin_glass_solver.py:45.
+// Each flip reads the tracked gains from LDS and its row from the cache (a
+// global access only for an extra's row or entries past RD_RS).
+constexpr int RD_M = 4;    // candidates per lane
+constexpr int RD_RS = 16;  // cached entries per candidate row
+constexpr int RD_X = 64;   // extras per round
+
+struct RoundLds {
+    size_t d, x, ap, sn, rci, rcw, ex, total;
+};
+
+template <typename WT, typename FT>
+__host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
+    auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+    RoundLds L;
+    size_t o = 0;
+    L.d = o;
+    o += al((size_t)n * sizeof(FT));
+    L.x = o;
+    o += al((size_t)n);
+    L.ap = o;
+    o += al((size_t)(n + 1) * 4);
+    L.sn = o;
+    o += al(64 * RD_M * 4);
+    L.rci = o;
+    o += al(64 * RD_M * RD_RS * 4);
+    L.rcw = o;
+    o += al(64 * RD_M * RD_RS * sizeof(WT));
+    L.ex = o;
+    o += al(RD_X * 4);
+    L.total = o;
+    return L;
+}
+
+template <typename WT, typename FT>
+__device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64_t max_steps, PhaseOut* out,
+                                   unsigned char* lds) {
+    constexpr int M = RD_M, RS = RD_RS;
+    static_assert(RS == 16, "the row prefetch maps 16 lanes to one row");
+    const int lane = threadIdx.x;
+    const int n = (int)g.n;
+    const RoundLds L = round_lds_layout<WT, FT>(n);
+    FT* D = (FT*)(lds + L.d);
+    uint8_t* x = lds + L.x;  // bit 0: the spin (set: -1); bit 1: tracked in this round
+    int32_t* AP = (int32_t*)(lds + L.ap);
+    int32_t* SN = (int32_t*)(lds + L.sn);  // candidate slot -> node (-1: empty)
+    int32_t* RCI = (int32_t*)(lds + L.rci);
+    WT* RCW = (WT*)(lds + L.rcw);
+    int32_t* EX = (int32_t*)(lds + L.ex);
+    for (int v = lane; v < n; v += 64) {
+        const uint8_t xv = xg[v];
+        x[v] = xv;
+        D[v] = xv ? -hg[v] : hg[v];
+    }
+    for (int v = lane; v <= n; v += 64) AP[v] = g.ap[v];
+    __syncthreads();
+    int64_t steps = 0;
+    double gsum = 0.0;
+    FT bv = lowest<FT>();
+    int bk = INT_MAX;
+    bool done = false;
+    while (!done) {
+        // scan: the lane's M best keys, best first (nodes rise within a lane,
+        // so an equal gain never displaces an earlier node)
+        FT cd[M];
+        int ck[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            cd[i] = lowest<FT>();
+            ck[i] = INT_MAX;
+        }
+        for (int v0 = lane; v0 < n; v0 += 512) {
+            FT d8[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) d8[u] = v0 + 64 * u < n ? D[v0 + 64 * u] : lowest<FT>();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const FT d = d8[u];
+                const int v = v0 + 64 * u;
+                if (d > cd[M - 1]) {
+#pragma unroll
+                    for (int i = M - 1; i > 0; --i) {
+                        const bool up = d > cd[i - 1];
+                        const bool here = d > cd[i];
+                        cd[i] = up ? cd[i - 1] : (here ? d : cd[i]);
+                        ck[i] = up ? ck[i - 1] : (here ? v : ck[i]);
+                    }
+                    if (d > cd[0]) {
+                        cd[0] = d;
+                        ck[0] = v;
+                    }
+                }
+            }
+        }
+        FT td = cd[M - 1];
+        int tk = ck[M - 1];
+        wave_argmax_dpp(td, tk);  // T: lanes with fewer than M nodes offer (lowest, INT_MAX)
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const int c = ck[i] < n ? ck[i] : -1;
+            SN[lane * M + i] = c;
+            if (c >= 0) x[c] |= 2u;
+        }
+        __syncthreads();
+        // candidate rows into the cache: entry j of slot s, 16 lanes per row
+#pragma unroll 1
+        for (int t0 = 0; t0 < M * 16; t0 += 16) {
+            int32_t vi[16];
+            WT vw[16];
+            int dst[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int sl = 4 * (t0 + u) + (lane >> 4), j = lane & 15;
+                const int c = SN[sl];
+                dst[u] = -1;
+                if (c >= 0) {
+                    const int e0 = AP[c];
+                    if (j < AP[c + 1] - e0) {
+                        dst[u] = sl * RS + j;
+                        vi[u] = g.ai[e0 + j];
+                        vw[u] = g.aw[e0 + j];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (dst[u] >= 0) {
+                    RCI[dst[u]] = vi[u];
+                    RCW[dst[u]] = vw[u];
+                }
+        }
+        __syncthreads();
+        int nx = 0;  // extras this round (wave-uniform)
+        int64_t round_flips = 0;
+        for (;;) {
+            // first maximum over the tracked nodes; the index carries the slot
+            // (node << 9 | slot + 1; 0 for an extra, whose row is not cached)
+            FT b = lowest<FT>();
+            int bp = INT_MAX;
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+                if (ck[i] < n) {
+                    const FT d = D[ck[i]];
+                    const int p = (ck[i] << 9) | (lane * M + i + 1);
+                    if (d > b || (d == b && p < bp)) {
+                        b = d;
+                        bp = p;
+                    }
+                }
+            if (lane < nx) {
+                const int c = EX[lane];
+                const FT d = D[c];
+                const int p = c << 9;
+                if (d > b || (d == b && p < bp)) {
+                    b = d;
+                    bp = p;
+                }
+            }
+            wave_argmax_dpp(b, bp);
+            const int k = bp == INT_MAX ? INT_MAX : bp >> 9;
+            const int slot = bp == INT_MAX ? -1 : (bp & 511) - 1;
+            if (!(b > td || (b == td && k < tk))) {
+                if (round_flips == 0) {  // only when there is no node at all
+                    bv = b;
+                    bk = k;
+                    done = true;
+                }
+                break;
+            }
+            bv = b;
+            bk = k;
+            if (!(b > 0) || steps >= max_steps) {
+                done = true;
+                break;
+            }
+            const int sk = (x[k] & 1u) ? -1 : 1;  // s_k before the flip
+            const int e0 = AP[k], deg = AP[k + 1] - e0;
+            bool full = false;
+            for (int j0 = 0; j0 < deg; j0 += 64) {
+                const int j = j0 + lane;
+                bool add = false;
+                int v = 0;
+                if (j < deg) {
+                    WT w;
+                    if (slot >= 0 && j < RS) {
+                        v = RCI[slot * RS + j];
+                        w = RCW[slot * RS + j];
+                    } else {
+                        v = g.ai[e0 + j];
+                        w = g.aw[e0 + j];
+                    }
+                    const uint8_t xv = x[v];
+                    const int sv = (xv & 1u) ? -1 : 1;
+                    const FT nd = D[v] - (FT)(2 * sk * sv) * (FT)w;
+                    D[v] = nd;
+                    add = !(xv & 2u) && (nd > td || (nd == td && v < tk));
+                }
+                const uint64_t m = __ballot(add);
+                const int cnt = __popcll(m);
+                if (nx + cnt > RD_X) {
+                    full = true;  // not tracked: this round ends after the flip
+                } else {
+                    if (add) {
+                        const int pos = nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        EX[pos] = v;
+                        x[v] |= 2u;
+                    }
+                    nx += cnt;
+                }
+            }
+            __syncthreads();
+            if (lane == 0) {
+                D[k] = -D[k];
+                x[k] ^= 1u;
+            }
+            __syncthreads();
+            ++steps;
+            ++round_flips;
+            gsum += (double)b;
+            if (full) break;
+        }
+        // untrack the round's candidates and extras
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+            if (ck[i] < n) x[ck[i]] &= 1u;
+        if (lane < nx) x[EX[lane]] &= 1u;
+        __syncthreads();
+    }
+    if (steps) {
+        for (int v = lane; v < n; v += 64) {
+            const uint8_t xv = x[v];
+            hg[v] = xv ? -D[v] : D[v];
+            xg[v] = xv;
+        }
+    }
+    if (lane == 0) {
+        out->gain = (double)bv;
+        out->k = bk;
+        out->l = -1;
+        out->accepts = steps;
+        out->evals = (int64_t)n * (steps + 1);
+        out->gain_sum = gsum;
+    }
+}
+
 // Integer-weight descent on packed keys.
 //   key_v = (Delta_v << (ib + 1)) | ((mask - v) << 1) | x_v,  mask = 2^ib - 1 >= n - 1.
 // The largest key carries the largest gain and, among equal gains, the lowest
@@ -1077,12 +1333,25 @@ static size_t descent2_lds(int64_t n) {
            (((size_t)ng * 4 + 15) & ~(size_t)15) + (size_t)n;
 }
 
+// mode 0: descent_wave_body; 1: the same with the row pointers staged in
+// LDS; 2: descent_round_body (sparse graphs)
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
-                                                    PhaseOut* out, const int32_t* gate, int stage_ap) {
+                                                    PhaseOut* out, const int32_t* gate, int mode) {
     extern __shared__ __align__(16) unsigned char lds[];
     if (gate && *gate) return;
-    descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds, stage_ap);
+    if (mode == 2 && g.use_adj)
+        descent_round_body<WT, FT>(g, xg, hg, max_steps, out, lds);
+    else
+        descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds, mode != 0 && g.use_adj);
+}
+
+// the round descent applies to sparse graphs whose gains, spins, row pointers
+// and candidate rows fit in LDS (node indices below 2^22: they share an int
+// with the candidate slot)
+template <typename WT, typename FT>
+static bool round_fits(int64_t n, size_t limit) {
+    return descent_variant() != 8 && n < (int64_t(1) << 22) && round_lds_layout<WT, FT>(n).total <= limit;
 }
 
 // LDS of descent_wave_body: gains and spins, plus the row pointers when staged
@@ -1119,13 +1388,14 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
     static const size_t limit2 = lds_limit((const void*)k_descent_wave2<WT, FT>);
     const bool sparse = (g.use_adj || g.W == nullptr) && descent_variant() == 2;
     const bool stage_ap = g.use_adj && descent_variant() != 7 && wave_lds<FT>(g.n, true) <= limit;
-    const size_t lds = wave_lds<FT>(g.n, stage_ap);
+    const bool rounds = g.use_adj && round_fits<WT, FT>(g.n, limit);
+    const size_t lds = rounds ? round_lds_layout<WT, FT>(g.n).total : wave_lds<FT>(g.n, stage_ap);
     if (sparse && descent2_lds<FT>(g.n) <= limit2)
         hipExtLaunchKernelGGL(k_descent_wave2<WT, FT>, dim3(1), dim3(64), (uint32_t)descent2_lds<FT>(g.n), s, e0, e1,
                               0, view<WT>(g), x, (FT*)h, max_steps, out, gate);
     else if (lds <= limit)
         hipExtLaunchKernelGGL(k_descent_wave<WT, FT>, dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, view<WT>(g), x,
-                              (FT*)h, max_steps, out, gate, stage_ap ? 1 : 0);
+                              (FT*)h, max_steps, out, gate, rounds ? 2 : stage_ap ? 1 : 0);
     else
         hipExtLaunchKernelGGL(k_descent<WT, FT>, dim3(1), dim3(1024), 0, s, e0, e1, 0, view<WT>(g), x, (FT*)h,
                               max_steps, out, gate);
@@ -2591,13 +2861,18 @@ __device__ __forceinline__ GView<WT> gview_of(const BInstRaw& r) {
     return v;
 }
 
+// mode as for k_descent_wave
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void kb_descent_lds(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
-                                                    int which, int stage_ap) {
+                                                    int which, int mode) {
     extern __shared__ __align__(16) unsigned char lds[];
     const BInstRaw& r = insts[list[blockIdx.x]];
-    descent_wave_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out, lds,
-                              stage_ap && r.use_adj);
+    if (mode == 2 && r.use_adj)
+        descent_round_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out,
+                                   lds);
+    else
+        descent_wave_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out,
+                                  lds, mode != 0 && r.use_adj);
 }
 
 __global__ __launch_bounds__(64) void kb_descent_key(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
@@ -2859,13 +3134,14 @@ static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, in
     static const size_t limit = lds_limit((const void*)kb_descent_lds<WT, FT>);
     static const size_t limit2 = lds_limit((const void*)kb_descent_lds2<WT, FT>);
     const bool stage_ap = descent_variant() != 7 && wave_lds<FT>(max_n, true) <= limit;
-    const size_t lds = wave_lds<FT>(max_n, stage_ap);
+    const bool rounds = round_fits<WT, FT>(max_n, limit);
+    const size_t lds = rounds ? round_lds_layout<WT, FT>(max_n).total : wave_lds<FT>(max_n, stage_ap);
     if (all_sparse && descent_variant() == 2 && descent2_lds<FT>(max_n) <= limit2)
         hipExtLaunchKernelGGL((kb_descent_lds2<WT, FT>), dim3((unsigned)L), dim3(64), descent2_lds<FT>(max_n), s, e0,
                               e1, 0, d, list, max_steps, which);
     else if (lds <= limit)
         hipExtLaunchKernelGGL((kb_descent_lds<WT, FT>), dim3((unsigned)L), dim3(64), lds, s, e0, e1, 0, d, list,
-                              max_steps, which, stage_ap ? 1 : 0);
+                              max_steps, which, rounds ? 2 : stage_ap ? 1 : 0);
     else
         hipExtLaunchKernelGGL((kb_descent_glb<WT, FT>), dim3((unsigned)L), dim3(1024), 0, s, e0, e1, 0, d, list,
                               max_steps, which);
