@@ -3358,6 +3358,168 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
     }
 }
 
+// LDS of the staged learned pass: gains, spins, ages, row pointers, the
+// visiting list and 64 cached candidate rows
+struct PassLds {
+    size_t d, x, age, ap, list, rci, rcw, total;
+};
+
+template <typename WT, typename FT>
+__host__ __device__ inline PassLds pass_lds_layout(int64_t n) {
+    auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+    PassLds L;
+    size_t o = 0;
+    L.d = o;
+    o += al((size_t)n * sizeof(FT));
+    L.x = o;
+    o += al((size_t)n);
+    L.age = o;
+    o += al((size_t)n);
+    L.ap = o;
+    o += al((size_t)(n + 1) * 4);
+    L.list = o;
+    o += al((size_t)n * 4);
+    L.rci = o;
+    o += al(64 * RD_RS * 4);
+    L.rcw = o;
+    o += al(64 * RD_RS * sizeof(WT));
+    L.total = o;
+    return L;
+}
+
+// The same pass with everything a flip touches in LDS: the nodes whose score
+// passes min_score are listed first (perm order), then each chunk of 64
+// listed nodes has its rows fetched into LDS in one batch of loads before its
+// flips, so a flip waits for no global access (rows longer than RD_RS read
+// their tail from global memory).  Ages stay in LDS until the end.
+template <typename WT, typename FT>
+__device__ void learned_pass_staged(const BInstRaw& r, const GView<WT>& g, float min_score, unsigned char* lds) {
+    constexpr int RS = RD_RS;
+    const int lane = threadIdx.x;
+    const int n = (int)r.n;
+    const PassLds L = pass_lds_layout<WT, FT>(n);
+    FT* D = (FT*)(lds + L.d);
+    uint8_t* x = lds + L.x;
+    uint8_t* age = lds + L.age;
+    int32_t* AP = (int32_t*)(lds + L.ap);
+    int32_t* LIST = (int32_t*)(lds + L.list);
+    int32_t* RCI = (int32_t*)(lds + L.rci);
+    WT* RCW = (WT*)(lds + L.rcw);
+    FT* hg = (FT*)r.h;
+    for (int v = lane; v < n; v += 64) {
+        const uint8_t xv = r.x[v];
+        x[v] = xv;
+        D[v] = xv ? -hg[v] : hg[v];
+        age[v] = r.age[v];
+    }
+    for (int v = lane; v <= n; v += 64) AP[v] = g.ap[v];
+    // the visiting list: perm order, score above min_score; 16 chunks of
+    // perm and score loads in flight per lane
+    int nl = 0;
+    for (int c0 = 0; c0 < n; c0 += 64 * 16) {
+        int pv[16];
+        bool ok[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int i = c0 + 64 * u + lane;
+            pv[u] = i < n ? r.perm[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) ok[u] = pv[u] >= 0 && r.score[pv[u]] > min_score;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const uint64_t m = __ballot(ok[u]);
+            if (ok[u])
+                LIST[nl + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = pv[u];
+            nl += __popcll(m);
+        }
+    }
+    __syncthreads();
+    int64_t flips = 0;
+    double gsum = 0.0, gmin = INFINITY;
+    for (int c0 = 0; c0 < nl; c0 += 64) {
+        // rows of this chunk's nodes: entry j of row i, 16 lanes per row
+        {
+            int32_t vi[16];
+            WT vw[16];
+            int dst[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int row = 4 * u + (lane >> 4), j = lane & 15;
+                dst[u] = -1;
+                if (c0 + row < nl) {
+                    const int c = LIST[c0 + row];
+                    const int e0 = AP[c];
+                    if (j < AP[c + 1] - e0) {
+                        dst[u] = row * RS + j;
+                        vi[u] = g.ai[e0 + j];
+                        vw[u] = g.aw[e0 + j];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (dst[u] >= 0) {
+                    RCI[dst[u]] = vi[u];
+                    RCW[dst[u]] = vw[u];
+                }
+        }
+        const bool valid = c0 + lane < nl;
+        const int v = valid ? LIST[c0 + lane] : 0;
+        __syncthreads();
+        FT d = valid ? D[v] : (FT)0;
+        uint64_t mask = __ballot(valid && d > 0);
+        while (mask) {
+            const int bl = __builtin_ctzll(mask);
+            const int k = __shfl(v, bl, 64);
+            const FT dd = __shfl(d, bl, 64);
+            gsum += (double)dd;
+            gmin = fmin(gmin, (double)dd);
+            ++flips;
+            const int sk = x[k] ? -1 : 1;
+            const int e0 = AP[k], deg = AP[k + 1] - e0;
+            for (int j = lane; j < deg; j += 64) {
+                int u;
+                WT w;
+                if (j < RS) {
+                    u = RCI[bl * RS + j];
+                    w = RCW[bl * RS + j];
+                } else {
+                    u = g.ai[e0 + j];
+                    w = g.aw[e0 + j];
+                }
+                const int su = x[u] ? -1 : 1;
+                D[u] -= (FT)(2 * sk * su) * (FT)w;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                D[k] = -D[k];
+                x[k] ^= 1u;
+                age[k] = 0;
+            }
+            __syncthreads();
+            d = valid ? D[v] : (FT)0;
+            mask = __ballot(valid && d > 0 && lane > bl);
+        }
+    }
+    __syncthreads();
+    for (int u = lane; u < n; u += 64) {
+        const uint8_t xv = x[u];
+        hg[u] = xv ? -D[u] : D[u];
+        r.x[u] = xv;
+        r.age[u] = age[u] < 255 ? (uint8_t)(age[u] + 1) : age[u];
+    }
+    if (lane == 0) {
+        r.out->accepts = flips;
+        r.out->gain_sum = gsum;
+        r.out->evals = n;
+        r.out->gain = gmin;  // the smallest accepted gain (+inf: none)
+        r.out->k = -1;
+        r.out->l = -1;
+    }
+}
+
 // One first-improvement pass in score order: nodes are visited in perm order
 // and flipped iff their exact gain is positive at that moment.  64 consecutive
 // candidates are evaluated at once (one per lane); a ballot finds the first
@@ -3368,6 +3530,10 @@ __global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, con
     extern __shared__ __align__(16) unsigned char lds[];
     const BInstRaw& r = insts[list[blockIdx.x]];
     const GView<WT> g = gview_of<WT>(r);
+    if (use_lds == 2 && r.use_adj) {
+        learned_pass_staged<WT, FT>(r, g, min_score, lds);
+        return;
+    }
     const int lane = threadIdx.x;
     const int64_t n = r.n;
     FT* hg = (FT*)r.h;
@@ -3448,9 +3614,15 @@ template <typename WT>
 static void b_learned_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, float min_score,
                            hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
-    const size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
+    // 2: the staged pass (sparse graphs), 1: gains and spins in LDS, 0: global
+    size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
     static const size_t limit = lds_limit((const void*)kb_learned_pass<WT, FT>);
-    const int use_lds = lds <= limit ? 1 : 0;
+    const size_t staged = pass_lds_layout<WT, FT>(max_n).total;
+    int use_lds = lds <= limit ? 1 : 0;
+    if (staged <= limit && descent_variant() != 8) {
+        use_lds = 2;
+        lds = staged;
+    }
     hipLaunchKernelGGL((kb_learned_pass<WT, FT>), dim3((unsigned)L), dim3(64), use_lds ? lds : 0, s, d, list, use_lds,
                        min_score);
 }
