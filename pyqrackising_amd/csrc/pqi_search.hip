@@ -557,19 +557,21 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
 }
 
 // Sparse-graph descent in rounds: the same moves as descent_wave_body with
-// a full scan only once per round.  A scan leaves each lane its RD_M best
-// (gain, node) keys -- the round's candidates, whose neighbour rows are then
-// fetched into LDS in one batch of parallel loads -- and the threshold T, the
-// largest RD_M-th key over the lanes.  Every node outside the tracked set
-// (candidates plus "extras") has a key <= T: an untouched node is below its
-// lane's RD_M-th key, and a neighbour whose gain a flip raises above T joins
-// the extras.  So while the best tracked key is above T it is the first
-// maximum over all n nodes, exactly the move of the full scan; the round ends
-// when it is not (or the extras are full) and the next scan starts.  Keys
-// order by gain, then lower node first, as in pyqrackising/spin_glass_solver.py:45.
-// Each flip reads the tracked gains from LDS and its row from the cache (a
-// global access only for an extra's row or entries past RD_RS).
-constexpr int RD_M = 4;    // candidates per lane
+// a full scan only once per round.  The scan gives each lane the 4th largest
+// gain among its nodes; T, the largest of these over the lanes, bounds every
+// node outside the round's candidates {v : D_v > T} (at most 3 per lane).
+// Their neighbour rows are fetched into LDS in one batch of loads.  A flip
+// that raises an untracked neighbour's gain above T adds it to the "extras",
+// so every untracked node stays <= T, and while the best tracked gain is
+// above T its first maximum (lowest node among equal gains) is the first
+// maximum over all n nodes: the move of the full scan
+// (pyqrackising/spin_glass_solver.py:37-52).  The round ends when it is not.
+// When the top gain itself does not exceed T (ties), the scan's own first
+// maximum is flipped and a new round starts.  Single wave: LDS accesses of
+// one wave complete in order, so phases are separated by compiler barriers
+// only.  A flip reads its row from the cache (global memory only for an
+// extra's row, or entries past RD_RS).
+constexpr int RD_C = 3;    // candidates per lane (nodes above a lane's 4th largest gain)
 constexpr int RD_RS = 16;  // cached entries per candidate row
 constexpr int RD_X = 64;   // extras per round
 
@@ -589,11 +591,11 @@ __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
     L.ap = o;
     o += al((size_t)(n + 1) * 4);
     L.sn = o;
-    o += al(64 * RD_M * 4);
+    o += al(64 * RD_C * 4);
     L.rci = o;
-    o += al(64 * RD_M * RD_RS * 4);
+    o += al(64 * RD_C * RD_RS * 4);
     L.rcw = o;
-    o += al(64 * RD_M * RD_RS * sizeof(WT));
+    o += al(64 * RD_C * RD_RS * sizeof(WT));
     L.ex = o;
     o += al(RD_X * 4);
     L.total = o;
@@ -603,18 +605,20 @@ __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
 template <typename WT, typename FT>
 __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64_t max_steps, PhaseOut* out,
                                    unsigned char* lds) {
-    constexpr int M = RD_M, RS = RD_RS;
-    static_assert(RS == 16, "the row prefetch maps 16 lanes to one row");
+    constexpr int RS = RD_RS, NC = 64 * RD_C;
+    static_assert(RS == 16, "the row fetch maps 16 lanes to one row");
     const int lane = threadIdx.x;
     const int n = (int)g.n;
     const RoundLds L = round_lds_layout<WT, FT>(n);
     FT* D = (FT*)(lds + L.d);
     uint8_t* x = lds + L.x;  // bit 0: the spin (set: -1); bit 1: tracked in this round
     int32_t* AP = (int32_t*)(lds + L.ap);
-    int32_t* SN = (int32_t*)(lds + L.sn);  // candidate slot -> node (-1: empty)
+    int32_t* SN = (int32_t*)(lds + L.sn);  // the round's candidates
     int32_t* RCI = (int32_t*)(lds + L.rci);
     WT* RCW = (WT*)(lds + L.rcw);
     int32_t* EX = (int32_t*)(lds + L.ex);
+    auto fmax_ = [](FT a, FT b) { return a > b ? a : b; };
+    auto fmin_ = [](FT a, FT b) { return a < b ? a : b; };
     for (int v = lane; v < n; v += 64) {
         const uint8_t xv = xg[v];
         x[v] = xv;
@@ -626,17 +630,61 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     double gsum = 0.0;
     FT bv = lowest<FT>();
     int bk = INT_MAX;
-    bool done = false;
-    while (!done) {
-        // scan: the lane's M best keys, best first (nodes rise within a lane,
-        // so an equal gain never displaces an earlier node)
-        FT cd[M];
-        int ck[M];
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            cd[i] = lowest<FT>();
-            ck[i] = INT_MAX;
+    FT T = lowest<FT>();
+    int nx = 0;  // extras this round (wave-uniform)
+    // Flip k (gain gk, spin byte xk, row e0 .. e0 + deg, cached as row slot
+    // when slot >= 0); with tracking, neighbours raised above T join the extras.
+    // Returns false when the extras overflowed (the round must end).
+    auto flip = [&](int k, FT gk, int xk, int e0, int deg, int slot, bool tracking) -> bool {
+        const int sk = (xk & 1) ? -1 : 1;  // s_k before the flip
+        bool fits = true;
+        for (int j0 = 0; j0 < deg; j0 += 64) {
+            const int j = j0 + lane;
+            bool add = false;
+            int v = 0;
+            if (j < deg) {
+                WT w;
+                if (slot >= 0 && j < RS) {
+                    v = RCI[slot * RS + j];
+                    w = RCW[slot * RS + j];
+                } else {
+                    v = g.ai[e0 + j];
+                    w = g.aw[e0 + j];
+                }
+                const uint8_t xv = x[v];
+                const int sv = (xv & 1u) ? -1 : 1;
+                const FT nd = D[v] - (FT)(2 * sk * sv) * (FT)w;
+                D[v] = nd;
+                add = tracking && !(xv & 2u) && nd > T;
+            }
+            const uint64_t m = __ballot(add);
+            const int cnt = __popcll(m);
+            if (nx + cnt > RD_X) {
+                fits = false;
+            } else {
+                if (add) {
+                    const int pos =
+                        nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    EX[pos] = v;
+                    x[v] |= 2u;
+                }
+                nx += cnt;
+            }
         }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            D[k] = -gk;
+            x[k] = (uint8_t)(xk ^ 1);
+        }
+        __builtin_amdgcn_wave_barrier();
+        return fits;
+    };
+    for (;;) {
+        // scan: the lane's first maximum and its 4 largest gains (a max/min
+        // insertion network; values only)
+        FT c0 = lowest<FT>(), c1 = c0, c2 = c0, c3 = c0;
+        FT lb = lowest<FT>();
+        int lk = INT_MAX;
         for (int v0 = lane; v0 < n; v0 += 512) {
             FT d8[8];
 #pragma unroll
@@ -644,157 +692,171 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const FT d = d8[u];
-                const int v = v0 + 64 * u;
-                if (d > cd[M - 1]) {
-#pragma unroll
-                    for (int i = M - 1; i > 0; --i) {
-                        const bool up = d > cd[i - 1];
-                        const bool here = d > cd[i];
-                        cd[i] = up ? cd[i - 1] : (here ? d : cd[i]);
-                        ck[i] = up ? ck[i - 1] : (here ? v : ck[i]);
-                    }
-                    if (d > cd[0]) {
-                        cd[0] = d;
-                        ck[0] = v;
-                    }
+                if (d > lb) {
+                    lb = d;
+                    lk = v0 + 64 * u;
                 }
+                const FT m0 = fmin_(c0, d);
+                c0 = fmax_(c0, d);
+                const FT m1 = fmin_(c1, m0);
+                c1 = fmax_(c1, m0);
+                const FT m2 = fmin_(c2, m1);
+                c2 = fmax_(c2, m1);
+                c3 = fmax_(c3, m2);
             }
         }
-        FT td = cd[M - 1];
-        int tk = ck[M - 1];
-        wave_argmax_dpp(td, tk);  // T: lanes with fewer than M nodes offer (lowest, INT_MAX)
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const int c = ck[i] < n ? ck[i] : -1;
-            SN[lane * M + i] = c;
-            if (c >= 0) x[c] |= 2u;
+        wave_argmax_dpp(lb, lk);
+        T = wave_reduce_dpp(c3, fmax_);
+        bv = lb;
+        bk = lk;
+        if (!(lb > 0) || steps >= max_steps) break;
+        if (!(lb > T)) {
+            // ties at the top: flip the scan's first maximum, then rescan
+            const uint8_t xk = x[lk];
+            const int e0 = AP[lk], deg = AP[lk + 1] - e0;
+            flip(lk, lb, xk, e0, deg, -1, false);
+            ++steps;
+            gsum += (double)lb;
+            continue;
         }
-        __syncthreads();
-        // candidate rows into the cache: entry j of slot s, 16 lanes per row
-#pragma unroll 1
-        for (int t0 = 0; t0 < M * 16; t0 += 16) {
-            int32_t vi[16];
-            WT vw[16];
-            int dst[16];
+        // the candidates: D_v > T (at most RD_C per lane)
+        int nc = 0;
+        for (int v0 = lane; v0 < n; v0 += 512) {
+            FT d8[8];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int sl = 4 * (t0 + u) + (lane >> 4), j = lane & 15;
-                const int c = SN[sl];
+            for (int u = 0; u < 8; ++u) d8[u] = v0 + 64 * u < n ? D[v0 + 64 * u] : lowest<FT>();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool ok = d8[u] > T;
+                const uint64_t m = __ballot(ok);
+                if (ok) {
+                    const int pos =
+                        nc + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    if (pos < NC) {
+                        SN[pos] = v0 + 64 * u;
+                        x[v0 + 64 * u] |= 2u;
+                    }
+                }
+                nc += __popcll(m);
+            }
+        }
+        nc = nc < NC ? nc : NC;  // never above NC: each lane has at most 3 nodes above T
+        __builtin_amdgcn_wave_barrier();
+        // the lane's candidates and their rows (registers), and the row cache
+        int cn[RD_C], ce0[RD_C], cdg[RD_C];
+#pragma unroll
+        for (int r = 0; r < RD_C; ++r) {
+            const int i = lane + 64 * r;
+            cn[r] = i < nc ? SN[i] : -1;
+            ce0[r] = cn[r] >= 0 ? AP[cn[r]] : 0;
+            cdg[r] = cn[r] >= 0 ? AP[cn[r] + 1] - ce0[r] : 0;
+        }
+        for (int b0 = 0; b0 < nc; b0 += 96) {
+            int32_t vi[24];
+            WT vw[24];
+            int dst[24];
+#pragma unroll
+            for (int u = 0; u < 24; ++u) {
+                const int row = b0 + 4 * u + (lane >> 4), j = lane & 15;
                 dst[u] = -1;
-                if (c >= 0) {
+                if (row < nc) {
+                    const int c = SN[row];
                     const int e0 = AP[c];
                     if (j < AP[c + 1] - e0) {
-                        dst[u] = sl * RS + j;
+                        dst[u] = row * RS + j;
                         vi[u] = g.ai[e0 + j];
                         vw[u] = g.aw[e0 + j];
                     }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 16; ++u)
+            for (int u = 0; u < 24; ++u)
                 if (dst[u] >= 0) {
                     RCI[dst[u]] = vi[u];
                     RCW[dst[u]] = vw[u];
                 }
         }
-        __syncthreads();
-        int nx = 0;  // extras this round (wave-uniform)
+        __builtin_amdgcn_wave_barrier();
+        nx = 0;
+        bool done = false;
         int64_t round_flips = 0;
         for (;;) {
-            // first maximum over the tracked nodes; the index carries the slot
-            // (node << 9 | slot + 1; 0 for an extra, whose row is not cached)
-            FT b = lowest<FT>();
-            int bp = INT_MAX;
+            // the lane's first maximum over its tracked nodes
+            FT bd = lowest<FT>();
+            int bn = INT_MAX, bs = -1, be = 0, bg = 0, bx = 0;
 #pragma unroll
-            for (int i = 0; i < M; ++i)
-                if (ck[i] < n) {
-                    const FT d = D[ck[i]];
-                    const int p = (ck[i] << 9) | (lane * M + i + 1);
-                    if (d > b || (d == b && p < bp)) {
-                        b = d;
-                        bp = p;
+            for (int r = 0; r < RD_C; ++r)
+                if (cn[r] >= 0) {
+                    const FT d = D[cn[r]];
+                    const int xv = x[cn[r]];
+                    if (d > bd || (d == bd && cn[r] < bn)) {
+                        bd = d;
+                        bn = cn[r];
+                        bs = lane + 64 * r;
+                        be = ce0[r];
+                        bg = cdg[r];
+                        bx = xv;
                     }
                 }
             if (lane < nx) {
                 const int c = EX[lane];
                 const FT d = D[c];
-                const int p = c << 9;
-                if (d > b || (d == b && p < bp)) {
-                    b = d;
-                    bp = p;
+                const int xv = x[c];
+                const int e0 = AP[c], e1 = AP[c + 1];
+                if (d > bd || (d == bd && c < bn)) {
+                    bd = d;
+                    bn = c;
+                    bs = -1;
+                    be = e0;
+                    bg = e1 - e0;
+                    bx = xv;
                 }
             }
-            wave_argmax_dpp(b, bp);
-            const int k = bp == INT_MAX ? INT_MAX : bp >> 9;
-            const int slot = bp == INT_MAX ? -1 : (bp & 511) - 1;
-            if (!(b > td || (b == td && k < tk))) {
-                if (round_flips == 0) {  // only when there is no node at all
-                    bv = b;
-                    bk = k;
-                    done = true;
-                }
-                break;
+            // the wave's first maximum: the largest gain, then the lowest node
+            const FT m = wave_reduce_dpp(bd, fmax_);
+            uint64_t eq = __ballot(bd == m && bn != INT_MAX);
+            if (__popcll(eq) > 1) {
+                const int c = ((eq >> lane) & 1) ? bn : INT_MAX;
+                const int mn = wave_reduce_dpp(c, [](int a, int b) { return a < b ? a : b; });
+                eq = __ballot(bn == mn && ((eq >> lane) & 1));
             }
-            bv = b;
+            if (eq == 0 || !(m > T)) break;  // an untracked node may hold the maximum: rescan
+            const int w = (int)__builtin_ctzll(eq);
+            const int k = __builtin_amdgcn_readlane(bn, w);
+            bv = m;
             bk = k;
-            if (!(b > 0) || steps >= max_steps) {
+            if (!(m > 0) || steps >= max_steps) {
                 done = true;
                 break;
             }
-            const int sk = (x[k] & 1u) ? -1 : 1;  // s_k before the flip
-            const int e0 = AP[k], deg = AP[k + 1] - e0;
-            bool full = false;
-            for (int j0 = 0; j0 < deg; j0 += 64) {
-                const int j = j0 + lane;
-                bool add = false;
-                int v = 0;
-                if (j < deg) {
-                    WT w;
-                    if (slot >= 0 && j < RS) {
-                        v = RCI[slot * RS + j];
-                        w = RCW[slot * RS + j];
-                    } else {
-                        v = g.ai[e0 + j];
-                        w = g.aw[e0 + j];
-                    }
-                    const uint8_t xv = x[v];
-                    const int sv = (xv & 1u) ? -1 : 1;
-                    const FT nd = D[v] - (FT)(2 * sk * sv) * (FT)w;
-                    D[v] = nd;
-                    add = !(xv & 2u) && (nd > td || (nd == td && v < tk));
-                }
-                const uint64_t m = __ballot(add);
-                const int cnt = __popcll(m);
-                if (nx + cnt > RD_X) {
-                    full = true;  // not tracked: this round ends after the flip
-                } else {
-                    if (add) {
-                        const int pos = nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        EX[pos] = v;
-                        x[v] |= 2u;
-                    }
-                    nx += cnt;
-                }
-            }
-            __syncthreads();
-            if (lane == 0) {
-                D[k] = -D[k];
-                x[k] ^= 1u;
-            }
-            __syncthreads();
+            const int slot = __builtin_amdgcn_readlane(bs, w);
+            const int e0 = __builtin_amdgcn_readlane(be, w);
+            const int deg = __builtin_amdgcn_readlane(bg, w);
+            const int xk = __builtin_amdgcn_readlane(bx, w);
+            const bool fits = flip(k, m, xk, e0, deg, slot, true);
             ++steps;
             ++round_flips;
-            gsum += (double)b;
-            if (full) break;
+            gsum += (double)m;
+            if (!fits) break;
         }
         // untrack the round's candidates and extras
 #pragma unroll
-        for (int i = 0; i < M; ++i)
-            if (ck[i] < n) x[ck[i]] &= 1u;
+        for (int r = 0; r < RD_C; ++r)
+            if (cn[r] >= 0) x[cn[r]] &= 1u;
         if (lane < nx) x[EX[lane]] &= 1u;
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
+        if (done) break;
+        if (round_flips == 0) {
+            // cannot happen (the scan's first maximum is a candidate above T);
+            // flipping it keeps every round making progress regardless
+            const uint8_t xk = x[lk];
+            const int e0 = AP[lk], deg = AP[lk + 1] - e0;
+            flip(lk, lb, xk, e0, deg, -1, false);
+            ++steps;
+            gsum += (double)lb;
+        }
     }
+    __syncthreads();
     if (steps) {
         for (int v = lane; v < n; v += 64) {
             const uint8_t xv = x[v];
