@@ -576,8 +576,47 @@ constexpr int RD_RS = 16;  // cached entries per candidate row
 constexpr int RD_X = 64;   // extras per round
 
 struct RoundLds {
-    size_t d, x, ap, sn, rci, rcw, ex, total;
+    size_t d, x, trk, ap, sn, rci, rcw, ex, total;
 };
+
+// maximum / minimum as one instruction (v_max_f64 etc.); gains are never NaN
+__device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ int32_t vmax(int32_t a, int32_t b) { return max(a, b); }
+__device__ __forceinline__ int32_t vmin(int32_t a, int32_t b) { return min(a, b); }
+
+// An order-preserving int32 key of a gain: the high word of a double with
+// its magnitude bits flipped when negative (so signed integer order follows
+// the value), or an integer gain itself.  Keys are coarser than gains, never
+// out of order.
+__device__ __forceinline__ int32_t gain_key(const double* D, int v) {
+    const int32_t hi = ((const int32_t*)D)[2 * v + 1];
+    return hi ^ ((hi >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ int32_t gain_key(const int32_t* D, int v) { return D[v]; }
+__device__ __forceinline__ int32_t hi_key(int32_t hi) { return hi ^ ((hi >> 31) & 0x7fffffff); }
+
+// keys of the gains of nodes b and b + 1 (b even; one 16- or 8-byte load)
+__device__ __forceinline__ void gain_key_pair(const double* D, int b, int32_t& k0, int32_t& k1) {
+    const int4 q = *(const int4*)(D + b);
+    k0 = hi_key(q.y);
+    k1 = hi_key(q.w);
+}
+__device__ __forceinline__ void gain_key_pair(const int32_t* D, int b, int32_t& k0, int32_t& k1) {
+    const int2 q = *(const int2*)(D + b);
+    k0 = q.x;
+    k1 = q.y;
+}
+
+// The largest gain whose key is at most k (k = INT_MIN: below every gain)
+__device__ __forceinline__ void key_bound(int32_t k, double& t) {
+    if (k == INT_MIN) {
+        t = -INFINITY;
+        return;
+    }
+    t = k < 0 ? __hiloint2double(k ^ 0x7fffffff, 0) : __hiloint2double(k, (int)0xFFFFFFFF);
+}
+__device__ __forceinline__ void key_bound(int32_t k, int32_t& t) { t = k; }
 
 template <typename WT, typename FT>
 __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
@@ -587,6 +626,8 @@ __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
     L.d = o;
     o += al((size_t)n * sizeof(FT));
     L.x = o;
+    o += al((size_t)n);
+    L.trk = o;
     o += al((size_t)n);
     L.ap = o;
     o += al((size_t)(n + 1) * 4);
@@ -611,17 +652,18 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     const int n = (int)g.n;
     const RoundLds L = round_lds_layout<WT, FT>(n);
     FT* D = (FT*)(lds + L.d);
-    uint8_t* x = lds + L.x;  // bit 0: the spin (set: -1); bit 1: tracked in this round
+    uint8_t* x = lds + L.x;      // the spins (1: -1)
+    uint8_t* trk = lds + L.trk;  // 1: tracked in this round
     int32_t* AP = (int32_t*)(lds + L.ap);
     int32_t* SN = (int32_t*)(lds + L.sn);  // the round's candidates
     int32_t* RCI = (int32_t*)(lds + L.rci);
     WT* RCW = (WT*)(lds + L.rcw);
     int32_t* EX = (int32_t*)(lds + L.ex);
-    auto fmax_ = [](FT a, FT b) { return a > b ? a : b; };
-    auto fmin_ = [](FT a, FT b) { return a < b ? a : b; };
+    auto fmax_ = [](FT a, FT b) { return vmax(a, b); };
     for (int v = lane; v < n; v += 64) {
         const uint8_t xv = xg[v];
         x[v] = xv;
+        trk[v] = 0;
         D[v] = xv ? -hg[v] : hg[v];
     }
     for (int v = lane; v <= n; v += 64) AP[v] = g.ap[v];
@@ -636,7 +678,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     // when slot >= 0); with tracking, neighbours raised above T join the extras.
     // Returns false when the extras overflowed (the round must end).
     auto flip = [&](int k, FT gk, int xk, int e0, int deg, int slot, bool tracking) -> bool {
-        const int sk = (xk & 1) ? -1 : 1;  // s_k before the flip
+        const int sk = xk ? -1 : 1;  // s_k before the flip
         bool fits = true;
         for (int j0 = 0; j0 < deg; j0 += 64) {
             const int j = j0 + lane;
@@ -651,11 +693,11 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
                     v = g.ai[e0 + j];
                     w = g.aw[e0 + j];
                 }
-                const uint8_t xv = x[v];
-                const int sv = (xv & 1u) ? -1 : 1;
+                const uint8_t xv = x[v], tv = trk[v];
+                const int sv = xv ? -1 : 1;
                 const FT nd = D[v] - (FT)(2 * sk * sv) * (FT)w;
                 D[v] = nd;
-                add = tracking && !(xv & 2u) && nd > T;
+                add = tracking && !tv && nd > T;
             }
             const uint64_t m = __ballot(add);
             const int cnt = __popcll(m);
@@ -666,7 +708,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
                     const int pos =
                         nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                     EX[pos] = v;
-                    x[v] |= 2u;
+                    trk[v] = 1;
                 }
                 nx += cnt;
             }
@@ -680,38 +722,55 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         return fits;
     };
     for (;;) {
-        // scan: the lane's first maximum and its 4 largest gains (a max/min
-        // insertion network; values only)
-        FT c0 = lowest<FT>(), c1 = c0, c2 = c0, c3 = c0;
-        FT lb = lowest<FT>();
-        int lk = INT_MAX;
-        for (int v0 = lane; v0 < n; v0 += 512) {
-            FT d8[8];
+        // scan: the lane's 4 largest gain keys (a max/min insertion network);
+        // KT is the largest 4th over the lanes and T the largest gain with a
+        // key <= KT, so a node outside {key > KT} has a gain <= T, and each
+        // lane has at most 3 nodes with key > KT
+        int32_t c0 = INT_MIN, c1 = INT_MIN, c2 = INT_MIN, c3 = INT_MIN;
+        auto insert = [&](int32_t k) {
+            const int32_t m0 = min(c0, k);
+            c0 = max(c0, k);
+            const int32_t m1 = min(c1, m0);
+            c1 = max(c1, m0);
+            const int32_t m2 = min(c2, m1);
+            c2 = max(c2, m1);
+            c3 = max(c3, m2);
+        };
+        // lane l holds the pairs of nodes 2 l + 128 i and 2 l + 128 i + 1
+        for (int b0 = 2 * lane; b0 < n; b0 += 512) {
+            int32_t ka[4], kb[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) d8[u] = v0 + 64 * u < n ? D[v0 + 64 * u] : lowest<FT>();
+            for (int u = 0; u < 4; ++u) {
+                const int b = b0 + 128 * u;
+                ka[u] = kb[u] = INT_MIN;
+                if (b < n) gain_key_pair(D, b, ka[u], kb[u]);
+                if (b + 1 >= n) kb[u] = INT_MIN;
+            }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const FT d = d8[u];
-                if (d > lb) {
-                    lb = d;
-                    lk = v0 + 64 * u;
-                }
-                const FT m0 = fmin_(c0, d);
-                c0 = fmax_(c0, d);
-                const FT m1 = fmin_(c1, m0);
-                c1 = fmax_(c1, m0);
-                const FT m2 = fmin_(c2, m1);
-                c2 = fmax_(c2, m1);
-                c3 = fmax_(c3, m2);
+            for (int u = 0; u < 4; ++u) {
+                insert(ka[u]);
+                insert(kb[u]);
             }
         }
-        wave_argmax_dpp(lb, lk);
-        T = wave_reduce_dpp(c3, fmax_);
-        bv = lb;
-        bk = lk;
-        if (!(lb > 0) || steps >= max_steps) break;
-        if (!(lb > T)) {
-            // ties at the top: flip the scan's first maximum, then rescan
+        const int32_t K0 = wave_max_i32(c0);
+        const int32_t KT = wave_max_i32(c3);
+        key_bound(KT, T);
+        if (K0 <= 0 || K0 <= KT || steps >= max_steps) {
+            // the first maximum itself (exact): the result, or the move to
+            // make directly when no key exceeds KT (ties) or gains are tiny
+            FT lb = lowest<FT>();
+            int lk = INT_MAX;
+            for (int v = lane; v < n; v += 64) {
+                const FT d = D[v];
+                if (d > lb) {
+                    lb = d;
+                    lk = v;
+                }
+            }
+            wave_argmax_dpp(lb, lk);
+            bv = lb;
+            bk = lk;
+            if (lk == INT_MAX || !(lb > 0) || steps >= max_steps) break;
             const uint8_t xk = x[lk];
             const int e0 = AP[lk], deg = AP[lk + 1] - e0;
             flip(lk, lb, xk, e0, deg, -1, false);
@@ -719,39 +778,45 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             gsum += (double)lb;
             continue;
         }
-        // the candidates: D_v > T (at most RD_C per lane)
-        int nc = 0;
-        for (int v0 = lane; v0 < n; v0 += 512) {
-            FT d8[8];
+        // the candidates: key > KT, at most RD_C per lane (same lane layout
+        // as the scan), kept in the lane's registers in ascending node order
+        int cn[RD_C], ncl = 0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) d8[u] = v0 + 64 * u < n ? D[v0 + 64 * u] : lowest<FT>();
+        for (int r = 0; r < RD_C; ++r) cn[r] = -1;
+        auto take = [&](bool ok, int v) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const bool ok = d8[u] > T;
-                const uint64_t m = __ballot(ok);
-                if (ok) {
-                    const int pos =
-                        nc + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    if (pos < NC) {
-                        SN[pos] = v0 + 64 * u;
-                        x[v0 + 64 * u] |= 2u;
-                    }
-                }
-                nc += __popcll(m);
+            for (int r = 0; r < RD_C; ++r) cn[r] = (ok && ncl == r) ? v : cn[r];
+            ncl += ok ? 1 : 0;
+        };
+        for (int b0 = 2 * lane; b0 < n; b0 += 512) {
+            int32_t ka[4], kb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int b = b0 + 128 * u;
+                ka[u] = kb[u] = INT_MIN;
+                if (b < n) gain_key_pair(D, b, ka[u], kb[u]);
+                if (b + 1 >= n) kb[u] = INT_MIN;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                take(ka[u] > KT, b0 + 128 * u);
+                take(kb[u] > KT, b0 + 128 * u + 1);
             }
         }
-        nc = nc < NC ? nc : NC;  // never above NC: each lane has at most 3 nodes above T
-        __builtin_amdgcn_wave_barrier();
-        // the lane's candidates and their rows (registers), and the row cache
-        int cn[RD_C], ce0[RD_C], cdg[RD_C];
 #pragma unroll
         for (int r = 0; r < RD_C; ++r) {
-            const int i = lane + 64 * r;
-            cn[r] = i < nc ? SN[i] : -1;
+            SN[lane + 64 * r] = cn[r];
+            if (cn[r] >= 0) trk[cn[r]] = 1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the lane's candidates and their rows (registers), and the row cache
+        int ce0[RD_C], cdg[RD_C];
+#pragma unroll
+        for (int r = 0; r < RD_C; ++r) {
             ce0[r] = cn[r] >= 0 ? AP[cn[r]] : 0;
             cdg[r] = cn[r] >= 0 ? AP[cn[r] + 1] - ce0[r] : 0;
         }
-        for (int b0 = 0; b0 < nc; b0 += 96) {
+        for (int b0 = 0; b0 < NC; b0 += 96) {
             int32_t vi[24];
             WT vw[24];
             int dst[24];
@@ -759,8 +824,8 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             for (int u = 0; u < 24; ++u) {
                 const int row = b0 + 4 * u + (lane >> 4), j = lane & 15;
                 dst[u] = -1;
-                if (row < nc) {
-                    const int c = SN[row];
+                const int c = SN[row];
+                if (c >= 0) {
                     const int e0 = AP[c];
                     if (j < AP[c + 1] - e0) {
                         dst[u] = row * RS + j;
@@ -781,51 +846,59 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         bool done = false;
         int64_t round_flips = 0;
         for (;;) {
-            // the lane's first maximum over its tracked nodes
-            FT bd = lowest<FT>();
+            // tracked gains: the lane's candidates (ascending nodes) and extra
+            FT d[RD_C];
+            int xr[RD_C];
+#pragma unroll
+            for (int r = 0; r < RD_C; ++r) {
+                d[r] = cn[r] >= 0 ? D[cn[r]] : lowest<FT>();
+                xr[r] = cn[r] >= 0 ? x[cn[r]] : 0;
+            }
+            int ec = -1, exx = 0, ee0 = 0, edg = 0;
+            FT ed = lowest<FT>();
+            if (lane < nx) {
+                ec = EX[lane];
+                ed = D[ec];
+                exx = x[ec];
+                ee0 = AP[ec];
+                edg = AP[ec + 1] - ee0;
+            }
+            FT lm = ed;
+#pragma unroll
+            for (int r = 0; r < RD_C; ++r) lm = vmax(lm, d[r]);
+            const FT m = wave_reduce_dpp(lm, fmax_);
+            if (!(m > T)) break;  // an untracked node may hold the maximum: rescan
+            // the lowest node of gain m: the lane's first candidate at m, or
+            // its extra when lower
             int bn = INT_MAX, bs = -1, be = 0, bg = 0, bx = 0;
 #pragma unroll
-            for (int r = 0; r < RD_C; ++r)
-                if (cn[r] >= 0) {
-                    const FT d = D[cn[r]];
-                    const int xv = x[cn[r]];
-                    if (d > bd || (d == bd && cn[r] < bn)) {
-                        bd = d;
-                        bn = cn[r];
-                        bs = lane + 64 * r;
-                        be = ce0[r];
-                        bg = cdg[r];
-                        bx = xv;
-                    }
+            for (int r = RD_C - 1; r >= 0; --r)
+                if (d[r] == m && cn[r] >= 0) {
+                    bn = cn[r];
+                    bs = lane + 64 * r;
+                    be = ce0[r];
+                    bg = cdg[r];
+                    bx = xr[r];
                 }
-            if (lane < nx) {
-                const int c = EX[lane];
-                const FT d = D[c];
-                const int xv = x[c];
-                const int e0 = AP[c], e1 = AP[c + 1];
-                if (d > bd || (d == bd && c < bn)) {
-                    bd = d;
-                    bn = c;
-                    bs = -1;
-                    be = e0;
-                    bg = e1 - e0;
-                    bx = xv;
-                }
+            if (ec >= 0 && ed == m && ec < bn) {
+                bn = ec;
+                bs = -1;
+                be = ee0;
+                bg = edg;
+                bx = exx;
             }
-            // the wave's first maximum: the largest gain, then the lowest node
-            const FT m = wave_reduce_dpp(bd, fmax_);
-            uint64_t eq = __ballot(bd == m && bn != INT_MAX);
+            uint64_t eq = __ballot(bn != INT_MAX);
             if (__popcll(eq) > 1) {
-                const int c = ((eq >> lane) & 1) ? bn : INT_MAX;
-                const int mn = wave_reduce_dpp(c, [](int a, int b) { return a < b ? a : b; });
-                eq = __ballot(bn == mn && ((eq >> lane) & 1));
+                const int mn = wave_reduce_dpp(bn, [](int a, int b) { return a < b ? a : b; });
+                eq = __ballot(bn == mn);
             }
-            if (eq == 0 || !(m > T)) break;  // an untracked node may hold the maximum: rescan
+            if (eq == 0) break;
             const int w = (int)__builtin_ctzll(eq);
             const int k = __builtin_amdgcn_readlane(bn, w);
             bv = m;
             bk = k;
             if (!(m > 0) || steps >= max_steps) {
+                bv = D[k];
                 done = true;
                 break;
             }
@@ -842,13 +915,26 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         // untrack the round's candidates and extras
 #pragma unroll
         for (int r = 0; r < RD_C; ++r)
-            if (cn[r] >= 0) x[cn[r]] &= 1u;
-        if (lane < nx) x[EX[lane]] &= 1u;
+            if (cn[r] >= 0) trk[cn[r]] = 0;
+        if (lane < nx) trk[EX[lane]] = 0;
         __builtin_amdgcn_wave_barrier();
         if (done) break;
         if (round_flips == 0) {
-            // cannot happen (the scan's first maximum is a candidate above T);
-            // flipping it keeps every round making progress regardless
+            // cannot happen (the node of key K0 > KT is a candidate above T);
+            // flipping the first maximum keeps every round making progress
+            FT lb = lowest<FT>();
+            int lk = INT_MAX;
+            for (int v = lane; v < n; v += 64) {
+                const FT d = D[v];
+                if (d > lb) {
+                    lb = d;
+                    lk = v;
+                }
+            }
+            wave_argmax_dpp(lb, lk);
+            bv = lb;
+            bk = lk;
+            if (lk == INT_MAX || !(lb > 0)) break;
             const uint8_t xk = x[lk];
             const int e0 = AP[lk], deg = AP[lk + 1] - e0;
             flip(lk, lb, xk, e0, deg, -1, false);
