@@ -468,6 +468,47 @@ __global__ __launch_bounds__(1024) void k_descent(GView<WT> g, uint8_t* x, FT* h
     descent_body<WT, FT>(g, x, h, max_steps, out);
 }
 
+// Single-wave staging of global arrays into LDS with 16 loads in flight per
+// lane (a plain element loop waits for each load before the next).
+// Spins and gains D_v = s_v h_v (and cleared flags when trk is given):
+template <typename FT>
+__device__ __forceinline__ void stage_gains(const uint8_t* xg, const FT* hg, int n, uint8_t* x, FT* D,
+                                            uint8_t* trk) {
+    const int lane = threadIdx.x;
+    for (int v0 = lane; v0 < n; v0 += 64 * 16) {
+        uint8_t xv[16];
+        FT hv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int v = v0 + 64 * u;
+            xv[u] = v < n ? xg[v] : 0;
+            hv[u] = v < n ? hg[v] : (FT)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int v = v0 + 64 * u;
+            if (v < n) {
+                x[v] = xv[u];
+                D[v] = xv[u] ? -hv[u] : hv[u];
+                if (trk) trk[v] = 0;
+            }
+        }
+    }
+}
+
+// cnt 32-bit words
+__device__ __forceinline__ void stage_words(const int32_t* src, int cnt, int32_t* dst) {
+    const int lane = threadIdx.x;
+    for (int i0 = lane; i0 < cnt; i0 += 64 * 16) {
+        int32_t w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) w[u] = i0 + 64 * u < cnt ? src[i0 + 64 * u] : 0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (i0 + 64 * u < cnt) dst[i0 + 64 * u] = w[u];
+    }
+}
+
 // Single-wavefront variant.  LDS holds the gains D_v = s_v h_v themselves, so
 // the argmax scan is one independent LDS load per element (unrolled), and a
 // flip of k updates D_k -> -D_k and D_v -= 2 w_vk s_k s_v for its neighbours.
@@ -483,13 +524,8 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
     // for them), so a flip waits for one global access (its row), not two
     int32_t* AP = (int32_t*)(lds + (((size_t)n * sizeof(FT) + 15) & ~(size_t)15) + (((size_t)n + 15) & ~(size_t)15));
     const int32_t* ap = stage_ap ? AP : g.ap;
-    for (int v = lane; v < n; v += 64) {
-        const uint8_t xv = xg[v];
-        x[v] = xv;
-        D[v] = xv ? -hg[v] : hg[v];
-    }
-    if (stage_ap)
-        for (int v = lane; v <= n; v += 64) AP[v] = g.ap[v];
+    stage_gains<FT>(xg, hg, n, x, D, nullptr);
+    if (stage_ap) stage_words(g.ap, n + 1, AP);
     __syncthreads();
     int64_t steps = 0, evals = 0;
     double gsum = 0.0;
@@ -579,6 +615,12 @@ struct RoundLds {
     size_t d, x, trk, ap, sn, rci, rcw, ex, total;
 };
 
+__device__ __forceinline__ double readlane_t(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ int32_t readlane_t(int32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
 // maximum / minimum as one instruction (v_max_f64 etc.); gains are never NaN
 __device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
@@ -660,13 +702,8 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     WT* RCW = (WT*)(lds + L.rcw);
     int32_t* EX = (int32_t*)(lds + L.ex);
     auto fmax_ = [](FT a, FT b) { return vmax(a, b); };
-    for (int v = lane; v < n; v += 64) {
-        const uint8_t xv = xg[v];
-        x[v] = xv;
-        trk[v] = 0;
-        D[v] = xv ? -hg[v] : hg[v];
-    }
-    for (int v = lane; v <= n; v += 64) AP[v] = g.ap[v];
+    stage_gains<FT>(xg, hg, n, x, D, trk);
+    stage_words(g.ap, n + 1, AP);
     __syncthreads();
     int64_t steps = 0;
     double gsum = 0.0;
@@ -687,8 +724,10 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             if (j < deg) {
                 WT w;
                 if (slot >= 0 && j < RS) {
-                    v = RCI[slot * RS + j];
-                    w = RCW[slot * RS + j];
+                    // through LDS-qualified pointers, so that the two sources
+                    // stay separate loads (not one flat load of either)
+                    v = ((const __attribute__((address_space(3))) int32_t*)RCI)[slot * RS + j];
+                    w = ((const __attribute__((address_space(3))) WT*)RCW)[slot * RS + j];
                 } else {
                     v = g.ai[e0 + j];
                     w = g.aw[e0 + j];
@@ -843,70 +882,77 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         }
         __builtin_amdgcn_wave_barrier();
         nx = 0;
-        bool done = false;
+        bool term = false;  // a termination condition met: the next scan settles it exactly
         int64_t round_flips = 0;
+        int myex = -1;  // this lane's extra (EX[lane])
         for (;;) {
-            // tracked gains: the lane's candidates (ascending nodes) and extra
-            FT d[RD_C];
-            int xr[RD_C];
+            // the lane's first maximum over its tracked nodes: candidates
+            // (ascending nodes, so a strict > keeps the lowest) and extra
+            FT bd = lowest<FT>();
+            int bn = -1, br = 0;
 #pragma unroll
             for (int r = 0; r < RD_C; ++r) {
-                d[r] = cn[r] >= 0 ? D[cn[r]] : lowest<FT>();
-                xr[r] = cn[r] >= 0 ? x[cn[r]] : 0;
-            }
-            int ec = -1, exx = 0, ee0 = 0, edg = 0;
-            FT ed = lowest<FT>();
-            if (lane < nx) {
-                ec = EX[lane];
-                ed = D[ec];
-                exx = x[ec];
-                ee0 = AP[ec];
-                edg = AP[ec + 1] - ee0;
-            }
-            FT lm = ed;
-#pragma unroll
-            for (int r = 0; r < RD_C; ++r) lm = vmax(lm, d[r]);
-            const FT m = wave_reduce_dpp(lm, fmax_);
-            if (!(m > T)) break;  // an untracked node may hold the maximum: rescan
-            // the lowest node of gain m: the lane's first candidate at m, or
-            // its extra when lower
-            int bn = INT_MAX, bs = -1, be = 0, bg = 0, bx = 0;
-#pragma unroll
-            for (int r = RD_C - 1; r >= 0; --r)
-                if (d[r] == m && cn[r] >= 0) {
+                const FT dr = D[cn[r] >= 0 ? cn[r] : 0];
+                if (cn[r] >= 0 && dr > bd) {
+                    bd = dr;
                     bn = cn[r];
-                    bs = lane + 64 * r;
-                    be = ce0[r];
-                    bg = cdg[r];
-                    bx = xr[r];
+                    br = r;
                 }
-            if (ec >= 0 && ed == m && ec < bn) {
-                bn = ec;
-                bs = -1;
-                be = ee0;
-                bg = edg;
-                bx = exx;
             }
-            uint64_t eq = __ballot(bn != INT_MAX);
+            {
+                const FT de = D[myex >= 0 ? myex : 0];
+                if (myex >= 0 && (de > bd || (de == bd && myex < bn))) {
+                    bd = de;
+                    bn = myex;
+                    br = RD_C;
+                }
+            }
+            // the wave's first maximum: a maximum over the gain keys, exact
+            // comparisons only where keys tie
+            const int32_t lkey = bn >= 0 ? gain_key(&bd, 0) : INT_MIN;
+            const int32_t K = wave_max_i32(lkey);
+            uint64_t eq = __ballot(bn >= 0 && lkey == K);
             if (__popcll(eq) > 1) {
-                const int mn = wave_reduce_dpp(bn, [](int a, int b) { return a < b ? a : b; });
-                eq = __ballot(bn == mn);
+                const bool in = (eq >> lane) & 1;
+                const FT mm = wave_reduce_dpp(in ? bd : lowest<FT>(), fmax_);
+                eq = __ballot(in && bd == mm);
+                if (__popcll(eq) > 1) {
+                    const bool in2 = (eq >> lane) & 1;
+                    const int mn = wave_reduce_dpp(in2 ? bn : INT_MAX, [](int a, int b) { return a < b ? a : b; });
+                    eq = __ballot(in2 && bn == mn);
+                }
             }
             if (eq == 0) break;
             const int w = (int)__builtin_ctzll(eq);
+            const FT m = readlane_t(bd, w);
             const int k = __builtin_amdgcn_readlane(bn, w);
-            bv = m;
-            bk = k;
+            if (!(m > T)) break;  // an untracked node may hold the maximum: rescan
             if (!(m > 0) || steps >= max_steps) {
-                bv = D[k];
-                done = true;
+                term = true;
                 break;
             }
-            const int slot = __builtin_amdgcn_readlane(bs, w);
-            const int e0 = __builtin_amdgcn_readlane(be, w);
-            const int deg = __builtin_amdgcn_readlane(bg, w);
-            const int xk = __builtin_amdgcn_readlane(bx, w);
+            const int rw = __builtin_amdgcn_readlane(br, w);
+            int slot = -1, e0, deg;
+            if (rw < RD_C) {
+                int ce = ce0[0], cg = cdg[0];
+#pragma unroll
+                for (int r = 1; r < RD_C; ++r)
+                    if (rw == r) {
+                        ce = ce0[r];
+                        cg = cdg[r];
+                    }
+                slot = w + 64 * rw;
+                e0 = __builtin_amdgcn_readlane(ce, w);
+                deg = __builtin_amdgcn_readlane(cg, w);
+            } else {
+                e0 = AP[k];
+                deg = AP[k + 1] - e0;
+            }
+            const int xk = x[k];
+            const int nx0 = nx;
             const bool fits = flip(k, m, xk, e0, deg, slot, true);
+            if (nx > nx0 && lane >= nx0 && lane < nx) myex = EX[lane];
+            __builtin_amdgcn_wave_barrier();
             ++steps;
             ++round_flips;
             gsum += (double)m;
@@ -916,10 +962,9 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
 #pragma unroll
         for (int r = 0; r < RD_C; ++r)
             if (cn[r] >= 0) trk[cn[r]] = 0;
-        if (lane < nx) trk[EX[lane]] = 0;
+        if (myex >= 0) trk[myex] = 0;
         __builtin_amdgcn_wave_barrier();
-        if (done) break;
-        if (round_flips == 0) {
+        if (round_flips == 0 && !term) {
             // cannot happen (the node of key K0 > KT is a candidate above T);
             // flipping the first maximum keeps every round making progress
             FT lb = lowest<FT>();
@@ -3554,13 +3599,16 @@ __device__ void learned_pass_staged(const BInstRaw& r, const GView<WT>& g, float
     int32_t* RCI = (int32_t*)(lds + L.rci);
     WT* RCW = (WT*)(lds + L.rcw);
     FT* hg = (FT*)r.h;
-    for (int v = lane; v < n; v += 64) {
-        const uint8_t xv = r.x[v];
-        x[v] = xv;
-        D[v] = xv ? -hg[v] : hg[v];
-        age[v] = r.age[v];
+    stage_gains<FT>(r.x, hg, n, x, D, nullptr);
+    stage_words(g.ap, n + 1, AP);
+    for (int v0 = lane; v0 < n; v0 += 64 * 16) {
+        uint8_t a[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u] = v0 + 64 * u < n ? r.age[v0 + 64 * u] : 0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (v0 + 64 * u < n) age[v0 + 64 * u] = a[u];
     }
-    for (int v = lane; v <= n; v += 64) AP[v] = g.ap[v];
     // the visiting list: perm order, score above min_score; 16 chunks of
     // perm and score loads in flight per lane
     int nl = 0;
