@@ -1942,24 +1942,32 @@ __device__ double seed_lowrank_one(const GView<WT>& g, const uint8_t* x, const F
     const uint64_t i = (uint64_t)(b % bs);
     const uint64_t gray = i ^ (i >> 1);
     const int64_t offset = (b / bs) << 6;
-    int64_t F[64];
-    int m = 0;
-    for (int j = 0; j < 64 && j < n; ++j) {
-        if (!((gray >> j) & 1u)) continue;
-        int64_t idx = seed_pos(n, offset, j);
-        int dup = -1;
-        for (int q = 0; q < m; ++q)
-            if (F[q] == idx) dup = q;
-        if (dup >= 0) F[dup] = F[--m];  // XOR twice cancels
-        else F[m++] = idx;
-    }
     AT acc = 0;
-    for (int q = 0; q < m; ++q) {
-        const int64_t k = F[q];
+    if (n >= 64) {
+        // the 64 positions are distinct: walk the set bits of the code
+        for (uint64_t rest = gray; rest; rest &= rest - 1) {
+            const int64_t k = seed_pos(n, offset, __builtin_ctzll(rest));
+            const int sk = x[k] ? -1 : 1;
+            acc += (AT)sk * (AT)h[k];
+            for (uint64_t r2 = rest & (rest - 1); r2; r2 &= r2 - 1) {
+                const int64_t l = seed_pos(n, offset, __builtin_ctzll(r2));
+                const int sl = x[l] ? -1 : 1;
+                acc -= (AT)(2 * sk * sl) * (AT)weight_of(g, k, l);
+            }
+        }
+        return (double)acc;
+    }
+    // n < 64: positions repeat, and a node flipped twice is not flipped; the
+    // set is a mask over the n nodes
+    uint64_t set = 0;
+    for (int j = 0; j < 64 && j < n; ++j)
+        if ((gray >> j) & 1u) set ^= 1ull << seed_pos(n, offset, j);
+    for (uint64_t rest = set; rest; rest &= rest - 1) {
+        const int64_t k = __builtin_ctzll(rest);
         const int sk = x[k] ? -1 : 1;
         acc += (AT)sk * (AT)h[k];
-        for (int r = q + 1; r < m; ++r) {
-            const int64_t l = F[r];
+        for (uint64_t r2 = rest & (rest - 1); r2; r2 &= r2 - 1) {
+            const int64_t l = __builtin_ctzll(r2);
             const int sl = x[l] ? -1 : 1;
             acc -= (AT)(2 * sk * sl) * (AT)weight_of(g, k, l);
         }
