@@ -182,7 +182,28 @@ __global__ __launch_bounds__(256) void k_infer_l1(const int32_t* __restrict__ pt
     if (v >= N) return;
     const float4 x = *(const float4*)(X + v * F0);
     float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
-    for (int32_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+    int32_t e = ptr[v];
+    const int32_t e1 = ptr[v + 1];
+    for (; e + 4 <= e1; e += 4) {
+        int32_t iu[4];
+        float wu[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            iu[u] = idx[e + u];
+            wu[u] = w[e + u];
+        }
+        float4 xu[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xu[u] = *(const float4*)(X + (int64_t)iu[u] * F0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            m0 += wu[u] * xu[u].x;
+            m1 += wu[u] * xu[u].y;
+            m2 += wu[u] * xu[u].z;
+            m3 += wu[u] * xu[u].w;
+        }
+    }
+    for (; e < e1; ++e) {
         const float we = w[e];
         const float4 xu = *(const float4*)(X + (int64_t)idx[e] * F0);
         m0 += we * xu.x;
@@ -258,7 +279,53 @@ __global__ __launch_bounds__(256) void k_infer_layer(const int32_t* __restrict__
 #pragma unroll
         for (int i = 0; i < 8; ++i) acc[i] = 0.f;
         if (v < N) {
-            for (int32_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+            // eight neighbours at a time: their indices, then their rows, in flight together
+            int32_t e = ptr[v];
+            const int32_t e1 = ptr[v + 1];
+            for (; e + 8 <= e1; e += 8) {
+                int32_t iu[8];
+                float wu[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    iu[u] = idx[e + u];
+                    wu[u] = w[e + u];
+                }
+                uint4 hv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) hv[u] = *(const uint4*)(Hin + (int64_t)iu[u] * HID + q * 8);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t hw[4] = {hv[u].x, hv[u].y, hv[u].z, hv[u].w};
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        acc[2 * i] += wu[u] * bf2f(hw[i] & 0xFFFFu);
+                        acc[2 * i + 1] += wu[u] * bf2f(hw[i] >> 16);
+                    }
+                }
+            }
+            if (e + 4 <= e1) {
+                int32_t iu[4];
+                float wu[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    iu[u] = idx[e + u];
+                    wu[u] = w[e + u];
+                }
+                uint4 hv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) hv[u] = *(const uint4*)(Hin + (int64_t)iu[u] * HID + q * 8);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t hw[4] = {hv[u].x, hv[u].y, hv[u].z, hv[u].w};
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        acc[2 * i] += wu[u] * bf2f(hw[i] & 0xFFFFu);
+                        acc[2 * i + 1] += wu[u] * bf2f(hw[i] >> 16);
+                    }
+                }
+                e += 4;
+            }
+            for (; e < e1; ++e) {
                 const float we = w[e];
                 const uint4 hv = *(const uint4*)(Hin + (int64_t)idx[e] * HID + q * 8);
                 const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
