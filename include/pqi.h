@@ -99,7 +99,11 @@ typedef struct {
 /* pqi_search_params.flags */
 #define PQI_FLAG_TIME_KERNELS 1 /* time the descent and main walk kernels with HIP events on the context stream
                                    (batched and learned searches: the descent launches of the whole call, the
-                                   same totals in every instance record) */
+                                   same totals in every instance record).  The walk is timed only where it is
+                                   launched on its own (host-decided loop, segment graphs); on the int8 fast
+                                   path and with the whole iteration captured as one graph it runs inside the
+                                   iteration (events inside a graph cannot be timed on this ROCm), so
+                                   walk_kernel_seconds and walk_launches stay 0 there */
 #define PQI_FLAG_HOST_LOOP 2    /* decide each phase on the host (one synchronisation per phase) instead of
                                    on the device (one per loop iteration); same result */
 #define PQI_FLAG_TEST_DRIFT 4   /* test hook (real weights): perturb the maintained field by
@@ -162,7 +166,13 @@ PQI_API int pqi_graph_get_info(const pqi_graph* g, pqi_graph_info* info);
 /* Candidate scoring.  Replaces run_cut_opencl + the calculate_cut(_sparse)
  * kernels (pyqrackising/maxcut_tfim.py:212-285, maxcut_tfim_sparse.py:227-306,
  * kernels.cl:44-94, 233-285).  bits: B x ceil(n/32) uint32.  values (optional, B)
- * receives cut(x_b) or E(x_b); *best_index the first maximum. */
+ * receives cut(x_b) or E(x_b); *best_index the first maximum.  Engine
+ * PQI_ENGINE_AUTO: exact for int8 and fp64 weights; for fp32 weights the
+ * split-BF16 engine, whose values are approximate, within
+ * 2e-5 * sum_{i<j} |w_ij| of the FP64 value (each weight keeps 2^-18 |w| after
+ * the hi/lo split, the rest is FP32 accumulation); *best_index and
+ * *best_value come from FP64 re-scores of the 16 best.  PQI_ENGINE_EXACT gives
+ * FP64 values for every candidate. */
 PQI_API int pqi_eval_batch(pqi_graph* g, const uint32_t* bits, int64_t B, int is_spin_glass,
                    double* values, int64_t* best_index, double* best_value);
 

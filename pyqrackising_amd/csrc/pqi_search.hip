@@ -2911,14 +2911,19 @@ __device__ void ls_report_body(const uint8_t* x, const int32_t* h, int64_t n, in
             rep->ctl = *c;
             rep->fix = *fix;
         }
-        // every store of wave 0 (the state and the report) before the number
+        // A fence orders only its own thread's stores: every thread that
+        // stored part of the state or the report (wave 0) fences them at
+        // system scope, and the barrier below puts all of that before lane
+        // 0's store of the number.  Moving the copy to other threads needs
+        // them fenced here too.
         __threadfence_system();
-        if (threadIdx.x == 0) {
-            const unsigned long long k = *seq + 1;
-            *seq = k;
-            __hip_atomic_store(&rep->seq, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            ls_begin_body(c);
-        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long k = *seq + 1;
+        *seq = k;
+        __hip_atomic_store(&rep->seq, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        ls_begin_body(c);
     }
 }
 
