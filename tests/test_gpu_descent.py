@@ -116,6 +116,49 @@ def test_descent_sparse_real():
     assert gain == pytest.approx(gr, rel=1e-9)
 
 
+def test_descent_sparse_real_ties():
+    """Sparse real weights in {+-0.5, +-1.5} (fp32, so the FP64 gain path): gains
+    are exact binary fractions with many exact ties, which the round descent
+    must break like the full scan (lowest node among equal gains), including
+    ties at its threshold.  Exact equality with the textbook ascent."""
+    log("descent sparse real with ties")
+    rng = np.random.default_rng(21)
+    n = 6000
+    lo, hi = graphs.erdos_renyi_edges_sparse(n, 6.0 / (n - 1), rng)
+    w = rng.choice([-1.5, -0.5, 0.5, 1.5], lo.size)
+    G = graphs.upper_csr(n, lo, hi, w, dtype=np.float32)
+    A = (G + G.T).tocsr().astype(np.float64)
+    A.sort_indices()
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = rng.integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, False)
+    xr, fr, gr = steepest(A, x0)
+    assert acc == fr and gain == gr
+    assert np.array_equal(x1, xr)
+
+
+def test_descent_sparse_int_beyond_key_range():
+    """CSR int8 weights whose row strengths exceed the packed-key range
+    (n = 4096, about 2100 neighbours of weight +-127 per node): the descent
+    keeps int32 gains in LDS and runs in rounds on integer keys (ties are
+    frequent).  Exact equality with the textbook ascent."""
+    log("descent sparse int beyond the key range")
+    rng = np.random.default_rng(22)
+    n = 4096
+    lo, hi = graphs.erdos_renyi_edges_sparse(n, 2100.0 / (n - 1), rng)
+    w = 127.0 * rng.choice([-1.0, 1.0], lo.size)
+    G = graphs.upper_csr(n, lo, hi, w, dtype=np.float32)
+    A = (G + G.T).tocsr().astype(np.int64)
+    A.sort_indices()
+    assert np.abs(A).sum(axis=1).max() >= 2 ** 18  # beyond 2^(30 - ib), ib = 12
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = rng.integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, False)
+    xr, fr, gr = steepest(A, x0)
+    assert acc == fr and gain == gr
+    assert np.array_equal(x1, xr)
+
+
 def test_descent_small_against_oracle(orc):
     """Oracle best single flip, applied until no gain, on small int graphs."""
     rng = np.random.default_rng(8)
