@@ -666,7 +666,7 @@ __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
     RoundLds L;
     size_t o = 0;
     L.d = o;
-    o += al((size_t)n * sizeof(FT));
+    o += al((size_t)(n + 1) * sizeof(FT));  // + a sentinel below every gain
     L.x = o;
     o += al((size_t)n);
     L.trk = o;
@@ -704,6 +704,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     auto fmax_ = [](FT a, FT b) { return vmax(a, b); };
     stage_gains<FT>(xg, hg, n, x, D, trk);
     stage_words(g.ap, n + 1, AP);
+    if (lane == 0) D[n] = lowest<FT>();  // the sentinel: empty candidate slots point here
     __syncthreads();
     int64_t steps = 0;
     double gsum = 0.0;
@@ -717,21 +718,10 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     auto flip = [&](int k, FT gk, int xk, int e0, int deg, int slot, bool tracking) -> bool {
         const int sk = xk ? -1 : 1;  // s_k before the flip
         bool fits = true;
-        for (int j0 = 0; j0 < deg; j0 += 64) {
-            const int j = j0 + lane;
+        // one neighbour per active lane: update its gain, track it if it rose above T
+        auto update = [&](bool act, int v, WT w) {
             bool add = false;
-            int v = 0;
-            if (j < deg) {
-                WT w;
-                if (slot >= 0 && j < RS) {
-                    // through LDS-qualified pointers, so that the two sources
-                    // stay separate loads (not one flat load of either)
-                    v = ((const __attribute__((address_space(3))) int32_t*)RCI)[slot * RS + j];
-                    w = ((const __attribute__((address_space(3))) WT*)RCW)[slot * RS + j];
-                } else {
-                    v = g.ai[e0 + j];
-                    w = g.aw[e0 + j];
-                }
+            if (act) {
                 const uint8_t xv = x[v], tv = trk[v];
                 const int sv = xv ? -1 : 1;
                 const FT nd = D[v] - (FT)(2 * sk * sv) * (FT)w;
@@ -739,17 +729,46 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
                 add = tracking && !tv && nd > T;
             }
             const uint64_t m = __ballot(add);
-            const int cnt = __popcll(m);
-            if (nx + cnt > RD_X) {
-                fits = false;
-            } else {
-                if (add) {
-                    const int pos =
-                        nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    EX[pos] = v;
-                    trk[v] = 1;
+            if (m) {
+                const int cnt = __popcll(m);
+                if (nx + cnt > RD_X) {
+                    fits = false;
+                } else {
+                    if (add) {
+                        const int pos = nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        EX[pos] = v;
+                        trk[v] = 1;
+                    }
+                    nx += cnt;
                 }
-                nx += cnt;
+            }
+        };
+        if (slot >= 0 && deg <= RS) {
+            // the whole row is cached: one entry per lane, no loop
+            const bool act = lane < deg;
+            const int c = slot * RS + (act ? lane : 0);
+            // through LDS-qualified pointers, so that cached and global
+            // entries stay separate loads (not one flat load of either)
+            const int v = ((const __attribute__((address_space(3))) int32_t*)RCI)[c];
+            const WT w = ((const __attribute__((address_space(3))) WT*)RCW)[c];
+            update(act, v, w);
+        } else {
+            for (int j0 = 0; j0 < deg; j0 += 64) {
+                const int j = j0 + lane;
+                const bool act = j < deg;
+                int v = 0;
+                WT w = (WT)0;
+                if (act) {
+                    if (slot >= 0 && j < RS) {
+                        v = ((const __attribute__((address_space(3))) int32_t*)RCI)[slot * RS + j];
+                        w = ((const __attribute__((address_space(3))) WT*)RCW)[slot * RS + j];
+                    } else {
+                        v = g.ai[e0 + j];
+                        w = g.aw[e0 + j];
+                    }
+                }
+                update(act, v, w);
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -886,52 +905,73 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         int64_t round_flips = 0;
         int myex = -1;  // this lane's extra (EX[lane])
         for (;;) {
-            // the lane's first maximum over its tracked nodes: candidates
-            // (ascending nodes, so a strict > keeps the lowest) and extra
-            FT bd = lowest<FT>();
-            int bn = -1, br = 0;
+            // the tracked gains: the lane's candidates (ascending nodes; empty
+            // slots read the sentinel) and its extra, all reads in flight together
+            FT d[RD_C + 1];
 #pragma unroll
-            for (int r = 0; r < RD_C; ++r) {
-                const FT dr = D[cn[r] >= 0 ? cn[r] : 0];
-                if (cn[r] >= 0 && dr > bd) {
-                    bd = dr;
-                    bn = cn[r];
-                    br = r;
-                }
+            for (int r = 0; r < RD_C; ++r) d[r] = D[cn[r] >= 0 ? cn[r] : n];
+            d[RD_C] = D[myex >= 0 ? myex : n];
+            // the wave's first maximum: a maximum of the gain keys, and when
+            // exactly one tracked node holds that key it is the winner;
+            // otherwise exact comparisons among the nodes holding it
+            int32_t kr[RD_C + 1];
+#pragma unroll
+            for (int r = 0; r <= RD_C; ++r) kr[r] = gain_key(&d[r], 0);
+            int32_t kl = kr[0];
+#pragma unroll
+            for (int r = 1; r <= RD_C; ++r) kl = max(kl, kr[r]);
+            const int32_t K = wave_max_i32(kl);
+            uint64_t mr[RD_C + 1];
+            int hits = 0;
+#pragma unroll
+            for (int r = 0; r <= RD_C; ++r) {
+                mr[r] = __ballot(kr[r] == K);
+                hits += __popcll(mr[r]);
             }
-            {
-                const FT de = D[myex >= 0 ? myex : 0];
-                if (myex >= 0 && (de > bd || (de == bd && myex < bn))) {
-                    bd = de;
-                    bn = myex;
-                    br = RD_C;
+            int rw = 0, w = 0;
+            if (hits == 1) {
+#pragma unroll
+                for (int r = RD_C; r >= 0; --r)
+                    if (mr[r]) rw = r;
+                w = (int)__builtin_ctzll(mr[rw]);
+            } else {
+                // several nodes share the top key: the largest gain among them,
+                // then the lowest node
+                FT bd = lowest<FT>();
+                int bn = INT_MAX, br = 0;
+#pragma unroll
+                for (int r = 0; r <= RD_C; ++r) {
+                    const int nd = r < RD_C ? cn[r] : myex;
+                    if (kr[r] == K && (d[r] > bd || (d[r] == bd && nd < bn))) {
+                        bd = d[r];
+                        bn = nd;
+                        br = r;
+                    }
                 }
+                const FT mm = wave_reduce_dpp(bd, fmax_);
+                const int c = bd == mm ? bn : INT_MAX;
+                const int mn = wave_reduce_dpp(c, [](int a, int b) { return a < b ? a : b; });
+                const uint64_t eqm = __ballot(bn == mn && bd == mm);
+                if (eqm == 0) break;  // no tracked node at all (cannot happen in a round)
+                w = (int)__builtin_ctzll(eqm);
+                rw = __builtin_amdgcn_readlane(br, w);
             }
-            // the wave's first maximum: a maximum over the gain keys, exact
-            // comparisons only where keys tie
-            const int32_t lkey = bn >= 0 ? gain_key(&bd, 0) : INT_MIN;
-            const int32_t K = wave_max_i32(lkey);
-            uint64_t eq = __ballot(bn >= 0 && lkey == K);
-            if (__popcll(eq) > 1) {
-                const bool in = (eq >> lane) & 1;
-                const FT mm = wave_reduce_dpp(in ? bd : lowest<FT>(), fmax_);
-                eq = __ballot(in && bd == mm);
-                if (__popcll(eq) > 1) {
-                    const bool in2 = (eq >> lane) & 1;
-                    const int mn = wave_reduce_dpp(in2 ? bn : INT_MAX, [](int a, int b) { return a < b ? a : b; });
-                    eq = __ballot(in2 && bn == mn);
+            // the winner's gain and node (rw is wave-uniform)
+            FT dsel = d[0];
+            int nsel = cn[0];
+#pragma unroll
+            for (int r = 1; r <= RD_C; ++r)
+                if (rw == r) {
+                    dsel = d[r];
+                    nsel = r < RD_C ? cn[r] : myex;
                 }
-            }
-            if (eq == 0) break;
-            const int w = (int)__builtin_ctzll(eq);
-            const FT m = readlane_t(bd, w);
-            const int k = __builtin_amdgcn_readlane(bn, w);
+            const FT m = readlane_t(dsel, w);
+            const int k = __builtin_amdgcn_readlane(nsel, w);
             if (!(m > T)) break;  // an untracked node may hold the maximum: rescan
             if (!(m > 0) || steps >= max_steps) {
                 term = true;
                 break;
             }
-            const int rw = __builtin_amdgcn_readlane(br, w);
             int slot = -1, e0, deg;
             if (rw < RD_C) {
                 int ce = ce0[0], cg = cdg[0];
