@@ -311,7 +311,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
 
     // fields and starting values
     BCHK(upload_list(list));
-    launch_b_field(kind, d_inst, d_list, count, max_n, s);
+    launch_b_field(kind, d_inst, d_list, count, max_n, all_sparse, s);
     BCHK(score(list));
     for (int64_t i = 0; i < count; ++i) {
         const pqi_graph* g = graphs[i];
@@ -396,7 +396,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             // a new round: the fp64 fields are recomputed exactly (as the
             // single-instance search does), so drift never carries over
             BCHK(upload_list(active));
-            launch_b_field(kind, d_inst, d_list, (int64_t)active.size(), max_n, s);
+            launch_b_field(kind, d_inst, d_list, (int64_t)active.size(), max_n, all_sparse, s);
             for (int32_t i : active) stats[i].field_checks++;
         }
 
@@ -622,7 +622,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
 
     // exact final scores (field recomputed from the state, as pqi_score does) and states
     BCHK(upload_list(list));
-    launch_b_field(kind, d_inst, d_list, count, max_n, s);
+    launch_b_field(kind, d_inst, d_list, count, max_n, all_sparse, s);
     BCHK(score(list));
     for (int64_t i = 0; i < count; ++i) {
         const pqi_graph* g = graphs[i];

@@ -364,7 +364,8 @@ struct BInstRaw {
 };
 
 // which = 0: the best state (x, h); 1: the reheat state (rx, rh)
-void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
+void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int all_sparse,
+                    hipStream_t s);
 // all_sparse: every listed instance updates through adjacency lists (enables the
 // two-level argmax variant); key_lds > 0: every instance has key_shift > 0 and
 // the packed-key descent runs with that much dynamic LDS; reg_r > 0: every

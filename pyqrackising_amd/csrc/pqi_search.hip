@@ -3356,11 +3356,15 @@ __global__ __launch_bounds__(256) void kb_field(const BInstRaw* insts, const int
     }
 }
 
+// Sparse rows take a thread each (max_n / 256 work groups per instance);
+// dense rows a wave each, in a loop over the grid, so a grid sized for
+// sparse rows also covers dense instances.
 template <typename WT>
-static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
+static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int all_sparse,
+                         hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
-    hipLaunchKernelGGL((kb_field<WT, FT>), dim3((unsigned)((max_n + 3) / 4), (unsigned)L), dim3(256), 0, s, d,
-                       list);
+    const int64_t gx = all_sparse ? (max_n + 255) / 256 : (max_n + 3) / 4;
+    hipLaunchKernelGGL((kb_field<WT, FT>), dim3((unsigned)gx, (unsigned)L), dim3(256), 0, s, d, list);
 }
 
 template <typename WT>
@@ -3466,8 +3470,9 @@ static void b_quad_impl(const BInstRaw* d, const int32_t* list, int64_t L, doubl
         }                                                           \
     } while (0)
 
-void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
-    PQI_KIND_DISPATCH(kind, b_field_impl, d, list, L, max_n, s);
+void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int all_sparse,
+                    hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_field_impl, d, list, L, max_n, all_sparse, s);
 }
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
                       int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
