@@ -185,7 +185,9 @@ def headline_c5(steps, warmup, log, with_cpu):
         short = [i for i, st in enumerate(lst) if st["max_energy"] < targets[i]]
         fb = []
         if short:
-            _, fb = _lib.batch_local_search([dgs[i] for i in short], [starts[i] for i in short], Pt)
+            # the plain batched search from the same starts, to the same targets
+            _, fb = _lib.learned_search(solver.net, [dgs[i] for i in short], [starts[i] for i in short], Pt,
+                                        targets=targets[short], passes=0)
         return lst, fb, short
 
     for w in range(warmup):

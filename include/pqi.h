@@ -302,7 +302,9 @@ PQI_API int pqi_mpnn_train_step(pqi_mpnn* m, const float* feats, const float* la
  * the loop of pqi_batch_local_search.  The network must be bound to exactly
  * these graphs, in this order.  targets (optional, count values: energy in
  * spin-glass mode, cut otherwise): an instance stops once it reaches its own.
- * p->max_rounds < 0 stops after the learned phase (no round of the loop). */
+ * p->max_rounds < 0 stops after the learned phase (no round of the loop).
+ * passes = 0 runs the plain batched loop (with targets, if given) and does
+ * not use the network, which may then be bound to other graphs. */
 PQI_API int pqi_learned_search(pqi_ctx* ctx, pqi_mpnn* net, pqi_graph* const* graphs, int64_t count,
                                uint8_t* const* thetas, const pqi_search_params* p, const double* targets,
                                int64_t passes, float min_score, pqi_search_stats* stats);

@@ -658,7 +658,9 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
                        pqi_search_stats* stats) {
     if (!net) return fail(PQI_ERR_INVALID, "null network");
     if (passes < 0) return fail(PQI_ERR_INVALID, "passes must be >= 0");
-    return run_batched(c, graphs, count, thetas, p, stats, net, targets, passes, min_score);
+    // no learned pass: the plain loop (with targets), the network is not used
+    pqi_mpnn* use = passes > 0 || (p && p->max_rounds < 0) ? net : nullptr;
+    return run_batched(c, graphs, count, thetas, p, stats, use, targets, passes, min_score);
 }
 
 }  // extern "C"

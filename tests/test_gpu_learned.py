@@ -177,3 +177,26 @@ def test_trained_network_gain():
         log(f"  trained over {other}: +{(tr.mean() / o.mean() - 1) * 100:.2f} % mean energy, higher on {wins}/32")
         assert tr.mean() > 1.005 * o.mean()
         assert wins >= 20
+
+
+def test_passes_zero_is_the_plain_search_to_targets():
+    """passes = 0: the plain batched loop with targets, the network unused (here
+    bound to other graphs).  With the plain search's own final energies as
+    targets it stops at the same states, with the same accepted moves, having
+    skipped only the last, non-improving iteration."""
+    log("learned search with passes=0 on graphs the network is not bound to")
+    rng = np.random.default_rng(93)
+    Gs = [er_uniform(int(rng.integers(200, 600)), 8.0, 800 + t) for t in range(6)]
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in Gs]
+    net = _lib.Mpnn(seed=3)
+    net.bind([_lib.DeviceGraph.csr(er_uniform(100, 4.0, 5))])
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=600)
+    xs, plain = _lib.batch_local_search(dgs, starts, p)
+    targets = np.array([s["max_energy"] for s in plain])
+    ys, st = _lib.learned_search(net, dgs, starts, p, targets=targets, passes=0)
+    for i in range(len(Gs)):
+        assert np.array_equal(ys[i], xs[i]), i
+        assert st[i]["max_energy"] == targets[i] and st[i]["learned_passes"] == 0
+        for k in ("single_accepts", "double_accepts", "seed_accepts", "walk_accepts", "reheat_accepts"):
+            assert st[i][k] == plain[i][k], (i, k)
