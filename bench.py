@@ -43,7 +43,7 @@ PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 # FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy of known size (see
 # the files)
 C5_TRAFFIC_FILE = os.path.join(PROFILES, "kb_descent_lds_c5_64x8192.md")
-C2_TRAFFIC_FILE = os.path.join(PROFILES, "k_descent_reg_c2_n2048.md")
+C2_TRAFFIC_FILE = os.path.join(PROFILES, "k_descent_plane_c2_n2048.md")
 
 
 def measured_traffic(path):
@@ -292,7 +292,7 @@ def c2_local_search(steps, warmup, log, with_cpu, cpu_full, walkers=128, seed_mu
     evals = st["single_evals"] + st["double_evals"] + st["gray_evals_computed"]
     total_evals = sum(s["single_evals"] + s["double_evals"] + s["gray_evals_computed"] for s in stats)
     total_ref = sum(s["single_evals"] + s["double_evals"] + s["gray_evals"] for s in stats)
-    # dominant kernel: the single-flip descent (k_descent_reg); algorithmic
+    # dominant kernel: the single-flip descent (k_descent_plane); algorithmic
     # bytes per flip evaluation: the gain (4 B int32) and the spin (1 B); per
     # accepted flip, 16 B per neighbour (index and weight read, gain read and
     # written) -- degree 3 here
@@ -317,14 +317,14 @@ def c2_local_search(steps, warmup, log, with_cpu, cpu_full, walkers=128, seed_mu
         "phase_seconds": {k: st[k] for k in ("seconds_single", "seconds_double", "seconds_seeds", "seconds_walk",
                                               "seconds_reheat")},
         "roofline": {
-            "kernel": "k_descent_reg<32, 4>",
+            "kernel": "k_descent_plane<1, 2, 3, 2>",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(C2_TRAFFIC_FILE),
-            "traffic_source": "profiles/k_descent_reg_c2_n2048.md",
+            "traffic_source": "profiles/k_descent_plane_c2_n2048.md",
             "avg_launch_ms": avg_launch_s * 1e3,
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },

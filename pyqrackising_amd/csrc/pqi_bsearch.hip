@@ -173,6 +173,24 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             reg_lds = std::max(reg_lds, t);
         }
     }
+    // bit-plane descent when every instance qualifies: planes for the
+    // largest row strength (max_n decides the words per lane)
+    int plane_s = -1;
+    if (reg_r > 0) {
+        int S = -1;
+        for (int64_t i = 0; i < count; ++i) {
+            if (!plane_descent_plan(*graphs[i], nullptr)) {
+                S = -1;
+                break;
+            }
+            S = std::max<int>(S, (int)graphs[i]->max_strength);
+        }
+        if (S >= 0) {
+            int half = 1;
+            for (int64_t i = 0; i < count; ++i) half &= plane_half(*graphs[i], S);
+            plane_s = 2 * S + half;
+        }
+    }
     const int64_t L0 = count;
     // shared records, one per instance in instance order, so that a phase's
     // results come back in one copy: phase records, top seed gains and walker
@@ -407,7 +425,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             const int64_t cap = 16 * max_n + 4096;
             BCHK(upload_list(run));
             launch_b_descent(kind, d_inst, d_list, (int64_t)run.size(), max_n, cap, 0, all_sparse, key_lds, reg_r,
-                             reg_stride, reg_lds, s, dev0, dev1);
+                             reg_stride, reg_lds, plane_s, s, dev0, dev1);
             BCHK(read_outs(run));
             descent_time();
             desc_pairs += (int64_t)run.size();
@@ -546,7 +564,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             BCHK(copies(rest3, wsrc, 1));
             BCHK(upload_list(rest3));
             launch_b_descent(kind, d_inst, d_list, (int64_t)rest3.size(), max_n, 0, 1, all_sparse, key_lds, reg_r,
-                             reg_stride, reg_lds, s, dev0, dev1);
+                             reg_stride, reg_lds, plane_s, s, dev0, dev1);
             BCHK(read_outs(rest3));
             descent_time();
             desc_pairs += (int64_t)rest3.size();  // best-only descents: no move, no write

@@ -27,6 +27,8 @@ struct Graph {
     int64_t nnz = 0;
     int64_t max_degree = 0;
     int64_t max_strength = -1;   // int8 weights: max_v sum_u |w_vu| (exact); -1 otherwise
+    bool unit_abs = false;       // int8 weights whose nonzeros are all +-1 (strength = degree)
+    int degree_parity = -1;      // 0: every degree even, 1: every degree odd, -1: mixed
     // int8 weights with adjacency lists: fixed-stride table, row v at v * stride,
     // entries (w << 24) | u, zero padded; stride = max degree rounded up so that
     // n * stride is a multiple of 4 (nullptr when not built)
@@ -330,6 +332,10 @@ size_t key_lds_bytes(const Graph& g, size_t limit);
 // Registers per lane of the register-resident packed-key descent of g (0: it
 // does not apply); *table_bytes receives the size of its neighbour table.
 int reg_descent_plan(const Graph& g, size_t* table_bytes);
+// words per lane (1 or 2) of the bit-plane descent for g, or 0 when it does not apply
+int plane_descent_plan(const Graph& g, size_t* table_bytes);
+// 1 when every degree of g has the parity of S (bit-plane values halved)
+int plane_half(const Graph& g, int64_t S);
 
 // ---- batched search: per-instance descriptors in device memory ----------------
 struct BInstRaw {
@@ -371,10 +377,12 @@ void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
 // the packed-key descent runs with that much dynamic LDS; reg_r > 0: every
 // instance takes the register-resident descent with reg_r registers per lane,
 // neighbour rows of at most reg_stride entries and neighbour tables of up to
-// reg_lds bytes (staged in LDS where they fit)
+// reg_lds bytes (staged in LDS where they fit); plane_s >= 0 (with reg_r > 0):
+// every instance takes the bit-plane descent, plane_s = 2 S + H with S the
+// largest row strength and H = 1 when every degree has the parity of S
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
                       int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
-                      hipStream_t s,
+                      int plane_s, hipStream_t s,
                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // dynamic LDS limit of the batched packed-key descent
 size_t b_key_lds_limit();

@@ -188,6 +188,11 @@ static int upload_adjacency(pqi_graph* g, const std::vector<int32_t>& ptr, const
     g->max_degree = md;
     if (g->kind == PQI_WEIGHTS_I8) {
         g->max_strength = (int64_t)ms;
+        g->unit_abs = std::all_of(w.begin(), w.end(), [](double v) { return std::fabs(v) == 1.0; });
+        int par = ptr.size() > 1 ? (int)((ptr[1] - ptr[0]) & 1) : 0;
+        for (size_t i = 0; i + 1 < ptr.size() && par >= 0; ++i)
+            if ((int)((ptr[i + 1] - ptr[i]) & 1) != par) par = -1;
+        g->degree_parity = par;
         // fixed-stride table for the packed-key descent when padding stays small
         const int64_t n = (int64_t)ptr.size() - 1;
         if (md > 0 && md <= 64 && n < (1 << 24) && n * md <= 4 * nnz) {

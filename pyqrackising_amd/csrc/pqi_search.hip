@@ -1363,6 +1363,320 @@ __global__ __launch_bounds__(64) void k_descent_reg(GView<int8_t> g, uint8_t* xg
     descent_reg_body<R, NB>(g, xg, hg, max_steps, out, lds, lds_bytes, ib);
 }
 
+// Bit-plane form of the descent for graphs whose nonzero weights are all +-1
+// (random regular +-1 graphs, unit weights), n <= 2048 NW, with a neighbour
+// table of at most NB entries per row.  Node v's gain is held as
+// c_v = (Delta_v + S) >> HS in [0, 2S >> HS] (S: the largest row strength,
+// which for +-1 weights is the largest degree; HS = 1 (H >= 1) when every
+// degree has the parity of S, so that Delta_v + S is always even; H = 2 when
+// also S = 2^NP - 1, so that a flipped node's -Delta_k + S is the complement
+// of its planes): bit b of c_v is bit
+// (v & 31) of plane word P[b][v >> 11] of lane (v >> 5) & 63, and its spin is
+// the same bit of X.  A step's first maximum comes from the planes, most
+// significant first: the remaining nodes whose bit b is set replace them
+// when there are any (one ballot per plane), and the lowest remaining node is
+// the first maximum of pyqrackising/spin_glass_solver.py:45 (a ballot, then
+// the lowest bit of one lane's word).  A flip of k changes each neighbour's
+// gain by -2 w s_k s_u = +-2: the neighbours to raise and to lower are
+// gathered into two bit masks and applied as one bit-sliced increment and
+// decrement of (Delta + S) >> 1, and k's own planes are rewritten with
+// -Delta_k.  A step costs O(NP + degree) vector instructions for all n gains.
+// Rows of at most 3 entries (NB = 3, one word per lane) are held in
+// registers: entry j of node v in register v & 31 of bank j of lane v >> 5,
+// read by a uniform index and one readlane.  Otherwise, while the step's last
+// ballots run, every lane loads the neighbour row of its own lowest remaining
+// node from the table in LDS, so the winning lane holds the row of k.  Same
+// moves, counters and result as descent_reg_body.
+template <int NW, int NP, int NB, int H>
+__device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                   PhaseOut* out, unsigned char* lds, size_t lds_bytes, int S) {
+    constexpr int HS = H ? 1 : 0;  // values halved
+    const int lane = threadIdx.x;
+    const int n = (int)g.n;
+    const int stride = (int)g.kstride;
+    uint32_t P[NP][NW], X[NW], V[NW];
+    const bool aligned = (((uintptr_t)xg | (uintptr_t)hg) & 15) == 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const int base = w * 2048 + lane * 32;
+        int32_t cv[32];
+        uint32_t xb = 0;
+        if (aligned && base + 32 <= n) {
+            const uint4* xs = (const uint4*)(xg + base);
+            const int4* hs = (const int4*)(hg + base);
+            const uint4 xa = xs[0], xc = xs[1];
+            int4 hq[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) hq[q] = hs[q];
+            const uint32_t xw[8] = {xa.x, xa.y, xa.z, xa.w, xc.x, xc.y, xc.z, xc.w};
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const uint32_t xv = ((xw[r >> 2] >> (8 * (r & 3))) & 0xFFu) != 0;
+                const int4 hp = hq[r >> 2];
+                const int32_t hv = (r & 3) == 0 ? hp.x : (r & 3) == 1 ? hp.y : (r & 3) == 2 ? hp.z : hp.w;
+                cv[r] = ((xv ? -hv : hv) + S) >> HS;
+                xb |= xv << r;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const int v = base + r;
+                cv[r] = 0;
+                if (v < n) {
+                    const uint32_t xv = xg[v] != 0;
+                    cv[r] = ((xv ? -hg[v] : hg[v]) + S) >> HS;
+                    xb |= xv << r;
+                }
+            }
+        }
+        X[w] = xb;
+        V[w] = base >= n ? 0u : (base + 32 <= n ? 0xFFFFFFFFu : ((1u << (n - base)) - 1u));
+#pragma unroll
+        for (int b = 0; b < NP; ++b) {
+            uint32_t pb = 0;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) pb |= (((uint32_t)cv[r] >> b) & 1u) << r;
+            P[b][w] = pb;
+        }
+    }
+    // rows of at most 3 entries in registers: three banks of 32, separate
+    // arrays so that a uniform dynamic index into each stays a register move.
+    // Entries with w = 0 (padding, past the stride, nodes past n) are stored
+    // as node 2048, whose lane field (u >> 5 = 64) matches no lane: a no-op
+    constexpr bool RR = NB == 3;
+    static_assert(!RR || NW == 1, "register rows hold one word of nodes per lane");
+    int32_t rr0[RR ? 32 : 1], rr1[RR ? 32 : 1], rr2[RR ? 32 : 1];
+    if constexpr (RR) {
+        auto rr_set = [&](int j, int r, int32_t v) {
+            v = (v >> 24) != 0 ? v : 2048;
+            if (j == 0) rr0[r] = v;
+            else if (j == 1) rr1[r] = v;
+            else rr2[r] = v;
+        };
+        const int base = lane * 32;
+        if (max_steps > 0 && stride == 3 && base + 32 <= n) {
+            // 96 consecutive entries in 16-byte pieces (rows start at 12 v bytes)
+            const int4* src = (const int4*)(g.kadj + base * 3);
+            int4 q[24];
+#pragma unroll
+            for (int i = 0; i < 24; ++i) q[i] = src[i];
+#pragma unroll
+            for (int r = 0; r < 32; ++r)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int e = 3 * r + j;
+                    const int4 v = q[e >> 2];
+                    rr_set(j, r, (e & 3) == 0 ? v.x : (e & 3) == 1 ? v.y : (e & 3) == 2 ? v.z : v.w);
+                }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 32; ++r)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int v = base + r;
+                    rr_set(j, r, (max_steps > 0 && j < stride && v < n) ? g.kadj[(int64_t)v * stride + j] : 0);
+                }
+        }
+    }
+    // otherwise the neighbour table (rows zero padded after their entries, the
+    // table to whole quads) is staged in LDS when it fits; with max_steps = 0
+    // nothing is staged and no row is used
+    __builtin_assume(stride >= 1);
+    const int quads = (n * stride + 3) >> 2;
+    const bool staged = !RR && max_steps > 0 && (size_t)quads * 16 <= lds_bytes;
+    if (staged) {
+        const int4* src = (const int4*)g.kadj;
+        int4* dst = (int4*)lds;
+#pragma unroll 12
+        for (int q = lane; q < quads; q += 64) dst[q] = src[q];
+    }
+    __syncthreads();
+    const int32_t step_cap = max_steps < INT_MAX ? (int32_t)max_steps : INT_MAX;
+    int32_t steps = 0;
+    int64_t gsum = 0;
+    int bv = 0, bk = 0;
+    for (;;) {
+        int L, p, ws = 0, cm = 0;
+        int32_t nb[NB];
+        {
+            uint32_t M[NW];
+#pragma unroll
+            for (int w = 0; w < NW; ++w) M[w] = V[w];
+#pragma unroll
+            for (int b = NP - 1; b >= 0; --b) {
+                uint32_t t[NW], any = 0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    t[w] = M[w] & P[b][w];
+                    any |= t[w];
+                }
+                const bool hit = __ballot(any != 0) != 0;  // wave-uniform
+#pragma unroll
+                for (int w = 0; w < NW; ++w) M[w] = hit ? t[w] : M[w];
+                cm |= hit ? (1 << b) : 0;
+            }
+            // the lowest remaining node: word ws, lane L, bit p.  n >= 1, so M
+            // is never empty (a plane only narrows it when its bits meet it)
+            uint64_t bal = __ballot(M[0] != 0);
+            uint32_t mw = M[0];
+            int cb = 0;
+            if constexpr (NW > 1) {
+                const uint64_t bal1 = __ballot(M[1] != 0);
+                ws = bal ? 0 : 1;
+                bal = bal ? bal : bal1;
+                cb = M[0] ? 0 : 2048;
+                mw = M[0] ? M[0] : M[1];
+            }
+            // this lane's lowest remaining node and its neighbour row, loaded
+            // while the winner is decided (entries past the stride repeat entry
+            // 0 and are never used; a lane with nothing left reads a valid row
+            // it never uses)
+            if constexpr (!RR) {
+                const int c = min(cb + lane * 32 + (int)__builtin_ctz(mw | 0x80000000u), n - 1);
+                if (staged) {
+                    const int32_t* row = (const int32_t*)lds + c * stride;
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) nb[j] = row[j < stride ? j : 0];
+                } else {
+                    const int32_t* row = g.kadj + (int64_t)c * stride;
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) nb[j] = row[j < stride ? j : 0];
+                }
+            }
+            L = (int)__builtin_ctzll(bal);
+            p = (int)__builtin_ctz(__builtin_amdgcn_readlane(mw, L));
+        }
+        if constexpr (RR) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) nb[j] = j == 0 ? rr0[p] : (j == 1 ? rr1[p] : rr2[p]);  // uniform index
+        }
+        bk = ws * 2048 + L * 32 + p;
+        bv = (cm << HS) - S;
+        uint32_t xl = X[0];
+        if constexpr (NW > 1) xl = ws ? X[1] : X[0];
+        const int32_t xkm = -(int32_t)((__builtin_amdgcn_readlane(xl, L) >> p) & 1u);  // -x_k
+        // The flip's new planes are formed before the stop test (bv <= 0 or
+        // the step cap, one sign test) is taken, so that test is off the
+        // dependency chain; they are kept only when the descent goes on.
+        // Neighbours: Delta_u changes by -2 w s_k s_u, i.e. +2 where
+        // s_u = -w s_k (u on side 1 when w s_k > 0, side 0 otherwise).
+        uint32_t inc[NW], all[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) inc[w] = all[w] = 0u;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (!RR && j >= stride) break;  // register rows: w = 0 entries past the stride
+            const int32_t a = __builtin_amdgcn_readlane(nb[j], L);
+            // lane of u, or 64 for a padding entry (register rows store it as
+            // node 2048; table rows are tested here)
+            const int lu = RR ? (int)__builtin_amdgcn_ubfe((uint32_t)a, 5u, 7u)
+                              : ((a >> 24) != 0 ? ((a >> 5) & 63) : 64);
+            const uint32_t m = lane == lu ? 1u << (a & 31) : 0u;
+            // fx = 0 when w > 0 and x_k = 0 or w < 0 and x_k = 1, else ~0
+            const uint32_t fx = (uint32_t)((a >> 31) ^ xkm);
+            if constexpr (NW == 1) {
+                inc[0] |= m & (X[0] ^ fx);
+                all[0] |= m;
+            } else {
+                const int u = a & 0xFFFFFF;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const uint32_t mm = (u >> 11) == w ? m : 0u;
+                    inc[w] |= mm & (X[w] ^ fx);
+                    all[w] |= mm;
+                }
+            }
+        }
+        const int cn = (S - bv) >> HS;  // k's new value: (-Delta_k + S) >> HS
+        uint32_t Q[NP][NW], Y[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t mk = (lane == L && w == ws) ? 1u << p : 0u;
+#pragma unroll
+            for (int b = 0; b < NP; ++b) {
+                if constexpr (H == 2) {
+                    Q[b][w] = P[b][w] ^ mk;  // S = 2^NP - 1: S - c is the complement
+                } else {
+                    const uint32_t nbits = (uint32_t)-((cn >> b) & 1);
+                    Q[b][w] = (P[b][w] & ~mk) | (mk & nbits);
+                }
+            }
+            Y[w] = X[w] ^ mk;
+            // +2 where inc, -2 where dec: a bit-sliced increment and decrement
+            // of (Delta + S) >> 1, planes 1 - HS and up (the value stays in
+            // range, so no carry leaves the top plane)
+            uint32_t cy = inc[w], bw = all[w] & ~inc[w];
+#pragma unroll
+            for (int b = 1 - HS; b < NP; ++b) {
+                const uint32_t ti = Q[b][w] & cy, td = bw & ~Q[b][w];
+                Q[b][w] ^= cy | bw;
+                cy = ti;
+                bw = td;
+            }
+        }
+        if (((bv - 1) | (step_cap - 1 - steps)) < 0) break;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+#pragma unroll
+            for (int b = 0; b < NP; ++b) P[b][w] = Q[b][w];
+            X[w] = Y[w];
+        }
+        ++steps;
+        gsum += bv;
+    }
+    if (steps) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const int base = w * 2048 + lane * 32;
+            int32_t hv[32];
+            uint32_t xw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                int32_t cr = 0;
+#pragma unroll
+                for (int b = 0; b < NP; ++b) cr |= (int32_t)((P[b][w] >> r) & 1u) << b;
+                const uint32_t xv = (X[w] >> r) & 1u;
+                const int32_t d = (cr << HS) - S;
+                hv[r] = xv ? -d : d;
+                xw[r >> 2] |= xv << (8 * (r & 3));
+            }
+            if (aligned && base + 32 <= n) {
+                int4* hs = (int4*)(hg + base);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) hs[q] = make_int4(hv[4 * q], hv[4 * q + 1], hv[4 * q + 2], hv[4 * q + 3]);
+                uint4* xs = (uint4*)(xg + base);
+                xs[0] = make_uint4(xw[0], xw[1], xw[2], xw[3]);
+                xs[1] = make_uint4(xw[4], xw[5], xw[6], xw[7]);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 32; ++r) {
+                    const int v = base + r;
+                    if (v < n) {
+                        hg[v] = hv[r];
+                        xg[v] = (uint8_t)((xw[r >> 2] >> (8 * (r & 3))) & 1u);
+                    }
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        out->gain = (double)bv;
+        out->k = bk;
+        out->l = -1;
+        out->accepts = steps;
+        out->evals = (int64_t)n * (steps + 1);
+        out->gain_sum = (double)gsum;
+    }
+}
+
+template <int NW, int NP, int NB, int H>
+__global__ __launch_bounds__(64) void k_descent_plane(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                                     PhaseOut* out, size_t lds_bytes, int S, const int32_t* gate) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
+    descent_plane_body<NW, NP, NB, H>(g, xg, hg, max_steps, out, lds, lds_bytes, S);
+}
+
 // Registers per lane of the register-resident descent for n nodes with a
 // neighbour table of `stride` entries per row, or 0 when it does not apply.
 static int reg_descent_regs(int64_t n, int64_t stride) {
@@ -1376,7 +1690,8 @@ static size_t reg_table_bytes(const Graph& g) {
     return (size_t)((g.n * g.key_adj_stride + 3) >> 2) * 16;
 }
 
-static int reg_descent_enabled() { return descent_variant() == 0; }
+// variant 9: the register-resident descent without the bit-plane form
+static int reg_descent_enabled() { return descent_variant() == 0 || descent_variant() == 9; }
 
 template <int R, int NB>
 __global__ __launch_bounds__(64) void k_descent_reg(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
@@ -1398,6 +1713,65 @@ int reg_descent_plan(const Graph& g, size_t* table_bytes) {
 
 // neighbour-row width of the register-resident descent: the stride rounded up to 4, 8 or 16
 static int reg_nb(int64_t stride) { return stride <= 4 ? 4 : (stride <= 8 ? 8 : 16); }
+
+// Bit-plane descent classes: S <= 3, 7, 15, 16 (a +-1 graph's degree is its
+// strength, so rows of at most 3 (4 with two words per lane), 8, 16, 16
+// entries), each with 3 - H, 4 - H, 5 - H, 6 - H planes for values up to
+// 2S >> H.
+static int plane_class(int64_t S) { return S <= 3 ? 0 : (S <= 7 ? 1 : (S <= 15 ? 2 : 3)); }
+
+template <int NW, int NP, int NB, int H>
+__global__ __launch_bounds__(64) void k_descent_plane(GView<int8_t> g, uint8_t* xg, int32_t* hg, int64_t max_steps,
+                                                     PhaseOut* out, size_t lds_bytes, int S, const int32_t* gate);
+
+static size_t plane_lds_cap() {
+    static const size_t cap = lds_limit((const void*)k_descent_plane<2, 6, 16, 0>);
+    return cap;
+}
+
+int plane_descent_plan(const Graph& g, size_t* table_bytes) {
+    if (descent_variant() != 0 || g.kind != PQI_WEIGHTS_I8 || !g.unit_abs || !g.key_adj || g.n < 1 || g.n > 4096 ||
+        g.max_strength < 0 || g.max_strength > 16 || g.key_adj_stride > g.max_strength)
+        return 0;
+    if (reg_table_bytes(g) > plane_lds_cap()) return 0;
+    if (table_bytes) *table_bytes = reg_table_bytes(g);
+    return g.n <= 2048 ? 1 : 2;
+}
+
+// whether every degree of g has the parity of S (then Delta + S is even)
+int plane_half(const Graph& g, int64_t S) { return g.degree_parity >= 0 && g.degree_parity == (int)(S & 1); }
+
+// H of the bit-plane kernels: 2 when the values are halved and S = 2^NP - 1
+// (S = 3, 7, 15 with NP = 2, 3, 4), 1 when only halved, 0 otherwise
+static int plane_h(int half, int64_t S) {
+    if (!half) return 0;
+    const int np = plane_class(S) + 2;
+    return S == (int64_t(1) << np) - 1 ? 2 : 1;
+}
+
+#define PQI_PLANE_H(NW, CLS, NB, H, KERNEL, ...)                         \
+    if ((H) == 2) KERNEL<NW, CLS + 2, NB, 2>(__VA_ARGS__);              \
+    else if ((H) == 1) KERNEL<NW, CLS + 2, NB, 1>(__VA_ARGS__);         \
+    else KERNEL<NW, CLS + 3, NB, 0>(__VA_ARGS__);
+#define PQI_PLANE_CLS(NW, CLS, H, KERNEL, ...)                           \
+    switch (CLS) {                                                      \
+        case 0: PQI_PLANE_H(NW, 0, (NW == 1 ? 3 : 4), H, KERNEL, __VA_ARGS__) break; \
+        case 1: PQI_PLANE_H(NW, 1, 8, H, KERNEL, __VA_ARGS__) break;    \
+        case 2: PQI_PLANE_H(NW, 2, 16, H, KERNEL, __VA_ARGS__) break;   \
+        default: PQI_PLANE_H(NW, 3, 16, H, KERNEL, __VA_ARGS__) break;  \
+    }
+#define PQI_PLANE_DISPATCH(NW, CLS, H, KERNEL, ...)                      \
+    switch (NW) {                                                       \
+        case 1: PQI_PLANE_CLS(1, CLS, H, KERNEL, __VA_ARGS__); break;   \
+        default: PQI_PLANE_CLS(2, CLS, H, KERNEL, __VA_ARGS__); break;  \
+    }
+
+template <int NW, int NP, int NB, int H>
+static void launch_plane(const GView<int8_t>& v, uint8_t* x, int32_t* h, int64_t max_steps, PhaseOut* out,
+                         size_t table, int S, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const int32_t* gate) {
+    hipExtLaunchKernelGGL((k_descent_plane<NW, NP, NB, H>), dim3(1), dim3(64), (uint32_t)table, s, e0, e1, 0, v,
+                          x, h, max_steps, out, table, S, gate);
+}
 
 #define PQI_REG_NB(R, NB, KERNEL, ...)                                   \
     switch (NB) {                                                       \
@@ -1602,6 +1976,13 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
     using FT = typename FieldOf<WT>::T;
     if constexpr (sizeof(WT) == 1) {
         size_t table = 0;
+        const int nw = plane_descent_plan(g, &table);
+        if (nw) {
+            const int S = (int)g.max_strength;
+            PQI_PLANE_DISPATCH(nw, plane_class(S), plane_h(plane_half(g, S), S), launch_plane, view<int8_t>(g), x,
+                               (int32_t*)h, max_steps, out, table, S, s, e0, e1, gate);
+            return;
+        }
         const int regs = reg_descent_plan(g, &table);
         if (regs) {
             PQI_REG_DISPATCH(regs, reg_nb(g.key_adj_stride), launch_reg, view<int8_t>(g), x, (int32_t*)h, max_steps,
@@ -3143,6 +3524,22 @@ __global__ __launch_bounds__(64) void kb_descent_reg(const BInstRaw* insts, cons
                             r.out, lds, lds_bytes, r.key_shift);
 }
 
+template <int NW, int NP, int NB, int H>
+__global__ __launch_bounds__(64) void kb_descent_plane(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
+                                                      int which, size_t lds_bytes, int S) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    descent_plane_body<NW, NP, NB, H>(gview_of<int8_t>(r), which ? r.rx : r.x, (int32_t*)(which ? r.rh : r.h),
+                                      max_steps, r.out, lds, lds_bytes, S);
+}
+
+template <int NW, int NP, int NB, int H>
+static void launch_b_plane(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_steps, int which,
+                           size_t table, int S, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL((kb_descent_plane<NW, NP, NB, H>), dim3((unsigned)L), dim3(64), table, s, e0, e1, 0, d,
+                          list, max_steps, which, table, S);
+}
+
 template <int R, int NB>
 static void launch_b_reg(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_steps, int which,
                          size_t table, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
@@ -3370,8 +3767,14 @@ static void b_field_impl(const BInstRaw* d, const int32_t* list, int64_t L, int6
 template <typename WT>
 static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
                            int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
-                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                           int plane_s, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     using FT = typename FieldOf<WT>::T;
+    if (sizeof(WT) == 1 && reg_r > 0 && plane_s >= 0) {
+        const int S = plane_s >> 1, half = plane_s & 1;
+        PQI_PLANE_DISPATCH(max_n <= 2048 ? 1 : 2, plane_class(S), plane_h(half, S), launch_b_plane, d, list, L,
+                           max_steps, which, reg_lds, S, s, e0, e1);
+        return;
+    }
     if (sizeof(WT) == 1 && reg_r > 0) {
         PQI_REG_DISPATCH(reg_r, reg_nb(reg_stride), launch_b_reg, d, list, L, max_steps, which, reg_lds, s, e0, e1);
         return;
@@ -3476,9 +3879,9 @@ void launch_b_field(int kind, const BInstRaw* d, const int32_t* list, int64_t L,
 }
 void launch_b_descent(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, int64_t max_steps,
                       int which, int all_sparse, size_t key_lds, int reg_r, int64_t reg_stride, size_t reg_lds,
-                      hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                      int plane_s, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     PQI_KIND_DISPATCH(kind, b_descent_impl, d, list, L, max_n, max_steps, which, all_sparse, key_lds, reg_r,
-                      reg_stride, reg_lds, s, e0, e1);
+                      reg_stride, reg_lds, plane_s, s, e0, e1);
 }
 void launch_b_double(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int which, hipStream_t s) {
     PQI_KIND_DISPATCH(kind, b_double_impl, d, list, L, which, s);

@@ -182,6 +182,53 @@ def test_descent_small_against_oracle(orc):
         assert np.array_equal(x1, xo)
 
 
+@pytest.mark.parametrize("n,d,seed", [(2048, 3, 21), (2046, 3, 22), (4096, 3, 23), (1500, 5, 24), (2500, 7, 25),
+                                       (1000, 9, 26), (2000, 15, 27), (2100, 16, 28), (70, 16, 29), (33, 2, 30)])
+def test_descent_bit_planes(n, d, seed):
+    """+-1 weights: the bit-plane descent (gains as Delta + S in 3 to 6
+    planes, one or two words per lane, a partial last word) against the
+    textbook steepest ascent: the same state, flip count and total gain."""
+    log(f"bit-plane descent n={n} d={d}")
+    G, A = regular_int(n, d, seed, 1)
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = np.random.default_rng(seed + 10).integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, False)
+    xr, fr, gr = steepest(A, x0)
+    assert acc == fr and gain == gr
+    assert np.array_equal(x1, xr)
+    # bounded descents continue where the last one stopped
+    x2 = x0
+    for cap in (1, 7, 100000):
+        x2, a2, _ = dg.single_flip_descent(x2, False, max_steps=cap)
+        assert a2 <= cap
+    assert np.array_equal(x2, xr)
+
+
+@pytest.mark.parametrize("n,d,drop,seed", [(2048, 3, 0.1, 31), (1900, 3, 0.3, 32), (3000, 3, 0.1, 33),
+                                            (1200, 10, 0.2, 34), (2048, 16, 0.05, 35)])
+def test_descent_bit_planes_irregular(n, d, drop, seed):
+    """+-1 weights on a regular graph with a share of its edges removed:
+    mixed degree parities (full, not halved, plane values) and rows shorter
+    than the stride (padding entries in the register rows and the table)."""
+    log(f"bit-plane descent irregular n={n} d={d} drop={drop}")
+    rng = np.random.default_rng(seed)
+    lo, hi = graphs.random_regular_edges(n, d, rng)
+    keep = rng.random(lo.size) >= drop
+    lo, hi = lo[keep], hi[keep]
+    w = rng.choice([-1.0, 1.0], lo.size)
+    G = graphs.upper_csr(n, lo, hi, w, dtype=np.float32)
+    A = (G + G.T).tocsr().astype(np.int64)
+    A.sort_indices()
+    deg = np.diff(A.indptr)
+    assert len(set(deg % 2)) == 2  # mixed parities
+    dg = _lib.DeviceGraph.csr(G)
+    x0 = rng.integers(0, 2, n).astype(np.uint8)
+    x1, acc, gain = dg.single_flip_descent(x0, False)
+    xr, fr, gr = steepest(A, x0)
+    assert acc == fr and gain == gr
+    assert np.array_equal(x1, xr)
+
+
 def test_descent_max_steps():
     G, A = regular_int(600, 4, 9)
     dg = _lib.DeviceGraph.csr(G)
