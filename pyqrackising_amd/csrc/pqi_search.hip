@@ -596,8 +596,8 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
 // a full scan only once per round.  The scan gives each lane the 4th largest
 // gain among its nodes; T, the largest of these over the lanes, bounds every
 // node outside the round's candidates {v : D_v > T} (at most 3 per lane).
-// Their neighbour rows are fetched into LDS in one batch of loads.  A flip
-// that raises an untracked neighbour's gain above T adds it to the "extras",
+// Their neighbour rows are fetched into registers in three batches of loads.
+// A flip that raises an untracked neighbour's gain above T adds it to the "extras",
 // so every untracked node stays <= T, and while the best tracked gain is
 // above T its first maximum (lowest node among equal gains) is the first
 // maximum over all n nodes: the move of the full scan
@@ -605,14 +605,14 @@ __device__ void descent_wave_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64
 // When the top gain itself does not exceed T (ties), the scan's own first
 // maximum is flipped and a new round starts.  Single wave: LDS accesses of
 // one wave complete in order, so phases are separated by compiler barriers
-// only.  A flip reads its row from the cache (global memory only for an
-// extra's row, or entries past RD_RS).
+// only.  A flip reads its row from registers (global memory only for an
+// extra's row, or rows longer than RD_RS).
 constexpr int RD_C = 3;    // candidates per lane (nodes above a lane's 4th largest gain)
 constexpr int RD_RS = 16;  // cached entries per candidate row
 constexpr int RD_X = 64;   // extras per round
 
 struct RoundLds {
-    size_t d, x, trk, ap, sn, rci, rcw, ex, total;
+    size_t d, x, trk, ap, sn, ex, total;
 };
 
 __device__ __forceinline__ double readlane_t(double v, int l) {
@@ -638,6 +638,10 @@ __device__ __forceinline__ int32_t gain_key(const double* D, int v) {
 __device__ __forceinline__ int32_t gain_key(const int32_t* D, int v) { return D[v]; }
 __device__ __forceinline__ int32_t hi_key(int32_t hi) { return hi ^ ((hi >> 31) & 0x7fffffff); }
 
+// the key of one gain held in registers
+__device__ __forceinline__ int32_t gain_key_of(double d) { return hi_key(__double2hiint(d)); }
+__device__ __forceinline__ int32_t gain_key_of(int32_t d) { return d; }
+
 // keys of the gains of nodes b and b + 1 (b even; one 16- or 8-byte load)
 __device__ __forceinline__ void gain_key_pair(const double* D, int b, int32_t& k0, int32_t& k1) {
     const int4 q = *(const int4*)(D + b);
@@ -660,13 +664,49 @@ __device__ __forceinline__ void key_bound(int32_t k, double& t) {
 }
 __device__ __forceinline__ void key_bound(int32_t k, int32_t& t) { t = k; }
 
+// The round descent's per-flip chain has two LDS round trips (the tracked
+// gains; the neighbours' spins, flags and gains).  The round's candidates'
+// neighbour rows live in registers: candidate c = lane +
+// 64 r (r < RD_C) has entry j of its row in register c >> 2 (banks of 24) of
+// lane 16 (c & 3) + j, so the winner's row is one uniform register read and
+// the 16 lanes of its group update one neighbour each.  The spins of a lane's
+// tracked nodes are bits of a register.  A flip: re-read the tracked gains
+// (LDS), each lane's exact best (highest gain, then lowest node), a wave
+// maximum of the best gains' int32 keys, the winner by ballot (exact
+// comparisons only when lanes share the top key), then the neighbours'
+// spins, flags and gains (LDS) and their new gains.  Scans and row fetches
+// load without bounds tests (the gains are padded to a multiple of 512 nodes
+// with the lowest gain), so every load of a stage is in flight at once.
+constexpr int RR_NQ = 64 * RD_C / 4;  // row registers per lane (48)
+constexpr int RR_BANK = RR_NQ / 2;    // banks of 24: a uniform index into each stays a register move
+
+// a weight as 32-bit words and back (8-byte weights take two)
+__device__ __forceinline__ void weight_words(float w, uint32_t& w0, uint32_t& w1) { w0 = __float_as_uint(w); }
+__device__ __forceinline__ void weight_words(double w, uint32_t& w0, uint32_t& w1) {
+    w0 = (uint32_t)__double2loint(w);
+    w1 = (uint32_t)__double2hiint(w);
+}
+__device__ __forceinline__ void weight_words(int8_t w, uint32_t& w0, uint32_t& w1) { w0 = (uint32_t)(int32_t)w; }
+template <typename WT>
+__device__ __forceinline__ WT weight_of_words(uint32_t w0, uint32_t w1);
+template <>
+__device__ __forceinline__ float weight_of_words<float>(uint32_t w0, uint32_t) { return __uint_as_float(w0); }
+template <>
+__device__ __forceinline__ double weight_of_words<double>(uint32_t w0, uint32_t w1) {
+    return __hiloint2double((int)w1, (int)w0);
+}
+template <>
+__device__ __forceinline__ int8_t weight_of_words<int8_t>(uint32_t w0, uint32_t) { return (int8_t)(int32_t)w0; }
+
 template <typename WT, typename FT>
 __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
     auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
     RoundLds L;
     size_t o = 0;
     L.d = o;
-    o += al((size_t)(n + 1) * sizeof(FT));  // + a sentinel below every gain
+    // + a sentinel below every gain at n, and the same up to a multiple of
+    // 512 nodes so that the scans load without bounds tests
+    o += al((size_t)((n + 1 + 511) & ~(int64_t)511) * sizeof(FT));
     L.x = o;
     o += al((size_t)n);
     L.trk = o;
@@ -675,10 +715,6 @@ __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
     o += al((size_t)(n + 1) * 4);
     L.sn = o;
     o += al(64 * RD_C * 4);
-    L.rci = o;
-    o += al(64 * RD_C * RD_RS * 4);
-    L.rcw = o;
-    o += al(64 * RD_C * RD_RS * sizeof(WT));
     L.ex = o;
     o += al(RD_X * 4);
     L.total = o;
@@ -687,9 +723,9 @@ __host__ __device__ inline RoundLds round_lds_layout(int64_t n) {
 
 template <typename WT, typename FT>
 __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int64_t max_steps, PhaseOut* out,
-                                   unsigned char* lds) {
-    constexpr int RS = RD_RS, NC = 64 * RD_C;
-    static_assert(RS == 16, "the row fetch maps 16 lanes to one row");
+                                    unsigned char* lds) {
+    constexpr int RS = RD_RS;
+    static_assert(RS == 16 && RD_C == 3, "16 lanes per row, 4 rows per register");
     const int lane = threadIdx.x;
     const int n = (int)g.n;
     const RoundLds L = round_lds_layout<WT, FT>(n);
@@ -698,13 +734,13 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     uint8_t* trk = lds + L.trk;  // 1: tracked in this round
     int32_t* AP = (int32_t*)(lds + L.ap);
     int32_t* SN = (int32_t*)(lds + L.sn);  // the round's candidates
-    int32_t* RCI = (int32_t*)(lds + L.rci);
-    WT* RCW = (WT*)(lds + L.rcw);
     int32_t* EX = (int32_t*)(lds + L.ex);
     auto fmax_ = [](FT a, FT b) { return vmax(a, b); };
+    auto imin_ = [](int a, int b) { return a < b ? a : b; };
     stage_gains<FT>(xg, hg, n, x, D, trk);
     stage_words(g.ap, n + 1, AP);
-    if (lane == 0) D[n] = lowest<FT>();  // the sentinel: empty candidate slots point here
+    const int nscan = (n + 511) >> 9;  // scan iterations of 512 nodes
+    for (int v = n + lane; v < ((n + 1 + 511) & ~511); v += 64) D[v] = lowest<FT>();  // sentinel and padding
     __syncthreads();
     int64_t steps = 0;
     double gsum = 0.0;
@@ -712,73 +748,57 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     int bk = INT_MAX;
     FT T = lowest<FT>();
     int nx = 0;  // extras this round (wave-uniform)
-    // Flip k (gain gk, spin byte xk, row e0 .. e0 + deg, cached as row slot
-    // when slot >= 0); with tracking, neighbours raised above T join the extras.
+    // the gain update of one neighbour per active lane (v, w) for a flip with
+    // spin s_k; with tracking, a neighbour raised above T joins the extras.
     // Returns false when the extras overflowed (the round must end).
-    auto flip = [&](int k, FT gk, int xk, int e0, int deg, int slot, bool tracking) -> bool {
-        const int sk = xk ? -1 : 1;  // s_k before the flip
+    auto update = [&](bool act, int v, WT w, int sk, bool tracking) -> bool {
+        bool add = false;
+        if (act) {
+            const uint8_t xv = x[v], tv = trk[v];
+            const FT nd = D[v] - (FT)(2 * sk * (xv ? -1 : 1)) * (FT)w;
+            D[v] = nd;
+            add = tracking && !tv && nd > T;
+        }
+        const uint64_t m = __ballot(add);
+        if (m) {
+            const int cnt = __popcll(m);
+            if (nx + cnt > RD_X) return false;
+            if (add) {
+                const int pos = nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                EX[pos] = v;
+                trk[v] = 1;
+            }
+            nx += cnt;
+        }
+        return true;
+    };
+    // a flip whose row comes from global memory, lanes over the entries
+    auto flip_global = [&](int k, int sk, bool tracking) -> bool {
+        const int e0 = AP[k], deg = AP[k + 1] - e0;
         bool fits = true;
-        // one neighbour per active lane: update its gain, track it if it rose above T
-        auto update = [&](bool act, int v, WT w) {
-            bool add = false;
-            if (act) {
-                const uint8_t xv = x[v], tv = trk[v];
-                const int sv = xv ? -1 : 1;
-                const FT nd = D[v] - (FT)(2 * sk * sv) * (FT)w;
-                D[v] = nd;
-                add = tracking && !tv && nd > T;
-            }
-            const uint64_t m = __ballot(add);
-            if (m) {
-                const int cnt = __popcll(m);
-                if (nx + cnt > RD_X) {
-                    fits = false;
-                } else {
-                    if (add) {
-                        const int pos = nx + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        EX[pos] = v;
-                        trk[v] = 1;
-                    }
-                    nx += cnt;
-                }
-            }
-        };
-        if (slot >= 0 && deg <= RS) {
-            // the whole row is cached: one entry per lane, no loop
-            const bool act = lane < deg;
-            const int c = slot * RS + (act ? lane : 0);
-            // through LDS-qualified pointers, so that cached and global
-            // entries stay separate loads (not one flat load of either)
-            const int v = ((const __attribute__((address_space(3))) int32_t*)RCI)[c];
-            const WT w = ((const __attribute__((address_space(3))) WT*)RCW)[c];
-            update(act, v, w);
-        } else {
-            for (int j0 = 0; j0 < deg; j0 += 64) {
-                const int j = j0 + lane;
-                const bool act = j < deg;
-                int v = 0;
-                WT w = (WT)0;
-                if (act) {
-                    if (slot >= 0 && j < RS) {
-                        v = ((const __attribute__((address_space(3))) int32_t*)RCI)[slot * RS + j];
-                        w = ((const __attribute__((address_space(3))) WT*)RCW)[slot * RS + j];
-                    } else {
-                        v = g.ai[e0 + j];
-                        w = g.aw[e0 + j];
-                    }
-                }
-                update(act, v, w);
-            }
+        for (int j0 = 0; j0 < deg; j0 += 64) {
+            const int j = j0 + lane;
+            const bool act = j < deg;
+            const int v = act ? g.ai[e0 + j] : 0;
+            const WT w = act ? g.aw[e0 + j] : (WT)0;
+            fits &= update(act, v, w, sk, tracking);
         }
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
-            D[k] = -gk;
-            x[k] = (uint8_t)(xk ^ 1);
-        }
-        __builtin_amdgcn_wave_barrier();
         return fits;
     };
+    // every lane stores the same values at the same addresses (no branch)
+    auto set_k = [&](int k, FT gk, int xk) {
+        __builtin_amdgcn_wave_barrier();
+        D[k] = -gk;
+        x[k] = (uint8_t)(xk ^ 1);
+        __builtin_amdgcn_wave_barrier();
+    };
+    // the round's candidate rows (banks of 24: a uniform index into each is a
+    // register move)
+    // weights as 32-bit words (8-byte weights in two banks of words)
+    constexpr int WW = sizeof(WT) == 8 ? 2 : 1;
+    int32_t riA[RR_BANK], riB[RR_BANK];
+    uint32_t wA0[RR_BANK], wB0[RR_BANK], wA1[WW == 2 ? RR_BANK : 1], wB1[WW == 2 ? RR_BANK : 1];
     for (;;) {
         // scan: the lane's 4 largest gain keys (a max/min insertion network);
         // KT is the largest 4th over the lanes and T the largest gain with a
@@ -795,15 +815,12 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             c3 = max(c3, m2);
         };
         // lane l holds the pairs of nodes 2 l + 128 i and 2 l + 128 i + 1
-        for (int b0 = 2 * lane; b0 < n; b0 += 512) {
+        // (the padding past n holds the lowest gain: loads without tests)
+        for (int it = 0; it < nscan; ++it) {
+            const int b0 = 2 * lane + 512 * it;
             int32_t ka[4], kb[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int b = b0 + 128 * u;
-                ka[u] = kb[u] = INT_MIN;
-                if (b < n) gain_key_pair(D, b, ka[u], kb[u]);
-                if (b + 1 >= n) kb[u] = INT_MIN;
-            }
+            for (int u = 0; u < 4; ++u) gain_key_pair(D, b0 + 128 * u, ka[u], kb[u]);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 insert(ka[u]);
@@ -829,15 +846,15 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             bv = lb;
             bk = lk;
             if (lk == INT_MAX || !(lb > 0) || steps >= max_steps) break;
-            const uint8_t xk = x[lk];
-            const int e0 = AP[lk], deg = AP[lk + 1] - e0;
-            flip(lk, lb, xk, e0, deg, -1, false);
+            const int xk = x[lk];
+            flip_global(lk, xk ? -1 : 1, false);
+            set_k(lk, lb, xk);
             ++steps;
             gsum += (double)lb;
             continue;
         }
         // the candidates: key > KT, at most RD_C per lane (same lane layout
-        // as the scan), kept in the lane's registers in ascending node order
+        // as the scan), in ascending node order; their spins as bits
         int cn[RD_C], ncl = 0;
 #pragma unroll
         for (int r = 0; r < RD_C; ++r) cn[r] = -1;
@@ -846,67 +863,82 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             for (int r = 0; r < RD_C; ++r) cn[r] = (ok && ncl == r) ? v : cn[r];
             ncl += ok ? 1 : 0;
         };
-        for (int b0 = 2 * lane; b0 < n; b0 += 512) {
+        for (int it = 0; it < nscan; ++it) {
+            const int b0 = 2 * lane + 512 * it;
             int32_t ka[4], kb[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int b = b0 + 128 * u;
-                ka[u] = kb[u] = INT_MIN;
-                if (b < n) gain_key_pair(D, b, ka[u], kb[u]);
-                if (b + 1 >= n) kb[u] = INT_MIN;
-            }
+            for (int u = 0; u < 4; ++u) gain_key_pair(D, b0 + 128 * u, ka[u], kb[u]);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 take(ka[u] > KT, b0 + 128 * u);
                 take(kb[u] > KT, b0 + 128 * u + 1);
             }
         }
+        uint32_t xs = 0;  // bit r: spin of slot r (RD_C: the extra)
+        int cdg[RD_C];
 #pragma unroll
         for (int r = 0; r < RD_C; ++r) {
             SN[lane + 64 * r] = cn[r];
-            if (cn[r] >= 0) trk[cn[r]] = 1;
+            cdg[r] = 0;
+            if (cn[r] >= 0) {
+                trk[cn[r]] = 1;
+                xs |= (uint32_t)(x[cn[r]] != 0) << r;
+                cdg[r] = AP[cn[r] + 1] - AP[cn[r]];
+            }
         }
         __builtin_amdgcn_wave_barrier();
-        // the lane's candidates and their rows (registers), and the row cache
-        int ce0[RD_C], cdg[RD_C];
+        // the candidates' rows into registers: register q of lane 16 g + j
+        // holds entry j of candidate 4 q + g (node, or -1 past the row)
 #pragma unroll
-        for (int r = 0; r < RD_C; ++r) {
-            ce0[r] = cn[r] >= 0 ? AP[cn[r]] : 0;
-            cdg[r] = cn[r] >= 0 ? AP[cn[r] + 1] - ce0[r] : 0;
-        }
-        for (int b0 = 0; b0 < NC; b0 += 96) {
-            int32_t vi[24];
-            WT vw[24];
-            int dst[24];
+        for (int q0 = 0; q0 < RR_NQ; q0 += 16) {
+            int32_t e0[16], dg[16];
+            // reads without tests (an empty slot reads row 0 and keeps no
+            // entry), so every read of a stage is in flight at once
+            int32_t cs[16];
 #pragma unroll
-            for (int u = 0; u < 24; ++u) {
-                const int row = b0 + 4 * u + (lane >> 4), j = lane & 15;
-                dst[u] = -1;
-                const int c = SN[row];
-                if (c >= 0) {
-                    const int e0 = AP[c];
-                    if (j < AP[c + 1] - e0) {
-                        dst[u] = row * RS + j;
-                        vi[u] = g.ai[e0 + j];
-                        vw[u] = g.aw[e0 + j];
-                    }
-                }
+            for (int u = 0; u < 16; ++u) cs[u] = SN[4 * (q0 + u) + (lane >> 4)];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int c = cs[u] >= 0 ? cs[u] : 0;
+                e0[u] = AP[c];
+                dg[u] = AP[c + 1];
             }
 #pragma unroll
-            for (int u = 0; u < 24; ++u)
-                if (dst[u] >= 0) {
-                    RCI[dst[u]] = vi[u];
-                    RCW[dst[u]] = vw[u];
+            for (int u = 0; u < 16; ++u) dg[u] = cs[u] >= 0 ? dg[u] - e0[u] : 0;
+            int32_t vi[16];
+            WT vw[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                // loads without tests (entry 0 past a row), so all are in flight
+                const bool ok = (lane & 15) < dg[u];
+                const int e = ok ? e0[u] + (lane & 15) : 0;
+                vi[u] = g.ai[e];
+                vw[u] = g.aw[e];
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) vi[u] = (lane & 15) < dg[u] ? vi[u] : -1;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int q = q0 + u;
+                uint32_t w0, w1 = 0;
+                weight_words(vw[u], w0, w1);
+                if (q < RR_BANK) {
+                    riA[q < RR_BANK ? q : 0] = vi[u];
+                    wA0[q < RR_BANK ? q : 0] = w0;
+                    if constexpr (WW == 2) wA1[q < RR_BANK ? q : 0] = w1;
+                } else {
+                    riB[q >= RR_BANK ? q - RR_BANK : 0] = vi[u];
+                    wB0[q >= RR_BANK ? q - RR_BANK : 0] = w0;
+                    if constexpr (WW == 2) wB1[q >= RR_BANK ? q - RR_BANK : 0] = w1;
                 }
+            }
         }
-        __builtin_amdgcn_wave_barrier();
         nx = 0;
         bool term = false;  // a termination condition met: the next scan settles it exactly
         int64_t round_flips = 0;
         int myex = -1;  // this lane's extra (EX[lane])
         for (;;) {
-            // the tracked gains: the lane's candidates (ascending nodes; empty
-            // slots read the sentinel) and its extra, all reads in flight together
+            // the tracked gains (empty slots read the sentinel), all in flight
             FT d[RD_C + 1];
 #pragma unroll
             for (int r = 0; r < RD_C; ++r) d[r] = D[cn[r] >= 0 ? cn[r] : n];
@@ -916,11 +948,8 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             // otherwise exact comparisons among the nodes holding it
             int32_t kr[RD_C + 1];
 #pragma unroll
-            for (int r = 0; r <= RD_C; ++r) kr[r] = gain_key(&d[r], 0);
-            int32_t kl = kr[0];
-#pragma unroll
-            for (int r = 1; r <= RD_C; ++r) kl = max(kl, kr[r]);
-            const int32_t K = wave_max_i32(kl);
+            for (int r = 0; r <= RD_C; ++r) kr[r] = gain_key_of(d[r]);
+            const int32_t K = wave_max_i32(max(max(kr[0], kr[1]), max(kr[2], kr[3])));
             uint64_t mr[RD_C + 1];
             int hits = 0;
 #pragma unroll
@@ -928,13 +957,9 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
                 mr[r] = __ballot(kr[r] == K);
                 hits += __popcll(mr[r]);
             }
-            int rw = 0, w = 0;
-            if (hits == 1) {
-#pragma unroll
-                for (int r = RD_C; r >= 0; --r)
-                    if (mr[r]) rw = r;
-                w = (int)__builtin_ctzll(mr[rw]);
-            } else {
+            int rw = mr[0] ? 0 : (mr[1] ? 1 : (mr[2] ? 2 : 3));
+            int w = (int)__builtin_ctzll(mr[0] ? mr[0] : (mr[1] ? mr[1] : (mr[2] ? mr[2] : mr[3])));
+            if (hits != 1) {
                 // several nodes share the top key: the largest gain among them,
                 // then the lowest node
                 FT bd = lowest<FT>();
@@ -950,48 +975,81 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
                 }
                 const FT mm = wave_reduce_dpp(bd, fmax_);
                 const int c = bd == mm ? bn : INT_MAX;
-                const int mn = wave_reduce_dpp(c, [](int a, int b) { return a < b ? a : b; });
-                const uint64_t eqm = __ballot(bn == mn && bd == mm);
-                if (eqm == 0) break;  // no tracked node at all (cannot happen in a round)
-                w = (int)__builtin_ctzll(eqm);
+                const int mn = wave_reduce_dpp(c, imin_);
+                w = (int)__builtin_ctzll(__ballot(bn == mn && bd == mm));
                 rw = __builtin_amdgcn_readlane(br, w);
             }
-            // the winner's gain and node (rw is wave-uniform)
-            FT dsel = d[0];
-            int nsel = cn[0];
-#pragma unroll
-            for (int r = 1; r <= RD_C; ++r)
-                if (rw == r) {
-                    dsel = d[r];
-                    nsel = r < RD_C ? cn[r] : myex;
-                }
+            // the winner's gain, node and spin (rw is wave-uniform)
+            const FT dsel = rw == 0 ? d[0] : (rw == 1 ? d[1] : (rw == 2 ? d[2] : d[3]));
+            const int nsel = rw == 0 ? cn[0] : (rw == 1 ? cn[1] : (rw == 2 ? cn[2] : myex));
             const FT m = readlane_t(dsel, w);
             const int k = __builtin_amdgcn_readlane(nsel, w);
+            const int xk = (int)((__builtin_amdgcn_readlane(xs, w) >> rw) & 1u);
             if (!(m > T)) break;  // an untracked node may hold the maximum: rescan
             if (!(m > 0) || steps >= max_steps) {
                 term = true;
                 break;
             }
-            int slot = -1, e0, deg;
-            if (rw < RD_C) {
-                int ce = ce0[0], cg = cdg[0];
-#pragma unroll
-                for (int r = 1; r < RD_C; ++r)
-                    if (rw == r) {
-                        ce = ce0[r];
-                        cg = cdg[r];
-                    }
-                slot = w + 64 * rw;
-                e0 = __builtin_amdgcn_readlane(ce, w);
-                deg = __builtin_amdgcn_readlane(cg, w);
-            } else {
-                e0 = AP[k];
-                deg = AP[k + 1] - e0;
-            }
-            const int xk = x[k];
             const int nx0 = nx;
-            const bool fits = flip(k, m, xk, e0, deg, slot, true);
-            if (nx > nx0 && lane >= nx0 && lane < nx) myex = EX[lane];
+            bool fits;
+            int deg = RS + 1;
+            if (rw < RD_C) {
+                const int cg = rw == 0 ? cdg[0] : (rw == 1 ? cdg[1] : cdg[2]);
+                deg = __builtin_amdgcn_readlane(cg, w);
+            }
+            if (deg <= RS) {
+                // the row from registers: candidate c, one entry per lane of
+                // its group; the other lanes read and write back the sentinel
+                const int c = w + 64 * rw;
+                const int q = c >> 2;
+                // one read from each bank, then a select of the values (a
+                // select of the two arrays' addresses would put them in memory)
+                const int qa = q < RR_BANK ? q : 0, qb = q >= RR_BANK ? q - RR_BANK : 0;
+                int32_t ia = riA[qa], ib = riB[qb];
+                uint32_t a0 = wA0[qa], b0 = wB0[qb], a1 = 0, b1 = 0;
+                if constexpr (WW == 2) {
+                    a1 = wA1[qa];
+                    b1 = wB1[qb];
+                }
+                __asm__("" : "+v"(ia), "+v"(ib), "+v"(a0), "+v"(b0), "+v"(a1), "+v"(b1));
+                const int vi = q < RR_BANK ? ia : ib;
+                const FT w2 = (FT)2 * (FT)weight_of_words<WT>(q < RR_BANK ? a0 : b0, q < RR_BANK ? a1 : b1);
+                const bool act = (lane >> 4) == (c & 3) && vi >= 0;
+                const int v = act ? vi : n;
+                const uint8_t xv = x[v], tv = trk[v];
+                const FT dv = D[v];
+                // D_v - 2 w s_k s_v (s_k s_v = 1 when the spins are equal);
+                // the sentinel stays the lowest gain (-inf, or kept as is)
+                const FT t = (int)(xv != 0) == xk ? w2 : -w2;
+                FT nd = dv - t;
+                if constexpr (!std::is_floating_point<FT>::value) nd = act ? nd : dv;
+                D[v] = nd;
+                const bool add = act && !tv && nd > T;
+                const uint64_t am = __ballot(add);
+                fits = true;
+                if (am) {
+                    const int cnt = __popcll(am);
+                    if (nx + cnt > RD_X) {
+                        fits = false;
+                    } else {
+                        if (add) {
+                            const int pos = nx + (int)__builtin_amdgcn_mbcnt_hi(
+                                                     (uint32_t)(am >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)am, 0u));
+                            EX[pos] = v;
+                            trk[v] = 1;
+                        }
+                        nx += cnt;
+                    }
+                }
+            } else {
+                fits = flip_global(k, xk ? -1 : 1, true);
+            }
+            set_k(k, m, xk);
+            if (lane == w) xs ^= 1u << rw;
+            if (nx > nx0 && lane >= nx0 && lane < nx) {
+                myex = EX[lane];
+                xs = (xs & ~(1u << RD_C)) | ((uint32_t)(x[myex] != 0) << RD_C);
+            }
             __builtin_amdgcn_wave_barrier();
             ++steps;
             ++round_flips;
@@ -1020,9 +1078,9 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             bv = lb;
             bk = lk;
             if (lk == INT_MAX || !(lb > 0)) break;
-            const uint8_t xk = x[lk];
-            const int e0 = AP[lk], deg = AP[lk + 1] - e0;
-            flip(lk, lb, xk, e0, deg, -1, false);
+            const int xk = x[lk];
+            flip_global(lk, xk ? -1 : 1, false);
+            set_k(lk, lb, xk);
             ++steps;
             gsum += (double)lb;
         }
@@ -3496,7 +3554,7 @@ __global__ __launch_bounds__(64) void kb_descent_lds(const BInstRaw* insts, cons
     const BInstRaw& r = insts[list[blockIdx.x]];
     if (mode == 2 && r.use_adj)
         descent_round_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out,
-                                   lds);
+                                    lds);
     else
         descent_wave_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out,
                                   lds, mode != 0 && r.use_adj);
