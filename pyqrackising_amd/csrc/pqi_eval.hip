@@ -468,10 +468,10 @@ __global__ __launch_bounds__(256) void k_batch_real(const WT* __restrict__ W, in
 // FP64 matrix-core re-score of a few candidates (C <= 16), real weights:
 // H = W S (rows x candidates) on v_mfma_f64_16x16x4f64, then
 // q_c = sum_i s_ci H_ic; W is streamed once.  A wave owns a block of 16 rows
-// and a split of the columns; per 16 columns each lane loads 4 consecutive
-// weights of row (lane mod 16) (one coalesced 16-byte load) and the 4 spins of
-// candidate (lane mod 16) at the same columns, and feeds 4 MFMAs
-// (A[row][k] = W, B[k][cand] = +-1, k = lane / 16).  Rows and columns up to
+// and a split of the columns; per 32 columns each lane loads 8 consecutive
+// weights of row (lane mod 16) (the row's four lanes read whole 128-byte
+// lines) and the 8 spins of candidate (lane mod 16) at the same columns, and
+// feeds 8 MFMAs (A[row][k] = W, B[k][cand] = +-1, k = lane / 16).  Rows and columns up to
 // n_pad are read without a branch: W is zero there.  D layout (measured on
 // gfx950): lane l, register r -> row 4 r + l / 16, candidate l mod 16.  Each
 // wave writes one partial per candidate; k_sum_parts adds them in a fixed
@@ -491,17 +491,20 @@ __global__ __launch_bounds__(256) void k_rescore_mfma(const WT* __restrict__ W, 
     const int r16 = lane & 15, q = lane >> 4;
     const int c = r16;  // this lane's candidate column of B and D
     const bool cl = c < C;
-    const WT* Wr = W + (i0 + r16) * ld + 4 * q;
+    // 32 columns per step: the four lanes of a row read 8 consecutive
+    // weights each, so every step reads whole 128-byte lines of the row;
+    // MFMA u takes columns j0 + 8 q + u (q = lane >> 4)
+    const WT* Wr = W + (i0 + r16) * ld + 8 * q;
     const uint32_t* brow = bits + (cl ? c : 0) * wpr;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t j0 = jb; j0 < je; j0 += 16) {
-        const uint32_t nib = cl ? (brow[j0 >> 5] >> ((j0 & 31) + 4 * q)) & 0xFu : 0u;
-        double a[4];
+    for (int64_t j0 = jb; j0 < je; j0 += 32) {
+        const uint32_t byte = cl ? (brow[j0 >> 5] >> (8 * q)) & 0xFFu : 0u;
+        double a[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) a[u] = (double)Wr[j0 + u];
+        for (int u = 0; u < 8; ++u) a[u] = (double)Wr[j0 + u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const double b = cl ? (((nib >> u) & 1u) ? -1.0 : 1.0) : 0.0;
+        for (int u = 0; u < 8; ++u) {
+            const double b = cl ? (((byte >> u) & 1u) ? -1.0 : 1.0) : 0.0;
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b, acc, 0, 0, 0);
         }
     }
@@ -515,13 +518,13 @@ __global__ __launch_bounds__(256) void k_rescore_mfma(const WT* __restrict__ W, 
     if (lane < 16 && cl) part[c * (nrb * ncs) + gw] = v;
 }
 
-// the FP64-MFMA grid: 16-row blocks x column splits (multiples of 16, at
+// the FP64-MFMA grid: 16-row blocks x column splits (multiples of 32, at
 // least 256 columns each), about 8192 waves in all
 static void mfma_grid(const Graph& g, int64_t* nrb, int64_t* ncs, int64_t* cols_per) {
     *nrb = (g.n + 15) / 16;
     const int64_t want = 8192;
     *ncs = std::max<int64_t>(1, std::min<int64_t>((want + *nrb - 1) / *nrb, g.n_pad / 256));
-    *cols_per = ((g.n_pad + *ncs - 1) / *ncs + 15) / 16 * 16;
+    *cols_per = ((g.n_pad + *ncs - 1) / *ncs + 31) / 32 * 32;
 }
 
 // q[m] = sum of part[m * nparts + p] over p: one work group per candidate,
