@@ -52,8 +52,9 @@ __global__ __launch_bounds__(256) void k_batch_grouped(const BatchInst* __restri
 
 // Wide tiles (256 candidates x 256 columns, pqi_tile_i8.h), persistent over a
 // tile list laid out for the XCDs: position p is run by a work group of XCD
-// p % 8, and the candidate tiles of one (instance, column block) sit at
-// positions of one residue, so they share that block of W through one L2.
+// p % 8, and all tiles of one instance sit at positions of one residue, its
+// candidate tiles of a column block next to each other, so they share that
+// block of W and the instance's candidate bits through one L2.
 // Empty positions (inst < 0) pad the per-XCD queues to one length.
 template <int BK, int NW>
 __global__ __launch_bounds__(64 * NW) void k_batch_wide(const BatchInst* __restrict__ insts,
@@ -251,26 +252,29 @@ int pqi_batch_create(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, int64_
     std::vector<BatchTile> sorted(tiles.size());
     for (size_t t = 0; t < order.size(); ++t) sorted[t] = tiles[order[t]];
     b->ntiles = (int64_t)sorted.size();
-    // wide tiles: groups (instance, 256-column block) of ceil(cands/256)
-    // candidate tiles, dealt longest first to the least loaded of 8 XCD queues
+    // wide tiles: each instance's tiles -- for every 256-column block, its
+    // ceil(cands/256) candidate tiles in a row -- go to one of 8 XCD queues,
+    // instances dealt longest first to the least loaded queue, so that one
+    // XCD's L2 serves both an instance's column blocks of W (to its candidate
+    // tiles) and its candidate bits (to all its column blocks)
     bool wide_ok = true;
     for (int64_t i = 0; i < count && wide_ok; ++i)
         wide_ok = graphs[i]->max_strength >= 0 && 256 * graphs[i]->max_strength < (int64_t(1) << 31);
     std::vector<BatchTile> wsorted;
     if (wide_ok) {
-        struct Group { int32_t inst, jb; int64_t cost; };
-        std::vector<Group> groups;
-        for (int64_t i = 0; i < count; ++i)
-            for (int64_t jb = 0; jb < (graphs[i]->n_pad + WIDE_BN - 1) / WIDE_BN; ++jb)
-                groups.push_back({(int32_t)i, (int32_t)jb, graphs[i]->n_pad});
-        std::stable_sort(groups.begin(), groups.end(), [](const Group& a, const Group& d) { return a.cost > d.cost; });
+        std::vector<int64_t> insts(count);
+        for (int64_t i = 0; i < count; ++i) insts[i] = i;
+        std::stable_sort(insts.begin(), insts.end(),
+                         [&](int64_t a, int64_t d) { return graphs[a]->n_pad > graphs[d]->n_pad; });
         const int64_t MBw = (cands + WIDE_BM - 1) / WIDE_BM;
         std::vector<std::vector<BatchTile>> queue(8);
         std::vector<int64_t> load(8, 0);
-        for (const Group& gr : groups) {
+        for (const int64_t i : insts) {
             const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            for (int64_t mb = 0; mb < MBw; ++mb) queue[x].push_back({gr.inst, (int32_t)mb, gr.jb, 0});
-            load[x] += gr.cost * MBw;
+            const int64_t JBi = (graphs[i]->n_pad + WIDE_BN - 1) / WIDE_BN;
+            for (int64_t jb = 0; jb < JBi; ++jb)
+                for (int64_t mb = 0; mb < MBw; ++mb) queue[x].push_back({(int32_t)i, (int32_t)mb, (int32_t)jb, 0});
+            load[x] += graphs[i]->n_pad * graphs[i]->n_pad * MBw;
         }
         size_t len = 0;
         for (auto& qx : queue) len = std::max(len, qx.size());
