@@ -4070,10 +4070,10 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
     }
 }
 
-// LDS of the staged learned pass: gains, spins, ages, row pointers, the
-// visiting list and 64 cached candidate rows
+// LDS of the staged learned pass: gains, spins, ages, row pointers and the
+// visiting list
 struct PassLds {
-    size_t d, x, age, ap, list, rci, rcw, total;
+    size_t d, x, age, ap, list, total;
 };
 
 template <typename WT, typename FT>
@@ -4082,31 +4082,28 @@ __host__ __device__ inline PassLds pass_lds_layout(int64_t n) {
     PassLds L;
     size_t o = 0;
     L.d = o;
-    o += al((size_t)n * sizeof(FT));
+    o += al((size_t)(n + 1) * sizeof(FT));  // + a slot that idle lanes read and write back
     L.x = o;
-    o += al((size_t)n);
+    o += al((size_t)(n + 1));
     L.age = o;
     o += al((size_t)n);
     L.ap = o;
     o += al((size_t)(n + 1) * 4);
     L.list = o;
     o += al((size_t)n * 4);
-    L.rci = o;
-    o += al(64 * RD_RS * 4);
-    L.rcw = o;
-    o += al(64 * RD_RS * sizeof(WT));
     L.total = o;
     return L;
 }
 
-// The same pass with everything a flip touches in LDS: the nodes whose score
-// passes min_score are listed first (perm order), then each chunk of 64
-// listed nodes has its rows fetched into LDS in one batch of loads before its
-// flips, so a flip waits for no global access (rows longer than RD_RS read
-// their tail from global memory).  Ages stay in LDS until the end.
+// The same pass with everything a flip touches in LDS or registers: the
+// nodes whose score passes min_score are listed first (perm order), then
+// each chunk of 64 listed nodes has its rows fetched into registers in one
+// batch of loads before its flips, so a flip waits for no global access (rows
+// longer than RD_RS come from global memory).  Ages stay in LDS until the end.
 template <typename WT, typename FT>
 __device__ void learned_pass_staged(const BInstRaw& r, const GView<WT>& g, float min_score, unsigned char* lds) {
     constexpr int RS = RD_RS;
+    constexpr int WW = sizeof(WT) == 8 ? 2 : 1;  // 32-bit words per weight
     const int lane = threadIdx.x;
     const int n = (int)r.n;
     const PassLds L = pass_lds_layout<WT, FT>(n);
@@ -4115,8 +4112,6 @@ __device__ void learned_pass_staged(const BInstRaw& r, const GView<WT>& g, float
     uint8_t* age = lds + L.age;
     int32_t* AP = (int32_t*)(lds + L.ap);
     int32_t* LIST = (int32_t*)(lds + L.list);
-    int32_t* RCI = (int32_t*)(lds + L.rci);
-    WT* RCW = (WT*)(lds + L.rcw);
     FT* hg = (FT*)r.h;
     stage_gains<FT>(r.x, hg, n, x, D, nullptr);
     stage_words(g.ap, n + 1, AP);
@@ -4127,6 +4122,10 @@ __device__ void learned_pass_staged(const BInstRaw& r, const GView<WT>& g, float
 #pragma unroll
         for (int u = 0; u < 16; ++u)
             if (v0 + 64 * u < n) age[v0 + 64 * u] = a[u];
+    }
+    if (lane == 0) {
+        D[n] = (FT)0;  // the idle lanes' slot
+        x[n] = 0;
     }
     // the visiting list: perm order, score above min_score; 16 chunks of
     // perm and score loads in flight per lane
@@ -4153,67 +4152,95 @@ __device__ void learned_pass_staged(const BInstRaw& r, const GView<WT>& g, float
     __syncthreads();
     int64_t flips = 0;
     double gsum = 0.0, gmin = INFINITY;
+    // the chunk's rows in registers: row i (the chunk's i-th node) has entry
+    // j in register i >> 2 of lane 16 (i & 3) + j (node, or -1 past the row)
+    int32_t ri[16];
+    uint32_t rw0[16], rw1[WW == 2 ? 16 : 1];
     for (int c0 = 0; c0 < nl; c0 += 64) {
-        // rows of this chunk's nodes: entry j of row i, 16 lanes per row
         {
-            int32_t vi[16];
-            WT vw[16];
-            int dst[16];
+            // every read of a stage in flight at once: no bounds tests
+            // (indices clamped, results masked)
+            int32_t cs[16], e0[16], e1[16];
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
-                const int row = 4 * u + (lane >> 4), j = lane & 15;
-                dst[u] = -1;
-                if (c0 + row < nl) {
-                    const int c = LIST[c0 + row];
-                    const int e0 = AP[c];
-                    if (j < AP[c + 1] - e0) {
-                        dst[u] = row * RS + j;
-                        vi[u] = g.ai[e0 + j];
-                        vw[u] = g.aw[e0 + j];
-                    }
-                }
+                const int i = c0 + 4 * u + (lane >> 4);
+                cs[u] = LIST[i < nl ? i : 0];
+                cs[u] = i < nl ? cs[u] : -1;
             }
 #pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (dst[u] >= 0) {
-                    RCI[dst[u]] = vi[u];
-                    RCW[dst[u]] = vw[u];
-                }
+            for (int u = 0; u < 16; ++u) {
+                const int c = cs[u] >= 0 ? cs[u] : 0;
+                e0[u] = AP[c];
+                e1[u] = AP[c + 1];
+            }
+            int32_t vi[16];
+            WT vw[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const bool ok = cs[u] >= 0 && (lane & 15) < e1[u] - e0[u];
+                const int e = ok ? e0[u] + (lane & 15) : 0;
+                vi[u] = g.ai[e];
+                vw[u] = g.aw[e];
+                e1[u] = ok;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                uint32_t w0, w1 = 0;
+                weight_words(vw[u], w0, w1);
+                ri[u] = e1[u] ? vi[u] : -1;
+                rw0[u] = w0;
+                if constexpr (WW == 2) rw1[u] = w1;
+            }
         }
+        // this lane's listed node: its spin, gain and row extent
         const bool valid = c0 + lane < nl;
-        const int v = valid ? LIST[c0 + lane] : 0;
-        __syncthreads();
+        const int v = LIST[valid ? c0 + lane : 0];
+        uint32_t xl = x[v];
+        const int e0l = AP[v], degl = AP[v + 1] - e0l;
         FT d = valid ? D[v] : (FT)0;
         uint64_t mask = __ballot(valid && d > 0);
         while (mask) {
             const int bl = __builtin_ctzll(mask);
-            const int k = __shfl(v, bl, 64);
-            const FT dd = __shfl(d, bl, 64);
+            const int k = __builtin_amdgcn_readlane(v, bl);
+            const FT dd = readlane_t(d, bl);
+            const int xk = (int)(__builtin_amdgcn_readlane(xl, bl) != 0);
+            const int deg = __builtin_amdgcn_readlane(degl, bl);
             gsum += (double)dd;
             gmin = fmin(gmin, (double)dd);
             ++flips;
-            const int sk = x[k] ? -1 : 1;
-            const int e0 = AP[k], deg = AP[k + 1] - e0;
-            for (int j = lane; j < deg; j += 64) {
-                int u;
-                WT w;
-                if (j < RS) {
-                    u = RCI[bl * RS + j];
-                    w = RCW[bl * RS + j];
-                } else {
-                    u = g.ai[e0 + j];
-                    w = g.aw[e0 + j];
+            if (deg <= RS) {
+                // the row from registers: one entry per lane of its group; the
+                // other lanes read and write back the idle slot n
+                const int q = bl >> 2;
+                int32_t ia = ri[q];
+                uint32_t a0 = rw0[q], a1 = 0;
+                if constexpr (WW == 2) a1 = rw1[q];
+                __asm__("" : "+v"(ia), "+v"(a0), "+v"(a1));
+                const bool act = (lane >> 4) == (bl & 3) && ia >= 0;
+                const int u = act ? ia : n;
+                const FT w2 = (FT)2 * (FT)weight_of_words<WT>(a0, a1);
+                const FT du = D[u];
+                const int xu = x[u] != 0;
+                // D_u - 2 w s_k s_u (s_k s_u = 1 when the spins are equal)
+                const FT nd = du - (xu == xk ? w2 : -w2);
+                D[u] = act ? nd : du;
+            } else {
+                const int e0 = __builtin_amdgcn_readlane(e0l, bl);
+                const int sk = xk ? -1 : 1;
+                for (int j = lane; j < deg; j += 64) {
+                    const int u = g.ai[e0 + j];
+                    const WT w = g.aw[e0 + j];
+                    const int su = x[u] ? -1 : 1;
+                    D[u] -= (FT)(2 * sk * su) * (FT)w;
                 }
-                const int su = x[u] ? -1 : 1;
-                D[u] -= (FT)(2 * sk * su) * (FT)w;
             }
-            __syncthreads();
-            if (lane == 0) {
-                D[k] = -D[k];
-                x[k] ^= 1u;
-                age[k] = 0;
-            }
-            __syncthreads();
+            __builtin_amdgcn_wave_barrier();
+            // every lane stores the same values at the same addresses
+            D[k] = -dd;
+            x[k] = (uint8_t)(xk ^ 1);
+            age[k] = 0;
+            xl = lane == bl ? (uint32_t)(xk ^ 1) : xl;
+            __builtin_amdgcn_wave_barrier();
             d = valid ? D[v] : (FT)0;
             mask = __ballot(valid && d > 0 && lane > bl);
         }
