@@ -42,7 +42,7 @@ PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 # HBM bytes per launch of each roofline kernel on its workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy of known size (see
 # the files)
-C5_TRAFFIC_FILE = os.path.join(PROFILES, "kb_descent_lds_c5_64x8192.md")
+C5_TRAFFIC_FILE = os.path.join(PROFILES, "kb_descent_round_c5_64x8192.md")
 C2_TRAFFIC_FILE = os.path.join(PROFILES, "k_descent_plane_c2_n2048.md")
 
 
@@ -203,7 +203,7 @@ def headline_c5(steps, warmup, log, with_cpu):
     for k, (lst, fb, short) in enumerate(recs):
         log(f"# step {k}: {64 - len(short)} of 64 reached in the learned search, {len(short)} by the fallback")
     ms_step = dt / steps * 1e3
-    # the dominant kernel: the FP64 CSR descent (kb_descent_lds), event-timed
+    # the dominant kernel: the FP64 CSR descent (kb_descent_round), event-timed
     desc_s = sum(r[0][0]["descent_kernel_seconds"] + (r[1][0]["descent_kernel_seconds"] if r[1] else 0.0)
                  for r in recs)
     desc_launches = sum(r[0][0]["descent_launches"] + (r[1][0]["descent_launches"] if r[1] else 0) for r in recs)
@@ -242,14 +242,14 @@ def headline_c5(steps, warmup, log, with_cpu):
             "network_training_s": train_s,
         },
         "roofline": {
-            "kernel": "kb_descent_lds<float, double>",
+            "kernel": "kb_descent_round<float, double>",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(C5_TRAFFIC_FILE),
-            "traffic_source": "profiles/kb_descent_lds_c5_64x8192.md",
+            "traffic_source": "profiles/kb_descent_round_c5_64x8192.md",
             "avg_launch_ms": avg_launch_s * 1e3,
             "launches_per_step": desc_launches / steps,
             "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -742,7 +742,9 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
     const int nscan = (n + 511) >> 9;  // scan iterations of 512 nodes
     for (int v = n + lane; v < ((n + 1 + 511) & ~511); v += 64) D[v] = lowest<FT>();  // sentinel and padding
     __syncthreads();
-    int64_t steps = 0;
+    // 32-bit step counters (a descent makes far fewer than 2^31 moves)
+    const int32_t step_cap = max_steps < INT_MAX ? (int32_t)max_steps : INT_MAX;
+    int32_t steps = 0;
     double gsum = 0.0;
     FT bv = lowest<FT>();
     int bk = INT_MAX;
@@ -830,7 +832,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         const int32_t K0 = wave_max_i32(c0);
         const int32_t KT = wave_max_i32(c3);
         key_bound(KT, T);
-        if (K0 <= 0 || K0 <= KT || steps >= max_steps) {
+        if (K0 <= 0 || K0 <= KT || steps >= step_cap) {
             // the first maximum itself (exact): the result, or the move to
             // make directly when no key exceeds KT (ties) or gains are tiny
             FT lb = lowest<FT>();
@@ -845,7 +847,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             wave_argmax_dpp(lb, lk);
             bv = lb;
             bk = lk;
-            if (lk == INT_MAX || !(lb > 0) || steps >= max_steps) break;
+            if (lk == INT_MAX || !(lb > 0) || steps >= step_cap) break;
             const int xk = x[lk];
             flip_global(lk, xk ? -1 : 1, false);
             set_k(lk, lb, xk);
@@ -935,7 +937,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         }
         nx = 0;
         bool term = false;  // a termination condition met: the next scan settles it exactly
-        int64_t round_flips = 0;
+        int32_t round_flips = 0;
         int myex = -1;  // this lane's extra (EX[lane])
         for (;;) {
             // the tracked gains (empty slots read the sentinel), all in flight
@@ -986,7 +988,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             const int k = __builtin_amdgcn_readlane(nsel, w);
             const int xk = (int)((__builtin_amdgcn_readlane(xs, w) >> rw) & 1u);
             if (!(m > T)) break;  // an untracked node may hold the maximum: rescan
-            if (!(m > 0) || steps >= max_steps) {
+            if (!(m > 0) || steps >= step_cap) {
                 term = true;
                 break;
             }
@@ -2005,10 +2007,17 @@ __global__ __launch_bounds__(64) void k_descent_wave(GView<WT> g, uint8_t* xg, F
                                                     PhaseOut* out, const int32_t* gate, int mode) {
     extern __shared__ __align__(16) unsigned char lds[];
     if (gate && *gate) return;
-    if (mode == 2 && g.use_adj)
-        descent_round_body<WT, FT>(g, xg, hg, max_steps, out, lds);
-    else
-        descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds, mode != 0 && g.use_adj);
+    descent_wave_body<WT, FT>(g, xg, hg, max_steps, out, lds, mode != 0 && g.use_adj);
+}
+
+// the round descent in a kernel of its own (its register allocation is not
+// shared with the full-scan body)
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void k_descent_round(GView<WT> g, uint8_t* xg, FT* hg, int64_t max_steps,
+                                                     PhaseOut* out, const int32_t* gate) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    if (gate && *gate) return;
+    descent_round_body<WT, FT>(g, xg, hg, max_steps, out, lds);
 }
 
 // the round descent applies to sparse graphs whose gains, spins, row pointers
@@ -2056,7 +2065,8 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
             return;
         }
     }
-    static const size_t limit = lds_limit((const void*)k_descent_wave<WT, FT>);
+    static const size_t limit = std::min(lds_limit((const void*)k_descent_wave<WT, FT>),
+                                         lds_limit((const void*)k_descent_round<WT, FT>));
     static const size_t limit2 = lds_limit((const void*)k_descent_wave2<WT, FT>);
     const bool sparse = (g.use_adj || g.W == nullptr) && descent_variant() == 2;
     const bool stage_ap = g.use_adj && descent_variant() != 7 && wave_lds<FT>(g.n, true) <= limit;
@@ -2065,9 +2075,12 @@ static void descent_impl(const Graph& g, uint8_t* x, void* h, int64_t max_steps,
     if (sparse && descent2_lds<FT>(g.n) <= limit2)
         hipExtLaunchKernelGGL(k_descent_wave2<WT, FT>, dim3(1), dim3(64), (uint32_t)descent2_lds<FT>(g.n), s, e0, e1,
                               0, view<WT>(g), x, (FT*)h, max_steps, out, gate);
+    else if (rounds)
+        hipExtLaunchKernelGGL(k_descent_round<WT, FT>, dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, view<WT>(g), x,
+                              (FT*)h, max_steps, out, gate);
     else if (lds <= limit)
         hipExtLaunchKernelGGL(k_descent_wave<WT, FT>, dim3(1), dim3(64), (uint32_t)lds, s, e0, e1, 0, view<WT>(g), x,
-                              (FT*)h, max_steps, out, gate, rounds ? 2 : stage_ap ? 1 : 0);
+                              (FT*)h, max_steps, out, gate, stage_ap ? 1 : 0);
     else
         hipExtLaunchKernelGGL(k_descent<WT, FT>, dim3(1), dim3(1024), 0, s, e0, e1, 0, view<WT>(g), x, (FT*)h,
                               max_steps, out, gate);
@@ -3546,7 +3559,17 @@ __device__ __forceinline__ GView<WT> gview_of(const BInstRaw& r) {
     return v;
 }
 
-// mode as for k_descent_wave
+// every listed instance sparse: the round descent in a kernel of its own
+template <typename WT, typename FT>
+__global__ __launch_bounds__(64) void kb_descent_round(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
+                                                      int which) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const BInstRaw& r = insts[list[blockIdx.x]];
+    descent_round_body<WT, FT>(gview_of<WT>(r), which ? r.rx : r.x, (FT*)(which ? r.rh : r.h), max_steps, r.out, lds);
+}
+
+// mode 2: the round descent for sparse instances (lists mixing sparse and
+// dense instances), 1 / 0 as for k_descent_wave
 template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void kb_descent_lds(const BInstRaw* insts, const int32_t* list, int64_t max_steps,
                                                     int which, int mode) {
@@ -3842,7 +3865,8 @@ static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, in
                               which, key_lds);
         return;
     }
-    static const size_t limit = lds_limit((const void*)kb_descent_lds<WT, FT>);
+    static const size_t limit = std::min(lds_limit((const void*)kb_descent_lds<WT, FT>),
+                                         lds_limit((const void*)kb_descent_round<WT, FT>));
     static const size_t limit2 = lds_limit((const void*)kb_descent_lds2<WT, FT>);
     const bool stage_ap = descent_variant() != 7 && wave_lds<FT>(max_n, true) <= limit;
     const bool rounds = round_fits<WT, FT>(max_n, limit);
@@ -3850,6 +3874,9 @@ static void b_descent_impl(const BInstRaw* d, const int32_t* list, int64_t L, in
     if (all_sparse && descent_variant() == 2 && descent2_lds<FT>(max_n) <= limit2)
         hipExtLaunchKernelGGL((kb_descent_lds2<WT, FT>), dim3((unsigned)L), dim3(64), descent2_lds<FT>(max_n), s, e0,
                               e1, 0, d, list, max_steps, which);
+    else if (rounds && all_sparse)
+        hipExtLaunchKernelGGL((kb_descent_round<WT, FT>), dim3((unsigned)L), dim3(64), lds, s, e0, e1, 0, d, list,
+                              max_steps, which);
     else if (lds <= limit)
         hipExtLaunchKernelGGL((kb_descent_lds<WT, FT>), dim3((unsigned)L), dim3(64), lds, s, e0, e1, 0, d, list,
                               max_steps, which, rounds ? 2 : stage_ap ? 1 : 0);
