@@ -1505,6 +1505,11 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
     // as node 2048, whose lane field (u >> 5 = 64) matches no lane: a no-op
     constexpr bool RR = NB == 3;
     static_assert(!RR || NW == 1, "register rows hold one word of nodes per lane");
+    // two planes of (Delta + S) / 2 with S = 3: plane 1 is the positive gains
+    constexpr bool SPEC = NW == 1 && NP == 2 && H == 2;
+    // register rows are also written to LDS (as the table, v * stride + j),
+    // when it fits, for the top-level batch below
+    const bool rows_lds = RR && max_steps > 0 && (size_t)n * stride * 4 <= lds_bytes;
     int32_t rr0[RR ? 32 : 1], rr1[RR ? 32 : 1], rr2[RR ? 32 : 1];
     if constexpr (RR) {
         auto rr_set = [&](int j, int r, int32_t v) {
@@ -1528,13 +1533,20 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
                     const int4 v = q[e >> 2];
                     rr_set(j, r, (e & 3) == 0 ? v.x : (e & 3) == 1 ? v.y : (e & 3) == 2 ? v.z : v.w);
                 }
+            if (rows_lds) {
+#pragma unroll
+                for (int i = 0; i < 24; ++i) ((int4*)lds)[lane * 24 + i] = q[i];
+            }
         } else {
 #pragma unroll
             for (int r = 0; r < 32; ++r)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     const int v = base + r;
-                    rr_set(j, r, (max_steps > 0 && j < stride && v < n) ? g.kadj[(int64_t)v * stride + j] : 0);
+                    const bool in_row = max_steps > 0 && j < stride && v < n;
+                    const int32_t a = in_row ? g.kadj[(int64_t)v * stride + j] : 0;
+                    rr_set(j, r, a);
+                    if (rows_lds && in_row) ((int32_t*)lds)[v * stride + j] = a;
                 }
         }
     }
@@ -1551,33 +1563,188 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
         for (int q = lane; q < quads; q += 64) dst[q] = src[q];
     }
     __syncthreads();
+    // top-level batch (register rows): bucket words, the (in, out) words of
+    // the independent-set rounds, and per-node byte counters
+    __shared__ uint32_t top_b[RR ? 64 : 1];
+    __shared__ uint64_t top_io[RR ? 64 : 1];
+    __shared__ __align__(16) uint32_t top_cnt[RR ? 512 : 4];
     const int32_t step_cap = max_steps < INT_MAX ? (int32_t)max_steps : INT_MAX;
     int32_t steps = 0;
     int64_t gsum = 0;
     int bv = 0, bk = 0;
+    if constexpr (RR) {
+        // the top-level narrowing of the first step, as in the loop below
+        uint32_t M[1] = {V[0]};
+        int cm = 0;
+#pragma unroll
+        for (int b = NP - 1; b >= 0; --b) {
+            const uint32_t t = M[0] & P[b][0];
+            const bool hit = __ballot(t != 0) != 0;
+            M[0] = hit ? t : M[0];
+            cm |= hit ? (1 << b) : 0;
+        }
+        const uint64_t bal = __ballot(M[0] != 0);
+        // Top-level batch.  When the largest gain is S, every node B of
+        // the bucket has Delta = S >= its row strength, so all its
+        // edges are unsatisfied and its flip lowers every neighbour by
+        // 2: no gain rises, and the steps take the bucket in index
+        // order, each flip dropping its bucket neighbours.  That is
+        // the lexicographically first maximal independent set of the
+        // bucket's own graph: node v is in it iff none of its lower
+        // bucket neighbours is.  It is found in rounds (a node is
+        // decided once its lower bucket neighbours are), flipped at
+        // once, and each neighbour is lowered by 2 per flipped
+        // neighbour.  Same moves and state as the single steps; taken
+        // before the first step, when the bucket spans at least 4 lanes
+        // and the step cap cannot fall inside it.  Afterwards no gain is
+        // S (each bucket node was flipped or lowered), and the single
+        // steps go on.
+        if (rows_lds && cm == ((2 * S) >> HS) && __popcll(bal) >= 4 && step_cap >= n) {
+            // Each lane walks its own bucket nodes (at most the busiest lane's
+            // count of steps, a few for a bucket spread over the wave), reading
+            // rows from LDS.  Entry j of node v is a lower bucket neighbour
+            // when bit r = v & 31 of lbj is set.
+            const int32_t* rows = (const int32_t*)lds;
+            const uint32_t B = M[0];
+            top_b[lane] = B;
+            __syncthreads();
+            uint32_t lb0 = 0u, lb1 = 0u, lb2 = 0u;
+            for (uint32_t bb = B; __ballot(bb != 0u);) {
+                if (bb) {
+                    const int r = __builtin_ctz(bb);
+                    bb &= bb - 1u;
+                    const int v = lane * 32 + r;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        if (j < stride) {
+                            const int32_t a = rows[v * stride + j];
+                            const int u = a & 0xFFF;
+                            const bool low = (a >> 24) != 0 && u < v && ((top_b[u >> 5] >> (u & 31)) & 1u);
+                            const uint32_t bit = (uint32_t)low << r;
+                            if (j == 0) lb0 |= bit;
+                            else if (j == 1) lb1 |= bit;
+                            else lb2 |= bit;
+                        }
+                    }
+                }
+            }
+            const uint32_t lba = lb0 | lb1 | lb2;
+            uint32_t in = B & ~lba, out = 0u, und = B & lba;
+            while (__ballot(und != 0u)) {
+                top_io[lane] = (uint64_t)in | ((uint64_t)out << 32);
+                __syncthreads();
+                uint32_t hit = 0u, blocked = 0u;
+                for (uint32_t uu = und; __ballot(uu != 0u);) {
+                    if (uu) {
+                        const int r = __builtin_ctz(uu);
+                        uu &= uu - 1u;
+                        const int v = lane * 32 + r;
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            const uint32_t lbj = j == 0 ? lb0 : (j == 1 ? lb1 : lb2);
+                            if (j < stride && ((lbj >> r) & 1u)) {
+                                const int u = rows[v * stride + j] & 0xFFF;
+                                const uint64_t st = top_io[u >> 5];
+                                hit |= (((uint32_t)st >> (u & 31)) & 1u) << r;
+                                blocked |= ((~(uint32_t)(st >> 32) >> (u & 31)) & 1u) << r;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                out |= und & hit;
+                const uint32_t join = und & ~hit & ~blocked;
+                in |= join;
+                und &= ~(hit | join);
+            }
+            // each neighbour's count of flipped neighbours (at most 3)
+            uint4* cnt4 = (uint4*)top_cnt;
+            cnt4[2 * lane] = make_uint4(0u, 0u, 0u, 0u);
+            cnt4[2 * lane + 1] = make_uint4(0u, 0u, 0u, 0u);
+            __syncthreads();
+            for (uint32_t ii = in; __ballot(ii != 0u);) {
+                if (ii) {
+                    const int r = __builtin_ctz(ii);
+                    ii &= ii - 1u;
+                    const int v = lane * 32 + r;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        if (j < stride) {
+                            const int32_t a = rows[v * stride + j];
+                            const int u = a & 0xFFF;
+                            if ((a >> 24) != 0)
+                                __hip_atomic_fetch_add(&top_cnt[u >> 2], 1u << (8 * (u & 3)), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const uint4 ca = cnt4[2 * lane], cb = cnt4[2 * lane + 1];
+            __syncthreads();
+            const uint32_t cw[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+            uint32_t d0 = 0u, d1 = 0u;  // count bits of this lane's 32 nodes
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                // bit 0 of each byte to bits 0..3 (the multiplier's cross
+                // terms stay below bit 24 or leave the word)
+                d0 |= (((cw[q] & 0x01010101u) * 0x01020408u) >> 24) << (4 * q);
+                d1 |= ((((cw[q] >> 1) & 0x01010101u) * 0x01020408u) >> 24) << (4 * q);
+            }
+            // c -= count (in units of 2 >> HS), a bit-sliced subtraction;
+            // the value stays >= 0 (Delta_u >= -deg u), then the flipped
+            // nodes take c = 0 (Delta = -S)
+            uint32_t bo = 0u;
+#pragma unroll
+            for (int b = 1 - HS; b < NP; ++b) {
+                const uint32_t sb = b == 1 - HS ? d0 : (b == 2 - HS ? d1 : 0u);
+                const uint32_t pb = P[b][0];
+                P[b][0] = pb ^ sb ^ bo;
+                bo = (~pb & (sb | bo)) | (sb & bo);
+            }
+#pragma unroll
+            for (int b = 0; b < NP; ++b) P[b][0] &= ~in;
+            X[0] ^= in;
+            const int m = wave_reduce_dpp((int)__popc(in), [](int a, int c) { return a + c; });
+            steps = m;
+            gsum = (int64_t)S * m;
+        }
+    }
     for (;;) {
         int L, p, ws = 0, cm = 0;
         int32_t nb[NB];
         {
             uint32_t M[NW];
+            uint64_t bal;
+            if constexpr (SPEC) {
+                // c in 0..3 and Delta > 0 iff c >= 2 (plane 1): both ballots at
+                // once.  With no positive gain cm = 0 stops the descent below,
+                // and the exact top is found after the loop
+                const uint32_t t1 = P[1][0], t2 = t1 & P[0][0];
+                const uint64_t bp = __ballot(t1 != 0u), b2 = __ballot(t2 != 0u);
+                M[0] = b2 ? t2 : t1;
+                bal = (b2 ? b2 : bp) | (1ull << 63);  // lane 63 when empty: in range
+                cm = b2 ? 3 : (bp ? 2 : 0);
+            } else {
 #pragma unroll
-            for (int w = 0; w < NW; ++w) M[w] = V[w];
+                for (int w = 0; w < NW; ++w) M[w] = V[w];
 #pragma unroll
-            for (int b = NP - 1; b >= 0; --b) {
-                uint32_t t[NW], any = 0;
+                for (int b = NP - 1; b >= 0; --b) {
+                    uint32_t t[NW], any = 0;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) {
-                    t[w] = M[w] & P[b][w];
-                    any |= t[w];
+                    for (int w = 0; w < NW; ++w) {
+                        t[w] = M[w] & P[b][w];
+                        any |= t[w];
+                    }
+                    const bool hit = __ballot(any != 0) != 0;  // wave-uniform
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) M[w] = hit ? t[w] : M[w];
+                    cm |= hit ? (1 << b) : 0;
                 }
-                const bool hit = __ballot(any != 0) != 0;  // wave-uniform
-#pragma unroll
-                for (int w = 0; w < NW; ++w) M[w] = hit ? t[w] : M[w];
-                cm |= hit ? (1 << b) : 0;
+                // the lowest remaining node: word ws, lane L, bit p.  n >= 1, so
+                // M is never empty (a plane only narrows it when its bits meet it)
+                bal = __ballot(M[0] != 0);
             }
-            // the lowest remaining node: word ws, lane L, bit p.  n >= 1, so M
-            // is never empty (a plane only narrows it when its bits meet it)
-            uint64_t bal = __ballot(M[0] != 0);
             uint32_t mw = M[0];
             int cb = 0;
             if constexpr (NW > 1) {
@@ -1604,7 +1771,7 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
                 }
             }
             L = (int)__builtin_ctzll(bal);
-            p = (int)__builtin_ctz(__builtin_amdgcn_readlane(mw, L));
+            p = (int)__builtin_ctz(__builtin_amdgcn_readlane(mw, L) | (SPEC ? 0x80000000u : 0u));
         }
         if constexpr (RR) {
 #pragma unroll
@@ -1683,6 +1850,24 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
         }
         ++steps;
         gsum += bv;
+    }
+    if constexpr (SPEC) {
+        if (bv <= 0) {
+            // the exact top of the final state: the largest value and its
+            // lowest node
+            uint32_t M = V[0];
+            int cm = 0;
+#pragma unroll
+            for (int b = NP - 1; b >= 0; --b) {
+                const uint32_t t = M & P[b][0];
+                const bool hit = __ballot(t != 0) != 0;
+                M = hit ? t : M;
+                cm |= hit ? (1 << b) : 0;
+            }
+            const int L = (int)__builtin_ctzll(__ballot(M != 0));
+            bk = L * 32 + (int)__builtin_ctz(__builtin_amdgcn_readlane(M, L));
+            bv = (cm << HS) - S;
+        }
     }
     if (steps) {
 #pragma unroll
