@@ -1608,106 +1608,117 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
             const uint32_t B = M[0];
             top_b[lane] = B;
             __syncthreads();
-            uint32_t lb0 = 0u, lb1 = 0u, lb2 = 0u;
+            // the (lower bucket neighbour u, bit r of node v) pairs of this
+            // lane's bucket nodes, at most 7 per lane, in 8 slots of LDS
+            // (top_cnt, not yet in use); a lane with more makes the batch
+            // fall back to single steps (nothing has moved yet)
+            uint32_t* pairs = top_cnt + lane * 8;
+            uint32_t lba = 0u;
+            int np = 0;
             for (uint32_t bb = B; __ballot(bb != 0u);) {
                 if (bb) {
+                    // branch-free: the three entries, then their bucket words,
+                    // each group of loads in flight together; a pair is
+                    // written at slot np & 7 and kept by advancing np (while
+                    // np <= 7 the slot written is never a kept one)
                     const int r = __builtin_ctz(bb);
                     bb &= bb - 1u;
                     const int v = lane * 32 + r;
+                    int32_t a[3];
+                    uint32_t bw[3];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) a[j] = rows[v * stride + min(j, stride - 1)];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) bw[j] = top_b[(a[j] & 0x7FF) >> 5];
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
-                        if (j < stride) {
-                            const int32_t a = rows[v * stride + j];
-                            const int u = a & 0xFFF;
-                            const bool low = (a >> 24) != 0 && u < v && ((top_b[u >> 5] >> (u & 31)) & 1u);
-                            const uint32_t bit = (uint32_t)low << r;
-                            if (j == 0) lb0 |= bit;
-                            else if (j == 1) lb1 |= bit;
-                            else lb2 |= bit;
-                        }
+                        const int u = a[j] & 0x7FF;
+                        const bool low = j < stride && (a[j] >> 24) != 0 && u < v && ((bw[j] >> (u & 31)) & 1u);
+                        pairs[np & 7] = ((uint32_t)r << 16) | (uint32_t)u;
+                        np += low ? 1 : 0;
+                        lba |= (uint32_t)low << r;
                     }
                 }
             }
-            const uint32_t lba = lb0 | lb1 | lb2;
-            uint32_t in = B & ~lba, out = 0u, und = B & lba;
-            while (__ballot(und != 0u)) {
-                top_io[lane] = (uint64_t)in | ((uint64_t)out << 32);
+            if (__ballot(np > 7) == 0) {
+                const uint4 pa = ((const uint4*)pairs)[0], pb = ((const uint4*)pairs)[1];
+                const uint32_t pr[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+                uint32_t in = B & ~lba, out = 0u, und = B & lba;
+                while (__ballot(und != 0u)) {
+                    top_io[lane] = (uint64_t)in | ((uint64_t)out << 32);
+                    __syncthreads();
+                    uint32_t hit = 0u, blocked = 0u;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        // slots past np hold stale values: any node (< 2048) is
+                        // a valid index, and the bits are dropped
+                        const int u = pr[k] & 0x7FF, r = (pr[k] >> 16) & 31;
+                        const uint64_t st = top_io[u >> 5];
+                        const uint32_t keep = k < np ? 1u : 0u;
+                        hit |= ((((uint32_t)st >> (u & 31)) & keep)) << r;
+                        blocked |= ((~(uint32_t)(st >> 32) >> (u & 31)) & keep) << r;
+                    }
+                    __syncthreads();
+                    hit &= und;
+                    blocked &= und;
+                    out |= hit;
+                    const uint32_t join = und & ~hit & ~blocked;
+                    in |= join;
+                    und &= ~(hit | join);
+                }
+                // each neighbour's count of flipped neighbours (at most 3)
+                uint4* cnt4 = (uint4*)top_cnt;
+                cnt4[2 * lane] = make_uint4(0u, 0u, 0u, 0u);
+                cnt4[2 * lane + 1] = make_uint4(0u, 0u, 0u, 0u);
                 __syncthreads();
-                uint32_t hit = 0u, blocked = 0u;
-                for (uint32_t uu = und; __ballot(uu != 0u);) {
-                    if (uu) {
-                        const int r = __builtin_ctz(uu);
-                        uu &= uu - 1u;
+                for (uint32_t ii = in; __ballot(ii != 0u);) {
+                    if (ii) {
+                        const int r = __builtin_ctz(ii);
+                        ii &= ii - 1u;
                         const int v = lane * 32 + r;
+                        int32_t a[3];
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) a[j] = rows[v * stride + min(j, stride - 1)];
 #pragma unroll
                         for (int j = 0; j < 3; ++j) {
-                            const uint32_t lbj = j == 0 ? lb0 : (j == 1 ? lb1 : lb2);
-                            if (j < stride && ((lbj >> r) & 1u)) {
-                                const int u = rows[v * stride + j] & 0xFFF;
-                                const uint64_t st = top_io[u >> 5];
-                                hit |= (((uint32_t)st >> (u & 31)) & 1u) << r;
-                                blocked |= ((~(uint32_t)(st >> 32) >> (u & 31)) & 1u) << r;
-                            }
+                            // padding entries (and entries past the stride) add 0
+                            const int u = a[j] & 0x7FF;
+                            const uint32_t add = (j < stride && (a[j] >> 24) != 0) ? 1u << (8 * (u & 3)) : 0u;
+                            __hip_atomic_fetch_add(&top_cnt[u >> 2], add, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     }
                 }
                 __syncthreads();
-                out |= und & hit;
-                const uint32_t join = und & ~hit & ~blocked;
-                in |= join;
-                und &= ~(hit | join);
-            }
-            // each neighbour's count of flipped neighbours (at most 3)
-            uint4* cnt4 = (uint4*)top_cnt;
-            cnt4[2 * lane] = make_uint4(0u, 0u, 0u, 0u);
-            cnt4[2 * lane + 1] = make_uint4(0u, 0u, 0u, 0u);
-            __syncthreads();
-            for (uint32_t ii = in; __ballot(ii != 0u);) {
-                if (ii) {
-                    const int r = __builtin_ctz(ii);
-                    ii &= ii - 1u;
-                    const int v = lane * 32 + r;
+                const uint4 ca = cnt4[2 * lane], cb = cnt4[2 * lane + 1];
+                __syncthreads();
+                const uint32_t cw[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+                uint32_t d0 = 0u, d1 = 0u;  // count bits of this lane's 32 nodes
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        if (j < stride) {
-                            const int32_t a = rows[v * stride + j];
-                            const int u = a & 0xFFF;
-                            if ((a >> 24) != 0)
-                                __hip_atomic_fetch_add(&top_cnt[u >> 2], 1u << (8 * (u & 3)), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
-                    }
+                for (int q = 0; q < 8; ++q) {
+                    // bit 0 of each byte to bits 0..3 (the multiplier's cross
+                    // terms stay below bit 24 or leave the word)
+                    d0 |= (((cw[q] & 0x01010101u) * 0x01020408u) >> 24) << (4 * q);
+                    d1 |= ((((cw[q] >> 1) & 0x01010101u) * 0x01020408u) >> 24) << (4 * q);
                 }
-            }
-            __syncthreads();
-            const uint4 ca = cnt4[2 * lane], cb = cnt4[2 * lane + 1];
-            __syncthreads();
-            const uint32_t cw[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-            uint32_t d0 = 0u, d1 = 0u;  // count bits of this lane's 32 nodes
+                // c -= count (in units of 2 >> HS), a bit-sliced subtraction;
+                // the value stays >= 0 (Delta_u >= -deg u), then the flipped
+                // nodes take c = 0 (Delta = -S)
+                uint32_t bo = 0u;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                // bit 0 of each byte to bits 0..3 (the multiplier's cross
-                // terms stay below bit 24 or leave the word)
-                d0 |= (((cw[q] & 0x01010101u) * 0x01020408u) >> 24) << (4 * q);
-                d1 |= ((((cw[q] >> 1) & 0x01010101u) * 0x01020408u) >> 24) << (4 * q);
-            }
-            // c -= count (in units of 2 >> HS), a bit-sliced subtraction;
-            // the value stays >= 0 (Delta_u >= -deg u), then the flipped
-            // nodes take c = 0 (Delta = -S)
-            uint32_t bo = 0u;
+                for (int b = 1 - HS; b < NP; ++b) {
+                    const uint32_t sb = b == 1 - HS ? d0 : (b == 2 - HS ? d1 : 0u);
+                    const uint32_t pb = P[b][0];
+                    P[b][0] = pb ^ sb ^ bo;
+                    bo = (~pb & (sb | bo)) | (sb & bo);
+                }
 #pragma unroll
-            for (int b = 1 - HS; b < NP; ++b) {
-                const uint32_t sb = b == 1 - HS ? d0 : (b == 2 - HS ? d1 : 0u);
-                const uint32_t pb = P[b][0];
-                P[b][0] = pb ^ sb ^ bo;
-                bo = (~pb & (sb | bo)) | (sb & bo);
+                for (int b = 0; b < NP; ++b) P[b][0] &= ~in;
+                X[0] ^= in;
+                const int m = wave_reduce_dpp((int)__popc(in), [](int a, int c) { return a + c; });
+                steps = m;
+                gsum = (int64_t)S * m;
             }
-#pragma unroll
-            for (int b = 0; b < NP; ++b) P[b][0] &= ~in;
-            X[0] ^= in;
-            const int m = wave_reduce_dpp((int)__popc(in), [](int a, int c) { return a + c; });
-            steps = m;
-            gsum = (int64_t)S * m;
         }
     }
     for (;;) {
