@@ -939,12 +939,18 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
         bool term = false;  // a termination condition met: the next scan settles it exactly
         int32_t round_flips = 0;
         int myex = -1;  // this lane's extra (EX[lane])
+        // the slots' gain addresses (the sentinel at n for empty slots), kept
+        // ready so that the four reads of a step go out together
+        int cnn[RD_C];
+#pragma unroll
+        for (int r = 0; r < RD_C; ++r) cnn[r] = cn[r] >= 0 ? cn[r] : n;
+        int exn = n;
         for (;;) {
             // the tracked gains (empty slots read the sentinel), all in flight
             FT d[RD_C + 1];
 #pragma unroll
-            for (int r = 0; r < RD_C; ++r) d[r] = D[cn[r] >= 0 ? cn[r] : n];
-            d[RD_C] = D[myex >= 0 ? myex : n];
+            for (int r = 0; r < RD_C; ++r) d[r] = D[cnn[r]];
+            d[RD_C] = D[exn];
             // the wave's first maximum: a maximum of the gain keys, and when
             // exactly one tracked node holds that key it is the winner;
             // otherwise exact comparisons among the nodes holding it
@@ -952,16 +958,16 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
 #pragma unroll
             for (int r = 0; r <= RD_C; ++r) kr[r] = gain_key_of(d[r]);
             const int32_t K = wave_max_i32(max(max(kr[0], kr[1]), max(kr[2], kr[3])));
-            uint64_t mr[RD_C + 1];
-            int hits = 0;
-#pragma unroll
-            for (int r = 0; r <= RD_C; ++r) {
-                mr[r] = __ballot(kr[r] == K);
-                hits += __popcll(mr[r]);
-            }
-            int rw = mr[0] ? 0 : (mr[1] ? 1 : (mr[2] ? 2 : 3));
-            int w = (int)__builtin_ctzll(mr[0] ? mr[0] : (mr[1] ? mr[1] : (mr[2] ? mr[2] : mr[3])));
-            if (hits != 1) {
+            // each lane: how many of its slots hold K, and the first; one
+            // ballot finds the lanes, and a single holder is the winner
+            const int h0 = kr[0] == K, h1 = kr[1] == K, h2 = kr[2] == K, h3 = kr[3] == K;
+            const int lane_hits = h0 + h1 + h2 + h3;
+            const int first = h0 ? 0 : (h1 ? 1 : (h2 ? 2 : 3));
+            const uint64_t holders = __ballot(lane_hits != 0);
+            int w = (int)__builtin_ctzll(holders);
+            const int wp = __builtin_amdgcn_readlane(first | (lane_hits << 2), w);
+            int rw = wp & 3;
+            if (((wp >> 2) != 1) | (__popcll(holders) != 1)) {
                 // several nodes share the top key: the largest gain among them,
                 // then the lowest node
                 FT bd = lowest<FT>();
@@ -1050,6 +1056,7 @@ __device__ void descent_round_body(const GView<WT>& g, uint8_t* xg, FT* hg, int6
             if (lane == w) xs ^= 1u << rw;
             if (nx > nx0 && lane >= nx0 && lane < nx) {
                 myex = EX[lane];
+                exn = myex;
                 xs = (xs & ~(1u << RD_C)) | ((uint32_t)(x[myex] != 0) << RD_C);
             }
             __builtin_amdgcn_wave_barrier();
