@@ -109,6 +109,10 @@ typedef struct {
 #define PQI_FLAG_TEST_DRIFT 4   /* test hook (real weights): perturb the maintained field by
                                    1e-3 ((v mod 7) - 3) before the first round, which the round's exact
                                    recompute must detect and repair */
+#define PQI_FLAG_FALLBACK 8     /* pqi_learned_search with targets and passes > 0: an instance that ends
+                                   below its target is searched again by the plain batched loop from its
+                                   own start, to the same target, within the same call; its record is then
+                                   the plain search's (fallback = 1), with the learned phase's counters kept */
 
 typedef struct {
     double max_energy; /* reference bookkeeping: start value plus accepted gains */
@@ -136,6 +140,11 @@ typedef struct {
     /* batched and learned searches: (instance, descent launch) pairs of the whole call, and those of them
      * whose descent moved (and wrote its state back); the same totals in every instance record */
     int64_t descent_instance_launches, descent_instance_writes;
+    /* pqi_learned_search with PQI_FLAG_FALLBACK: 1 when the instance's result comes from the plain
+     * fallback search, else 0 (descent timing totals and pairs then cover both searches of the call) */
+    int64_t fallback;
+    /* with PQI_FLAG_FALLBACK, a fallback record: the single-flip accepts of the learned search it replaces */
+    int64_t replaced_single_accepts;
 } pqi_search_stats;
 
 /* Thread-local message for the last failing call on this thread. */

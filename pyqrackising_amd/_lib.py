@@ -9,6 +9,7 @@ import ctypes
 import os
 import threading
 import weakref
+from collections.abc import Mapping
 
 import numpy as np
 
@@ -29,6 +30,7 @@ SEEDS_AUTO, SEEDS_MFMA, SEEDS_LOWRANK = 0, 1, 2
 FLAG_TIME_KERNELS = 1
 FLAG_HOST_LOOP = 2
 FLAG_TEST_DRIFT = 4
+FLAG_FALLBACK = 8
 ENGINE_AUTO, ENGINE_EXACT, ENGINE_BF16 = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -102,10 +104,65 @@ class SearchStats(ctypes.Structure):
         ("learned_min_gain", ctypes.c_double),
         ("descent_instance_launches", ctypes.c_int64),
         ("descent_instance_writes", ctypes.c_int64),
+        ("fallback", ctypes.c_int64),
+        ("replaced_single_accepts", ctypes.c_int64),
     ]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_STATS_NAMES = tuple(f for f, _ in SearchStats._fields_)
+_STATS_DTYPE = np.dtype([(f, "<f8" if t is ctypes.c_double else "<i8") for f, t in SearchStats._fields_])
+assert _STATS_DTYPE.itemsize == ctypes.sizeof(SearchStats)
+
+
+class StatsRecord(Mapping):
+    """One instance's pqi_search_stats of a batched call: a read-only mapping
+    from the field names to Python numbers (columns converted once per call)."""
+
+    __slots__ = ("_cols", "_i")
+
+    def __init__(self, cols, i):
+        self._cols = cols
+        self._i = i
+
+    def __getitem__(self, key):
+        return self._cols[key][self._i]
+
+    def __iter__(self):
+        return iter(_STATS_NAMES)
+
+    def __len__(self):
+        return len(_STATS_NAMES)
+
+    def __repr__(self):
+        return repr(dict(self))
+
+
+def _stats_records(st):
+    """The records of a ctypes SearchStats array, one StatsRecord each."""
+    arr = np.frombuffer(st, dtype=_STATS_DTYPE)
+    cols = {f: arr[f].tolist() for f in _STATS_NAMES}
+    return [StatsRecord(cols, i) for i in range(len(st))]
+
+
+def _batch_states(graphs, thetas):
+    """Copies of the start states (one block when all n agree) and the
+    pointer array the C calls take."""
+    k = len(graphs)
+    ns = [g.n for g in graphs]
+    if k and all(n == ns[0] for n in ns):
+        block = np.array(thetas, dtype=np.uint8)  # a copy
+        if block.shape != (k, ns[0]):
+            raise ValueError("theta shape does not match its graph")
+        base, n = block.ctypes.data, ns[0]
+        return list(block), (ctypes.c_void_p * k)(*range(base, base + k * n, n))
+    states = [np.ascontiguousarray(t, dtype=np.uint8).copy() for t in thetas]
+    for g, t in zip(graphs, states):
+        if t.shape != (g.n,):
+            raise ValueError("theta shape does not match its graph")
+    return states, (ctypes.c_void_p * k)(*[t.ctypes.data for t in states])
 
 
 class BnbStats(ctypes.Structure):
@@ -567,16 +624,12 @@ def batch_local_search(graphs, thetas, params):
     graphs = list(graphs)
     ctx = graphs[0].ctx
     k = len(graphs)
-    states = [np.ascontiguousarray(t, dtype=np.uint8).copy() for t in thetas]
-    for g, t in zip(graphs, states):
-        if t.shape != (g.n,):
-            raise ValueError("theta shape does not match its graph")
+    states, tarr = _batch_states(graphs, thetas)
     garr = (ctypes.c_void_p * k)(*[g._h.value for g in graphs])
-    tarr = (ctypes.c_void_p * k)(*[t.ctypes.data for t in states])
     st = (SearchStats * k)()
     with ctx.lock:
         check(load().pqi_batch_local_search(ctx.handle, garr, k, tarr, ctypes.byref(params), st))
-    return states, [s.as_dict() for s in st]
+    return states, _stats_records(st)
 
 
 class Mpnn:
@@ -652,21 +705,26 @@ class Mpnn:
             pass
 
 
-def learned_search(net, graphs, thetas, params, targets=None, passes=8, min_score=-np.inf):
-    """pqi_learned_search: returns (states, list of stats dicts)."""
+def learned_search(net, graphs, thetas, params, targets=None, passes=8, min_score=-np.inf, fallback=False):
+    """pqi_learned_search: returns (states, list of stats mappings).  With
+    fallback (targets and passes > 0 required), PQI_FLAG_FALLBACK: instances
+    that end below their target are searched again by the plain batched loop
+    from their starts within the same call (their record's "fallback" is 1)."""
     graphs = list(graphs)
     ctx = graphs[0].ctx
     k = len(graphs)
-    states = [np.ascontiguousarray(t, dtype=np.uint8).copy() for t in thetas]
+    states, tarr = _batch_states(graphs, thetas)
     garr = (ctypes.c_void_p * k)(*[g._h.value for g in graphs])
-    tarr = (ctypes.c_void_p * k)(*[t.ctypes.data for t in states])
     tg = None if targets is None else np.ascontiguousarray(targets, dtype=np.float64)
+    if fallback:
+        params = SearchParams.from_buffer_copy(params)
+        params.flags |= FLAG_FALLBACK
     st = (SearchStats * k)()
     with ctx.lock:
         check(load().pqi_learned_search(ctx.handle, net._h, garr, k, tarr, ctypes.byref(params),
                                         None if tg is None else _p(tg), int(passes), ctypes.c_float(min_score),
                                         st))
-    return states, [s.as_dict() for s in st]
+    return states, _stats_records(st)
 
 
 def make_params(n, is_spin_glass=True, gray_iterations=None, walkers=None, seed_multiple=None,

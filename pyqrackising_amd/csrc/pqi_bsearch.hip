@@ -678,7 +678,60 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
     if (passes < 0) return fail(PQI_ERR_INVALID, "passes must be >= 0");
     // no learned pass: the plain loop (with targets), the network is not used
     pqi_mpnn* use = passes > 0 || (p && p->max_rounds < 0) ? net : nullptr;
-    return run_batched(c, graphs, count, thetas, p, stats, use, targets, passes, min_score);
+    const bool fallback = p && (p->flags & PQI_FLAG_FALLBACK) && targets && passes > 0 && count > 0 && thetas;
+    if (!fallback) return run_batched(c, graphs, count, thetas, p, stats, use, targets, passes, min_score);
+    // the starts, for the instances that fall short
+    std::vector<std::vector<uint8_t>> starts(count);
+    for (int64_t i = 0; i < count; ++i) {
+        if (!graphs[i] || !thetas[i]) return fail(PQI_ERR_INVALID, "null graph or state");
+        starts[i].assign(thetas[i], thetas[i] + graphs[i]->n);
+    }
+    int rc = run_batched(c, graphs, count, thetas, p, stats, use, targets, passes, min_score);
+    if (rc != PQI_OK) return rc;
+    std::vector<int64_t> shortl;
+    for (int64_t i = 0; i < count; ++i) {
+        stats[i].fallback = 0;
+        if (stats[i].max_energy < targets[i]) shortl.push_back(i);
+    }
+    if (shortl.empty()) return PQI_OK;
+    // the plain batched loop from their starts to the same targets
+    const int64_t k = (int64_t)shortl.size();
+    std::vector<pqi_graph*> gs(k);
+    std::vector<uint8_t*> ts(k);
+    std::vector<double> tg(k);
+    std::vector<pqi_search_stats> fs(k);
+    for (int64_t t = 0; t < k; ++t) {
+        const int64_t i = shortl[t];
+        gs[t] = graphs[i];
+        ts[t] = starts[i].data();
+        tg[t] = targets[i];
+    }
+    // the learned call's descent totals (the same in every record)
+    const double l_s = stats[0].descent_kernel_seconds;
+    const int64_t l_launches = stats[0].descent_launches, l_pairs = stats[0].descent_instance_launches,
+                  l_writes = stats[0].descent_instance_writes;
+    rc = run_batched(c, gs.data(), k, ts.data(), p, fs.data(), nullptr, tg.data(), 0, 0.f);
+    if (rc != PQI_OK) return rc;
+    for (int64_t t = 0; t < k; ++t) {
+        const int64_t i = shortl[t];
+        std::memcpy(thetas[i], starts[i].data(), graphs[i]->n);
+        pqi_search_stats r = fs[t];
+        r.learned_accepts = stats[i].learned_accepts;
+        r.learned_passes = stats[i].learned_passes;
+        r.seconds_learned = stats[i].seconds_learned;
+        r.learned_min_gain = stats[i].learned_min_gain;
+        r.fallback = 1;
+        r.replaced_single_accepts = stats[i].single_accepts;
+        stats[i] = r;
+    }
+    // every record carries the call's descent totals: both searches
+    for (int64_t i = 0; i < count; ++i) {
+        stats[i].descent_kernel_seconds = l_s + fs[0].descent_kernel_seconds;
+        stats[i].descent_launches = l_launches + fs[0].descent_launches;
+        stats[i].descent_instance_launches = l_pairs + fs[0].descent_instance_launches;
+        stats[i].descent_instance_writes = l_writes + fs[0].descent_instance_writes;
+    }
+    return PQI_OK;
 }
 
 }  // extern "C"

@@ -93,8 +93,8 @@ class LearnedSolver:
     def bind(self, dgs):
         return self.net.bind(dgs)
 
-    def solve(self, dgs, starts, params, targets=None, passes=8, min_score=-np.inf, bind=True):
+    def solve(self, dgs, starts, params, targets=None, passes=8, min_score=-np.inf, bind=True, fallback=False):
         if bind:
             self.net.bind(dgs)
         return _lib.learned_search(self.net, dgs, starts, params, targets=targets, passes=passes,
-                                   min_score=min_score)
+                                   min_score=min_score, fallback=fallback)

@@ -32,7 +32,42 @@ def test_structs_match_header_sizes():
     # field order/size of the three ABI structs (all 8-byte aligned members)
     assert ctypes.sizeof(_lib.GraphInfo) == 40
     assert ctypes.sizeof(_lib.SearchParams) == 48
-    assert ctypes.sizeof(_lib.SearchStats) == 8 * 33
+    assert ctypes.sizeof(_lib.SearchStats) == 8 * 35
+
+
+def test_stats_records_match_the_structs():
+    # the column conversion of a batched call's records gives each field's
+    # value, as the per-struct conversion does
+    st = (_lib.SearchStats * 5)()
+    for i in range(5):
+        st[i].max_energy = 1.5 * i - 2.0
+        st[i].rounds = 7 + i
+        st[i].learned_min_gain = float("inf") if i == 2 else -0.25 * i
+        st[i].fallback = i % 2
+        st[i].replaced_single_accepts = 100 * i
+    recs = _lib._stats_records(st)
+    assert len(recs) == 5
+    for i in range(5):
+        d = st[i].as_dict()
+        assert dict(recs[i]) == d
+        assert list(recs[i]) == list(d)
+        assert type(recs[i]["rounds"]) is int and type(recs[i]["max_energy"]) is float
+
+
+def test_batch_states_are_copies_with_pointers():
+    class G:
+        def __init__(self, n):
+            self.n = n
+
+    rng = np.random.default_rng(0)
+    for ns in ([16, 16, 16], [5, 9, 7]):
+        th = [rng.integers(0, 2, n).astype(np.uint8) for n in ns]
+        states, tarr = _lib._batch_states([G(n) for n in ns], th)
+        for t, s_, p in zip(th, states, tarr):
+            assert np.array_equal(t, s_) and s_.ctypes.data == p
+            assert not np.shares_memory(t, s_)
+    with pytest.raises(ValueError):
+        _lib._batch_states([G(4), G(4)], [np.zeros(4, np.uint8), np.zeros(5, np.uint8)])
 
 
 def test_version_and_error_plumbing():

@@ -200,3 +200,53 @@ def test_passes_zero_is_the_plain_search_to_targets():
         assert st[i]["max_energy"] == targets[i] and st[i]["learned_passes"] == 0
         for k in ("single_accepts", "double_accepts", "seed_accepts", "walk_accepts", "reheat_accepts"):
             assert st[i][k] == plain[i][k], (i, k)
+
+
+def test_fallback_in_call_matches_two_calls():
+    """PQI_FLAG_FALLBACK: the instances that end below their target are
+    searched again by the plain batched loop from their starts inside the same
+    call.  The result equals the learned search followed by a separate plain
+    search (passes = 0) of the short instances: the same states and counters,
+    the learned counters kept on the fallback records, and the replaced
+    search's single-flip accepts recorded apart.  With the untrained network
+    and two passes, 5 of the 8 instances fall back (asserted non-empty); one
+    target is set above what the plain search reaches."""
+    log("learned search with the in-call fallback vs two calls")
+    rng = np.random.default_rng(97)
+    Gs = [er_uniform(int(rng.integers(300, 700)), 8.0, 900 + t) for t in range(8)]
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in Gs]
+    net = _lib.Mpnn(seed=4)
+    net.bind(dgs)
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=800)
+    _, plain = _lib.batch_local_search(dgs, starts, p)
+    targets = np.array([s["max_energy"] for s in plain])
+    targets[0] += 1.0
+    counters = ("rounds", "single_accepts", "double_accepts", "seed_accepts", "walk_accepts", "reheat_accepts",
+                "single_evals", "double_evals", "seed_evals", "gray_evals")
+    ys, st = _lib.learned_search(net, dgs, starts, p, targets=targets, passes=2, fallback=True)
+    xs, ls = _lib.learned_search(net, dgs, starts, p, targets=targets, passes=2)
+    short = [i for i in range(len(Gs)) if ls[i]["max_energy"] < targets[i]]
+    assert short
+    zs, fs = _lib.learned_search(net, [dgs[i] for i in short], [starts[i] for i in short], p,
+                                 targets=targets[short], passes=0)
+    for i in range(len(Gs)):
+        if i in short:
+            t = short.index(i)
+            assert st[i]["fallback"] == 1, i
+            assert np.array_equal(ys[i], zs[t]), i
+            assert st[i]["max_energy"] == fs[t]["max_energy"], i
+            for k in counters:
+                assert st[i][k] == fs[t][k], (i, k)
+            assert st[i]["replaced_single_accepts"] == ls[i]["single_accepts"], i
+            for k in ("learned_accepts", "learned_passes"):
+                assert st[i][k] == ls[i][k], (i, k)
+        else:
+            assert st[i]["fallback"] == 0 and st[i]["replaced_single_accepts"] == 0, i
+            assert np.array_equal(ys[i], xs[i]), i
+            assert st[i]["max_energy"] == ls[i]["max_energy"] >= targets[i], i
+            for k in counters + ("learned_accepts", "learned_passes"):
+                assert st[i][k] == ls[i][k], (i, k)
+    # the call's descent pairs cover both searches, the same in every record
+    pairs = ls[0]["descent_instance_launches"] + fs[0]["descent_instance_launches"]
+    assert all(s["descent_instance_launches"] == pairs for s in st)
