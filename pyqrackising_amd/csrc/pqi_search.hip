@@ -161,6 +161,27 @@ __device__ __forceinline__ T wave_reduce_dpp(T v, Op op) {
 
 __device__ __forceinline__ int32_t max3i(int32_t a, int32_t b, int32_t c) { return max(max(a, b), c); }
 
+// Four one-lane masks at once: m_j = (lane == l_j) ? b_j : 0 for wave-uniform
+// lanes l_j.  The four compares write four scalar pairs and go out before the
+// four selects, so the selects do not wait on each other (compares through
+// VCC would serialise them).
+__device__ __forceinline__ void lane_masks4(int lane, int l0, int l1, int l2, int l3, uint32_t b0, uint32_t b1,
+                                            uint32_t b2, uint32_t b3, uint32_t& m0, uint32_t& m1, uint32_t& m2,
+                                            uint32_t& m3) {
+    uint64_t k0, k1, k2, k3;
+    __asm__(
+        "v_cmp_eq_u32_e64 %4, %8, %12\n\t"
+        "v_cmp_eq_u32_e64 %5, %9, %12\n\t"
+        "v_cmp_eq_u32_e64 %6, %10, %12\n\t"
+        "v_cmp_eq_u32_e64 %7, %11, %12\n\t"
+        "v_cndmask_b32_e64 %0, 0, %13, %4\n\t"
+        "v_cndmask_b32_e64 %1, 0, %14, %5\n\t"
+        "v_cndmask_b32_e64 %2, 0, %15, %6\n\t"
+        "v_cndmask_b32_e64 %3, 0, %16, %7"
+        : "=&v"(m0), "=&v"(m1), "=&v"(m2), "=v"(m3), "=&s"(k0), "=&s"(k1), "=&s"(k2), "=&s"(k3)
+        : "s"(l0), "s"(l1), "s"(l2), "s"(l3), "v"(lane), "v"(b0), "v"(b1), "v"(b2), "v"(b3));
+}
+
 // Integer wave maximum; the identity (INT_MIN) as the DPP fill value lets
 // each step fold into one v_max_i32 with a DPP source.
 __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
@@ -1808,9 +1829,25 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
         uint32_t inc[NW], all[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) inc[w] = all[w] = 0u;
+        uint32_t mk1 = 0u;  // register rows: k's own bit, formed with the neighbours' masks
+        if constexpr (RR) {
+            int32_t a[3];
+            int lu[3];
+            uint32_t bit[3], m[3], fx[3];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (!RR && j >= stride) break;  // register rows: w = 0 entries past the stride
+            for (int j = 0; j < 3; ++j) {
+                a[j] = __builtin_amdgcn_readlane(nb[j], L);
+                lu[j] = (int)__builtin_amdgcn_ubfe((uint32_t)a[j], 5u, 7u);  // 64: a padding entry
+                bit[j] = 1u << (a[j] & 31);
+                fx[j] = (uint32_t)((a[j] >> 31) ^ xkm);
+            }
+            lane_masks4(lane, lu[0], lu[1], lu[2], L, bit[0], bit[1], bit[2], 1u << p, m[0], m[1], m[2], mk1);
+            inc[0] = (m[0] & (X[0] ^ fx[0])) | (m[1] & (X[0] ^ fx[1])) | (m[2] & (X[0] ^ fx[2]));
+            all[0] = m[0] | m[1] | m[2];
+        }
+#pragma unroll
+        for (int j = 0; j < (RR ? 0 : NB); ++j) {
+            if (!RR && j >= stride) break;
             const int32_t a = __builtin_amdgcn_readlane(nb[j], L);
             // lane of u, or 64 for a padding entry (register rows store it as
             // node 2048; table rows are tested here)
@@ -1836,7 +1873,7 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
         uint32_t Q[NP][NW], Y[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            const uint32_t mk = (lane == L && w == ws) ? 1u << p : 0u;
+            const uint32_t mk = RR ? mk1 : ((lane == L && w == ws) ? 1u << p : 0u);
 #pragma unroll
             for (int b = 0; b < NP; ++b) {
                 if constexpr (H == 2) {
