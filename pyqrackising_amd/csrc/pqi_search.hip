@@ -1759,10 +1759,25 @@ __device__ void descent_plane_body(const GView<int8_t>& g, uint8_t* xg, int32_t*
                 // c in 0..3 and Delta > 0 iff c >= 2 (plane 1): both ballots at
                 // once.  With no positive gain cm = 0 stops the descent below,
                 // and the exact top is found after the loop
-                const uint32_t t1 = P[1][0], t2 = t1 & P[0][0];
-                const uint64_t bp = __ballot(t1 != 0u), b2 = __ballot(t2 != 0u);
-                M[0] = b2 ? t2 : t1;
-                bal = (b2 ? b2 : bp) | (1ull << 63);  // lane 63 when empty: in range
+                // (two compares, one scalar test feeding both selects, the
+                // empty case mapped to lane 63 so the lane stays in range)
+                const uint32_t t1 = P[1][0];
+                uint32_t mt, t2;
+                uint64_t bp, b2, sel;
+                __asm__(
+                    "v_and_b32 %[t2], %[t1], %[p0]\n\t"
+                    "v_cmp_ne_u32_e64 %[b2], 0, %[t2]\n\t"
+                    "v_cmp_ne_u32_e64 %[bp], 0, %[t1]\n\t"
+                    "s_cmp_lg_u64 %[b2], 0\n\t"
+                    "s_cselect_b64 %[bal], %[b2], %[bp]\n\t"
+                    "s_cselect_b64 %[sel], -1, 0\n\t"
+                    "s_bitset1_b64 %[bal], 63\n\t"
+                    "v_cndmask_b32_e64 %[mt], %[t1], %[t2], %[sel]"
+                    : [t2] "=&v"(t2), [b2] "=&s"(b2), [bp] "=&s"(bp), [bal] "=&s"(bal), [sel] "=&s"(sel),
+                      [mt] "=v"(mt)
+                    : [t1] "v"(t1), [p0] "v"(P[0][0])
+                    : "scc");
+                M[0] = mt;
                 cm = b2 ? 3 : (bp ? 2 : 0);
             } else {
 #pragma unroll
