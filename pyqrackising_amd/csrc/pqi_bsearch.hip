@@ -12,7 +12,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pqi_internal.h"
@@ -67,10 +69,15 @@ namespace pqi {
 // with exact gains over the spins scored above min_score), stopping early when
 // no instance flips.  With `targets`,
 // an instance stops as soon as its value (energy in spin-glass mode, cut
-// otherwise) reaches its target.
+// otherwise) reaches its target.  `set` selects the context's resources: 0 its
+// stream, batch arena and staging; 1 the second stream's, so that a second
+// search can run beside a first one.  `first_descent`, with targets, is called
+// once after the first round's descent with the instances still below their
+// targets (before the rest of the round).
+using BelowFn = std::function<void(const std::vector<int32_t>&)>;
 static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint8_t* const* thetas,
                        const pqi_search_params* p, pqi_search_stats* stats, pqi_mpnn* net, const double* targets,
-                       int64_t passes, float min_score) {
+                       int64_t passes, float min_score, int set = 0, const BelowFn* first_descent = nullptr) {
     if (!c || !graphs || !thetas || !p || !stats) return fail(PQI_ERR_INVALID, "null argument");
     if (count < 1) return fail(PQI_ERR_INVALID, "count must be >= 1");
     if (p->walkers < 1 || p->seed_multiple < 1) return fail(PQI_ERR_INVALID, "walkers and seed_multiple must be >= 1");
@@ -87,7 +94,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             if (thetas[i][v] > 1) return fail(PQI_ERR_INVALID, "theta entries must be 0 or 1");
     }
     if (hipSetDevice(c->device) != hipSuccess) return fail(PQI_ERR_HIP, "hipSetDevice failed");
-    hipStream_t s = c->stream;
+    hipStream_t s = set ? c->stream2 : c->stream;
+    if (!s) return fail(PQI_ERR_INVALID, "no second stream");
     void* arena = nullptr;
     const size_t fb = kind == PQI_WEIGHTS_I8 ? 4 : 8;
     const int64_t G = p->walkers;
@@ -200,17 +208,17 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
                         align256(L0 * sizeof(BInstRaw));
     {
-        const int e = batch_arena(c, total + recs + misc, &arena);
+        const int e = batch_arena(c, set, total + recs + misc, &arena);
         if (e != PQI_OK) return e;
     }
     // pinned host side: the states, then the records, then a bump area for
     // the lists and moves uploaded between two synchronisations
     const size_t bump_cap = align256(L0 * 16) * 6;
     {
-        const int e = ensure_pinned(c, 2, xbytes + rec_out + 3 * rec_g + align256(L0 * 8) + bump_cap);
+        const int e = ensure_pinned(c, set ? 3 : 2, xbytes + rec_out + 3 * rec_g + align256(L0 * 8) + bump_cap);
         if (e != PQI_OK) return e;
     }
-    char* pin = (char*)c->pin[2].p;
+    char* pin = (char*)c->pin[set ? 3 : 2].p;
     uint8_t* pin_x = (uint8_t*)pin;
     PhaseOut* pin_out = (PhaseOut*)(pin + xbytes);
     double* pin_seld = (double*)(pin + xbytes + rec_out);
@@ -448,6 +456,10 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             for (int32_t i : active)
                 if (!reached(i)) open.push_back(i);
             active.swap(open);
+            if (first_descent) {
+                (*first_descent)(active);
+                first_descent = nullptr;
+            }
             if (active.empty()) {
                 for (int32_t i : round_set) stats[i].seconds_single += t_single / (double)round_set.size();
                 break;
@@ -686,36 +698,94 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
         if (!graphs[i] || !thetas[i]) return fail(PQI_ERR_INVALID, "null graph or state");
         starts[i].assign(thetas[i], thetas[i] + graphs[i]->n);
     }
-    int rc = run_batched(c, graphs, count, thetas, p, stats, use, targets, passes, min_score);
+    if (!c->stream2) {
+        if (hipSetDevice(c->device) != hipSuccess) return fail(PQI_ERR_HIP, "hipSetDevice failed");
+        if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess)
+            return fail(PQI_ERR_HIP, "hipStreamCreate failed");
+    }
+    // The plain search of an instance depends only on its start and target, so
+    // it is started, on the second stream and a thread of its own, for every
+    // instance still below its target after the learned search's first
+    // descent, while the learned search finishes its round.  An instance that
+    // ends below its target was below it then (the value only rises), so the
+    // fallback covers it; the others' fallbacks are discarded.
+    struct Fallback {
+        std::vector<int32_t> list;
+        std::vector<pqi_graph*> gs;
+        std::vector<uint8_t*> ts;
+        std::vector<double> tg;
+        std::vector<pqi_search_stats> st;
+        std::thread th;
+        int rc = PQI_OK;
+        std::string err;
+    };
+    auto start_fallback = [&](Fallback& f, const std::vector<int32_t>& below, int set) {
+        const int64_t k = (int64_t)below.size();
+        f.list = below;
+        f.gs.resize(k);
+        f.ts.resize(k);
+        f.tg.resize(k);
+        f.st.resize(k);
+        for (int64_t t = 0; t < k; ++t) {
+            f.gs[t] = graphs[below[t]];
+            f.ts[t] = starts[below[t]].data();
+            f.tg[t] = targets[below[t]];
+        }
+        f.th = std::thread([&f, c, p, k, set]() {
+            f.rc = run_batched(c, f.gs.data(), k, f.ts.data(), p, f.st.data(), nullptr, f.tg.data(), 0, 0.f, set);
+            if (f.rc != PQI_OK) f.err = pqi_last_error();
+        });
+    };
+    Fallback fb;
+    const BelowFn hook = [&](const std::vector<int32_t>& below) {
+        if (!below.empty()) start_fallback(fb, below, 1);
+    };
+    int rc = run_batched(c, graphs, count, thetas, p, stats, use, targets, passes, min_score, 0, &hook);
+    if (fb.th.joinable()) fb.th.join();
     if (rc != PQI_OK) return rc;
-    std::vector<int64_t> shortl;
+    if (fb.rc != PQI_OK) return fail(fb.rc, fb.err);
+    std::vector<int32_t> shortl;
     for (int64_t i = 0; i < count; ++i) {
         stats[i].fallback = 0;
-        if (stats[i].max_energy < targets[i]) shortl.push_back(i);
+        if (stats[i].max_energy < targets[i]) shortl.push_back((int32_t)i);
     }
     if (shortl.empty()) return PQI_OK;
-    // the plain batched loop from their starts to the same targets
-    const int64_t k = (int64_t)shortl.size();
-    std::vector<pqi_graph*> gs(k);
-    std::vector<uint8_t*> ts(k);
-    std::vector<double> tg(k);
-    std::vector<pqi_search_stats> fs(k);
-    for (int64_t t = 0; t < k; ++t) {
-        const int64_t i = shortl[t];
-        gs[t] = graphs[i];
-        ts[t] = starts[i].data();
-        tg[t] = targets[i];
+    // (a short instance the early fallback did not cover: the learned search
+    // never reached its first descent, e.g. max_rounds < 0; searched now)
+    std::vector<int32_t> slot(count, -1);
+    for (size_t t = 0; t < fb.list.size(); ++t) slot[fb.list[t]] = (int32_t)t;
+    std::vector<int32_t> missing;
+    for (int32_t i : shortl)
+        if (slot[i] < 0) missing.push_back(i);
+    Fallback fb2;
+    if (!missing.empty()) {
+        start_fallback(fb2, missing, 0);
+        fb2.th.join();
+        if (fb2.rc != PQI_OK) return fail(fb2.rc, fb2.err);
     }
-    // the learned call's descent totals (the same in every record)
+    // the learned call's descent totals (the same in every record), and the
+    // fallbacks' (each run's totals are the same in all of its records)
+    double f_s = 0.0;
+    int64_t f_launches = 0, f_pairs = 0, f_writes = 0;
+    for (const Fallback* f : {&fb, &fb2})
+        if (!f->st.empty()) {
+            f_s += f->st[0].descent_kernel_seconds;
+            f_launches += f->st[0].descent_launches;
+            f_pairs += f->st[0].descent_instance_launches;
+            f_writes += f->st[0].descent_instance_writes;
+        }
     const double l_s = stats[0].descent_kernel_seconds;
     const int64_t l_launches = stats[0].descent_launches, l_pairs = stats[0].descent_instance_launches,
                   l_writes = stats[0].descent_instance_writes;
-    rc = run_batched(c, gs.data(), k, ts.data(), p, fs.data(), nullptr, tg.data(), 0, 0.f);
-    if (rc != PQI_OK) return rc;
-    for (int64_t t = 0; t < k; ++t) {
-        const int64_t i = shortl[t];
+    for (int32_t i : shortl) {
+        const bool early = slot[i] >= 0;
+        int32_t t = early ? slot[i] : -1;
+        if (!early)
+            for (size_t u = 0; u < fb2.list.size(); ++u)
+                if (fb2.list[u] == i) t = (int32_t)u;
+        const Fallback& f = early ? fb : fb2;
         std::memcpy(thetas[i], starts[i].data(), graphs[i]->n);
-        pqi_search_stats r = fs[t];
+        pqi_search_stats r = f.st[t];
         r.learned_accepts = stats[i].learned_accepts;
         r.learned_passes = stats[i].learned_passes;
         r.seconds_learned = stats[i].seconds_learned;
@@ -726,10 +796,10 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
     }
     // every record carries the call's descent totals: both searches
     for (int64_t i = 0; i < count; ++i) {
-        stats[i].descent_kernel_seconds = l_s + fs[0].descent_kernel_seconds;
-        stats[i].descent_launches = l_launches + fs[0].descent_launches;
-        stats[i].descent_instance_launches = l_pairs + fs[0].descent_instance_launches;
-        stats[i].descent_instance_writes = l_writes + fs[0].descent_instance_writes;
+        stats[i].descent_kernel_seconds = l_s + f_s;
+        stats[i].descent_launches = l_launches + f_launches;
+        stats[i].descent_instance_launches = l_pairs + f_pairs;
+        stats[i].descent_instance_writes = l_writes + f_writes;
     }
     return PQI_OK;
 }

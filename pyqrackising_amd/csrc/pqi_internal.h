@@ -89,7 +89,10 @@ struct pqi_ctx {
     // being written by the previous call's last, idle iteration never lands on
     // the next call's input)
     // [2] the batched search's state copies, phase records and lists
-    pqi::Pinned pin[3];
+    // [3] the same for a second batched search running beside the first
+    //     (the learned search's fallback, on stream2)
+    pqi::Pinned pin[4];
+    hipStream_t stream2 = nullptr;  // created on first use, default priority
     pqi::LsGraphs ls;
 };
 
@@ -122,7 +125,8 @@ int noalloc_init(int device);
 // demand (the stream is synchronised before an old buffer is freed).
 int ensure_pinned(pqi_ctx* c, int which, size_t bytes);
 // The batched search's device arena, kept by the context and grown on demand.
-int batch_arena(pqi_ctx* c, size_t bytes, void** out);
+// set 0: the context stream's arena; set 1: the second stream's
+int batch_arena(pqi_ctx* c, int set, size_t bytes, void** out);
 // Drift check of an incrementally maintained fp64 field: h is recomputed
 // exactly from W and x and overwritten; entries that had moved by more than
 // 1e-9 max(1, |h_exact|) are counted.  Skipped when gate is non-null and

@@ -44,7 +44,7 @@ int fail(int code, const std::string& msg) {
 // Scratch slots of a context.
 enum Slot {
     S_X, S_H, S_RX, S_RH, S_FIX, S_WX, S_WH, S_WE, S_WEV, S_SEL, S_SELD, S_DELTA, S_Q, S_BITS, S_BATCH,
-    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_STAGE, S_BARENA, S_COUNT
+    S_OUT, S_DBL, S_SCORE, S_HB, S_QB, S_RANK, S_CTL, S_OUT2, S_CUM, S_STAGE, S_BARENA, S_BARENA2, S_COUNT
 };
 
 static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
@@ -53,8 +53,9 @@ static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
     if (bytes == 0) bytes = 16;
     if (b.bytes < bytes) {
         // queued work may still use the old buffer (a local search returns with
-        // its last, idle iteration still on the stream)
-        if (b.p) HIPCHK(hipStreamSynchronize(c->stream));
+        // its last, idle iteration still on the stream; the second batched
+        // stream's arena likewise)
+        if (b.p) HIPCHK(slot == S_BARENA2 ? hipStreamSynchronize(c->stream2) : hipStreamSynchronize(c->stream));
         if (b.p) HIPCHK(hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
@@ -66,7 +67,7 @@ static int ensure(pqi_ctx* c, int slot, size_t bytes, void** out) {
     return PQI_OK;
 }
 
-int batch_arena(pqi_ctx* c, size_t bytes, void** out) { return ensure(c, S_BARENA, bytes, out); }
+int batch_arena(pqi_ctx* c, int set, size_t bytes, void** out) { return ensure(c, set ? S_BARENA2 : S_BARENA, bytes, out); }
 
 // dst row r = src row r (width bytes) followed by zeros up to dpitch
 __global__ void k_restride_words(uint32_t* __restrict__ dst, int64_t dwords, const uint32_t* __restrict__ src,
@@ -298,6 +299,7 @@ int pqi_destroy(pqi_ctx* c) {
     if (!c) return PQI_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     ls_graphs_reset(c);
     if (c->ls.host_ctl) (void)hipHostFree(c->ls.host_ctl);
     if (c->ls.seq_dev) (void)hipFree(c->ls.seq_dev);
@@ -306,6 +308,7 @@ int pqi_destroy(pqi_ctx* c) {
     for (auto& b : c->bufs)
         if (b.p) (void)hipFree(b.p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
     return PQI_OK;
 }
@@ -530,7 +533,7 @@ static int pin_ensure(pqi_ctx* c, int which, size_t bytes) {
     Pinned& pb = c->pin[which];
     if (pb.bytes >= bytes) return PQI_OK;
     if (pb.p) {
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipStreamSynchronize(which == 3 ? c->stream2 : c->stream));
         (void)hipHostFree(pb.p);
         pb = Pinned{};
     }

@@ -205,7 +205,8 @@ def test_passes_zero_is_the_plain_search_to_targets():
 def test_fallback_in_call_matches_two_calls():
     """PQI_FLAG_FALLBACK: the instances that end below their target are
     searched again by the plain batched loop from their starts inside the same
-    call.  The result equals the learned search followed by a separate plain
+    call (started on a second stream after the learned search's first descent,
+    beside the rest of its round).  The result equals the learned search followed by a separate plain
     search (passes = 0) of the short instances: the same states and counters,
     the learned counters kept on the fallback records, and the replaced
     search's single-flip accepts recorded apart.  With the untrained network
@@ -247,6 +248,9 @@ def test_fallback_in_call_matches_two_calls():
             assert st[i]["max_energy"] == ls[i]["max_energy"] >= targets[i], i
             for k in counters + ("learned_accepts", "learned_passes"):
                 assert st[i][k] == ls[i][k], (i, k)
-    # the call's descent pairs cover both searches, the same in every record
-    pairs = ls[0]["descent_instance_launches"] + fs[0]["descent_instance_launches"]
-    assert all(s["descent_instance_launches"] == pairs for s in st)
+    # the call's descent pairs cover both searches, the same in every record;
+    # the fallback starts after the learned search's first descent, for every
+    # instance still below its target then (a superset of the short ones)
+    pairs = {s["descent_instance_launches"] for s in st}
+    assert len(pairs) == 1
+    assert pairs.pop() >= ls[0]["descent_instance_launches"] + fs[0]["descent_instance_launches"]
