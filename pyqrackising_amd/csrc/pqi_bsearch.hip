@@ -11,8 +11,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -20,6 +22,61 @@
 #include "pqi_internal.h"
 
 namespace pqi {
+
+// One job at a time on a thread of its own, kept for the context's lifetime
+// (no thread creation, and the HIP runtime's per-thread set-up done once).
+struct Worker {
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::function<void()> job;
+    bool pending = false, done = true, quit = false;
+
+    Worker() {
+        th = std::thread([this]() {
+            for (;;) {
+                std::function<void()> f;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [this]() { return pending || quit; });
+                    if (!pending) return;
+                    f = std::move(job);
+                    pending = false;
+                }
+                f();
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    done = true;
+                }
+                cv.notify_all();
+            }
+        });
+    }
+    void run(std::function<void()> f) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            job = std::move(f);
+            pending = true;
+            done = false;
+        }
+        cv.notify_all();
+    }
+    void wait() {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [this]() { return done; });
+    }
+};
+
+void worker_stop(Worker* w) {
+    w->wait();
+    {
+        std::lock_guard<std::mutex> lk(w->mu);
+        w->quit = true;
+    }
+    w->cv.notify_all();
+    w->th.join();
+    delete w;
+}
 
 namespace {
 
@@ -715,10 +772,12 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
         std::vector<uint8_t*> ts;
         std::vector<double> tg;
         std::vector<pqi_search_stats> st;
-        std::thread th;
+        bool started = false;
         int rc = PQI_OK;
         std::string err;
     };
+    if (!c->worker) c->worker = new Worker();
+    Worker* wk = c->worker;
     auto start_fallback = [&](Fallback& f, const std::vector<int32_t>& below, int set) {
         const int64_t k = (int64_t)below.size();
         f.list = below;
@@ -731,7 +790,8 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
             f.ts[t] = starts[below[t]].data();
             f.tg[t] = targets[below[t]];
         }
-        f.th = std::thread([&f, c, p, k, set]() {
+        f.started = true;
+        wk->run([&f, c, p, k, set]() {
             f.rc = run_batched(c, f.gs.data(), k, f.ts.data(), p, f.st.data(), nullptr, f.tg.data(), 0, 0.f, set);
             if (f.rc != PQI_OK) f.err = pqi_last_error();
         });
@@ -741,7 +801,7 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
         if (!below.empty()) start_fallback(fb, below, 1);
     };
     int rc = run_batched(c, graphs, count, thetas, p, stats, use, targets, passes, min_score, 0, &hook);
-    if (fb.th.joinable()) fb.th.join();
+    if (fb.started) wk->wait();
     if (rc != PQI_OK) return rc;
     if (fb.rc != PQI_OK) return fail(fb.rc, fb.err);
     std::vector<int32_t> shortl;
@@ -760,7 +820,7 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
     Fallback fb2;
     if (!missing.empty()) {
         start_fallback(fb2, missing, 0);
-        fb2.th.join();
+        wk->wait();
         if (fb2.rc != PQI_OK) return fail(fb2.rc, fb2.err);
     }
     // the learned call's descent totals (the same in every record), and the

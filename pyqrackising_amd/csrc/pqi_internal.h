@@ -47,6 +47,11 @@ struct Buffer {
     size_t bytes = 0;
 };
 
+// A host thread that runs one job at a time for a context (the learned
+// search's fallback, beside the caller's thread); defined in pqi_bsearch.hip.
+struct Worker;
+void worker_stop(Worker* w);
+
 // Pinned host memory with its device-visible address.
 struct Pinned {
     void* p = nullptr;
@@ -93,6 +98,7 @@ struct pqi_ctx {
     //     (the learned search's fallback, on stream2)
     pqi::Pinned pin[4];
     hipStream_t stream2 = nullptr;  // created on first use, default priority
+    pqi::Worker* worker = nullptr;  // the thread that drives stream2 (created on first use)
     pqi::LsGraphs ls;
 };
 

@@ -298,6 +298,7 @@ int pqi_create(int device, pqi_ctx** out) {
 int pqi_destroy(pqi_ctx* c) {
     if (!c) return PQI_OK;
     (void)hipSetDevice(c->device);
+    if (c->worker) worker_stop(c->worker);
     (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     ls_graphs_reset(c);
