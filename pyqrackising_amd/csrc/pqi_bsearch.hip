@@ -405,6 +405,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     }
 
     auto reached = [&](int32_t i) { return targets && stats[i].max_energy >= targets[i]; };
+    int64_t learned_moves = 0;
 
     // learned phase
     if (net && passes > 0) {
@@ -425,6 +426,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
             for (size_t t = 0; t < run.size(); ++t) {
                 pqi_search_stats& st = stats[run[t]];
                 st.learned_accepts += outs[t].accepts;
+                learned_moves += outs[t].accepts;
                 st.learned_passes++;
                 if (outs[t].accepts > 0) st.learned_min_gain = std::min(st.learned_min_gain, outs[t].gain);
                 st.max_energy += scale * outs[t].gain_sum;
@@ -462,6 +464,8 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     std::vector<int32_t> active;
     for (int32_t i : list)
         if (!reached(i) && !(net && p->max_rounds < 0)) active.push_back(i);
+    // the fields are the exact ones of the starts until a move is made
+    bool field_exact = learned_moves == 0, first_round = true;
     std::vector<double> walk_gain(count), reheat(count);
     std::vector<double> we(G);
     std::vector<int64_t> wev(G);
@@ -477,11 +481,17 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         for (int32_t i : active) stats[i].rounds++;
         if (kind != PQI_WEIGHTS_I8) {
             // a new round: the fp64 fields are recomputed exactly (as the
-            // single-instance search does), so drift never carries over
-            BCHK(upload_list(active));
-            launch_b_field(kind, d_inst, d_list, (int64_t)active.size(), max_n, all_sparse, s);
+            // single-instance search does), so drift never carries over.  In
+            // the first round with no learned move before it, the fields are
+            // still the exact ones computed from the starts, so the launch
+            // (which would reproduce them bit for bit) is skipped
+            if (!(field_exact && first_round)) {
+                BCHK(upload_list(active));
+                launch_b_field(kind, d_inst, d_list, (int64_t)active.size(), max_n, all_sparse, s);
+            }
             for (int32_t i : active) stats[i].field_checks++;
         }
+        first_round = false;
 
         // single-flip descent, bounded launches repeated while the bound is hit
         double t0 = now_s();
