@@ -538,11 +538,15 @@ struct pqi_mpnn {
     float *M3 = nullptr, *Z3 = nullptr, *H3 = nullptr, *score = nullptr, *y = nullptr, *ds = nullptr;
     float *dA = nullptr, *dB = nullptr, *dC = nullptr, *part = nullptr;
     double* lpart = nullptr;
+    // the BF16 copy of Theta_2/3 in `part` matches P (cleared when P changes
+    // or a backward pass reuses `part`)
+    bool theta_bf16 = false;
 };
 
 namespace pqi {
 
 static void mpnn_free_batch(pqi_mpnn* m) {
+    m->theta_bf16 = false;
     float** fs[] = {&m->X0, &m->M1, &m->Z1, &m->H1, &m->M2, &m->Z2, &m->H2, &m->M3, &m->Z3,
                     &m->H3, &m->score, &m->y, &m->ds, &m->dA, &m->dB, &m->dC, &m->part, &m->w};
     for (float** f : fs)
@@ -599,7 +603,10 @@ static void forward_infer(pqi_mpnn* m, hipStream_t s) {
     }
     const int64_t N = m->N;
     uint16_t* Tb = (uint16_t*)m->part;
-    hipLaunchKernelGGL(k_theta_bf16, dim3((unsigned)((4 * HID * HID + 255) / 256)), dim3(256), 0, s, m->P, Tb);
+    if (!m->theta_bf16) {
+        hipLaunchKernelGGL(k_theta_bf16, dim3((unsigned)((4 * HID * HID + 255) / 256)), dim3(256), 0, s, m->P, Tb);
+        m->theta_bf16 = true;
+    }
     const int64_t t1 = (N + 255) / 256, tl = (N + 127) / 128;
     hipLaunchKernelGGL(k_infer_l1, dim3((unsigned)(8 * ((t1 + 7) / 8))), dim3(256), 0, s, m->ptr, m->idx, m->w, m->X0,
                        m->P, N, t1, (uint16_t*)m->H1);
@@ -612,6 +619,7 @@ static void forward_infer(pqi_mpnn* m, hipStream_t s) {
 
 // loss and gradient into m->G; returns the loss through *loss (after sync)
 static int backward(pqi_mpnn* m, hipStream_t s, double* loss) {
+    m->theta_bf16 = false;
     const int64_t N = m->N;
     const unsigned g4 = (unsigned)((N + 3) / 4);
     const int64_t nb = (N + 255) / 256;
@@ -740,6 +748,7 @@ int pqi_mpnn_get_params(pqi_mpnn* m, float* out) {
 
 int pqi_mpnn_set_params(pqi_mpnn* m, const float* in) {
     if (!m || !in) return fail(PQI_ERR_INVALID, "null argument");
+    m->theta_bf16 = false;
     if (hipMemcpy(m->P, in, NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess) return fail(PQI_ERR_HIP, "copy failed");
     std::vector<float> zeros(NPARAM, 0.f);
     (void)hipMemcpy(m->m1, zeros.data(), NPARAM * 4, hipMemcpyHostToDevice);
@@ -858,6 +867,7 @@ int pqi_mpnn_loss_grad(pqi_mpnn* m, const float* feats, const float* labels, dou
 int pqi_mpnn_train_step(pqi_mpnn* m, const float* feats, const float* labels, float lr, double* loss) {
     int r = pqi_mpnn_loss_grad(m, feats, labels, loss, nullptr);
     if (r != PQI_OK) return r;
+    m->theta_bf16 = false;  // P changes below
     m->steps++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, (float)m->steps), c2 = 1.f - std::pow(b2, (float)m->steps);
