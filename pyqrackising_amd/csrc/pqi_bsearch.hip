@@ -363,7 +363,17 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         std::memcpy(hp, src, bytes);
         return hipMemcpyAsync(dst, hp, bytes, hipMemcpyHostToDevice, s);
     };
-    auto upload_list = [&](const std::vector<int32_t>& l) -> hipError_t { return upload(d_list, l.data(), l.size() * 4); };
+    // the list now in d_list (only this lambda writes it): a phase over the
+    // same instances as the previous one uploads nothing
+    std::vector<int32_t> list_on_device;
+    bool list_known = false;
+    auto upload_list = [&](const std::vector<int32_t>& l) -> hipError_t {
+        if (list_known && l == list_on_device) return hipSuccess;
+        const hipError_t e = upload(d_list, l.data(), l.size() * 4);
+        list_known = e == hipSuccess;
+        if (list_known) list_on_device = l;
+        return e;
+    };
     auto read_outs = [&](const std::vector<int32_t>& l) -> hipError_t {
         hipError_t e = hipMemcpyAsync(pin_out, d_out, count * sizeof(PhaseOut), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = sync();
