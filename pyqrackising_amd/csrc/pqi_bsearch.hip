@@ -262,8 +262,11 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     // totals (G per instance), ages of the learned phase
     const size_t rec_out = align256(L0 * sizeof(PhaseOut)), rec_g = align256(L0 * G * 8);
     const size_t recs = xbytes + rec_out + 3 * rec_g + (net ? xbytes : 0);
+    // learned passes between two synchronisations (one record per pass each)
+    constexpr int64_t PG = 4;
+    const size_t lp_gate = net ? align256(L0 * sizeof(LpGate)) : 0, lp_outs = net ? align256(L0 * PG * sizeof(PhaseOut)) : 0;
     const size_t misc = align256(L0 * 4) + align256(L0 * 16) + align256(L0 * 4) + align256(L0 * 8) +
-                        align256(L0 * sizeof(BInstRaw));
+                        align256(L0 * sizeof(BInstRaw)) + lp_gate + lp_outs;
     {
         const int e = batch_arena(c, set, total + recs + misc, &arena);
         if (e != PQI_OK) return e;
@@ -272,7 +275,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     // the lists and moves uploaded between two synchronisations
     const size_t bump_cap = align256(L0 * 16) * 6;
     {
-        const int e = ensure_pinned(c, set ? 3 : 2, xbytes + rec_out + 3 * rec_g + align256(L0 * 8) + bump_cap);
+        const int e = ensure_pinned(c, set ? 3 : 2, xbytes + rec_out + 3 * rec_g + align256(L0 * 8) + bump_cap + lp_outs);
         if (e != PQI_OK) return e;
     }
     char* pin = (char*)c->pin[set ? 3 : 2].p;
@@ -283,6 +286,7 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     int64_t* pin_wev = (int64_t*)(pin + xbytes + rec_out + 2 * rec_g);
     double* pin_q = (double*)(pin + xbytes + rec_out + 3 * rec_g);
     char* pin_bump = pin + xbytes + rec_out + 3 * rec_g + align256(L0 * 8);
+    PhaseOut* pin_lp = (PhaseOut*)(pin_bump + bump_cap);
     size_t bump_used = 0;
     char* a = (char*)arena;
     char* rec = a + total;
@@ -331,6 +335,10 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
     double* d_q = (double*)m;
     m += align256(L0 * 8);
     BInstRaw* d_inst = (BInstRaw*)m;
+    m += align256(L0 * sizeof(BInstRaw));
+    LpGate* d_gate = (LpGate*)m;
+    m += lp_gate;
+    PhaseOut* d_lp = (PhaseOut*)m;
     BCHK(hipMemcpyAsync(d_inst, h_inst.data(), count * sizeof(BInstRaw), hipMemcpyHostToDevice, s));
     for (int64_t i = 0; i < count; ++i) std::memcpy(pin_x + xoff[i], thetas[i], h_inst[i].n);
     BCHK(hipMemcpyAsync(d_xall, pin_x, xbytes, hipMemcpyHostToDevice, s));
@@ -424,25 +432,52 @@ static int run_batched(pqi_ctx* c, pqi_graph* const* graphs, int64_t count, uint
         std::vector<int32_t> run;
         for (int32_t i : list)
             if (!reached(i)) run.push_back(i);
-        for (int64_t pass = 0; pass < passes && !run.empty(); ++pass) {
-            BCHK(upload_list(list));
-            launch_b_features(kind, d_inst, d_list, count, max_n, s);
-            mpnn_forward_device(net, s);
-            BCHK(upload_list(run));
-            launch_b_order(d_inst, d_list, (int64_t)run.size(), s);
-            launch_b_learned_pass(kind, d_inst, d_list, (int64_t)run.size(), max_n, min_score, s);
-            BCHK(read_outs(run));
-            std::vector<int32_t> again;
-            for (size_t t = 0; t < run.size(); ++t) {
-                pqi_search_stats& st = stats[run[t]];
-                st.learned_accepts += outs[t].accepts;
-                learned_moves += outs[t].accepts;
-                st.learned_passes++;
-                if (outs[t].accepts > 0) st.learned_min_gain = std::min(st.learned_min_gain, outs[t].gain);
-                st.max_energy += scale * outs[t].gain_sum;
-                if (outs[t].accepts > 0 && !reached(run[t])) again.push_back(run[t]);
+        // Up to PG passes between two synchronisations: each instance's gate
+        // (its value, target and activity) is kept on the device, a pass over
+        // an inactive instance returns at once, and the host then replays the
+        // group's records with the same rule (an instance goes on while its
+        // pass moved and it is below its target).
+        if (!run.empty()) {
+            std::vector<LpGate> gates(count);
+            for (int64_t i = 0; i < count; ++i) {
+                gates[i].energy = stats[i].max_energy;
+                gates[i].target = targets ? targets[i] : INFINITY;
+                gates[i].scale = scale;
+                gates[i].active = 0;
+                gates[i].pad = 0;
+                gates[i].outs = d_lp + i * PG;
             }
-            run.swap(again);
+            for (int32_t i : run) gates[i].active = 1;
+            BCHK(upload(d_gate, gates.data(), count * sizeof(LpGate)));
+        }
+        for (int64_t pass0 = 0; pass0 < passes && !run.empty(); pass0 += PG) {
+            const int64_t group = std::min<int64_t>(PG, passes - pass0);
+            const std::vector<int32_t> launched = run;  // a superset of every pass's actives in the group
+            for (int64_t q = 0; q < group; ++q) {
+                BCHK(upload_list(list));
+                launch_b_features(kind, d_inst, d_list, count, max_n, s);
+                mpnn_forward_device(net, s);
+                BCHK(upload_list(launched));
+                launch_b_order(d_inst, d_list, (int64_t)launched.size(), s);
+                launch_b_learned_pass(kind, d_inst, d_list, (int64_t)launched.size(), max_n, min_score, d_gate, (int)q,
+                                      s);
+            }
+            BCHK(hipMemcpyAsync(pin_lp, d_lp, count * PG * sizeof(PhaseOut), hipMemcpyDeviceToHost, s));
+            BCHK(sync());
+            for (int64_t q = 0; q < group && !run.empty(); ++q) {
+                std::vector<int32_t> again;
+                for (int32_t i : run) {
+                    const PhaseOut& o = pin_lp[i * PG + q];
+                    pqi_search_stats& st = stats[i];
+                    st.learned_accepts += o.accepts;
+                    learned_moves += o.accepts;
+                    st.learned_passes++;
+                    if (o.accepts > 0) st.learned_min_gain = std::min(st.learned_min_gain, o.gain);
+                    st.max_energy += scale * o.gain_sum;
+                    if (o.accepts > 0 && !reached(i)) again.push_back(i);
+                }
+                run.swap(again);
+            }
         }
         const double dt = now_s() - t0;
         for (int64_t i = 0; i < count; ++i) stats[i].seconds_learned += dt / (double)count;

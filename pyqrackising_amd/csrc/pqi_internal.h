@@ -413,8 +413,19 @@ void launch_b_quad(int kind, const BInstRaw* d, const int32_t* list, int64_t L, 
 // pass over the nodes in that order with exact gains (out->accepts, out->gain_sum)
 void launch_b_features(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s);
 void launch_b_order(const BInstRaw* d, const int32_t* list, int64_t L, hipStream_t s);
+// Learned passes without a synchronisation per pass: an instance's running
+// value (start value plus scale times each pass's gain sum, in pass order),
+// its target, and whether it still takes passes (its last pass moved and it
+// is below its target); each pass's record goes to outs[pass].
+struct LpGate {
+    double energy, target, scale;
+    int32_t active, pad;
+    PhaseOut* outs;
+};
+// gate: nullptr, or one LpGate per instance (an inactive instance's launch
+// returns at once; an active one records its pass and updates its gate)
 void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n,
-                           float min_score, hipStream_t s);
+                           float min_score, LpGate* gate, int pass, hipStream_t s);
 
 // candidate sampler (pqi_sampler.hip): shots rows of packed candidates from the
 // Hamming-weight CDF and the neighbour-repulsion picker, shot indices shot0..

@@ -4549,15 +4549,35 @@ __device__ void learned_pass_staged(const BInstRaw& r, const GView<WT>& g, float
 // candidates are evaluated at once (one per lane); a ballot finds the first
 // positive one, it is applied, and the later lanes re-evaluate.
 template <typename WT, typename FT>
+__device__ void learned_pass_plain(const BInstRaw& r, const GView<WT>& g, int use_lds, float min_score,
+                                   unsigned char* lds);
+
+template <typename WT, typename FT>
 __global__ __launch_bounds__(64) void kb_learned_pass(const BInstRaw* insts, const int32_t* list, int use_lds,
-                                                     float min_score) {
+                                                     float min_score, LpGate* gate, int pass) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const BInstRaw& r = insts[list[blockIdx.x]];
+    const int i = list[blockIdx.x];
+    if (gate && !gate[i].active) return;  // wave-uniform: this instance takes no more passes
+    const BInstRaw& r = insts[i];
     const GView<WT> g = gview_of<WT>(r);
-    if (use_lds == 2 && r.use_adj) {
+    if (use_lds == 2 && r.use_adj)
         learned_pass_staged<WT, FT>(r, g, min_score, lds);
-        return;
+    else
+        learned_pass_plain<WT, FT>(r, g, use_lds, min_score, lds);
+    if (gate && threadIdx.x == 0) {
+        // the record lane 0 has just written; the same bookkeeping as the
+        // host's (scale is 1 or 2, so the product is exact)
+        LpGate& gt = gate[i];
+        const PhaseOut o = *r.out;
+        gt.outs[pass] = o;
+        gt.energy = gt.energy + gt.scale * o.gain_sum;
+        gt.active = (o.accepts > 0 && gt.energy < gt.target) ? 1 : 0;
     }
+}
+
+template <typename WT, typename FT>
+__device__ void learned_pass_plain(const BInstRaw& r, const GView<WT>& g, int use_lds, float min_score,
+                                   unsigned char* lds) {
     const int lane = threadIdx.x;
     const int64_t n = r.n;
     FT* hg = (FT*)r.h;
@@ -4636,7 +4656,7 @@ static void b_features_impl(const BInstRaw* d, const int32_t* list, int64_t L, i
 
 template <typename WT>
 static void b_learned_impl(const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, float min_score,
-                           hipStream_t s) {
+                           LpGate* gate, int pass, hipStream_t s) {
     using FT = typename FieldOf<WT>::T;
     // 2: the staged pass (sparse graphs), 1: gains and spins in LDS, 0: global
     size_t lds = (((size_t)max_n * sizeof(FT) + 15) & ~(size_t)15) + (size_t)max_n;
@@ -4648,7 +4668,7 @@ static void b_learned_impl(const BInstRaw* d, const int32_t* list, int64_t L, in
         lds = staged;
     }
     hipLaunchKernelGGL((kb_learned_pass<WT, FT>), dim3((unsigned)L), dim3(64), use_lds ? lds : 0, s, d, list, use_lds,
-                       min_score);
+                       min_score, gate, pass);
 }
 
 void launch_b_features(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n, hipStream_t s) {
@@ -4658,8 +4678,8 @@ void launch_b_order(const BInstRaw* d, const int32_t* list, int64_t L, hipStream
     hipLaunchKernelGGL(kb_order, dim3((unsigned)L), dim3(256), 0, s, d, list);
 }
 void launch_b_learned_pass(int kind, const BInstRaw* d, const int32_t* list, int64_t L, int64_t max_n,
-                           float min_score, hipStream_t s) {
-    PQI_KIND_DISPATCH(kind, b_learned_impl, d, list, L, max_n, min_score, s);
+                           float min_score, LpGate* gate, int pass, hipStream_t s) {
+    PQI_KIND_DISPATCH(kind, b_learned_impl, d, list, L, max_n, min_score, gate, pass, s);
 }
 
 }  // namespace pqi
