@@ -4271,12 +4271,114 @@ __global__ __launch_bounds__(256) void kb_features(const BInstRaw* insts, const 
 }
 
 // Stable counting sort into 256 score buckets, highest scores first; within a
-// bucket nodes keep index order.  One work group per instance.
+// bucket nodes keep index order.  One work group per instance.  Up to 8192
+// nodes, each thread's 32 scores are read once, all in flight, and kept in
+// registers for the range, the counts and the scatter.
+__device__ void order_in_registers(const BInstRaw& r, int t) {
+    constexpr int J = 32, KC = 4;
+    __shared__ float lo_w[4], hi_w[4];
+    __shared__ int32_t cnt[256];
+    __shared__ int32_t wcnt[KC * 4][256];
+    const int n = (int)r.n;
+    const int lane = t & 63, wid = t >> 6;
+    float sc[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int v = t + 256 * j;
+        sc[j] = r.score[v < n ? v : 0];
+    }
+    float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        if (t + 256 * j < n) {
+            lo = fminf(lo, sc[j]);
+            hi = fmaxf(hi, sc[j]);
+        }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, m));
+        hi = fmaxf(hi, __shfl_xor(hi, m));
+    }
+    if (lane == 0) {
+        lo_w[wid] = lo;
+        hi_w[wid] = hi;
+    }
+    cnt[t] = 0;
+#pragma unroll
+    for (int w = 0; w < KC * 4; ++w) wcnt[w][t] = 0;
+    __syncthreads();
+    lo = fminf(fminf(lo_w[0], lo_w[1]), fminf(lo_w[2], lo_w[3]));
+    hi = fmaxf(fmaxf(hi_w[0], hi_w[1]), fmaxf(hi_w[2], hi_w[3]));
+    const float span = hi > lo ? hi - lo : 1.f;
+    auto bucket = [&](float x) {
+        int b = (int)((hi - x) / span * 255.f);
+        return b < 0 ? 0 : (b > 255 ? 255 : b);
+    };
+    int bk[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        bk[j] = bucket(sc[j]);
+        if (t + 256 * j < n) atomicAdd(&cnt[bk[j]], 1);
+    }
+    __syncthreads();
+    if (t == 0) {
+        int32_t run = 0;
+        for (int b = 0; b < 256; ++b) {
+            const int32_t c = cnt[b];
+            cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    // KC chunks of 256 nodes per round, as in the general path below
+#pragma unroll
+    for (int c = 0; c < J; c += KC) {
+        if (256 * c >= n) break;  // uniform
+        int rank[KC];
+        bool ok[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            ok[k] = t + 256 * (c + k) < n;
+            uint64_t same = __ballot(ok[k]);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                const uint64_t m = __ballot(ok[k] && ((bk[c + k] >> bit) & 1));
+                same &= ((bk[c + k] >> bit) & 1) ? m : ~m;
+            }
+            rank[k] = __popcll(same & ((1ull << lane) - 1ull));
+            if (ok[k] && rank[k] == 0) wcnt[4 * k + wid][bk[c + k]] = __popcll(same);
+        }
+        __syncthreads();
+        {
+            int32_t run = cnt[t];
+#pragma unroll
+            for (int w = 0; w < KC * 4; ++w) {
+                const int32_t x = wcnt[w][t];
+                wcnt[w][t] = run;
+                run += x;
+            }
+            cnt[t] = run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KC; ++k)
+            if (ok[k]) r.perm[wcnt[4 * k + wid][bk[c + k]] + rank[k]] = t + 256 * (c + k);
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < KC * 4; ++w) wcnt[w][t] = 0;
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int32_t* list) {
     __shared__ float lo_s[256], hi_s[256];
     __shared__ int32_t cnt[256];
     const BInstRaw& r = insts[list[blockIdx.x]];
     const int t = threadIdx.x;
+    if (r.n <= 8192) {
+        order_in_registers(r, t);
+        return;
+    }
     float lo = INFINITY, hi = -INFINITY;
     for (int64_t v = t; v < r.n; v += 256) {
         const float sc = r.score[v];
@@ -4304,7 +4406,8 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
     for (int64_t v = t; v < r.n; v += 256) atomicAdd(&cnt[bucket(r.score[v])], 1);
     __syncthreads();
     // bucket starts (exclusive prefix of the counts)
-    __shared__ int32_t wcnt[4][256];
+    constexpr int KC = 4;  // 256-node chunks per round
+    __shared__ int32_t wcnt[KC * 4][256];
     if (t == 0) {
         int32_t run = 0;
         for (int b = 0; b < 256; ++b) {
@@ -4314,29 +4417,38 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
         }
     }
 #pragma unroll
-    for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+    for (int w = 0; w < KC * 4; ++w) wcnt[w][t] = 0;
     __syncthreads();
-    // stable scatter, 256 nodes in index order at a time: the lanes of a wave
-    // with the same bucket are found by 8 ballots (one per bucket bit) and
-    // ranked by lane; each bucket's offsets advance wave by wave
+    // stable scatter, KC x 256 nodes in index order per round: in each chunk
+    // the lanes of a wave with the same bucket are found by 8 ballots (one per
+    // bucket bit) and ranked by lane; each bucket's offsets advance over the
+    // (chunk, wave) groups in node order, so a round takes three barriers
     const int lane = t & 63, wid = t >> 6;
-    for (int64_t c0 = 0; c0 < r.n; c0 += 256) {
-        const int64_t v = c0 + t;
-        const bool ok = v < r.n;
-        const int b = ok ? bucket(r.score[v]) : 0;
-        uint64_t same = __ballot(ok);
+    for (int64_t c0 = 0; c0 < r.n; c0 += KC * 256) {
+        int b[KC], rank[KC];
+        bool ok[KC];
 #pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-            const uint64_t m = __ballot(ok && ((b >> bit) & 1));
-            same &= ((b >> bit) & 1) ? m : ~m;
+        for (int k = 0; k < KC; ++k) {
+            const int64_t v = c0 + 256 * k + t;
+            ok[k] = v < r.n;
+            b[k] = ok[k] ? bucket(r.score[v]) : 0;
         }
-        const int rank = __popcll(same & ((1ull << lane) - 1ull));
-        if (ok && rank == 0) wcnt[wid][b] = __popcll(same);
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            uint64_t same = __ballot(ok[k]);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                const uint64_t m = __ballot(ok[k] && ((b[k] >> bit) & 1));
+                same &= ((b[k] >> bit) & 1) ? m : ~m;
+            }
+            rank[k] = __popcll(same & ((1ull << lane) - 1ull));
+            if (ok[k] && rank[k] == 0) wcnt[4 * k + wid][b[k]] = __popcll(same);
+        }
         __syncthreads();
         {
             int32_t run = cnt[t];  // thread t: bucket t
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
+            for (int w = 0; w < KC * 4; ++w) {
                 const int32_t c = wcnt[w][t];
                 wcnt[w][t] = run;
                 run += c;
@@ -4344,10 +4456,12 @@ __global__ __launch_bounds__(256) void kb_order(const BInstRaw* insts, const int
             cnt[t] = run;
         }
         __syncthreads();
-        if (ok) r.perm[wcnt[wid][b] + rank] = (int32_t)v;
+#pragma unroll
+        for (int k = 0; k < KC; ++k)
+            if (ok[k]) r.perm[wcnt[4 * k + wid][b[k]] + rank[k]] = (int32_t)(c0 + 256 * k + t);
         __syncthreads();
 #pragma unroll
-        for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+        for (int w = 0; w < KC * 4; ++w) wcnt[w][t] = 0;
         __syncthreads();
     }
 }
