@@ -112,7 +112,8 @@ typedef struct {
 #define PQI_FLAG_FALLBACK 8     /* pqi_learned_search with targets and passes > 0: an instance that ends
                                    below its target is searched again by the plain batched loop from its
                                    own start, to the same target, within the same call; its record is then
-                                   the plain search's (fallback = 1), with the learned phase's counters kept */
+                                   the plain search's (fallback = 1), with the learned phase's counters kept.
+                                   The fallback runs the loop with max_rounds 0 when the caller's is < 0 */
 
 typedef struct {
     double max_energy; /* reference bookkeeping: start value plus accepted gains */

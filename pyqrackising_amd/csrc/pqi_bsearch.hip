@@ -831,6 +831,10 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
         int rc = PQI_OK;
         std::string err;
     };
+    // the fallback is the plain search: max_rounds < 0 (learned passes only)
+    // limits the learned search alone
+    pqi_search_params pf = *p;
+    if (pf.max_rounds < 0) pf.max_rounds = 0;
     if (!c->worker) c->worker = new Worker();
     Worker* wk = c->worker;
     auto start_fallback = [&](Fallback& f, const std::vector<int32_t>& below, int set) {
@@ -846,8 +850,8 @@ int pqi_learned_search(pqi_ctx* c, pqi_mpnn* net, pqi_graph* const* graphs, int6
             f.tg[t] = targets[below[t]];
         }
         f.started = true;
-        wk->run([&f, c, p, k, set]() {
-            f.rc = run_batched(c, f.gs.data(), k, f.ts.data(), p, f.st.data(), nullptr, f.tg.data(), 0, 0.f, set);
+        wk->run([&f, c, &pf, k, set]() {
+            f.rc = run_batched(c, f.gs.data(), k, f.ts.data(), &pf, f.st.data(), nullptr, f.tg.data(), 0, 0.f, set);
             if (f.rc != PQI_OK) f.err = pqi_last_error();
         });
     };

@@ -254,3 +254,36 @@ def test_fallback_in_call_matches_two_calls():
     pairs = {s["descent_instance_launches"] for s in st}
     assert len(pairs) == 1
     assert pairs.pop() >= ls[0]["descent_instance_launches"] + fs[0]["descent_instance_launches"]
+
+
+def test_fallback_after_learned_only_search():
+    """PQI_FLAG_FALLBACK with max_rounds < 0 (the learned passes only, no round
+    of the loop): no first descent, so the fallback of every short instance
+    runs after the learned search, as the plain search (max_rounds 0), with
+    the same records as two calls."""
+    log("learned-only search with the in-call fallback vs two calls")
+    rng = np.random.default_rng(101)
+    Gs = [er_uniform(int(rng.integers(300, 600)), 8.0, 950 + t) for t in range(5)]
+    dgs = [_lib.DeviceGraph.csr(G) for G in Gs]
+    starts = [rng.integers(0, 2, G.shape[0]).astype(np.uint8) for G in Gs]
+    net = _lib.Mpnn(seed=6)
+    net.bind(dgs)
+    p = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=500)
+    _, plain = _lib.batch_local_search(dgs, starts, p)
+    targets = np.array([s["max_energy"] for s in plain])
+    q = _lib.make_params(0, is_spin_glass=1, walkers=4, seed_multiple=2, gray_iterations=500, max_rounds=-1)
+    ys, st = _lib.learned_search(net, dgs, starts, q, targets=targets, passes=2, fallback=True)
+    xs, ls = _lib.learned_search(net, dgs, starts, q, targets=targets, passes=2)
+    short = [i for i in range(len(Gs)) if ls[i]["max_energy"] < targets[i]]
+    assert short  # learned passes alone stop short of the plain search's energies
+    zs, fs = _lib.learned_search(net, [dgs[i] for i in short], [starts[i] for i in short], p,
+                                 targets=targets[short], passes=0)
+    for i in range(len(Gs)):
+        if i in short:
+            t = short.index(i)
+            assert st[i]["fallback"] == 1 and np.array_equal(ys[i], zs[t]), i
+            assert st[i]["max_energy"] == fs[t]["max_energy"] >= targets[i], i
+            assert st[i]["single_accepts"] == fs[t]["single_accepts"], i
+            assert st[i]["learned_accepts"] == ls[i]["learned_accepts"], i
+        else:
+            assert st[i]["fallback"] == 0 and np.array_equal(ys[i], xs[i]), i
