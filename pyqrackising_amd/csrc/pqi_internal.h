@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <cstddef>
 
 #include <string>
 #include <vector>
@@ -274,8 +275,24 @@ struct LsReport {
     LsCtl ctl;
     FieldRepair fix;
     double cut, energy;  // exact score of the state after the iteration
+    unsigned long long check;  // ls_report_check of this report, its number and its state
     unsigned long long seq;  // written last: the number of the report
 };
+// Check word of a report: its number, its 8-byte words before `check` (control
+// record, repairs, score; the structs have no padding) and the sum of its
+// state's terms, each word weighted by an odd multiplier of its position, so
+// that a record left by another report (another number) or read in part fails it.
+__host__ __device__ inline unsigned long long ls_state_term(unsigned long long word, unsigned long long j) {
+    return (word ^ j) * (0xA24BAED4963EE407ull + 2ull * j);
+}
+__host__ __device__ inline unsigned long long ls_report_check(const LsReport& r, unsigned long long xsum,
+                                                              unsigned long long k) {
+    const unsigned long long* w = (const unsigned long long*)&r;
+    unsigned long long h = xsum + k * 0x9E3779B97F4A7C15ull;
+    for (unsigned long long i = 0; i < offsetof(LsReport, check) / 8; ++i)
+        h += (w[i] ^ i) * (0xD6E8FEB86659FD93ull + 2ull * i);
+    return h;
+}
 // int8 weights with adjacency lists: the whole start of a local search in one launch
 // (theta_dev: the caller's 0/1 bytes in pinned host memory, device-visible address)
 void launch_ls_start_i8(const Graph& g, const uint8_t* theta_dev, uint8_t* x, void* h, uint8_t* rx, void* rh,

@@ -567,6 +567,42 @@ static int wait_report(pqi_ctx* c, const LsReport* r, unsigned long long want) {
     }
 }
 
+// A report read whole: its number, and a check word that matches its record
+// and its state (n bytes) as read now.
+static bool report_whole(const LsReport* r, const uint8_t* xo, int64_t n, unsigned long long want) {
+    LsReport loc;
+    std::memcpy(&loc, r, sizeof(loc));
+    if (loc.seq != want) return false;
+    unsigned long long xs = 0;
+    const int64_t n8 = n >> 3;
+    for (int64_t j = 0; j < n8; ++j) {
+        unsigned long long w;
+        std::memcpy(&w, xo + 8 * j, 8);
+        xs += ls_state_term(w, (unsigned long long)j);
+    }
+    if (n & 7) {
+        unsigned long long t = 0;
+        for (int64_t v = n8 << 3; v < n; ++v) t |= (unsigned long long)xo[v] << (8 * (v - (n8 << 3)));
+        xs += ls_state_term(t, (unsigned long long)n8);
+    }
+    return ls_report_check(loc, xs, want) == loc.check;
+}
+
+// Waits until the report numbered `want` reads whole; after about a thousand
+// tries the stream is drained and the report read once more (then a mismatch
+// is an error).
+static int settle_report(pqi_ctx* c, const LsReport* r, const uint8_t* xo, int64_t n, unsigned long long want) {
+    for (uint32_t spins = 1;; ++spins) {
+        if (report_whole(r, xo, n, want)) return PQI_OK;
+        if ((spins & 0x3FF) == 0) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (report_whole(r, xo, n, want)) return PQI_OK;
+            return fail(PQI_ERR_HIP, "local search: an iteration report does not match its check word");
+        }
+        __builtin_ia32_pause();
+    }
+}
+
 static int score_device(pqi_graph* g, const uint8_t* x, double* cut, double* en) {
     pqi_ctx* c = g->ctx;
     double* out;
@@ -1404,7 +1440,16 @@ static int local_search_device(pqi_graph* g, uint8_t* theta, const pqi_search_pa
                 st->descent_kernel_seconds += dtimer[slot]->seconds();
                 st->descent_launches++;
             }
-            if (ls_stopped(&hc)) break;
+            if (ls_stopped(&hc)) {
+                // The stop, the state and the score are taken only from a
+                // report whose check word matches: a number seen before the
+                // rest of its report would otherwise leave the slot's previous
+                // record (another call's final, stopped one) to be read.  If
+                // the whole report goes on, so does the loop.
+                CHK(settle_report(c, rep[slot], xout[slot], n, want[slot]));
+                std::memcpy(&hc, &rep[slot]->ctl, sizeof(hc));
+                if (ls_stopped(&hc)) break;
+            }
             slot ^= 1;
         }
         c->ls.seq_dirty = false;

@@ -3640,23 +3640,39 @@ __device__ void ls_report_body(const uint8_t* x, const int32_t* h, int64_t n, in
         const uint8_t xv = x[v];
         q += xv ? -(long long)h[v] : (long long)h[v];
     }
+    long long xs = 0;  // the state's terms of the check word (wave 0)
     if (threadIdx.x < 64) {
         // 16-byte pieces, then the ragged end byte by byte (x and x_out are 256-byte aligned)
         const int64_t nv = n >> 4;
         for (int64_t k = threadIdx.x; k < nv; k += 64) ((uint4*)x_out)[k] = ((const uint4*)x)[k];
         for (int64_t v = (nv << 4) + threadIdx.x; v < n; v += 64) x_out[v] = x[v];
+        const int64_t n8 = n >> 3;
+        for (int64_t j = threadIdx.x; j < n8; j += 64)
+            xs += (long long)ls_state_term(((const unsigned long long*)x)[j], (unsigned long long)j);
+        if (threadIdx.x == 0 && (n & 7)) {
+            unsigned long long t = 0;
+            for (int64_t v = n8 << 3; v < n; ++v) t |= (unsigned long long)x[v] << (8 * (v - (n8 << 3)));
+            xs += (long long)ls_state_term(t, (unsigned long long)n8);
+        }
     }
     q = wave_sum(q);
+    xs = wave_sum(xs);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = q;
     __syncthreads();
     if (threadIdx.x < 64) {
         if (threadIdx.x == 0) {
             long long qs = 0;
             for (int i = 0; i < (int)(blockDim.x >> 6); ++i) qs += part[i];
-            rep->cut = (double)((2 * w_up_i - qs) / 4);
-            rep->energy = (double)(-qs / 2);
-            rep->ctl = *c;
-            rep->fix = *fix;
+            LsReport loc;
+            loc.ctl = *c;
+            loc.fix = *fix;
+            loc.cut = (double)((2 * w_up_i - qs) / 4);
+            loc.energy = (double)(-qs / 2);
+            rep->cut = loc.cut;
+            rep->energy = loc.energy;
+            rep->ctl = loc.ctl;
+            rep->fix = loc.fix;
+            rep->check = ls_report_check(loc, (unsigned long long)xs, *seq + 1);
         }
         // A fence orders only its own thread's stores: every thread that
         // stored part of the state or the report (wave 0) fences them at
