@@ -177,3 +177,42 @@ def test_local_search_at_the_benchmark_configuration(ctx):
         assert st[k] == r[k], k
     assert st["cut_value"] == r["cut_value"] and st["min_energy"] == r["min_energy"]
     assert st["max_energy"] == -r["min_energy"]
+
+
+def test_back_to_back_searches_read_their_own_reports(ctx):
+    """Searches on one context, back to back, alternating a short search (a
+    state that stops after its first report) with the benchmark's search:
+    every benchmark search must reproduce the oracle's record, whatever the
+    report slots still hold from the call before (a report is taken only when
+    its check word matches its number, record and state)."""
+    sys.path.insert(0, ROOT)
+    from bench import c2_instance
+
+    gold = load_c2_golden()
+    r = gold["result"]
+    W, x0 = c2_instance()
+    n = W.shape[0]
+    xc = np.unpackbits(np.frombuffer(bytes.fromhex(gold["state_packed_hex"]), np.uint8), bitorder="little")[:n]
+    dg = _lib.DeviceGraph.dense(W)
+    p = _lib.make_params(n, is_spin_glass=1, gray_iterations=n * n, walkers=128, seed_multiple=1,
+                         seed_engine=_lib.SEEDS_MFMA)
+    rng = np.random.default_rng(77)
+    m = 96
+    lo, hi = graphs.random_regular_edges(m, 3, rng)
+    Ws = graphs.dense_from_edges(m, lo, hi, graphs.pm1(lo.size, rng), dtype=np.int8)
+    ds = _lib.DeviceGraph.dense(Ws)
+    ps = _lib.make_params(m, is_spin_glass=1, gray_iterations=64, walkers=4, seed_multiple=4)
+    ys, _ = ds.local_search(rng.integers(0, 2, m).astype(np.uint8), ps)
+    ys_first = None
+    for k in range(40):
+        # the short search starts from its own optimum: it stops after one iteration
+        y, sts = ds.local_search(ys.copy(), ps)
+        if ys_first is None:
+            ys_first = (y.copy(), sts["cut_value"], sts["rounds"])
+        assert np.array_equal(y, ys_first[0]) and sts["cut_value"] == ys_first[1], k
+        x, st = dg.local_search(x0, p)
+        if not (np.array_equal(x, xc) and st["cut_value"] == r["cut_value"]):
+            log(f"  call {k}: GPU cut {st['cut_value']}, oracle cut {r['cut_value']}, rounds {st['rounds']}")
+        assert np.array_equal(x, xc) and st["cut_value"] == r["cut_value"], k
+        assert st["single_accepts"] == r["single_accepts"] and st["gray_evals"] == r["gray_evals"], k
+    log(f"  40 pairs of back-to-back searches, short search rounds {ys_first[2]}")
